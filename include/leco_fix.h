@@ -1,0 +1,138 @@
+/*
+ * leco_fix.h -- C ABI of the MI355X (gfx950) LeCo fixed-partition integer codec.
+ *
+ * Drop-in for Codecset::Leco_int<T> (headers/piecewise_fix_integer_template.h:14-338
+ * of the LeCo reference), T in {uint32_t, uint64_t}.  Every stream this
+ * library writes is byte-identical, block by block, to what Leco_int<T>::
+ * encodeArray8_int writes for the same block; a column stream is the
+ * concatenation of its blocks, block b at byte offsets[b].
+ *
+ * Conventions
+ *  - "d_" pointers are device (HBM) pointers, "h_" pointers are host pointers.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *    Column calls are asynchronous on that stream; host-buffer block calls
+ *    are synchronous.
+ *  - Block b of a column of n values with block size bs holds values
+ *    [b*bs, min((b+1)*bs, n)); there are ceil(n/bs) blocks
+ *    (experiments/leco_int_fix_template.cpp:98-127).
+ *  - A stream buffer must be readable for LECO_FIX_TAIL_PAD bytes past its
+ *    last block: decode and lookup read whole 16-byte words.
+ *  - Every call returns LECO_FIX_OK (0) or a positive error code; nothing
+ *    aborts.  leco_fix_strerror() names a code.
+ */
+#ifndef LECO_FIX_H_
+#define LECO_FIX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LECO_FIX_OK 0
+#define LECO_FIX_EINVAL 1   /* bad argument (size, width, null pointer, capacity) */
+#define LECO_FIX_EHIP 2     /* a HIP runtime call or kernel launch failed */
+#define LECO_FIX_ENOMEM 3   /* device or host allocation failed */
+
+#define LECO_FIX_TAIL_PAD 64
+
+/* Per-block parameters, kept next to a column stream so that a point lookup
+ * needs one table read and one payload read.  For a raw block (bits == width)
+ * theta0/theta1 are unused and payload_offset points at the raw values. */
+typedef struct leco_fix_block_param {
+    double theta0;           /* centred intercept, as stored in the block header */
+    double theta1;           /* slope, as stored in the block header */
+    uint64_t payload_offset; /* stream byte offset of the packed residuals / raw values */
+    uint32_t bits;           /* residual bit width l (byte 0 of the block) */
+    uint32_t length;         /* values in the block */
+} leco_fix_block_param;
+
+const char *leco_fix_strerror(int code);
+
+/* ---- sizes ------------------------------------------------------------- */
+
+/* ceil(n / block_size): the `blocks` count of Leco_int<T>::init
+ * (piecewise_fix_integer_template.h:22-26, leco_int_fix_template.cpp:100-105). */
+uint64_t leco_fix_block_count(uint64_t n, uint32_t block_size);
+
+/* Bytes a column stream buffer needs in the worst case (every block at the
+ * largest of raw and l = width-1), plus LECO_FIX_TAIL_PAD.  width_bits is 32 or 64. */
+uint64_t leco_fix_stream_bound(uint64_t n, uint32_t block_size, uint32_t width_bits);
+
+/* Size of one encoded block with bit width l (bit_write.h:257-323 and
+ * piecewise_fix_integer_template.h:118-137). */
+uint64_t leco_fix_block_bytes(uint32_t bits, uint32_t width_bits, uint32_t length);
+
+/* ---- column calls on device memory (asynchronous on `stream`) ----------- */
+
+/* Encodes a column: per block, Leco_int<T>::encodeArray8_int
+ * (piecewise_fix_integer_template.h:27-140).  Writes the concatenated block
+ * streams to d_stream, block start offsets to d_offsets[0..nb] (d_offsets[nb]
+ * = total bytes) and the lookup table to d_params[0..nb-1].
+ * stream_capacity must be >= leco_fix_stream_bound(n, block_size, width). */
+int leco_fix_encode_column_u32(const uint32_t *d_in, uint64_t n, uint32_t block_size,
+                               uint8_t *d_stream, uint64_t stream_capacity,
+                               uint64_t *d_offsets, leco_fix_block_param *d_params, void *stream);
+int leco_fix_encode_column_u64(const uint64_t *d_in, uint64_t n, uint32_t block_size,
+                               uint8_t *d_stream, uint64_t stream_capacity,
+                               uint64_t *d_offsets, leco_fix_block_param *d_params, void *stream);
+
+/* Full decode: per block, Leco_int<T>::decodeArray8
+ * (piecewise_fix_integer_template.h:142-204) into d_out[0..n).
+ * Deviation: a block with l == width-1 is decoded, where the reference
+ * copies it as raw bytes (:171-177) and returns wrong values. */
+int leco_fix_decode_column_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                               uint32_t block_size, uint32_t *d_out, void *stream);
+int leco_fix_decode_column_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                               uint32_t block_size, uint64_t *d_out, void *stream);
+
+/* Range decode: values [begin, end) of the column into d_out[0..end-begin). */
+int leco_fix_decode_range_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                              uint32_t block_size, uint64_t begin, uint64_t end,
+                              uint32_t *d_out, void *stream);
+int leco_fix_decode_range_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                              uint32_t block_size, uint64_t begin, uint64_t end,
+                              uint64_t *d_out, void *stream);
+
+/* Point lookups: d_out[q] = value d_idx[q] of the column; per query
+ * Leco_int<T>::randomdecodeArray8(block idx/bs, idx%bs)
+ * (piecewise_fix_integer_template.h:305-337, leco_int_fix_template.cpp:189-193).
+ * An index >= n yields 0. */
+int leco_fix_lookup_u32(const uint8_t *d_stream, const leco_fix_block_param *d_params,
+                        uint64_t n, uint32_t block_size, const uint64_t *d_idx, uint64_t nq,
+                        uint32_t *d_out, void *stream);
+int leco_fix_lookup_u64(const uint8_t *d_stream, const leco_fix_block_param *d_params,
+                        uint64_t n, uint32_t block_size, const uint64_t *d_idx, uint64_t nq,
+                        uint64_t *d_out, void *stream);
+
+/* Rebuilds d_params from block headers, for a stream written elsewhere. */
+int leco_fix_build_params(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                          uint32_t block_size, uint32_t width_bits,
+                          leco_fix_block_param *d_params, void *stream);
+
+/* ---- one block in host memory (synchronous; the Leco_int<T> surface) ---- */
+
+/* Leco_int<T>::encodeArray8_int(data, length, res, nvalue)
+ * (piecewise_fix_integer_template.h:27): encodes h_data[0..length) as one
+ * block into h_res and stores the byte count in *bytes_written. */
+int leco_fix_encode_block_u32(const uint32_t *h_data, uint64_t length, uint8_t *h_res,
+                              uint64_t res_capacity, uint64_t *bytes_written);
+int leco_fix_encode_block_u64(const uint64_t *h_data, uint64_t length, uint8_t *h_res,
+                              uint64_t res_capacity, uint64_t *bytes_written);
+
+/* Leco_int<T>::decodeArray8(in, length, out, nvalue) (:142): decodes one
+ * block of `length` values.  The block's size follows from its header. */
+int leco_fix_decode_block_u32(const uint8_t *h_in, uint64_t length, uint32_t *h_out);
+int leco_fix_decode_block_u64(const uint8_t *h_in, uint64_t length, uint64_t *h_out);
+
+/* Leco_int<T>::randomdecodeArray8(in, to_find, out, nvalue) (:305): value
+ * to_find of one block. */
+int leco_fix_random_decode_u32(const uint8_t *h_in, uint32_t to_find, uint32_t *h_value);
+int leco_fix_random_decode_u64(const uint8_t *h_in, uint32_t to_find, uint64_t *h_value);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LECO_FIX_H_ */

@@ -1,0 +1,265 @@
+// C ABI of the gfx950 LeCo_fix codec (declared in include/leco_fix.h).
+//
+// Column calls validate their arguments on the host and enqueue kernels on
+// the caller's stream.  The one-block host-buffer calls reproduce the
+// Leco_int<T> surface (piecewise_fix_integer_template.h:27,142,305): they
+// stage the block through a per-thread device workspace, run the same
+// kernels and copy the result back, synchronously.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include "leco_internal.h"
+
+using leco::kMaxBlockSize;
+
+namespace {
+
+int hip_status(hipError_t e) { return e == hipSuccess ? LECO_FIX_OK : LECO_FIX_EHIP; }
+
+bool valid_bs(uint32_t bs) { return bs >= 1 && bs <= kMaxBlockSize; }
+
+template <typename T>
+int encode_column(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t cap, uint64_t *offs,
+                  leco_fix_block_param *prm, void *st) {
+    if (n == 0 || !valid_bs(bs) || !in || !stream || !offs || !prm) return LECO_FIX_EINVAL;
+    if (cap < leco_fix_stream_bound(n, bs, 8 * sizeof(T))) return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_encode<T>(in, n, bs, stream, offs, prm, (hipStream_t)st));
+}
+
+template <typename T>
+int decode_range(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, uint64_t lo, uint64_t hi,
+                 T *out, void *st) {
+    if (!valid_bs(bs) || !stream || !offs || !out || lo >= hi || hi > n) return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_decode<T>(stream, offs, n, bs, lo, hi, out, (hipStream_t)st));
+}
+
+template <typename T>
+int lookup(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, uint32_t bs, const uint64_t *idx,
+           uint64_t nq, T *out, void *st) {
+    if (!valid_bs(bs) || n == 0 || !stream || !prm) return LECO_FIX_EINVAL;
+    if (nq == 0) return LECO_FIX_OK;
+    if (!idx || !out) return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_lookup<T>(stream, prm, n, bs, idx, nq, out, (hipStream_t)st));
+}
+
+// ---- per-thread staging for the one-block host-buffer calls -------------
+
+struct Staging {
+    hipStream_t stream = nullptr;
+    void *dev = nullptr;
+    size_t cap = 0;
+    ~Staging() {
+        if (dev) (void)hipFree(dev);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    // Device buffer of at least `bytes`; 256-byte aligned sub-buffers are
+    // carved by the caller.
+    int reserve(size_t bytes) {
+        if (!stream && hipStreamCreate(&stream) != hipSuccess) return LECO_FIX_EHIP;
+        if (bytes <= cap) return LECO_FIX_OK;
+        if (dev) (void)hipFree(dev);
+        dev = nullptr;
+        cap = 0;
+        if (hipMalloc(&dev, bytes) != hipSuccess) return LECO_FIX_ENOMEM;
+        cap = bytes;
+        return LECO_FIX_OK;
+    }
+};
+
+thread_local Staging g_stage;
+
+size_t up256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+template <typename T>
+int encode_block_host(const T *h, uint64_t len, uint8_t *res, uint64_t rcap, uint64_t *written) {
+    if (!h || !res || !written || len == 0 || len > kMaxBlockSize) return LECO_FIX_EINVAL;
+    const uint32_t bs = (uint32_t)len;
+    const size_t in_b = up256(len * sizeof(T));
+    const size_t st_b = up256(leco_fix_stream_bound(len, bs, 8 * sizeof(T)));
+    const size_t off_b = up256(2 * sizeof(uint64_t));
+    const size_t prm_b = up256(sizeof(leco_fix_block_param));
+    int rc = g_stage.reserve(in_b + st_b + off_b + prm_b);
+    if (rc) return rc;
+    uint8_t *base = (uint8_t *)g_stage.dev;
+    T *d_in = (T *)base;
+    uint8_t *d_st = base + in_b;
+    uint64_t *d_off = (uint64_t *)(base + in_b + st_b);
+    leco_fix_block_param *d_prm = (leco_fix_block_param *)(base + in_b + st_b + off_b);
+    hipStream_t s = g_stage.stream;
+    if (hipMemcpyAsync(d_in, h, len * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (leco::launch_encode<T>(d_in, len, bs, d_st, d_off, d_prm, s) != hipSuccess) return LECO_FIX_EHIP;
+    uint64_t offs[2];
+    if (hipMemcpyAsync(offs, d_off, sizeof(offs), hipMemcpyDeviceToHost, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return LECO_FIX_EHIP;
+    if (offs[1] > rcap) return LECO_FIX_EINVAL;
+    if (hipMemcpyAsync(res, d_st, offs[1], hipMemcpyDeviceToHost, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (hipStreamSynchronize(s) != hipSuccess) return LECO_FIX_EHIP;
+    *written = offs[1];
+    return LECO_FIX_OK;
+}
+
+// Bytes of an encoded block of `len` values, from its header byte.
+template <typename T>
+uint64_t host_block_bytes(const uint8_t *h_in, uint64_t len) {
+    constexpr uint32_t W = 8 * sizeof(T);
+    const uint32_t l = h_in[0];
+    if (l == 127) return 1 + sizeof(T);
+    if (l == 126) return 1 + 2 * sizeof(T);
+    return leco::block_bytes(l, W, (uint32_t)len);
+}
+
+template <typename T>
+int decode_block_host(const uint8_t *h_in, uint64_t len, T *h_out) {
+    if (!h_in || !h_out || len == 0 || len > kMaxBlockSize) return LECO_FIX_EINVAL;
+    const uint64_t bytes = host_block_bytes<T>(h_in, len);
+    const size_t st_b = up256(bytes + LECO_FIX_TAIL_PAD);
+    const size_t off_b = up256(2 * sizeof(uint64_t));
+    const size_t out_b = up256(len * sizeof(T));
+    int rc = g_stage.reserve(st_b + off_b + out_b);
+    if (rc) return rc;
+    uint8_t *base = (uint8_t *)g_stage.dev;
+    uint8_t *d_st = base;
+    uint64_t *d_off = (uint64_t *)(base + st_b);
+    T *d_out = (T *)(base + st_b + off_b);
+    hipStream_t s = g_stage.stream;
+    const uint64_t offs[2] = {0, bytes};
+    if (hipMemcpyAsync(d_st, h_in, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (hipMemsetAsync(d_st + bytes, 0, st_b - bytes, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (hipMemcpyAsync(d_off, offs, sizeof(offs), hipMemcpyHostToDevice, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (leco::launch_decode<T>(d_st, d_off, len, (uint32_t)len, 0, len, d_out, s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    if (hipMemcpyAsync(h_out, d_out, len * sizeof(T), hipMemcpyDeviceToHost, s) != hipSuccess) return LECO_FIX_EHIP;
+    return hip_status(hipStreamSynchronize(s));
+}
+
+template <typename T>
+int random_decode_host(const uint8_t *h_in, uint32_t to_find, T *h_value) {
+    constexpr uint32_t W = 8 * sizeof(T);
+    if (!h_in || !h_value) return LECO_FIX_EINVAL;
+    const uint32_t l = h_in[0];
+    // Only the header and the bytes holding values [0, to_find] are read.
+    const uint64_t len = (uint64_t)to_find + 1;
+    uint64_t bytes;
+    if (l >= W) bytes = 1 + len * sizeof(T);
+    else if (l == 0) bytes = 17;
+    else bytes = 17 + (len * l + 7) / 8;
+    const size_t st_b = up256(bytes + LECO_FIX_TAIL_PAD);
+    const size_t off_b = up256(2 * sizeof(uint64_t));
+    const size_t prm_b = up256(sizeof(leco_fix_block_param));
+    const size_t q_b = up256(2 * sizeof(uint64_t));
+    int rc = g_stage.reserve(st_b + off_b + prm_b + q_b);
+    if (rc) return rc;
+    uint8_t *base = (uint8_t *)g_stage.dev;
+    uint8_t *d_st = base;
+    uint64_t *d_off = (uint64_t *)(base + st_b);
+    leco_fix_block_param *d_prm = (leco_fix_block_param *)(base + st_b + off_b);
+    uint64_t *d_q = (uint64_t *)(base + st_b + off_b + prm_b);
+    T *d_v = (T *)(d_q + 1);
+    hipStream_t s = g_stage.stream;
+    const uint64_t offs[2] = {0, bytes};
+    if (hipMemcpyAsync(d_st, h_in, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (hipMemsetAsync(d_st + bytes, 0, st_b - bytes, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (hipMemcpyAsync(d_off, offs, sizeof(offs), hipMemcpyHostToDevice, s) != hipSuccess) return LECO_FIX_EHIP;
+    const uint64_t q = to_find;
+    if (hipMemcpyAsync(d_q, &q, sizeof(q), hipMemcpyHostToDevice, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (leco::launch_build_params(d_st, d_off, len, (uint32_t)len, W, d_prm, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (leco::launch_lookup<T>(d_st, d_prm, len, (uint32_t)len, d_q, 1, d_v, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (hipMemcpyAsync(h_value, d_v, sizeof(T), hipMemcpyDeviceToHost, s) != hipSuccess) return LECO_FIX_EHIP;
+    return hip_status(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *leco_fix_strerror(int code) {
+    switch (code) {
+    case LECO_FIX_OK: return "ok";
+    case LECO_FIX_EINVAL: return "invalid argument";
+    case LECO_FIX_EHIP: return "HIP runtime error";
+    case LECO_FIX_ENOMEM: return "out of memory";
+    default: return "unknown error";
+    }
+}
+
+uint64_t leco_fix_block_count(uint64_t n, uint32_t block_size) {
+    return block_size ? (n + block_size - 1) / block_size : 0;
+}
+
+uint64_t leco_fix_block_bytes(uint32_t bits, uint32_t width_bits, uint32_t length) {
+    return leco::block_bytes(bits, width_bits, length);
+}
+
+uint64_t leco_fix_stream_bound(uint64_t n, uint32_t bs, uint32_t w) {
+    if (n == 0 || bs == 0 || (w != 32 && w != 64)) return 0;
+    const uint64_t nb = (n + bs - 1) / bs, last = n - (nb - 1) * bs;
+    auto worst = [w](uint32_t m) {
+        uint64_t a = leco::block_bytes(w - 1, w, m), b = leco::block_bytes(w, w, m);
+        return a > b ? a : b;
+    };
+    return (nb - 1) * worst(bs) + worst((uint32_t)last) + LECO_FIX_TAIL_PAD;
+}
+
+int leco_fix_encode_column_u32(const uint32_t *d_in, uint64_t n, uint32_t bs, uint8_t *d_stream, uint64_t cap,
+                               uint64_t *d_offsets, leco_fix_block_param *d_params, void *stream) {
+    return encode_column<uint32_t>(d_in, n, bs, d_stream, cap, d_offsets, d_params, stream);
+}
+int leco_fix_encode_column_u64(const uint64_t *d_in, uint64_t n, uint32_t bs, uint8_t *d_stream, uint64_t cap,
+                               uint64_t *d_offsets, leco_fix_block_param *d_params, void *stream) {
+    return encode_column<uint64_t>(d_in, n, bs, d_stream, cap, d_offsets, d_params, stream);
+}
+
+int leco_fix_decode_column_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                               uint32_t *d_out, void *stream) {
+    return decode_range<uint32_t>(d_stream, d_offsets, n, bs, 0, n, d_out, stream);
+}
+int leco_fix_decode_column_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                               uint64_t *d_out, void *stream) {
+    return decode_range<uint64_t>(d_stream, d_offsets, n, bs, 0, n, d_out, stream);
+}
+
+int leco_fix_decode_range_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                              uint64_t begin, uint64_t end, uint32_t *d_out, void *stream) {
+    return decode_range<uint32_t>(d_stream, d_offsets, n, bs, begin, end, d_out, stream);
+}
+int leco_fix_decode_range_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                              uint64_t begin, uint64_t end, uint64_t *d_out, void *stream) {
+    return decode_range<uint64_t>(d_stream, d_offsets, n, bs, begin, end, d_out, stream);
+}
+
+int leco_fix_lookup_u32(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n, uint32_t bs,
+                        const uint64_t *d_idx, uint64_t nq, uint32_t *d_out, void *stream) {
+    return lookup<uint32_t>(d_stream, d_params, n, bs, d_idx, nq, d_out, stream);
+}
+int leco_fix_lookup_u64(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n, uint32_t bs,
+                        const uint64_t *d_idx, uint64_t nq, uint64_t *d_out, void *stream) {
+    return lookup<uint64_t>(d_stream, d_params, n, bs, d_idx, nq, d_out, stream);
+}
+
+int leco_fix_build_params(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                          uint32_t w, leco_fix_block_param *d_params, void *stream) {
+    if (n == 0 || !valid_bs(bs) || (w != 32 && w != 64) || !d_stream || !d_offsets || !d_params)
+        return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_build_params(d_stream, d_offsets, n, bs, w, d_params, (hipStream_t)stream));
+}
+
+int leco_fix_encode_block_u32(const uint32_t *h, uint64_t len, uint8_t *res, uint64_t cap, uint64_t *written) {
+    return encode_block_host<uint32_t>(h, len, res, cap, written);
+}
+int leco_fix_encode_block_u64(const uint64_t *h, uint64_t len, uint8_t *res, uint64_t cap, uint64_t *written) {
+    return encode_block_host<uint64_t>(h, len, res, cap, written);
+}
+int leco_fix_decode_block_u32(const uint8_t *h_in, uint64_t len, uint32_t *h_out) {
+    return decode_block_host<uint32_t>(h_in, len, h_out);
+}
+int leco_fix_decode_block_u64(const uint8_t *h_in, uint64_t len, uint64_t *h_out) {
+    return decode_block_host<uint64_t>(h_in, len, h_out);
+}
+int leco_fix_random_decode_u32(const uint8_t *h_in, uint32_t to_find, uint32_t *v) {
+    return random_decode_host<uint32_t>(h_in, to_find, v);
+}
+int leco_fix_random_decode_u64(const uint8_t *h_in, uint32_t to_find, uint64_t *v) {
+    return random_decode_host<uint64_t>(h_in, to_find, v);
+}
+
+}  // extern "C"

@@ -1,0 +1,99 @@
+// Device helpers shared by the gfx950 LeCo_fix kernels.
+//
+// Numerics: every kernel is compiled with -ffp-contract=off, so the only fused
+// multiply-adds are the explicit __builtin_fma calls below.  They sit where
+// the reference's -O3 -march=native x86 build fuses (lr.h:72-83,
+// piecewise_fix_integer_template.h:40,54, bit_read.h:247,1187), which makes
+// theta and every prediction bit-identical to the reference CPU path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace leco {
+
+constexpr uint64_t kX86DefaultNaN = 0xFFF8000000000000ull;
+constexpr int kWave = 64;
+
+// x86 CVTTSD2SI (r64): truncation toward zero; NaN or out of [-2^63, 2^63)
+// gives INT64_MIN.  The reference converts predictions with a plain
+// (int64_t)/(long long) cast, which is this instruction on x86.
+__device__ __forceinline__ int64_t cvtt64(double x) {
+    if (!(x >= -9223372036854775808.0 && x < 9223372036854775808.0)) return INT64_MIN;
+    return (int64_t)x;
+}
+
+// p(i) = cvtt64(theta0 + theta1 * i) with one rounding (fma).
+__device__ __forceinline__ int64_t predict(double t0, double t1, uint32_t i) {
+    return cvtt64(__builtin_fma(t1, (double)i, t0));
+}
+
+// Bit length (lr.h:116-130: 64 - clzll(v)).
+__device__ __forceinline__ int bitlen64(uint64_t v) { return v ? 64 - __clzll((long long)v) : 0; }
+
+// Sign/magnitude residual of y against prediction p
+// (piecewise_fix_integer_template.h:55-64): the comparison is signed int64
+// for 32-bit values and unsigned for 64-bit values.
+template <int W>
+__device__ __forceinline__ uint64_t residual_field(uint64_t y, int64_t p, int l, uint64_t *mag_out) {
+    bool pos = (W == 64) ? (y > (uint64_t)p) : ((int64_t)y > p);
+    uint64_t mag = pos ? y - (uint64_t)p : (uint64_t)p - y;
+    if (W == 32) mag &= 0xFFFFFFFFull;
+    *mag_out = mag;
+    if (l <= 0) return 0;
+    return (mag & ((1ull << (l - 1)) - 1)) | ((uint64_t)pos << (l - 1));
+}
+
+// Value from a prediction and an l-bit field (bit_read.h:238-254).
+__device__ __forceinline__ uint64_t apply_field(int64_t p, uint64_t field, int l) {
+    uint64_t mag = field & ((1ull << (l - 1)) - 1);
+    return ((field >> (l - 1)) & 1) ? (uint64_t)p + mag : (uint64_t)p - mag;
+}
+
+__device__ __forceinline__ uint64_t low_mask(int l) { return l >= 64 ? ~0ull : ((1ull << l) - 1); }
+
+// Little-endian 8-byte read at any byte address, from two aligned loads.
+// Reads up to 15 bytes past addr+8; callers keep that inside the buffer.
+__device__ __forceinline__ uint64_t load_u64_any(const uint8_t *p) {
+    uintptr_t a = (uintptr_t)p;
+    const uint64_t *q = (const uint64_t *)(a & ~(uintptr_t)7);
+    int s = (int)(a & 7) * 8;
+    uint64_t w0 = q[0];
+    if (!s) return w0;
+    uint64_t w1 = q[1];
+    return (w0 >> s) | (w1 << (64 - s));
+}
+
+// `bits` bits starting at bit `bit` of the bytes at base (l <= 64).
+__device__ __forceinline__ uint64_t load_bits(const uint8_t *base, uint64_t bit, int bits) {
+    const uint8_t *p = base + (bit >> 3);
+    uintptr_t a = (uintptr_t)p;
+    const uint64_t *q = (const uint64_t *)(a & ~(uintptr_t)7);
+    int s = (int)(a & 7) * 8 + (int)(bit & 7);
+    uint64_t w0 = q[0], w1 = q[1];
+    uint64_t v = s ? (w0 >> s) | (w1 << (64 - s)) : w0;
+    return v & low_mask(bits);
+}
+
+__device__ __forceinline__ double bits_to_double(uint64_t b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ uint64_t double_to_bits(double d) { return (uint64_t)__double_as_longlong(d); }
+
+// ---- wave / workgroup reductions (wave64) ----------------------------------
+
+__device__ __forceinline__ int64_t wave_max_i64(int64_t v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) { int64_t u = __shfl_xor((long long)v, o, kWave); v = u > v ? u : v; }
+    return v;
+}
+__device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) { int64_t u = __shfl_xor((long long)v, o, kWave); v = u < v ? u : v; }
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        uint64_t u = (uint64_t)__shfl_xor((long long)v, o, kWave);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+}  // namespace leco

@@ -1,0 +1,82 @@
+"""Loads libleco_fix.so (the gfx950 HIP library) and declares its C ABI.
+
+There is no CPU fallback: if the library is not built, loading fails with an
+error that says how to build it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libleco_fix.so")
+
+c_u8p = ctypes.c_void_p
+c_u64 = ctypes.c_uint64
+c_u32 = ctypes.c_uint32
+c_ptr = ctypes.c_void_p
+c_int = ctypes.c_int
+
+LECO_FIX_OK = 0
+LECO_FIX_TAIL_PAD = 64
+
+
+class LecoError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: {strerror(code)} (code {code})")
+        self.code = code
+
+
+class BlockParam(ctypes.Structure):
+    """leco_fix_block_param (include/leco_fix.h)."""
+    _fields_ = [("theta0", ctypes.c_double), ("theta1", ctypes.c_double),
+                ("payload_offset", ctypes.c_uint64), ("bits", ctypes.c_uint32), ("length", ctypes.c_uint32)]
+
+
+# name -> (restype, argtypes); every function declared in include/leco_fix.h
+SIGNATURES = {
+    "leco_fix_strerror": (ctypes.c_char_p, [c_int]),
+    "leco_fix_block_count": (c_u64, [c_u64, c_u32]),
+    "leco_fix_stream_bound": (c_u64, [c_u64, c_u32, c_u32]),
+    "leco_fix_block_bytes": (c_u64, [c_u32, c_u32, c_u32]),
+    "leco_fix_build_params": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_u32, c_ptr, c_ptr]),
+}
+for _w in ("u32", "u64"):
+    SIGNATURES.update({
+        f"leco_fix_encode_column_{_w}": (c_int, [c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr, c_ptr]),
+        f"leco_fix_decode_column_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_ptr]),
+        f"leco_fix_decode_range_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_u64, c_u64, c_ptr, c_ptr]),
+        f"leco_fix_lookup_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr]),
+        f"leco_fix_encode_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr, c_u64, ctypes.POINTER(c_u64)]),
+        f"leco_fix_decode_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr]),
+        f"leco_fix_random_decode_{_w}": (c_int, [c_ptr, c_u32, c_ptr]),
+    })
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python learn-to-compress_amd/build.py` "
+                              "(or __graft_entry__.build()) first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def strerror(code: int) -> str:
+    try:
+        return lib().leco_fix_strerror(code).decode()
+    except ImportError:
+        return "library not loaded"
+
+
+def check(code: int, what: str) -> None:
+    if code != LECO_FIX_OK:
+        raise LecoError(code, what)

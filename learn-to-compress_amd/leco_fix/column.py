@@ -1,0 +1,129 @@
+"""A compressed column resident in HBM, over the C ABI's column calls.
+
+torch provides the device allocations and the stream; every byte of codec
+work runs in the HIP kernels of libleco_fix.so.  Values are held in int32 /
+int64 tensors and read as uint32 / uint64 (same bits).
+
+Reference behaviour mirrored: experiments/leco_int_fix_template.cpp:98-213
+partitions a column into fixed blocks (:98-106), encodes each block
+(:111-127), decodes all blocks (:142-153) and looks values up by
+block = idx / block_size, position = idx % block_size (:189-193).
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from ._lib import LECO_FIX_TAIL_PAD, BlockParam, check, lib
+
+_STORAGE = {32: torch.int32, 64: torch.int64}
+PARAM_BYTES = ctypes.sizeof(BlockParam)
+
+
+def width_of(t: torch.Tensor) -> int:
+    if t.dtype in (torch.int32, torch.uint32):
+        return 32
+    if t.dtype in (torch.int64, torch.uint64):
+        return 64
+    raise TypeError(f"columns hold 32- or 64-bit integers, not {t.dtype}")
+
+
+def _stream_handle(stream) -> int:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream
+
+
+def _as_storage(t: torch.Tensor) -> torch.Tensor:
+    w = width_of(t)
+    return t.view(_STORAGE[w]) if t.dtype != _STORAGE[w] else t
+
+
+class Column:
+    """One column: concatenated block streams + offsets + param table."""
+
+    def __init__(self, stream: torch.Tensor, offsets: torch.Tensor, params: torch.Tensor,
+                 n: int, block_size: int, width: int, nbytes: int):
+        self.stream = stream          # uint8 [capacity], >= nbytes + tail pad
+        self.offsets = offsets        # int64 [nb + 1] (uint64 bits)
+        self.params = params          # uint8 [nb * 32] (leco_fix_block_param)
+        self.n = n
+        self.block_size = block_size
+        self.width = width
+        self.nbytes = nbytes
+
+    @property
+    def num_blocks(self) -> int:
+        return int(lib().leco_fix_block_count(self.n, self.block_size))
+
+    @property
+    def compression_ratio(self) -> float:
+        return self.nbytes / (self.n * self.width // 8)
+
+    @classmethod
+    def encode(cls, values: torch.Tensor, block_size: int, stream=None, compact: bool = False) -> "Column":
+        """Leco_int<T>::encodeArray8_int over every block of `values` (device tensor)."""
+        if not values.is_cuda:
+            raise ValueError("Column.encode takes a device tensor")
+        values = _as_storage(values.contiguous())
+        w = width_of(values)
+        n = values.numel()
+        L = lib()
+        bound = int(L.leco_fix_stream_bound(n, block_size, w))
+        nb = int(L.leco_fix_block_count(n, block_size))
+        dev = values.device
+        st = torch.empty(bound, dtype=torch.uint8, device=dev)
+        offs = torch.empty(nb + 1, dtype=torch.int64, device=dev)
+        prm = torch.empty(nb * PARAM_BYTES, dtype=torch.uint8, device=dev)
+        fn = L.leco_fix_encode_column_u32 if w == 32 else L.leco_fix_encode_column_u64
+        check(fn(values.data_ptr(), n, block_size, st.data_ptr(), bound, offs.data_ptr(), prm.data_ptr(),
+                 _stream_handle(stream)), "encode_column")
+        nbytes = int(offs[nb].item())
+        col = cls(st, offs, prm, n, block_size, w, nbytes)
+        if compact:
+            col.compact()
+        return col
+
+    def compact(self) -> None:
+        """Shrinks the stream buffer to the encoded size plus the tail pad."""
+        st = torch.zeros(self.nbytes + LECO_FIX_TAIL_PAD, dtype=torch.uint8, device=self.stream.device)
+        st[:self.nbytes].copy_(self.stream[:self.nbytes])
+        self.stream = st
+
+    def stream_bytes(self) -> bytes:
+        return self.stream[:self.nbytes].cpu().numpy().tobytes()
+
+    def decode(self, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """Leco_int<T>::decodeArray8 over every block."""
+        return self.decode_range(0, self.n, out=out, stream=stream)
+
+    def decode_range(self, begin: int, end: int, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        if out is None:
+            out = torch.empty(end - begin, dtype=_STORAGE[self.width], device=self.stream.device)
+        L = lib()
+        fn = L.leco_fix_decode_range_u32 if self.width == 32 else L.leco_fix_decode_range_u64
+        check(fn(self.stream.data_ptr(), self.offsets.data_ptr(), self.n, self.block_size, begin, end,
+                 out.data_ptr(), _stream_handle(stream)), "decode_range")
+        return out
+
+    def lookup(self, idx: torch.Tensor, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+        """Leco_int<T>::randomdecodeArray8 for every index in idx (int64 device tensor)."""
+        idx = idx.contiguous()
+        if idx.dtype not in (torch.int64, torch.uint64):
+            idx = idx.to(torch.int64)
+        if out is None:
+            out = torch.empty(idx.numel(), dtype=_STORAGE[self.width], device=self.stream.device)
+        L = lib()
+        fn = L.leco_fix_lookup_u32 if self.width == 32 else L.leco_fix_lookup_u64
+        check(fn(self.stream.data_ptr(), self.params.data_ptr(), self.n, self.block_size, idx.data_ptr(),
+                 idx.numel(), out.data_ptr(), _stream_handle(stream)), "lookup")
+        return out
+
+    def rebuild_params(self, stream=None) -> torch.Tensor:
+        """Param table recomputed from the block headers (leco_fix_build_params)."""
+        prm = torch.empty_like(self.params)
+        check(lib().leco_fix_build_params(self.stream.data_ptr(), self.offsets.data_ptr(), self.n,
+                                          self.block_size, self.width, prm.data_ptr(), _stream_handle(stream)),
+              "build_params")
+        return prm

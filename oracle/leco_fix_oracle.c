@@ -1,0 +1,384 @@
+/*
+ * leco_fix_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A CPU restatement of LeCo's fixed-length-partition integer codec,
+ * Codecset::Leco_int<T> (headers/piecewise_fix_integer_template.h:14-338),
+ * for T = uint32_t and T = uint64_t.  It is the checker for the GPU library:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * it.  The product path (learn-to-compress_amd/) never links, calls or falls
+ * back to it.
+ *
+ * Every function names the reference lines it restates.  The reference itself
+ * cannot be compiled here (headers/common.h:12-19,34,65 needs Boost, GMP and
+ * Eigen, none of which is installed), so this file is pinned by:
+ *   - the independent pure-Python restatement tests/spec_leco.py (exact
+ *     rational fma, a bit-at-a-time writer/reader) and the fixtures it wrote
+ *     under tests/golden/;
+ *   - closed forms: exact-linear blocks give theta1 = b, theta0 = a; constant
+ *     blocks give theta1 = 0 and l = 0; partition and size formulas;
+ *   - invariants: decode(encode(x)) == x, lookup(i) == decode[i],
+ *     every residual magnitude < 2^(l-1).
+ * Parity unpinned: the exact theta bits of the fit in the rounding regime
+ * (partial sums of y or i*y above 2^53).  They follow the recipe below, which
+ * is what GCC's -O3 -march=native build of lr.h produces on an FMA-capable x86
+ * host (GNU mode contracts a*b+c into fma); no run of the reference confirms
+ * it.  Also unpinned: the x86 default-NaN bits used for one-value blocks.
+ *
+ * FP recipe (build with -ffp-contract=off so only the explicit fma() fuse):
+ *   fit      lr.h:66-85   sx += x; sy += y; sxx = fma(x,x,sxx); sxy = fma(x,y,sxy)
+ *                         ccc = fma(sxy,m,-(sx*sy)); xxx = fma(sxx,m,-(sx*sx))
+ *                         t1 = ccc/xxx; t0 = fma(-t1,sx,sy)/m
+ *   predict  piecewise_fix_integer_template.h:40,54; bit_read.h:247,1187
+ *                         p(i) = cvtt64(fma(t1,(double)i,t0))
+ * cvtt64 is x86 CVTTSD2SI r64: truncate toward zero, NaN or out-of-range
+ * gives INT64_MIN.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ORACLE_API __attribute__((visibility("default")))
+
+static const uint64_t X86_DEFAULT_NAN = 0xFFF8000000000000ull;
+
+static double bits_to_double(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
+
+/* x86 CVTTSD2SI with a 64-bit destination. */
+ORACLE_API int64_t leco_oracle_cvtt64(double x)
+{
+    if (!(x >= -9223372036854775808.0 && x < 9223372036854775808.0))
+        return INT64_MIN;
+    return (int64_t)x;
+}
+
+static inline uint64_t load_val(const void *y, int width, uint64_t i)
+{
+    return width == 32 ? (uint64_t)((const uint32_t *)y)[i] : ((const uint64_t *)y)[i];
+}
+
+static inline void store_val(void *y, int width, uint64_t i, uint64_t v)
+{
+    if (width == 32) ((uint32_t *)y)[i] = (uint32_t)v;
+    else ((uint64_t *)y)[i] = v;
+}
+
+/* bits_int_T (lr.h:116-130): 64 - clzll(v), 0 for v == 0. */
+static inline int bitlen64(uint64_t v) { return v ? 64 - __builtin_clzll(v) : 0; }
+
+static inline int64_t predict(double t0, double t1, uint64_t i)
+{
+    return leco_oracle_cvtt64(fma(t1, (double)i, t0));
+}
+
+/* lr_int_T<T>::caltheta, lr.h:66-85, under the FMA recipe above. */
+ORACLE_API void leco_oracle_fit(const void *y, int width, uint32_t m, double *t0, double *t1)
+{
+    double sx = 0, sy = 0, sxx = 0, sxy = 0;
+    for (uint32_t i = 0; i < m; i++) {
+        double xi = (double)i, yi = (double)load_val(y, width, i);
+        sx = sx + xi;
+        sy = sy + yi;
+        sxx = fma(xi, xi, sxx);
+        sxy = fma(xi, yi, sxy);
+    }
+    double md = (double)m;
+    double ccc = fma(sxy, md, -(sx * sy));
+    double xxx = fma(sxx, md, -(sx * sx));
+    if (xxx == 0.0) {
+        /* 0/0 for a one-value block: x86 returns its default NaN, and the
+           later fnmadd/div/add steps carry it into theta0 unchanged. */
+        *t0 = *t1 = bits_to_double(X86_DEFAULT_NAN);
+        return;
+    }
+    double th1 = ccc / xxx;
+    *t1 = th1;
+    *t0 = fma(-th1, sx, sy) / md;
+}
+
+/* Payload size of a packed block, bit_write.h:263-321: the writer emits
+   E+1 bytes (E = ceil(l*n/8)) and then 8 - E%8 zero bytes. */
+ORACLE_API uint64_t leco_oracle_payload_bytes(int l, uint32_t n)
+{
+    uint64_t e = ((uint64_t)l * n + 7) / 8;
+    return 8 * (e / 8) + 9;
+}
+
+ORACLE_API uint64_t leco_oracle_block_bytes(int l, int width, uint32_t n)
+{
+    if (l >= width) return 1 + (uint64_t)n * (width / 8);
+    return 17 + (l ? leco_oracle_payload_bytes(l, n) : 0);
+}
+
+/* Largest block a column of n values at block size bs can need. */
+ORACLE_API uint64_t leco_oracle_stream_bound(uint64_t n, uint32_t bs, int width)
+{
+    if (n == 0) return 0;
+    uint64_t nb = (n + bs - 1) / bs, last = n - (nb - 1) * bs, tot = 0;
+    uint64_t full = leco_oracle_block_bytes(width - 1, width, bs);
+    uint64_t fraw = leco_oracle_block_bytes(width, width, bs);
+    uint64_t lb = leco_oracle_block_bytes(width - 1, width, (uint32_t)last);
+    uint64_t lraw = leco_oracle_block_bytes(width, width, (uint32_t)last);
+    tot = (nb - 1) * (full > fraw ? full : fraw) + (lb > lraw ? lb : lraw);
+    return tot;
+}
+
+/* Leco_int<T>::encodeArray8_int, piecewise_fix_integer_template.h:27-140,
+   with write_delta_int_T (bit_write.h:257-323).  Returns bytes written. */
+ORACLE_API uint64_t leco_oracle_encode_block(const void *y, int width, uint32_t m, uint8_t *out)
+{
+    double t0, t1;
+    leco_oracle_fit(y, width, m, &t0, &t1);
+
+    /* pass 1 (:36-45): int64 residuals under the first theta; wrapping */
+    int64_t mx = INT64_MIN, mn = INT64_MAX;
+    for (uint32_t i = 0; i < m; i++) {
+        uint64_t yi = load_val(y, width, i);
+        int64_t d = (int64_t)(yi - (uint64_t)predict(t0, t1, i));
+        if (d > mx) mx = d;
+        if (d < mn) mn = d;
+    }
+    /* :46 -- the int64 sum wraps; x/2. is exact, so fused or not is equal */
+    if (!isnan(t0))
+        t0 = t0 + (double)(int64_t)((uint64_t)mx + (uint64_t)mn) * 0.5;
+
+    /* pass 2 (:48-72): sign/magnitude residuals; the comparison is signed
+       int64 for uint32 and unsigned for uint64 (usual conversions) */
+    uint64_t wmask = width == 32 ? 0xFFFFFFFFull : ~0ull;
+    uint64_t maxmag = 0;
+    for (uint32_t i = 0; i < m; i++) {
+        uint64_t yi = load_val(y, width, i);
+        int64_t p = predict(t0, t1, i);
+        int pos = width == 64 ? (yi > (uint64_t)p) : ((int64_t)yi > p);
+        uint64_t mag = (pos ? yi - (uint64_t)p : (uint64_t)p - yi) & wmask;
+        if (mag > maxmag) maxmag = mag;
+    }
+    /* :108-117 */
+    int l = maxmag ? bitlen64(maxmag) + 1 : 0;
+    if (l > width) l = width;
+    out[0] = (uint8_t)l;
+    if (l == width) { /* :120-128 raw block */
+        for (uint32_t i = 0; i < m; i++) {
+            uint64_t v = load_val(y, width, i);
+            memcpy(out + 1 + (uint64_t)i * (width / 8), &v, width / 8);
+        }
+        return 1 + (uint64_t)m * (width / 8);
+    }
+    memcpy(out + 1, &t0, 8);
+    memcpy(out + 9, &t1, 8);
+    if (!l) return 17;
+
+    /* write_delta_int_T: field k = (mag & (2^(l-1)-1)) | sign << (l-1),
+       packed LSB-first at bit k*l; the remaining bytes are zero. */
+    uint64_t pbytes = leco_oracle_payload_bytes(l, m);
+    uint8_t *pay = out + 17;
+    memset(pay, 0, pbytes);
+    unsigned __int128 acc = 0;
+    int nacc = 0;
+    uint64_t wr = 0;
+    uint64_t low = ((uint64_t)1 << (l - 1)) - 1;
+    for (uint32_t i = 0; i < m; i++) {
+        uint64_t yi = load_val(y, width, i);
+        int64_t p = predict(t0, t1, i);
+        int pos = width == 64 ? (yi > (uint64_t)p) : ((int64_t)yi > p);
+        uint64_t mag = (pos ? yi - (uint64_t)p : (uint64_t)p - yi) & wmask;
+        uint64_t field = (mag & low) | ((uint64_t)pos << (l - 1));
+        acc |= (unsigned __int128)field << nacc;
+        nacc += l;
+        while (nacc >= 8) {
+            pay[wr++] = (uint8_t)acc;
+            acc >>= 8;
+            nacc -= 8;
+        }
+    }
+    if (nacc > 0) pay[wr++] = (uint8_t)acc;
+    return 17 + pbytes;
+}
+
+static inline uint64_t read_field(const uint8_t *pay, uint64_t bit, int l)
+{
+    uint64_t byte = bit >> 3;
+    int sh = (int)(bit & 7);
+    unsigned __int128 win = 0;
+    int need = (sh + l + 7) / 8;
+    for (int k = 0; k < need; k++)
+        win |= (unsigned __int128)pay[byte + k] << (8 * k);
+    win >>= sh;
+    return l == 64 ? (uint64_t)win : (uint64_t)win & (((uint64_t)1 << l) - 1);
+}
+
+static inline uint64_t apply_residual(int64_t p, uint64_t field, int l)
+{
+    uint64_t mag = field & (((uint64_t)1 << (l - 1)) - 1);
+    int pos = (int)((field >> (l - 1)) & 1);
+    return pos ? (uint64_t)p + mag : (uint64_t)p - mag;
+}
+
+/* Leco_int<T>::decodeArray8, piecewise_fix_integer_template.h:142-204, with
+   read_all_bit_fix<T> (bit_read.h:222-275).  ref_compat = 1 also copies the
+   reference's handling of l == width-1 as a raw block (:171-177), which
+   returns garbage; ref_compat = 0 decodes that block correctly. */
+ORACLE_API void leco_oracle_decode_block(const uint8_t *in, int width, uint32_t m, void *out, int ref_compat)
+{
+    int l = in[0];
+    uint32_t bpv = width / 8;
+    if (l == 127) { uint64_t v = 0; memcpy(&v, in + 1, bpv); store_val(out, width, 0, v); return; }
+    if (l == 126) {
+        uint64_t v = 0; memcpy(&v, in + 1, bpv); store_val(out, width, 0, v);
+        v = 0; memcpy(&v, in + 1 + bpv, bpv); store_val(out, width, 1, v);
+        return;
+    }
+    if (l >= width || (ref_compat && l == width - 1)) {
+        memcpy(out, in + 1, (uint64_t)m * bpv);
+        return;
+    }
+    double t0, t1;
+    memcpy(&t0, in + 1, 8);
+    memcpy(&t1, in + 9, 8);
+    const uint8_t *pay = in + 17;
+    for (uint32_t i = 0; i < m; i++) {
+        int64_t p = predict(t0, t1, i);
+        uint64_t v = l ? apply_residual(p, read_field(pay, (uint64_t)i * l, l), l) : (uint64_t)p;
+        store_val(out, width, i, v);
+    }
+}
+
+/* Leco_int<T>::randomdecodeArray8, :305-337, with read_bit_fix_int_wo_round
+   (bit_read.h:1170-1199).  For l == 0 the reference converts the double
+   straight to T (:334); this returns (T)cvtt64 like the full decoder, which
+   is the same value for every stream the encoder writes below 2^63. */
+ORACLE_API uint64_t leco_oracle_lookup_block(const uint8_t *in, int width, uint32_t j)
+{
+    int l = in[0];
+    uint64_t wmask = width == 32 ? 0xFFFFFFFFull : ~0ull;
+    if (l >= width) {
+        uint64_t v = 0;
+        memcpy(&v, in + 1 + (uint64_t)j * (width / 8), width / 8);
+        return v;
+    }
+    double t0, t1;
+    memcpy(&t0, in + 1, 8);
+    memcpy(&t1, in + 9, 8);
+    int64_t p = predict(t0, t1, j);
+    if (!l) return (uint64_t)p & wmask;
+    return apply_residual(p, read_field(in + 17, (uint64_t)j * l, l), l) & wmask;
+}
+
+/* ---- column level: experiments/leco_int_fix_template.cpp:98-213 ---------- */
+
+typedef struct {
+    const void *y; int width; uint64_t n; uint32_t bs;
+    uint8_t *stream; const uint64_t *offsets; uint64_t *sizes;
+    void *out; const uint64_t *idx; uint64_t nq;
+    uint64_t lo, hi;
+} job_t;
+
+static void *enc_worker(void *arg)
+{
+    job_t *j = (job_t *)arg;
+    uint64_t bound = leco_oracle_block_bytes(j->width, j->width, j->bs);
+    uint64_t b2 = leco_oracle_block_bytes(j->width - 1, j->width, j->bs);
+    if (b2 > bound) bound = b2;
+    for (uint64_t b = j->lo; b < j->hi; b++) {
+        uint64_t s = b * j->bs, m = j->n - s < j->bs ? j->n - s : j->bs;
+        const uint8_t *src = (const uint8_t *)j->y + s * (j->width / 8);
+        j->sizes[b] = leco_oracle_encode_block(src, j->width, (uint32_t)m, j->stream + b * bound);
+    }
+    return NULL;
+}
+
+static void *dec_worker(void *arg)
+{
+    job_t *j = (job_t *)arg;
+    for (uint64_t b = j->lo; b < j->hi; b++) {
+        uint64_t s = b * j->bs, m = j->n - s < j->bs ? j->n - s : j->bs;
+        leco_oracle_decode_block(j->stream + j->offsets[b], j->width, (uint32_t)m,
+                                 (uint8_t *)j->out + s * (j->width / 8), 0);
+    }
+    return NULL;
+}
+
+static void *ra_worker(void *arg)
+{
+    job_t *j = (job_t *)arg;
+    for (uint64_t q = j->lo; q < j->hi; q++) {
+        uint64_t i = j->idx[q], b = i / j->bs;
+        uint64_t v = leco_oracle_lookup_block(j->stream + j->offsets[b], j->width, (uint32_t)(i - b * j->bs));
+        store_val(j->out, j->width, q, v);
+    }
+    return NULL;
+}
+
+static void run_jobs(void *(*fn)(void *), job_t proto, uint64_t total, int threads)
+{
+    if (threads < 1) threads = 1;
+    if ((uint64_t)threads > total) threads = total ? (int)total : 1;
+    pthread_t tid[256];
+    job_t jobs[256];
+    if (threads > 256) threads = 256;
+    for (int t = 0; t < threads; t++) {
+        jobs[t] = proto;
+        jobs[t].lo = total * t / threads;
+        jobs[t].hi = total * (t + 1) / threads;
+        if (threads == 1) { fn(&jobs[t]); return; }
+        pthread_create(&tid[t], NULL, fn, &jobs[t]);
+    }
+    for (int t = 0; t < threads; t++) pthread_join(tid[t], NULL);
+}
+
+/* Encodes a column block by block (the driver loop :111-127) into one
+   contiguous stream: block b's bytes sit at offsets[b]..offsets[b+1].
+   Returns the stream size.  threads > 1 encodes into bound-sized slots of a
+   scratch buffer in parallel and compacts afterwards. */
+ORACLE_API uint64_t leco_oracle_encode_column(const void *y, int width, uint64_t n, uint32_t bs,
+                                              uint8_t *stream, uint64_t *offsets, int threads)
+{
+    uint64_t nb = (n + bs - 1) / bs;
+    offsets[0] = 0;
+    if (threads <= 1) {
+        for (uint64_t b = 0; b < nb; b++) {
+            uint64_t s = b * bs, m = n - s < bs ? n - s : bs;
+            offsets[b + 1] = offsets[b] +
+                leco_oracle_encode_block((const uint8_t *)y + s * (width / 8), width, (uint32_t)m,
+                                         stream + offsets[b]);
+        }
+        return offsets[nb];
+    }
+    uint64_t bound = leco_oracle_block_bytes(width, width, bs);
+    uint64_t b2 = leco_oracle_block_bytes(width - 1, width, bs);
+    if (b2 > bound) bound = b2;
+    uint8_t *scratch = (uint8_t *)malloc(nb * bound);
+    uint64_t *sizes = (uint64_t *)malloc(nb * sizeof(uint64_t));
+    if (!scratch || !sizes) { free(scratch); free(sizes); return 0; }
+    job_t p = {0};
+    p.y = y; p.width = width; p.n = n; p.bs = bs; p.stream = scratch; p.sizes = sizes;
+    run_jobs(enc_worker, p, nb, threads);
+    for (uint64_t b = 0; b < nb; b++) {
+        memcpy(stream + offsets[b], scratch + b * bound, sizes[b]);
+        offsets[b + 1] = offsets[b] + sizes[b];
+    }
+    free(scratch);
+    free(sizes);
+    return offsets[nb];
+}
+
+/* The timed decode loop :142-153 (without the in-loop verification). */
+ORACLE_API void leco_oracle_decode_column(const uint8_t *stream, const uint64_t *offsets, int width,
+                                          uint64_t n, uint32_t bs, void *out, int threads)
+{
+    job_t p = {0};
+    p.width = width; p.n = n; p.bs = bs; p.stream = (uint8_t *)stream; p.offsets = offsets; p.out = out;
+    run_jobs(dec_worker, p, (n + bs - 1) / bs, threads);
+}
+
+/* The random-access loop :187-207: block idx/bs, position idx%bs. */
+ORACLE_API void leco_oracle_lookup_column(const uint8_t *stream, const uint64_t *offsets, int width,
+                                          uint64_t n, uint32_t bs, const uint64_t *idx, uint64_t nq,
+                                          void *out, int threads)
+{
+    job_t p = {0};
+    p.width = width; p.n = n; p.bs = bs; p.stream = (uint8_t *)stream; p.offsets = offsets;
+    p.idx = idx; p.nq = nq; p.out = out;
+    run_jobs(ra_worker, p, nq, threads);
+}

@@ -1,0 +1,153 @@
+"""ctypes wrapper of the CPU oracle (oracle/leco_fix_oracle.c) -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker of the GPU library; nothing in learn-to-compress_amd/ imports it.
+Each function mirrors the reference call it restates (see the C file).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_leco.so")
+
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle_leco.so"], check=True)
+    return LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.leco_oracle_cvtt64.argtypes = [ctypes.c_double]
+        L.leco_oracle_cvtt64.restype = ctypes.c_int64
+        L.leco_oracle_fit.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+        L.leco_oracle_fit.restype = None
+        L.leco_oracle_payload_bytes.argtypes = [ctypes.c_int, ctypes.c_uint32]
+        L.leco_oracle_payload_bytes.restype = ctypes.c_uint64
+        L.leco_oracle_block_bytes.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32]
+        L.leco_oracle_block_bytes.restype = ctypes.c_uint64
+        L.leco_oracle_stream_bound.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int]
+        L.leco_oracle_stream_bound.restype = ctypes.c_uint64
+        L.leco_oracle_encode_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p]
+        L.leco_oracle_encode_block.restype = ctypes.c_uint64
+        L.leco_oracle_decode_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
+                                               ctypes.c_void_p, ctypes.c_int]
+        L.leco_oracle_decode_block.restype = None
+        L.leco_oracle_lookup_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32]
+        L.leco_oracle_lookup_block.restype = ctypes.c_uint64
+        L.leco_oracle_encode_column.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+        L.leco_oracle_encode_column.restype = ctypes.c_uint64
+        L.leco_oracle_decode_column.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64,
+                                                ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int]
+        L.leco_oracle_decode_column.restype = None
+        L.leco_oracle_lookup_column.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64,
+                                                ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
+                                                ctypes.c_void_p, ctypes.c_int]
+        L.leco_oracle_lookup_column.restype = None
+        _lib = L
+    return _lib
+
+
+def _width(a: np.ndarray) -> int:
+    if a.dtype == np.uint32:
+        return 32
+    if a.dtype == np.uint64:
+        return 64
+    raise TypeError(f"oracle takes uint32 or uint64 arrays, not {a.dtype}")
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def cvtt64(x: float) -> int:
+    return lib().leco_oracle_cvtt64(x)
+
+
+def fit(y: np.ndarray) -> tuple[float, float]:
+    """lr_int_T<T>::caltheta (lr.h:66-85) -> (theta0, theta1)."""
+    y = np.ascontiguousarray(y)
+    t0, t1 = ctypes.c_double(), ctypes.c_double()
+    lib().leco_oracle_fit(_ptr(y), _width(y), len(y), ctypes.byref(t0), ctypes.byref(t1))
+    return t0.value, t1.value
+
+
+def block_bytes(l: int, width: int, n: int) -> int:
+    return lib().leco_oracle_block_bytes(l, width, n)
+
+
+def stream_bound(n: int, bs: int, width: int) -> int:
+    return lib().leco_oracle_stream_bound(n, bs, width)
+
+
+def encode_block(y: np.ndarray) -> bytes:
+    """Leco_int<T>::encodeArray8_int for one block."""
+    y = np.ascontiguousarray(y)
+    w = _width(y)
+    out = np.zeros(max(block_bytes(w, w, len(y)), block_bytes(w - 1, w, len(y))) + 64, np.uint8)
+    nb = lib().leco_oracle_encode_block(_ptr(y), w, len(y), _ptr(out))
+    return out[:nb].tobytes()
+
+
+def decode_block(blk: bytes, n: int, dtype, ref_compat: bool = False) -> np.ndarray:
+    """Leco_int<T>::decodeArray8 for one block of n values."""
+    dtype = np.dtype(dtype)
+    src = np.frombuffer(bytes(blk) + bytes(64), np.uint8)
+    out = np.zeros(max(n, 2), dtype)
+    lib().leco_oracle_decode_block(_ptr(src), 8 * dtype.itemsize, n, _ptr(out), int(ref_compat))
+    return out[:n]
+
+
+def lookup_block(blk: bytes, j: int, dtype) -> int:
+    """Leco_int<T>::randomdecodeArray8(blk, j)."""
+    dtype = np.dtype(dtype)
+    src = np.frombuffer(bytes(blk) + bytes(64), np.uint8)
+    return int(lib().leco_oracle_lookup_block(_ptr(src), 8 * dtype.itemsize, j))
+
+
+def encode_column(y: np.ndarray, bs: int, threads: int = 1) -> tuple[np.ndarray, np.ndarray]:
+    """Column of blocks -> (stream bytes as uint8 array, offsets[nb+1])."""
+    y = np.ascontiguousarray(y)
+    w = _width(y)
+    n = len(y)
+    nb = (n + bs - 1) // bs
+    stream = np.zeros(stream_bound(n, bs, w) + 64, np.uint8)
+    offs = np.zeros(nb + 1, np.uint64)
+    total = lib().leco_oracle_encode_column(_ptr(y), w, n, bs, _ptr(stream), _ptr(offs), threads)
+    return stream[:total].copy(), offs
+
+
+def decode_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, dtype, threads: int = 1,
+                  out: np.ndarray | None = None) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    padded = stream if len(stream) >= int(offs[-1]) + 64 else np.concatenate([stream, np.zeros(64, np.uint8)])
+    if out is None:
+        out = np.empty(n, dtype)
+    lib().leco_oracle_decode_column(_ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs, _ptr(out), threads)
+    return out
+
+
+def lookup_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, idx: np.ndarray, dtype,
+                  threads: int = 1) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    idx = np.ascontiguousarray(idx, np.uint64)
+    padded = np.concatenate([stream, np.zeros(64, np.uint8)])
+    out = np.empty(len(idx), dtype)
+    lib().leco_oracle_lookup_column(_ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs, _ptr(idx), len(idx),
+                                    _ptr(out), threads)
+    return out
