@@ -135,7 +135,9 @@ __global__ __launch_bounds__(kThreads) void k_resid(const T *__restrict__ in, ui
         for (int w = 0; w < kThreads / kWave; ++w) mg = rmag[w] > mg ? rmag[w] : mg;
         int l = mg ? bitlen64(mg) + 1 : 0;
         if (l > W) l = W;
-        prm[b].theta0 = t0;
+        // a raw block stores no theta; its table entry carries zeros
+        prm[b].theta0 = l == W ? 0.0 : t0;
+        if (l == W) prm[b].theta1 = 0.0;
         prm[b].bits = (uint32_t)l;
         offs[b + 1] = block_bytes((uint32_t)l, W, m);
     }
