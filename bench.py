@@ -89,6 +89,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64_000_000)
     ap.add_argument("--cpu-repeat", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-reps", type=int, default=3,
+                    help="plain device copies of the decoded column (a known-size HBM reference kernel)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -140,6 +142,14 @@ def main() -> int:
     log("lookup timing")
     ms_lk, _ = timed(torch, lambda: col.lookup(idx, out=lk), 5, 1)
 
+    copy_gbs = None
+    if args.copy_reps > 0:
+        log("device copy reference")
+        dst = torch.empty_like(out)
+        ms_cp, _ = timed(torch, lambda: dst.copy_(out), args.copy_reps, 1)
+        copy_gbs = 2 * raw_bytes / (ms_cp * 1e-3) / 1e9
+        del dst
+
     cpu = None
     if not args.no_cpu:
         sample = min(args.cpu_sample, n)
@@ -149,6 +159,15 @@ def main() -> int:
                "sample": f"first {sample} values of the column ({(sample + bs - 1) // bs} blocks), "
                          f"decoded {args.cpu_repeat}x by the C oracle, 1 thread",
                "all_cores_value": round(gbsn, 3), "all_cores": nthr}
+
+    traffic, traffic_src = None, None
+    pmc_file = os.path.join(ROOT, "profiles", f"decode_pwl{n // 1_000_000}M_u64_bs{bs}_pmc.json")
+    if os.path.exists(pmc_file):
+        with open(pmc_file) as f:
+            k = json.load(f)["kernels"].get("k_decode")
+        if k:
+            traffic = int(k["read_bytes"] + k["write_bytes"])
+            traffic_src = os.path.relpath(pmc_file, ROOT) + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
 
     dec_gbs = raw_bytes / (ms_dec * 1e-3) / 1e9
     achieved = (comp + raw_bytes) / (ms_dec * 1e-3) / 1e9
@@ -176,6 +195,7 @@ def main() -> int:
             "lookups_per_s": round(args.lookups / (ms_lk * 1e-3), 0),
             "lookup_batch_ms": round(ms_lk, 3),
             "decode_wall_ms_per_step": round(wall_dec * 1e3 / args.steps, 4),
+            "device_copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
         },
         "roofline": {
             "bound": "hbm",
@@ -183,7 +203,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "kernel": "k_decode<uint64_t,64>",
             "bytes_per_launch": comp + raw_bytes,
         },

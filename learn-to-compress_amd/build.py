@@ -56,7 +56,19 @@ def build() -> str:
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "leco_int_driver.cpp")
+DRIVER = os.path.join(ROOT, "tests", "cpp", "leco_int_driver")
+
+
+def build_driver() -> str:
+    """The C++ driver over include/leco_int.hpp (host code: g++ links the library)."""
+    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
+          "-o", DRIVER, "-L", LIBDIR, "-lleco_fix", "-Wl,-rpath,$ORIGIN/../../learn-to-compress_amd/lib"])
+    return DRIVER
 
 
 if __name__ == "__main__":

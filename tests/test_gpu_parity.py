@@ -168,3 +168,16 @@ def test_leco_int_host_surface(torch_dev):
             assert np.array_equal(codec.decodeArray8(blk, len(y), None, b), y)
             for j in {0, len(y) // 2, len(y) - 1}:
                 assert codec.randomdecodeArray8(blk, j, None, b) == int(y[j])
+
+
+@pytest.mark.parametrize("width", [32, 64])
+def test_cpp_leco_int_driver(torch_dev, width):
+    # the reference driver's flow, in C++ against include/leco_int.hpp
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "tests", "cpp", "leco_int_driver")
+    say(f"cpp driver width={width}")
+    r = subprocess.run([exe, "60000", "7", str(width)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.strip().endswith("OK")
