@@ -131,6 +131,55 @@ int leco_fix_decode_block_u64(const uint8_t *h_in, uint64_t length, uint64_t *h_
 int leco_fix_random_decode_u32(const uint8_t *h_in, uint32_t to_find, uint32_t *h_value);
 int leco_fix_random_decode_u64(const uint8_t *h_in, uint32_t to_find, uint64_t *h_value);
 
+/* ---- batches: many columns in one launch ------------------------------ */
+
+/* One column of a batch. d_in is read by encode, d_out written by decode;
+ * either may be NULL when that operation is not used. */
+typedef struct leco_fix_column_desc {
+    const void *d_in;
+    void *d_out;
+    uint64_t n;
+    uint32_t block_size;
+    uint32_t width_bits; /* 32 or 64 */
+} leco_fix_column_desc;
+
+/* A batch owns a device block table over all blocks of its columns (column
+ * order), one arena holding every column's stream back to back, the arena
+ * offsets (total_blocks + 1) and the param table (total_blocks).  Column c's
+ * blocks are global blocks [first, first + nb); its stream, offsets and
+ * params are (arena, offsets + first, params + first) and work with every
+ * column call above. */
+typedef struct leco_fix_batch leco_fix_batch;
+
+int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco_fix_batch **out);
+int leco_fix_batch_destroy(leco_fix_batch *batch);
+/* Encodes every column: fit, then one workgroup per block in ticket order
+ * (arena offsets by decoupled look-back), then the param fix-up. */
+int leco_fix_batch_encode(leco_fix_batch *batch, void *stream);
+/* Decodes every column with one persistent launch over the block table. */
+int leco_fix_batch_decode(leco_fix_batch *batch, void *stream);
+/* Same as encode (op 0) / decode (op 1), replayed from a hipGraph captured on
+ * first use. */
+int leco_fix_batch_graph_launch(leco_fix_batch *batch, int op, void *stream);
+/* Changes where decode writes column c (the graph picks it up). */
+int leco_fix_batch_set_output(leco_fix_batch *batch, uint32_t c, void *d_out);
+/* Persistent decode grid, in workgroups (0 = occupancy-derived default). */
+int leco_fix_batch_set_decode_grid(leco_fix_batch *batch, uint32_t workgroups);
+int leco_fix_batch_info(const leco_fix_batch *batch, uint64_t *total_blocks, uint64_t *arena_capacity,
+                        uint8_t **d_arena, uint64_t **d_offsets, leco_fix_block_param **d_params);
+int leco_fix_batch_column(const leco_fix_batch *batch, uint32_t c, uint64_t *first_block, uint64_t *num_blocks);
+
+/* Block-size autotuner (after search_bsize::search_block_size,
+ * headers/search_blocksize.h:14-114): for each column, encodes `windows`
+ * sample windows of `window` values at each candidate block size in one
+ * batched GPU encode, and picks the size with the fewest encoded bytes;
+ * sizes within `tolerance` (relative) of the best count as ties and the
+ * largest of them wins (fewer blocks decode faster).  Writes out_bs[ncols].
+ * Synchronous. */
+int leco_fix_autotune_block_sizes(const leco_fix_column_desc *cols, uint32_t ncols, const uint32_t *candidates,
+                                  uint32_t ncand, uint32_t windows, uint32_t window, double tolerance,
+                                  uint32_t *out_bs, uint64_t *out_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

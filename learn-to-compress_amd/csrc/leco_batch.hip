@@ -1,0 +1,255 @@
+// Batched encode/decode of many columns (C ABI in include/leco_fix.h) and
+// the block-size autotuner.
+//
+// A batch keeps a device block table (column descriptors + the global index
+// of each column's first block), one arena for every column's stream, and
+// the arena offsets / param tables.  Encode and decode are single launches
+// (plus a fit and a fix-up kernel for encode) over all blocks of all columns;
+// both can be replayed from a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "leco_internal.h"
+
+using leco::BatchColumn;
+
+struct leco_fix_batch {
+    uint32_t ncols = 0;
+    uint64_t total = 0;
+    std::vector<BatchColumn> h_cols;
+    std::vector<uint64_t> h_first;
+    BatchColumn *d_cols = nullptr;
+    uint64_t *d_first = nullptr;
+    uint8_t *d_arena = nullptr;
+    uint64_t arena_cap = 0;
+    uint64_t *d_offs = nullptr;
+    leco_fix_block_param *d_prm = nullptr;
+    uint32_t *d_widths = nullptr;
+    uint32_t decode_grid = 0;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+};
+
+namespace {
+
+int hip_status(hipError_t e) { return e == hipSuccess ? LECO_FIX_OK : LECO_FIX_EHIP; }
+
+void free_batch(leco_fix_batch *b) {
+    for (auto &g : b->graph)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
+    (void)hipFree(b->d_cols);
+    (void)hipFree(b->d_first);
+    (void)hipFree(b->d_arena);
+    (void)hipFree(b->d_offs);
+    (void)hipFree(b->d_prm);
+    (void)hipFree(b->d_widths);
+    delete b;
+}
+
+bool valid_desc(const leco_fix_column_desc &c) {
+    return c.n > 0 && c.block_size >= 1 && c.block_size <= leco::kMaxBlockSize &&
+           (c.width_bits == 32 || c.width_bits == 64);
+}
+
+int enqueue(leco_fix_batch *b, int op, hipStream_t st) {
+    if (op == 0)
+        return hip_status(leco::launch_batch_encode(b->d_cols, b->d_first, b->ncols, b->total, b->d_arena, b->d_offs,
+                                                    b->d_prm, b->d_widths, st));
+    return hip_status(leco::launch_batch_decode(b->d_cols, b->d_first, b->ncols, b->total, b->d_arena, b->d_offs,
+                                                b->decode_grid, st));
+}
+
+uint32_t default_decode_grid() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+        return 1024;
+    return (uint32_t)cus * 8;  // 8 workgroups (32 waves) per CU
+}
+
+}  // namespace
+
+extern "C" {
+
+int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco_fix_batch **out) {
+    if (!cols || !out || ncols == 0) return LECO_FIX_EINVAL;
+    for (uint32_t c = 0; c < ncols; ++c)
+        if (!valid_desc(cols[c])) return LECO_FIX_EINVAL;
+    leco_fix_batch *b = new (std::nothrow) leco_fix_batch;
+    if (!b) return LECO_FIX_ENOMEM;
+    b->ncols = ncols;
+    b->h_cols.resize(ncols);
+    b->h_first.resize(ncols + 1);
+    uint64_t arena = 0;
+    for (uint32_t c = 0; c < ncols; ++c) {
+        const leco_fix_column_desc &d = cols[c];
+        b->h_cols[c] = BatchColumn{d.d_in, d.d_out, d.n, d.block_size, d.width_bits};
+        b->h_first[c] = b->total;
+        const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
+        const uint64_t last = d.n - (nb - 1) * d.block_size;
+        arena += (nb - 1) * leco::block_bound(d.width_bits, d.block_size) +
+                 leco::block_bound(d.width_bits, (uint32_t)last);
+        b->total += nb;
+    }
+    b->h_first[ncols] = b->total;
+    b->arena_cap = arena + LECO_FIX_TAIL_PAD;
+    b->decode_grid = default_decode_grid();
+    bool ok = hipMalloc(&b->d_cols, ncols * sizeof(BatchColumn)) == hipSuccess &&
+              hipMalloc(&b->d_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
+              hipMalloc(&b->d_arena, b->arena_cap) == hipSuccess &&
+              hipMalloc(&b->d_offs, (b->total + 1) * sizeof(uint64_t)) == hipSuccess &&
+              hipMalloc(&b->d_prm, b->total * sizeof(leco_fix_block_param)) == hipSuccess &&
+              hipMalloc(&b->d_widths, b->total * sizeof(uint32_t)) == hipSuccess;
+    if (!ok) {
+        free_batch(b);
+        return LECO_FIX_ENOMEM;
+    }
+    if (hipMemcpy(b->d_cols, b->h_cols.data(), ncols * sizeof(BatchColumn), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->d_first, b->h_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemset(b->d_arena + arena, 0, LECO_FIX_TAIL_PAD) != hipSuccess) {
+        free_batch(b);
+        return LECO_FIX_EHIP;
+    }
+    *out = b;
+    return LECO_FIX_OK;
+}
+
+int leco_fix_batch_destroy(leco_fix_batch *b) {
+    if (!b) return LECO_FIX_EINVAL;
+    free_batch(b);
+    return LECO_FIX_OK;
+}
+
+int leco_fix_batch_encode(leco_fix_batch *b, void *stream) {
+    if (!b) return LECO_FIX_EINVAL;
+    for (const BatchColumn &c : b->h_cols)
+        if (!c.in) return LECO_FIX_EINVAL;
+    return enqueue(b, 0, (hipStream_t)stream);
+}
+
+int leco_fix_batch_decode(leco_fix_batch *b, void *stream) {
+    if (!b) return LECO_FIX_EINVAL;
+    for (const BatchColumn &c : b->h_cols)
+        if (!c.out) return LECO_FIX_EINVAL;
+    return enqueue(b, 1, (hipStream_t)stream);
+}
+
+int leco_fix_batch_graph_launch(leco_fix_batch *b, int op, void *stream) {
+    if (!b || (op != 0 && op != 1)) return LECO_FIX_EINVAL;
+    for (const BatchColumn &c : b->h_cols)
+        if (op == 0 ? !c.in : !c.out) return LECO_FIX_EINVAL;
+    if (!b->graph[op]) {
+        if (!b->cap_stream && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess)
+            return LECO_FIX_EHIP;
+        hipGraph_t g = nullptr;
+        if (hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeRelaxed) != hipSuccess) return LECO_FIX_EHIP;
+        int rc = enqueue(b, op, b->cap_stream);
+        hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+        if (rc != LECO_FIX_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return LECO_FIX_EHIP;
+        }
+        e = hipGraphInstantiate(&b->graph[op], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            b->graph[op] = nullptr;
+            return LECO_FIX_EHIP;
+        }
+    }
+    return hip_status(hipGraphLaunch(b->graph[op], (hipStream_t)stream));
+}
+
+int leco_fix_batch_set_output(leco_fix_batch *b, uint32_t c, void *d_out) {
+    if (!b || c >= b->ncols) return LECO_FIX_EINVAL;
+    b->h_cols[c].out = d_out;
+    return hip_status(hipMemcpy(&b->d_cols[c], &b->h_cols[c], sizeof(BatchColumn), hipMemcpyHostToDevice));
+}
+
+int leco_fix_batch_set_decode_grid(leco_fix_batch *b, uint32_t wgs) {
+    if (!b) return LECO_FIX_EINVAL;
+    b->decode_grid = wgs ? wgs : default_decode_grid();
+    if (b->graph[1]) {  // the captured launch has the old grid baked in
+        (void)hipGraphExecDestroy(b->graph[1]);
+        b->graph[1] = nullptr;
+    }
+    return LECO_FIX_OK;
+}
+
+int leco_fix_batch_info(const leco_fix_batch *b, uint64_t *total, uint64_t *cap, uint8_t **arena, uint64_t **offs,
+                        leco_fix_block_param **prm) {
+    if (!b) return LECO_FIX_EINVAL;
+    if (total) *total = b->total;
+    if (cap) *cap = b->arena_cap;
+    if (arena) *arena = b->d_arena;
+    if (offs) *offs = b->d_offs;
+    if (prm) *prm = b->d_prm;
+    return LECO_FIX_OK;
+}
+
+int leco_fix_batch_column(const leco_fix_batch *b, uint32_t c, uint64_t *first, uint64_t *nb) {
+    if (!b || c >= b->ncols) return LECO_FIX_EINVAL;
+    if (first) *first = b->h_first[c];
+    if (nb) *nb = b->h_first[c + 1] - b->h_first[c];
+    return LECO_FIX_OK;
+}
+
+int leco_fix_autotune_block_sizes(const leco_fix_column_desc *cols, uint32_t ncols, const uint32_t *cand,
+                                  uint32_t ncand, uint32_t windows, uint32_t window, double tol, uint32_t *out_bs,
+                                  uint64_t *out_bytes, void *stream) {
+    if (!cols || !cand || !out_bs || ncols == 0 || ncand == 0 || windows == 0 || window == 0) return LECO_FIX_EINVAL;
+    for (uint32_t j = 0; j < ncand; ++j)
+        if (cand[j] == 0 || cand[j] > leco::kMaxBlockSize || window % cand[j]) return LECO_FIX_EINVAL;
+    // tuning batch: column c, window k, candidate j -> one entry
+    std::vector<leco_fix_column_desc> tune;
+    std::vector<uint32_t> owner;
+    for (uint32_t c = 0; c < ncols; ++c) {
+        const leco_fix_column_desc &d = cols[c];
+        if (!d.d_in || !valid_desc(d)) return LECO_FIX_EINVAL;
+        const uint64_t slots = d.n / window;  // whole windows, aligned to every candidate
+        const uint32_t nw = slots == 0 ? 1 : (uint32_t)std::min<uint64_t>(windows, slots);
+        for (uint32_t k = 0; k < nw; ++k) {
+            const uint64_t start = slots == 0 ? 0 : (slots * k / nw) * window;
+            const uint64_t len = slots == 0 ? d.n : window;
+            for (uint32_t j = 0; j < ncand; ++j) {
+                tune.push_back({(const uint8_t *)d.d_in + start * (d.width_bits / 8), nullptr, len, cand[j],
+                                d.width_bits});
+                owner.push_back(c * ncand + j);
+            }
+        }
+    }
+    leco_fix_batch *b = nullptr;
+    int rc = leco_fix_batch_create(tune.data(), (uint32_t)tune.size(), &b);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    rc = leco_fix_batch_encode(b, st);
+    std::vector<uint64_t> offs(b->total + 1);
+    if (rc == LECO_FIX_OK &&
+        (hipMemcpyAsync(offs.data(), b->d_offs, offs.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st) !=
+             hipSuccess ||
+         hipStreamSynchronize(st) != hipSuccess))
+        rc = LECO_FIX_EHIP;
+    if (rc == LECO_FIX_OK) {
+        std::vector<uint64_t> bytes((size_t)ncols * ncand, 0);
+        for (size_t e = 0; e < tune.size(); ++e) bytes[owner[e]] += offs[b->h_first[e + 1]] - offs[b->h_first[e]];
+        for (uint32_t c = 0; c < ncols; ++c) {
+            uint64_t best = UINT64_MAX;
+            for (uint32_t j = 0; j < ncand; ++j) best = std::min(best, bytes[(size_t)c * ncand + j]);
+            uint32_t pick = 0;
+            for (uint32_t j = 0; j < ncand; ++j)
+                if ((double)bytes[(size_t)c * ncand + j] <= (double)best * (1.0 + tol) && cand[j] > pick)
+                    pick = cand[j];
+            out_bs[c] = pick;
+            if (out_bytes)
+                for (uint32_t j = 0; j < ncand; ++j) out_bytes[(size_t)c * ncand + j] = bytes[(size_t)c * ncand + j];
+        }
+    }
+    free_batch(b);
+    return rc;
+}
+
+}  // extern "C"

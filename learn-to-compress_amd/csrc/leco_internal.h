@@ -1,4 +1,5 @@
-// Host-side launchers for the gfx950 kernels (defined in leco_kernels.hip).
+// Types shared by host code and kernels, and the kernel launchers
+// (defined in leco_kernels.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,6 +21,45 @@ __host__ __device__ inline uint64_t block_bytes(uint32_t l, uint32_t w, uint32_t
     if (l >= w) return 1 + (uint64_t)n * (w / 8);
     return 17 + (l ? payload_bytes(l, n) : 0);
 }
+// Largest byte size of a block of n values (raw or l = w-1).
+__host__ __device__ inline uint64_t block_bound(uint32_t w, uint32_t n) {
+    uint64_t a = block_bytes(w - 1, w, n), b = block_bytes(w, w, n);
+    return a > b ? a : b;
+}
+
+// One column of a batch, as the device sees it.
+struct BatchColumn {
+    const void *in;   // values to encode (device)
+    void *out;        // decoded values (device)
+    uint64_t n;
+    uint32_t bs;
+    uint32_t width;   // 32 or 64
+};
+
+// A resident column of the request server.
+struct ServeColumn {
+    const uint8_t *stream;
+    const uint64_t *offsets;
+    const leco_fix_block_param *params;
+    uint64_t n;
+    uint32_t bs;
+    uint32_t width;
+};
+
+enum : uint32_t { kServeLookup = 0, kServeRange = 1 };
+
+// One workgroup's share of a served batch: a chunk of a point-lookup request,
+// or one block of a range-decode request.
+struct ServeItem {
+    uint32_t kind;
+    uint32_t column;
+    uint32_t count;        // lookup: indices in this chunk
+    uint32_t reserved;
+    const uint64_t *idx;   // lookup: indices
+    void *out;             // lookup: results; range: where value `lo` goes
+    uint64_t block;        // range: block id
+    uint64_t lo, hi;       // range: [lo, hi) in column positions
+};
 
 // Each launcher enqueues on `st` and returns the hipGetLastError() of the launch.
 template <typename T>
@@ -33,5 +73,19 @@ hipError_t launch_lookup(const uint8_t *stream, const leco_fix_block_param *prm,
                          const uint64_t *idx, uint64_t nq, T *out, hipStream_t st);
 hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
                                uint32_t w, leco_fix_block_param *prm, hipStream_t st);
+
+// Batched encode of every block of ncols columns into one arena: offs holds
+// total+1 arena offsets (offs[first[c] + b] for block b of column c), prm the
+// total per-block params, widths total uint32 of scratch.  first[c] is the
+// global index of column c's block 0; first[ncols] = total.
+hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
+                               uint8_t *arena, uint64_t *offs, leco_fix_block_param *prm, uint32_t *widths,
+                               hipStream_t st);
+// Batched decode: a persistent grid of `grid` workgroups strides over the blocks.
+hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
+                               const uint8_t *arena, const uint64_t *offs, uint32_t grid, hipStream_t st);
+// Served batch: `grid` workgroups stride over nitems items.
+hipError_t launch_serve(const ServeColumn *cols, const ServeItem *items, uint32_t nitems, uint32_t grid,
+                        hipStream_t st);
 
 }  // namespace leco

@@ -1,0 +1,90 @@
+// Device-side constants, shared-memory layouts and block addressing used by
+// the kernels in leco_kernels.hip.
+#pragma once
+
+#include "leco_device.h"
+#include "leco_internal.h"
+
+namespace leco {
+
+constexpr int kThreads = 256;                       // workgroup of the per-block kernels (4 waves)
+constexpr uint32_t kChunk = 2048;                   // values staged through LDS at a time
+constexpr int kLdsBytes = kChunk * 8 + 64;          // one chunk of raw uint64 + alignment slack
+constexpr int kVPT = 16;                            // values per lane kept in registers (4096 per block)
+constexpr int kFitRows = 64;                        // blocks per fit wave (one lane each)
+constexpr int kFitTK = 32;                          // values per block row in one fit tile
+constexpr int kFitPer = kFitRows * kFitTK / kWave;  // tile values loaded per lane
+
+// Look-back status word (in leco_fix_block_param::payload_offset while the
+// encoder runs): 2 flag bits + 62-bit byte count.
+constexpr uint64_t kStatusAggregate = 1ull << 62;   // value = this block's size
+constexpr uint64_t kStatusInclusive = 2ull << 62;   // value = bytes up to and including this block
+constexpr uint64_t kStatusMask = 3ull << 62;
+constexpr uint64_t kStatusValue = (1ull << 62) - 1;
+constexpr uint32_t kSpinLimit = 1u << 26;              // look-back wait bound (s_sleep 1 each)
+
+__device__ __forceinline__ void store_status(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t load_status(const uint64_t *p) {
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct EncodeShared {
+    alignas(16) uint32_t buf[kLdsBytes / 4];
+    int64_t rmax[kThreads / kWave], rmin[kThreads / kWave];
+    uint64_t rmag[kThreads / kWave];
+    uint64_t off, end;
+    double t0;
+    int l;
+};
+
+// Where block b's values live.
+struct RowRef {
+    const uint8_t *base;
+    uint32_t m;  // 0 when b is past the last block
+    int width;
+};
+
+// Blocks of one column.
+struct PlainRows {
+    const uint8_t *in;
+    uint64_t n;
+    uint32_t bs;
+    int width;
+    uint64_t nb;
+    __device__ __forceinline__ RowRef row(uint64_t b) const {
+        if (b >= nb) return {nullptr, 0, width};
+        const uint64_t s = b * bs;
+        return {in + s * (width / 8), (uint32_t)(n - s < bs ? n - s : bs), width};
+    }
+};
+
+// Largest c with first[c] <= it (first[0] = 0 <= it < first[ncols]).
+__device__ __forceinline__ uint32_t find_column(const uint64_t *__restrict__ first, uint32_t ncols, uint64_t it) {
+    uint32_t lo = 0, hi = ncols;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (first[mid] <= it) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Blocks of many columns, in column order.
+struct BatchRows {
+    const BatchColumn *cols;
+    const uint64_t *first;
+    uint32_t ncols;
+    uint64_t total;
+    __device__ __forceinline__ RowRef row(uint64_t it) const {
+        if (it >= total) return {nullptr, 0, 64};
+        const uint32_t c = find_column(first, ncols, it);
+        const BatchColumn col = cols[c];
+        const uint64_t s = (it - first[c]) * col.bs;
+        return {(const uint8_t *)col.in + s * (col.width / 8), (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs),
+                (int)col.width};
+    }
+};
+
+}  // namespace leco

@@ -5,67 +5,89 @@
 // offs[b]), exactly the bytes Leco_int<T>::encodeArray8_int writes per block,
 // plus a 32-byte leco_fix_block_param per block for point lookups.
 //
-// Encode = fit (one lane per block) -> residual/bit width (one workgroup per
-// block) -> exclusive scan of block sizes -> pack (one workgroup per block).
-// Decode = one workgroup per block; lookup = one lane per query.
-#include "leco_device.h"
-#include "leco_internal.h"
+// Encode = K1 fit (one lane per block) -> K2 residuals + bit width + offset
+// (decoupled look-back) + pack, one workgroup per block, values kept in
+// registers -> K3 fix-up of payload offsets.  Decode = one workgroup per
+// block; lookup = one lane per query.  The batched variants run the same
+// per-block bodies over a device table of blocks from many columns.
+#include "leco_kernels.h"
 
 namespace leco {
-
-constexpr int kThreads = 256;           // workgroup of the per-block kernels (4 waves)
-constexpr int kFitTK = 32;              // values per block row in one fit tile
-constexpr uint32_t kChunk = 2048;       // values staged through LDS at a time
-constexpr int kLdsBytes = kChunk * 8 + 64;
 
 // ---------------------------------------------------------------------------
 // K1 fit: lr_int_T<T>::caltheta (lr.h:66-85).  The sums over y and i*y are
 // sequential FP64 chains in the reference, so each lane owns one block and
-// replays its chain in index order; 64 blocks per wave.  A tile of 64 blocks x
-// kFitTK values is loaded row by row (coalesced) into LDS and read back one
-// block per lane.  sum(i) and sum(i*i) are exact and formed in closed form.
+// replays its chain in index order, 64 blocks per wave.  Tiles of 64 blocks x
+// kFitTK values are loaded row by row (coalesced), converted and parked in
+// LDS, and read back one block per lane; the next tile's loads are in flight
+// while the current tile's chains run.  sum(i) and sum(i*i) are exact and
+// formed in closed form.  Also clears each block's look-back status word
+// (payload_offset) and, in workgroup 0, the ticket counter.
 // ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(64) void k_fit(const T *__restrict__ in, uint64_t n, uint32_t bs, uint64_t nb,
-                                            leco_fix_block_param *__restrict__ prm) {
-    __shared__ T tile[64][kFitTK + 1];
+template <class Rows>
+__global__ __launch_bounds__(64) void k_fit(Rows rows, leco_fix_block_param *__restrict__ prm,
+                                            unsigned long long *__restrict__ ticket) {
+    __shared__ double tile[kFitRows][kFitTK + 1];
+    __shared__ const uint8_t *rbase[kFitRows];
+    __shared__ uint32_t rlen[kFitRows];
+    __shared__ int rwid[kFitRows];
     const int lane = threadIdx.x;
-    const uint64_t b0 = (uint64_t)blockIdx.x * 64;
-    const uint64_t b = b0 + lane;
-    const uint64_t rows = nb - b0 < 64 ? nb - b0 : 64;
-    uint32_t m = 0;
-    if (b < nb) {
-        uint64_t s = b * bs;
-        m = (uint32_t)(n - s < bs ? n - s : bs);
+    const uint64_t b = (uint64_t)blockIdx.x * kFitRows + lane;
+    if (blockIdx.x == 0 && lane == 0) *ticket = 0;
+
+    RowRef me = rows.row(b);
+    rbase[lane] = me.base;
+    rlen[lane] = me.m;
+    rwid[lane] = me.width;
+    uint32_t maxm = me.m;
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        uint32_t u = (uint32_t)__shfl_xor((int)maxm, o, kWave);
+        maxm = u > maxm ? u : maxm;
     }
+    __syncthreads();
+
+    uint64_t reg[kFitPer];
+    auto load = [&](uint32_t k0) {
+#pragma unroll
+        for (int it = 0; it < kFitPer; ++it) {
+            const int r = it * (kWave / kFitTK) + lane / kFitTK;
+            const uint32_t k = k0 + (uint32_t)(lane % kFitTK);
+            uint64_t v = 0;
+            if (k < rlen[r]) {
+                const uint8_t *p = rbase[r];
+                v = rwid[r] == 64 ? ((const uint64_t *)p)[k] : (uint64_t)((const uint32_t *)p)[k];
+            }
+            reg[it] = v;
+        }
+    };
+
     double sy = 0.0, sxy = 0.0;
-    for (uint32_t k0 = 0; k0 < bs; k0 += kFitTK) {
-#pragma unroll 8
-        for (int it = 0; it < 64 * kFitTK / 64; ++it) {
-            const int r = it * (64 / kFitTK) + lane / kFitTK;
-            const int c = lane % kFitTK;
-            T v = 0;
-            if ((uint64_t)r < rows && k0 + c < bs) {
-                uint64_t idx = (b0 + r) * bs + k0 + c;
-                if (idx < n) v = in[idx];
-            }
-            tile[r][c] = v;
-        }
+    if (maxm) load(0);
+    for (uint32_t k0 = 0; k0 < maxm; k0 += kFitTK) {
+#pragma unroll
+        for (int it = 0; it < kFitPer; ++it)
+            tile[it * (kWave / kFitTK) + lane / kFitTK][lane % kFitTK] = (double)reg[it];
         __syncthreads();
-        if (k0 < m) {
-            const uint32_t kend = m - k0 < (uint32_t)kFitTK ? m - k0 : (uint32_t)kFitTK;
-            for (uint32_t c = 0; c < kend; ++c) {
-                const double yi = (double)tile[lane][c];
-                const double xi = (double)(k0 + c);
+        if (k0 + kFitTK < maxm) load(k0 + kFitTK);
+        if (k0 + kFitTK <= me.m) {
+#pragma unroll
+            for (int c = 0; c < kFitTK; ++c) {
+                const double yi = tile[lane][c];
                 sy = sy + yi;
-                sxy = __builtin_fma(xi, yi, sxy);
+                sxy = __builtin_fma((double)(k0 + c), yi, sxy);
+            }
+        } else if (k0 < me.m) {
+            for (uint32_t c = 0; c < me.m - k0; ++c) {
+                const double yi = tile[lane][c];
+                sy = sy + yi;
+                sxy = __builtin_fma((double)(k0 + c), yi, sxy);
             }
         }
         __syncthreads();
     }
-    if (b >= nb) return;
-    const uint64_t mm = m;
-    const double md = (double)m;
+    if (!me.m) return;
+    const uint64_t mm = me.m;
+    const double md = (double)me.m;
     const double sx = (double)(mm * (mm - 1) / 2);
     const double sxx = (double)((mm - 1) * mm * (2 * mm - 1) / 6);
     const double ccc = __builtin_fma(sxy, md, -(sx * sy));
@@ -78,147 +100,138 @@ __global__ __launch_bounds__(64) void k_fit(const T *__restrict__ in, uint64_t n
         t1 = ccc / xxx;
         t0 = __builtin_fma(-t1, sx, sy) / md;
     }
-    prm[b].theta0 = t0;
-    prm[b].theta1 = t1;
-    prm[b].length = m;
+    leco_fix_block_param *p = prm + b;
+    p->theta0 = t0;
+    p->theta1 = t1;
+    p->payload_offset = 0;  // look-back status: not ready
+    p->length = me.m;
 }
 
 // ---------------------------------------------------------------------------
-// K2 residuals: piecewise_fix_integer_template.h:36-117.  Pass 1 takes the
-// min/max int64 residual under the fitted line and centres theta0 on it;
-// pass 2 takes the largest sign/magnitude residual, which sets the bit width l.
-// Writes theta0 (centred), l and the block's byte size (into offs[b+1]).
+// K2 encode body: piecewise_fix_integer_template.h:36-137.
+//   pass 1: min/max int64 residual under the fitted line; theta0 += (max+min)/2
+//   pass 2: largest sign/magnitude residual -> bit width l (lr.h:116-130)
+//   offset: decoupled look-back over the preceding blocks' sizes
+//   pack:   header [l][theta0][theta1] (or [l] + raw values when l == W), then
+//           field k = (mag & (2^(l-1)-1)) | sign<<(l-1) LSB-first, then zero
+//           padding (bit_write.h:257-323).  kChunk values at a time are OR-ed
+//           into an LDS image aligned like their HBM destination and written
+//           with 16-byte stores; byte stores only at the block's ragged ends.
+// The block's values stay in registers (kVPT per lane) across all passes.
 // ---------------------------------------------------------------------------
 template <typename T, int W>
-__global__ __launch_bounds__(kThreads) void k_resid(const T *__restrict__ in, uint64_t n, uint32_t bs,
-                                                    leco_fix_block_param *__restrict__ prm,
-                                                    uint64_t *__restrict__ offs) {
-    __shared__ int64_t rmax[kThreads / kWave], rmin[kThreads / kWave];
-    __shared__ uint64_t rmag[kThreads / kWave];
-    const uint64_t b = blockIdx.x;
+__device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint32_t m, leco_fix_block_param *prm,
+                                                  uint64_t b, uint64_t *__restrict__ offs,
+                                                  uint8_t *__restrict__ out, EncodeShared &sh) {
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
-    const uint64_t s = b * bs;
-    const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
-    const T *y = in + s;
     double t0 = prm[b].theta0;
     const double t1 = prm[b].theta1;
 
+    uint64_t v[kVPT];
+#pragma unroll
+    for (int k = 0; k < kVPT; ++k) {
+        const uint32_t j = tid + k * kThreads;
+        v[k] = j < m ? (uint64_t)y[j] : 0;
+    }
+
+    // pass 1
     int64_t mx = INT64_MIN, mn = INT64_MAX;
-    for (uint32_t j = tid; j < m; j += kThreads) {
+#pragma unroll
+    for (int k = 0; k < kVPT; ++k) {
+        const uint32_t j = tid + k * kThreads;
+        if (j < m) {
+            const int64_t d = (int64_t)(v[k] - (uint64_t)predict(t0, t1, j));
+            mx = d > mx ? d : mx;
+            mn = d < mn ? d : mn;
+        }
+    }
+    for (uint32_t j = tid + kVPT * kThreads; j < m; j += kThreads) {
         const int64_t d = (int64_t)((uint64_t)y[j] - (uint64_t)predict(t0, t1, j));
         mx = d > mx ? d : mx;
         mn = d < mn ? d : mn;
     }
     mx = wave_max_i64(mx);
     mn = wave_min_i64(mn);
-    if (lane == 0) { rmax[wid] = mx; rmin[wid] = mn; }
+    if (lane == 0) { sh.rmax[wid] = mx; sh.rmin[wid] = mn; }
     __syncthreads();
-    mx = rmax[0];
-    mn = rmin[0];
+    mx = sh.rmax[0];
+    mn = sh.rmin[0];
+#pragma unroll
     for (int w = 1; w < kThreads / kWave; ++w) {
-        mx = rmax[w] > mx ? rmax[w] : mx;
-        mn = rmin[w] < mn ? rmin[w] : mn;
+        mx = sh.rmax[w] > mx ? sh.rmax[w] : mx;
+        mn = sh.rmin[w] < mn ? sh.rmin[w] : mn;
     }
     // :46 theta0 += (max + min) / 2.  (int64 sum wraps; x * 0.5 is exact)
     if (!__builtin_isnan(t0)) t0 = t0 + (double)(int64_t)((uint64_t)mx + (uint64_t)mn) * 0.5;
 
-    uint64_t mg = 0;
-    for (uint32_t j = tid; j < m; j += kThreads) {
-        uint64_t mag;
+    // pass 2
+    uint64_t mg = 0, mag;
+#pragma unroll
+    for (int k = 0; k < kVPT; ++k) {
+        const uint32_t j = tid + k * kThreads;
+        if (j < m) {
+            residual_field<W>(v[k], predict(t0, t1, j), 0, &mag);
+            mg = mag > mg ? mag : mg;
+        }
+    }
+    for (uint32_t j = tid + kVPT * kThreads; j < m; j += kThreads) {
         residual_field<W>((uint64_t)y[j], predict(t0, t1, j), 0, &mag);
         mg = mag > mg ? mag : mg;
     }
     mg = wave_max_u64(mg);
-    if (lane == 0) rmag[wid] = mg;
+    if (lane == 0) sh.rmag[wid] = mg;
     __syncthreads();
+
+    // bit width, size and look-back (one lane)
     if (tid == 0) {
-        for (int w = 0; w < kThreads / kWave; ++w) mg = rmag[w] > mg ? rmag[w] : mg;
+#pragma unroll
+        for (int w = 0; w < kThreads / kWave; ++w) mg = sh.rmag[w] > mg ? sh.rmag[w] : mg;
         int l = mg ? bitlen64(mg) + 1 : 0;
         if (l > W) l = W;
-        // a raw block stores no theta; its table entry carries zeros
-        prm[b].theta0 = l == W ? 0.0 : t0;
+        const uint64_t size = block_bytes((uint32_t)l, W, m);
+        uint64_t *status = &prm[b].payload_offset;
+        uint64_t excl = 0;
+        if (b == 0) {
+            store_status(status, kStatusInclusive | size);
+        } else {
+            store_status(status, kStatusAggregate | size);
+            // Every lower block belongs to a workgroup that is already running
+            // and publishes its size without waiting, so this ends; the spin
+            // bound (~seconds) only keeps a broken invariant from hanging the GPU.
+            uint32_t spins = 0;
+            for (uint64_t p = b - 1;; --p) {
+                uint64_t s;
+                while (((s = load_status(&prm[p].payload_offset)) & kStatusMask) == 0 && ++spins < kSpinLimit)
+                    __builtin_amdgcn_s_sleep(1);
+                excl += s & kStatusValue;
+                if ((s & kStatusMask) == kStatusInclusive || spins >= kSpinLimit || p == 0) break;
+            }
+            store_status(status, kStatusInclusive | (excl + size));
+        }
+        offs[b + 1] = excl + size;
+        sh.off = excl;
+        sh.end = excl + size;
+        sh.l = l;
+        sh.t0 = l == W ? 0.0 : t0;
+        prm[b].theta0 = sh.t0;
         if (l == W) prm[b].theta1 = 0.0;
         prm[b].bits = (uint32_t)l;
-        offs[b + 1] = block_bytes((uint32_t)l, W, m);
     }
-}
+    __syncthreads();
+    const int l = sh.l;
+    const uint64_t off = sh.off, end = sh.end;
+    t0 = sh.t0;
 
-// ---------------------------------------------------------------------------
-// K3 scan: offs[1..nb] hold block sizes; turns offs[0..nb] into the exclusive
-// prefix (offs[0] = 0, offs[nb] = stream bytes).  One workgroup.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_scan(uint64_t *__restrict__ offs, uint64_t nb) {
-    constexpr int PER = 8;
-    __shared__ uint64_t wsum[1024 / kWave];
-    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-    uint64_t carry = 0;
-    if (tid == 0) offs[0] = 0;
-    for (uint64_t base = 0; base < nb; base += 1024 * PER) {
-        uint64_t v[PER], loc = 0;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            const uint64_t i = base + (uint64_t)tid * PER + k;
-            v[k] = i < nb ? offs[1 + i] : 0;
-            loc += v[k];
-        }
-        uint64_t x = loc;
-        for (int o = 1; o < kWave; o <<= 1) {
-            uint64_t u = (uint64_t)__shfl_up((long long)x, o, kWave);
-            if (lane >= o) x += u;
-        }
-        if (lane == kWave - 1) wsum[wid] = x;
-        __syncthreads();
-        if (wid == 0) {
-            uint64_t w = lane < 1024 / kWave ? wsum[lane] : 0;
-            for (int o = 1; o < 1024 / kWave; o <<= 1) {
-                uint64_t u = (uint64_t)__shfl_up((long long)w, o, kWave);
-                if (lane >= o) w += u;
-            }
-            if (lane < 1024 / kWave) wsum[lane] = w;
-        }
-        __syncthreads();
-        uint64_t run = carry + (wid ? wsum[wid - 1] : 0) + x - loc;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            run += v[k];
-            const uint64_t i = base + (uint64_t)tid * PER + k;
-            if (i < nb) offs[1 + i] = run;
-        }
-        carry += wsum[1024 / kWave - 1];
-        __syncthreads();
-    }
-}
-
-// ---------------------------------------------------------------------------
-// K4 pack: the block byte string (piecewise_fix_integer_template.h:118-137,
-// write_delta_int_T bit_write.h:257-323).  Header [l][theta0][theta1] (or [l]
-// + raw values when l == W), then field k = (mag & (2^(l-1)-1)) | sign<<(l-1)
-// LSB-first at bit 17*8 + k*l, then zero padding to offs[b+1].  kChunk values
-// at a time are OR-ed into an LDS image aligned like their HBM destination,
-// then written with 16-byte stores (byte stores only at the block's ragged
-// ends, which neighbouring workgroups share).
-// ---------------------------------------------------------------------------
-template <typename T, int W>
-__global__ __launch_bounds__(kThreads) void k_pack(const T *__restrict__ in, uint64_t n, uint32_t bs,
-                                                   leco_fix_block_param *__restrict__ prm,
-                                                   const uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
-    __shared__ __align__(16) uint32_t buf[kLdsBytes / 4];
-    const uint64_t b = blockIdx.x;
-    const int tid = threadIdx.x;
-    const uint64_t s = b * bs;
-    const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
-    const int l = (int)prm[b].bits;
-    const double t0 = prm[b].theta0, t1 = prm[b].theta1;
-    const uint64_t off = offs[b], end = offs[b + 1];
+    // pack
     const bool raw = (l == W);
     const uint32_t hdr = raw ? 1 : 17;
-    if (tid == 0) prm[b].payload_offset = off + hdr;
-    const T *y = in + s;
     const uint32_t nchunks = l == 0 ? 1 : (m + kChunk - 1) / kChunk;
-    const uint64_t cbytes = (uint64_t)(kChunk / 8) * l;  // payload bytes per full chunk
+    const uint64_t cbytes = (uint64_t)(kChunk / 8) * l;
     const uint64_t t0b = double_to_bits(t0), t1b = double_to_bits(t1);
-
-    for (uint32_t c = 0; c < nchunks; ++c) {
+    uint32_t *buf = sh.buf;
+    // value(kk) gives the value at chunk position tid + kk*kThreads; kk is a
+    // compile-time index so that register-cached values stay in registers
+    auto pack_chunk = [&](uint32_t c, auto value) {
         const uint64_t g0 = c == 0 ? off : off + hdr + c * cbytes;
         const uint64_t g1 = c + 1 == nchunks ? end : off + hdr + (c + 1) * cbytes;
         const uint64_t a0 = g0 & ~15ull;
@@ -238,21 +251,25 @@ __global__ __launch_bounds__(kThreads) void k_pack(const T *__restrict__ in, uin
             const uint64_t pbit = 8ull * (lb + (c == 0 ? hdr : 0));
             const uint32_t j0 = c * kChunk;
             const uint32_t jn = m - j0 < kChunk ? m - j0 : kChunk;
-            for (uint32_t j = tid; j < jn; j += kThreads) {
-                const uint32_t i = j0 + j;
-                const uint64_t v = (uint64_t)y[i];
-                uint64_t field, mag;
-                if (raw) field = v;
-                else field = residual_field<W>(v, predict(t0, t1, i), l, &mag);
-                const uint64_t q = pbit + (uint64_t)j * l;
-                const uint32_t dw = (uint32_t)(q >> 5);
-                const int sh = (int)(q & 31);
-                const uint32_t d0 = (uint32_t)(field << sh);
-                const uint32_t d1 = (uint32_t)(sh ? field >> (32 - sh) : field >> 32);
-                const uint32_t d2 = sh ? (uint32_t)(field >> (64 - sh)) : 0u;
-                if (d0) atomicOr(&buf[dw], d0);
-                if (sh + l > 32 && d1) atomicOr(&buf[dw + 1], d1);
-                if (sh + l > 64 && d2) atomicOr(&buf[dw + 2], d2);
+#pragma unroll
+            for (int kk = 0; kk < (int)(kChunk / kThreads); ++kk) {
+                const uint32_t j = tid + kk * kThreads;  // position inside the chunk
+                if (j < jn) {
+                    const uint32_t i = j0 + j;
+                    const uint64_t yv = value(kk, i);
+                    uint64_t field, mg2;
+                    if (raw) field = yv;
+                    else field = residual_field<W>(yv, predict(t0, t1, i), l, &mg2);
+                    const uint64_t q = pbit + (uint64_t)j * l;
+                    const uint32_t dw = (uint32_t)(q >> 5);
+                    const int shb = (int)(q & 31);
+                    const uint32_t d0 = (uint32_t)(field << shb);
+                    const uint32_t d1 = (uint32_t)(shb ? field >> (32 - shb) : field >> 32);
+                    const uint32_t d2 = shb ? (uint32_t)(field >> (64 - shb)) : 0u;
+                    if (d0) atomicOr(&buf[dw], d0);
+                    if (shb + l > 32 && d1) atomicOr(&buf[dw + 1], d1);
+                    if (shb + l > 64 && d2) atomicOr(&buf[dw + 2], d2);
+                }
             }
         }
         __syncthreads();
@@ -269,34 +286,58 @@ __global__ __launch_bounds__(kThreads) void k_pack(const T *__restrict__ in, uin
             }
         }
         __syncthreads();
-    }
+    };
+    static_assert(kVPT == 2 * (kChunk / kThreads), "two chunks of values are register-cached");
+    pack_chunk(0, [&](int kk, uint32_t) { return v[kk]; });
+    if (nchunks > 1) pack_chunk(1, [&](int kk, uint32_t) { return v[kChunk / kThreads + kk]; });
+    for (uint32_t c = 2; c < nchunks; ++c) pack_chunk(c, [&](int, uint32_t i) { return (uint64_t)y[i]; });
+}
+
+// One workgroup per block, in ticket order (the look-back needs every lower
+// block to be owned by a running workgroup).
+template <class Rows>
+__global__ __launch_bounds__(kThreads) void k_encode(Rows rows, leco_fix_block_param *__restrict__ prm,
+                                                     uint64_t *__restrict__ offs, uint8_t *__restrict__ out,
+                                                     unsigned long long *__restrict__ ticket) {
+    __shared__ EncodeShared sh;
+    __shared__ uint64_t s_b;
+    if (threadIdx.x == 0) s_b = atomicAdd(ticket, 1ull);
+    __syncthreads();
+    const uint64_t b = s_b;
+    const RowRef r = rows.row(b);
+    if (r.width == 64)
+        encode_block_body<uint64_t, 64>((const uint64_t *)r.base, r.m, prm, b, offs, out, sh);
+    else
+        encode_block_body<uint32_t, 32>((const uint32_t *)r.base, r.m, prm, b, offs, out, sh);
+}
+
+// K3: payload offsets for the param table, offs[0] = 0 (it held the ticket).
+__global__ __launch_bounds__(kThreads) void k_encode_fixup(leco_fix_block_param *__restrict__ prm,
+                                                           uint64_t *__restrict__ offs, uint64_t nb,
+                                                           const uint32_t *__restrict__ widths, uint32_t width) {
+    const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (b >= nb) return;
+    const uint32_t w = widths ? widths[b] : width;
+    const uint64_t excl = (load_status(&prm[b].payload_offset) & kStatusValue) -
+                          block_bytes(prm[b].bits, w, prm[b].length);
+    prm[b].payload_offset = excl + (prm[b].bits >= w ? 1 : 17);
+    if (b == 0) offs[0] = 0;
 }
 
 // ---------------------------------------------------------------------------
-// K5 decode: Leco_int<T>::decodeArray8 (:142-204) with read_all_bit_fix
-// (bit_read.h:222-275).  One workgroup per block: the block's payload is
-// staged through LDS kChunk values at a time with aligned 16-byte loads, then
-// each lane extracts its field, adds it to cvtt64(fma(i, theta1, theta0)) and
-// stores.  Writes values [lo, hi) of the column to out[global - lo].
-// A block with l == W-1 is decoded, not copied as raw (the reference's
-// :171-177 treats it as raw and returns wrong values).
+// K5 decode body: Leco_int<T>::decodeArray8 (:142-204) with read_all_bit_fix
+// (bit_read.h:222-275).  The payload is staged through LDS kChunk values at a
+// time with aligned 16-byte loads; each lane extracts its field, adds it to
+// cvtt64(fma(i, theta1, theta0)) and stores.  Writes block values [jlo, jhi)
+// to o[j].  A block with l == W-1 is decoded, not copied as raw (the
+// reference's :171-177 treats it as raw and returns wrong values).
 // ---------------------------------------------------------------------------
 template <typename T, int W>
-__global__ __launch_bounds__(kThreads) void k_decode(const uint8_t *__restrict__ st,
-                                                     const uint64_t *__restrict__ offs, uint64_t n,
-                                                     uint32_t bs, uint64_t b_first, uint64_t lo, uint64_t hi,
-                                                     T *__restrict__ out) {
-    __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
-    const uint64_t b = b_first + blockIdx.x;
+__device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st, uint64_t off, uint32_t m,
+                                                  uint32_t jlo, uint32_t jhi, T *__restrict__ o,
+                                                  uint64_t *__restrict__ buf) {
     const int tid = threadIdx.x;
-    const uint64_t s = b * bs;
-    const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
-    const uint32_t jlo = lo > s ? (uint32_t)(lo - s) : 0;
-    const uint32_t jhi = hi - s < m ? (uint32_t)(hi - s) : m;
-    T *o = out - lo + s;  // o[j] is column value s + j
-    const uint64_t off = offs[b];
     const int l = st[off];
-
     if (l == 127 || l == 126) {  // :152-170, single/double raw value blocks
         const uint32_t cnt = l == 127 ? 1 : 2;
         if ((uint32_t)tid < cnt && (uint32_t)tid >= jlo && (uint32_t)tid < jhi)
@@ -333,29 +374,39 @@ __global__ __launch_bounds__(kThreads) void k_decode(const uint8_t *__restrict__
             const uint64_t q = lbit + (uint64_t)(j - j0) * lw;
             const uint32_t w = (uint32_t)(q >> 6);
             const int sh = (int)(q & 63);
-            const uint64_t v = sh ? (buf[w] >> sh) | (buf[w + 1] << (64 - sh)) : buf[w];
-            const uint64_t field = v & low_mask(lw);
+            const uint64_t vv = sh ? (buf[w] >> sh) | (buf[w + 1] << (64 - sh)) : buf[w];
+            const uint64_t field = vv & low_mask(lw);
             o[j] = raw ? (T)field : (T)apply_field(predict(t0, t1, j), field, l);
         }
         __syncthreads();
     }
 }
 
+// One workgroup per block of one column; values [lo, hi) go to out[global - lo].
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_decode(const uint8_t *__restrict__ st,
+                                                     const uint64_t *__restrict__ offs, uint64_t n, uint32_t bs,
+                                                     uint64_t b_first, uint64_t lo, uint64_t hi,
+                                                     T *__restrict__ out) {
+    __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
+    const uint64_t b = b_first + blockIdx.x;
+    const uint64_t s = b * bs;
+    const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
+    const uint32_t jlo = lo > s ? (uint32_t)(lo - s) : 0;
+    const uint32_t jhi = hi - s < m ? (uint32_t)(hi - s) : m;
+    decode_block_body<T, W>(st, offs[b], m, jlo, jhi, out - lo + s, buf);
+}
+
 // ---------------------------------------------------------------------------
 // K6 lookup: Leco_int<T>::randomdecodeArray8 (:305-337) with
-// read_bit_fix_int_wo_round (bit_read.h:1170-1199).  One lane per query:
-// the block's parameters come from the param table, the field from an
-// unaligned bit window read as two aligned 8-byte words.
+// read_bit_fix_int_wo_round (bit_read.h:1170-1199).  One lane per query: the
+// block's parameters come from the param table, the field from an unaligned
+// bit window read as two aligned 8-byte words.
 // ---------------------------------------------------------------------------
-template <typename T, int W>
-__global__ __launch_bounds__(kThreads) void k_lookup(const uint8_t *__restrict__ st,
-                                                     const leco_fix_block_param *__restrict__ prm, uint64_t n,
-                                                     uint32_t bs, const uint64_t *__restrict__ idx, uint64_t nq,
-                                                     T *__restrict__ out) {
-    const uint64_t q = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (q >= nq) return;
-    const uint64_t i = idx[q];
-    if (i >= n) { out[q] = 0; return; }
+template <int W>
+__device__ __forceinline__ uint64_t lookup_one(const uint8_t *__restrict__ st,
+                                               const leco_fix_block_param *__restrict__ prm, uint32_t bs,
+                                               uint64_t i) {
     uint64_t b;
     uint32_t j;
     if (i <= 0xFFFFFFFFull) {
@@ -368,14 +419,20 @@ __global__ __launch_bounds__(kThreads) void k_lookup(const uint8_t *__restrict__
     }
     const leco_fix_block_param p = prm[b];
     const int l = (int)p.bits;
-    uint64_t v;
-    if (l >= W) {
-        v = load_bits(st + p.payload_offset, (uint64_t)j * W, W);
-    } else {
-        const int64_t pr = predict(p.theta0, p.theta1, j);
-        v = l ? apply_field(pr, load_bits(st + p.payload_offset, (uint64_t)j * l, l), l) : (uint64_t)pr;
-    }
-    out[q] = (T)v;
+    if (l >= W) return load_bits(st + p.payload_offset, (uint64_t)j * W, W);
+    const int64_t pr = predict(p.theta0, p.theta1, j);
+    return l ? apply_field(pr, load_bits(st + p.payload_offset, (uint64_t)j * l, l), l) : (uint64_t)pr;
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_lookup(const uint8_t *__restrict__ st,
+                                                     const leco_fix_block_param *__restrict__ prm, uint64_t n,
+                                                     uint32_t bs, const uint64_t *__restrict__ idx, uint64_t nq,
+                                                     T *__restrict__ out) {
+    const uint64_t q = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= nq) return;
+    const uint64_t i = idx[q];
+    out[q] = i < n ? (T)lookup_one<W>(st, prm, bs, i) : (T)0;
 }
 
 // Param table from block headers (for streams this library did not write).
@@ -402,6 +459,73 @@ __global__ __launch_bounds__(kThreads) void k_build_params(const uint8_t *__rest
 }
 
 // ---------------------------------------------------------------------------
+// Batched kernels: blocks of many columns (mixed widths and block sizes) as
+// one global list, items in column order.  Encode writes one arena; decode is
+// a persistent kernel striding over the items.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_batch_decode(const BatchColumn *__restrict__ cols,
+                                                           const uint64_t *__restrict__ first, uint32_t ncols,
+                                                           uint64_t total, const uint8_t *__restrict__ arena,
+                                                           const uint64_t *__restrict__ offs) {
+    __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
+    for (uint64_t it = blockIdx.x; it < total; it += gridDim.x) {
+        const uint32_t c = find_column(first, ncols, it);
+        const BatchColumn col = cols[c];
+        const uint64_t b = it - first[c];
+        const uint64_t s = b * col.bs;
+        const uint32_t m = (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs);
+        if (col.width == 64)
+            decode_block_body<uint64_t, 64>(arena, offs[it], m, 0, m, (uint64_t *)col.out + s, buf);
+        else
+            decode_block_body<uint32_t, 32>(arena, offs[it], m, 0, m, (uint32_t *)col.out + s, buf);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_batch_widths(const BatchColumn *__restrict__ cols,
+                                                           const uint64_t *__restrict__ first, uint32_t ncols,
+                                                           uint64_t total, uint32_t *__restrict__ widths) {
+    const uint64_t it = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (it >= total) return;
+    widths[it] = cols[find_column(first, ncols, it)].width;
+}
+
+// ---------------------------------------------------------------------------
+// Served batch: one launch for a mixed list of point-lookup chunks and
+// range-decode blocks from many requests (continuous batching).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void k_serve(const ServeColumn *__restrict__ cols,
+                                                    const ServeItem *__restrict__ items, uint32_t nitems) {
+    __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const ServeItem w = items[it];
+        const ServeColumn col = cols[w.column];
+        if (w.kind == kServeLookup) {
+            for (uint32_t q = threadIdx.x; q < w.count; q += kThreads) {
+                const uint64_t i = w.idx[q];
+                if (col.width == 64)
+                    ((uint64_t *)w.out)[q] = i < col.n ? lookup_one<64>(col.stream, col.params, col.bs, i) : 0;
+                else
+                    ((uint32_t *)w.out)[q] =
+                        i < col.n ? (uint32_t)lookup_one<32>(col.stream, col.params, col.bs, i) : 0u;
+            }
+        } else {
+            // one block of a range [lo, hi): w.block, output base w.out = value lo
+            const uint64_t s = w.block * col.bs;
+            const uint32_t m = (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs);
+            const uint32_t jlo = w.lo > s ? (uint32_t)(w.lo - s) : 0;
+            const uint32_t jhi = w.hi - s < m ? (uint32_t)(w.hi - s) : m;
+            if (col.width == 64)
+                decode_block_body<uint64_t, 64>(col.stream, col.offsets[w.block], m, jlo, jhi,
+                                                (uint64_t *)w.out - w.lo + s, buf);
+            else
+                decode_block_body<uint32_t, 32>(col.stream, col.offsets[w.block], m, jlo, jhi,
+                                                (uint32_t *)w.out - w.lo + s, buf);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -409,10 +533,12 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
                          leco_fix_block_param *prm, hipStream_t st) {
     constexpr int W = 8 * sizeof(T);
     const uint64_t nb = (n + bs - 1) / bs;
-    k_fit<T><<<dim3((unsigned)((nb + 63) / 64)), dim3(64), 0, st>>>(in, n, bs, nb, prm);
-    k_resid<T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(in, n, bs, prm, offs);
-    k_scan<<<dim3(1), dim3(1024), 0, st>>>(offs, nb);
-    k_pack<T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(in, n, bs, prm, offs, stream);
+    PlainRows rows{(const uint8_t *)in, n, bs, W, nb};
+    unsigned long long *ticket = (unsigned long long *)offs;  // offs[0] is the ticket until the fix-up
+    k_fit<PlainRows><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(64), 0, st>>>(rows, prm, ticket);
+    k_encode<PlainRows><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream, ticket);
+    k_encode_fixup<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, nb,
+                                                                                                nullptr, W);
     return hipGetLastError();
 }
 
@@ -439,6 +565,32 @@ hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint
     const uint64_t nb = (n + bs - 1) / bs;
     k_build_params<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(stream, offs, n, bs,
                                                                                                  nb, w, prm);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
+                               uint8_t *arena, uint64_t *offs, leco_fix_block_param *prm, uint32_t *widths,
+                               hipStream_t st) {
+    BatchRows rows{cols, first, ncols, total};
+    unsigned long long *ticket = (unsigned long long *)offs;
+    k_batch_widths<<<dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(cols, first, ncols,
+                                                                                                    total, widths);
+    k_fit<BatchRows><<<dim3((unsigned)((total + kFitRows - 1) / kFitRows)), dim3(64), 0, st>>>(rows, prm, ticket);
+    k_encode<BatchRows><<<dim3((unsigned)total), dim3(kThreads), 0, st>>>(rows, prm, offs, arena, ticket);
+    k_encode_fixup<<<dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, total,
+                                                                                                    widths, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
+                               const uint8_t *arena, const uint64_t *offs, uint32_t grid, hipStream_t st) {
+    k_batch_decode<<<dim3(grid), dim3(kThreads), 0, st>>>(cols, first, ncols, total, arena, offs);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve(const ServeColumn *cols, const ServeItem *items, uint32_t nitems, uint32_t grid,
+                        hipStream_t st) {
+    k_serve<<<dim3(grid), dim3(kThreads), 0, st>>>(cols, items, nitems);
     return hipGetLastError();
 }
 

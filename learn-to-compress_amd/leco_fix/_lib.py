@@ -52,6 +52,29 @@ for _w in ("u32", "u64"):
         f"leco_fix_random_decode_{_w}": (c_int, [c_ptr, c_u32, c_ptr]),
     })
 
+
+
+class ColumnDesc(ctypes.Structure):
+    """leco_fix_column_desc (include/leco_fix.h)."""
+    _fields_ = [("d_in", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("n", ctypes.c_uint64),
+                ("block_size", ctypes.c_uint32), ("width_bits", ctypes.c_uint32)]
+
+
+_pp = ctypes.POINTER(ctypes.c_void_p)
+SIGNATURES.update({
+    "leco_fix_batch_create": (c_int, [c_ptr, c_u32, _pp]),
+    "leco_fix_batch_destroy": (c_int, [c_ptr]),
+    "leco_fix_batch_encode": (c_int, [c_ptr, c_ptr]),
+    "leco_fix_batch_decode": (c_int, [c_ptr, c_ptr]),
+    "leco_fix_batch_graph_launch": (c_int, [c_ptr, c_int, c_ptr]),
+    "leco_fix_batch_set_output": (c_int, [c_ptr, c_u32, c_ptr]),
+    "leco_fix_batch_set_decode_grid": (c_int, [c_ptr, c_u32]),
+    "leco_fix_batch_info": (c_int, [c_ptr, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64), _pp, _pp, _pp]),
+    "leco_fix_batch_column": (c_int, [c_ptr, c_u32, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)]),
+    "leco_fix_autotune_block_sizes": (c_int, [c_ptr, c_u32, c_ptr, c_u32, c_u32, c_u32, ctypes.c_double, c_ptr,
+                                              c_ptr, c_ptr]),
+})
+
 _lib = None
 
 

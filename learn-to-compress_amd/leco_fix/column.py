@@ -52,6 +52,7 @@ class Column:
         self.block_size = block_size
         self.width = width
         self.nbytes = nbytes
+        self.base = 0                 # stream byte of block 0 (nonzero inside a batch arena)
 
     @property
     def num_blocks(self) -> int:
@@ -92,7 +93,7 @@ class Column:
         self.stream = st
 
     def stream_bytes(self) -> bytes:
-        return self.stream[:self.nbytes].cpu().numpy().tobytes()
+        return self.stream[self.base:self.base + self.nbytes].cpu().numpy().tobytes()
 
     def decode(self, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
         """Leco_int<T>::decodeArray8 over every block."""

@@ -1,0 +1,110 @@
+"""GPU: batched encode/decode of many mixed columns, hipGraph replay, and the
+block-size autotuner, against the plain path and the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", file=sys.stderr, flush=True)
+
+
+def to_dev(torch, a):
+    st = np.int32 if a.dtype == np.uint32 else np.int64
+    return torch.from_numpy(np.ascontiguousarray(a).view(st)).to("cuda")
+
+
+def mixed_columns(seed, count=12, max_n=40_000):
+    from leco_fix import datagen
+    rng = np.random.default_rng(seed)
+    dists = list(datagen.DISTRIBUTIONS.values())
+    cols = []
+    for c in range(count):
+        width = 32 if c % 2 else 64
+        n = int(np.exp(rng.uniform(np.log(50), np.log(max_n))))
+        bs = int(rng.choice([1, 3, 256, 1000, 2048, 4096, 5000, 16384]))
+        cols.append((dists[c % 4](n, width, int(rng.integers(1 << 30))), bs))
+    return cols
+
+
+def test_batch_matches_plain_and_oracle(oracle_mod):
+    import torch
+    from leco_fix.batch import Batch
+    cols = mixed_columns(1)
+    vals = [to_dev(torch, v) for v, _ in cols]
+    say(f"batch of {len(cols)} mixed columns")
+    b = Batch(vals, [bs for _, bs in cols])
+    b.encode()
+    outs = b.decode()
+    torch.cuda.synchronize()
+    for c, (v, bs) in enumerate(cols):
+        want_st, want_off = oracle_mod.encode_column(v, bs)
+        assert b.column_bytes(c) == want_st.tobytes(), c
+        col = b.column(c)
+        assert np.array_equal(col.offsets.cpu().numpy().view(np.uint64) - np.uint64(col.base), want_off)
+        assert torch.equal(outs[c], vals[c]), c
+        idx = torch.randint(0, len(v), (3000,), device="cuda")
+        assert torch.equal(col.lookup(idx), vals[c][idx]), c
+        a = len(v) // 3
+        assert torch.equal(col.decode_range(a, len(v)), vals[c][a:]), c
+
+
+def test_batch_graph_replay_is_stable():
+    import torch
+    from leco_fix.batch import Batch
+    cols = mixed_columns(2, count=9)
+    vals = [to_dev(torch, v) for v, _ in cols]
+    b = Batch(vals, [bs for _, bs in cols])
+    b.encode(graph=True)
+    torch.cuda.synchronize()
+    first = [b.column_bytes(c) for c in range(len(cols))]
+    for rep in range(4):
+        say(f"graph replay {rep}")
+        for o in b.outs:
+            o.zero_()
+        b.encode(graph=True)
+        b.decode(graph=True)
+        torch.cuda.synchronize()
+        for c in range(len(cols)):
+            assert torch.equal(b.outs[c], vals[c]), (rep, c)
+            assert b.column_bytes(c) == first[c], (rep, c)
+        _ = torch.randn(1000, device="cuda").sum().item()  # eager work between replays
+
+
+def test_batch_decode_grid_sizes():
+    import torch
+    from leco_fix.batch import Batch
+    cols = mixed_columns(3, count=6)
+    vals = [to_dev(torch, v) for v, _ in cols]
+    b = Batch(vals, [bs for _, bs in cols])
+    b.encode()
+    for grid in [1, 7, 256, 4096]:
+        b.set_decode_grid(grid)
+        for o in b.outs:
+            o.zero_()
+        b.decode(graph=grid > 7)
+        torch.cuda.synchronize()
+        assert all(torch.equal(o, v) for o, v in zip(b.outs, vals)), grid
+
+
+def test_autotune_bytes_match_oracle(oracle_mod):
+    import torch
+    from leco_fix.batch import autotune_block_sizes
+    from leco_fix import datagen
+    cand = (256, 1024, 4096)
+    window, windows = 8192, 3
+    cols = [datagen.sorted_uniform(50_000, 64, 5), datagen.stepwise(30_000, 32, 6),
+            datagen.linear_sigma(5_000, 32, 7)]   # the last is shorter than a window
+    bs, nbytes = autotune_block_sizes([to_dev(torch, v) for v in cols], cand, windows, window, tolerance=0.0)
+    for c, v in enumerate(cols):
+        slots = len(v) // window
+        starts = [0] if slots == 0 else [(slots * k // min(windows, slots)) * window
+                                          for k in range(min(windows, slots))]
+        lens = [len(v)] if slots == 0 else [window] * len(starts)
+        for j, cb in enumerate(cand):
+            want = sum(len(oracle_mod.encode_column(v[s:s + ln], cb)[0]) for s, ln in zip(starts, lens))
+            assert int(nbytes[c, j]) == want, (c, cb)
+        assert bs[c] == cand[int(np.argmin(nbytes[c]))] or nbytes[c][cand.index(bs[c])] == nbytes[c].min()
