@@ -180,6 +180,78 @@ int leco_fix_autotune_block_sizes(const leco_fix_column_desc *cols, uint32_t nco
                                   uint32_t ncand, uint32_t windows, uint32_t window, double tolerance,
                                   uint32_t *out_bs, uint64_t *out_bytes, void *stream);
 
+/* ---- request server: resident columns, continuous batching ------------- */
+
+#define LECO_FIX_REQ_LOOKUP 0 /* count indices h_idx[0..count) of `column` */
+#define LECO_FIX_REQ_RANGE 1  /* values [begin, begin+count) of `column` */
+#define LECO_FIX_REQ_INSERT 2 /* encode d_values (count values, block_size, width_bits) into slot `column` */
+#define LECO_FIX_REQ_EVICT 3  /* drop slot `column` once no admitted request uses it */
+
+#define LECO_FIX_SERVE_BATCHED 0 /* continuous batching into shared launches */
+#define LECO_FIX_SERVE_PLAIN 1   /* one launch + synchronize per request, in arrival order */
+
+typedef struct leco_fix_server_config {
+    uint32_t slots;               /* batches in flight at once (one stream each); 0 = 4 */
+    uint32_t max_batch_requests;  /* requests per batch; 0 = 8192 */
+    uint64_t max_batch_values;    /* lookup indices + range values per batch; 0 = 2^26 */
+    uint64_t reserve_bytes;       /* device memory kept free; the pools take the rest */
+    uint64_t result_pool_bytes;   /* result pool (0 = half of what the reserve leaves) */
+    uint64_t column_pool_bytes;   /* column pool for inserted columns (0 = the other half) */
+} leco_fix_server_config;
+
+typedef struct leco_fix_request {
+    uint32_t kind;           /* LECO_FIX_REQ_* */
+    uint32_t column;         /* column slot */
+    uint64_t count;          /* indices, range length, or values to insert */
+    uint64_t begin;          /* range start */
+    const uint64_t *h_idx;   /* lookup indices (host) */
+    void *h_out;             /* if set, results are copied here (host) before the request completes */
+    const void *d_values;    /* insert: values in HBM */
+    uint32_t block_size;     /* insert */
+    uint32_t width_bits;     /* insert */
+    double arrival_s;        /* arrival time, seconds after the run starts (non-decreasing) */
+} leco_fix_request;
+
+typedef struct leco_fix_request_result {
+    double latency_s;        /* completion - arrival */
+    double queued_s;         /* launch - arrival */
+    uint64_t batch;          /* batch it ran in */
+    int status;              /* LECO_FIX_OK, or an error code (e.g. evicted column) */
+    int reserved;
+} leco_fix_request_result;
+
+typedef struct leco_fix_server_stats {
+    double wall_s;           /* first arrival to last completion */
+    uint64_t batches;
+    uint64_t completed;
+    uint64_t failed;
+    uint64_t max_batch_requests;
+    uint64_t values_out;     /* lookup results + range values produced */
+    uint64_t bytes_out;
+    uint64_t encoded_values; /* inserted values encoded */
+    uint64_t encoded_bytes;  /* their raw bytes */
+    double encode_gpu_s;     /* summed encode time of inserts (HIP events) */
+    uint64_t peak_result_bytes;
+    uint64_t peak_column_bytes;
+} leco_fix_server_stats;
+
+typedef struct leco_fix_server leco_fix_server;
+
+int leco_fix_server_create(const leco_fix_server_config *cfg, leco_fix_server **out);
+int leco_fix_server_destroy(leco_fix_server *srv);
+/* Makes an encoded column (in caller memory) servable in slot `column`. */
+int leco_fix_server_attach_column(leco_fix_server *srv, uint32_t column, const uint8_t *d_stream,
+                                  const uint64_t *d_offsets, const leco_fix_block_param *d_params, uint64_t n,
+                                  uint32_t block_size, uint32_t width_bits);
+/* Serves a request trace in real time (arrivals by the host clock) and
+ * fills one result per request; results[i] follows reqs[i]. */
+int leco_fix_server_run(leco_fix_server *srv, const leco_fix_request *reqs, uint64_t nreq, int mode,
+                        leco_fix_request_result *results, leco_fix_server_stats *stats);
+/* Device pointers of an inserted column (valid until it is evicted). */
+int leco_fix_server_column(const leco_fix_server *srv, uint32_t column, const uint8_t **d_stream,
+                           const uint64_t **d_offsets, const leco_fix_block_param **d_params, uint64_t *n,
+                           uint32_t *block_size, uint32_t *width_bits, uint64_t *stream_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,646 @@
+// Request server over resident compressed columns (C ABI in include/leco_fix.h).
+//
+// Continuous batching: a host loop admits arriving point-lookup and
+// range-decode requests into shared launches of k_serve.  Up to `slots`
+// batches are in flight at once, one stream each; a request that arrives
+// while others are on the GPU joins the next batch as soon as a slot and
+// result memory are free, so batches overlap on the device.  Results live in
+// a ring-allocated pool; columns inserted while traffic runs are encoded on a
+// separate stream into a first-fit column pool and evicted once no admitted
+// request uses them.  The pools are carved once from free device memory minus
+// a reserve, which is the memory admission.
+//
+// The plain mode serves the same trace with one launch and one synchronize
+// per request (the per-request launch path), for comparison.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "leco_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+constexpr uint32_t kLookupChunk = 1024;   // indices per served work item
+constexpr uint64_t kAlign = 256;
+constexpr uint32_t kMaxColumns = 1u << 16;
+constexpr uint64_t kIdxCap = 1ull << 22;  // staged lookup indices per batch
+constexpr uint64_t kItemCap = 1ull << 18; // work items per batch
+constexpr uint32_t kServeGrid = 4096;     // workgroups per served launch at most
+
+uint64_t up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// Ring over [0, cap): allocations are made in order and reclaimed once every
+// older allocation has been released too.
+class Ring {
+public:
+    void init(uint64_t cap) { cap_ = cap; }
+    bool alloc(uint64_t n, uint64_t *off, uint64_t *id) {
+        n = up(n ? n : 1, kAlign);
+        if (n > cap_) return false;
+        uint64_t b;
+        if (q_.empty()) {
+            b = 0;
+        } else {
+            const uint64_t tail = q_.front().b;
+            if (head_ > tail) {  // used [tail, head): free [head, cap) and [0, tail)
+                if (head_ + n <= cap_) b = head_;
+                else if (n <= tail) b = 0;
+                else return false;
+            } else {             // wrapped, used [tail, cap) + [0, head): free [head, tail)
+                if (head_ + n <= tail) b = head_;
+                else return false;
+            }
+        }
+        head_ = b + n;
+        used_ += n;
+        q_.push_back({b, n, false, next_});
+        *off = b;
+        *id = next_++;
+        return true;
+    }
+    void release(uint64_t id) {
+        for (auto &a : q_)
+            if (a.id == id) { a.freed = true; break; }
+        while (!q_.empty() && q_.front().freed) {
+            used_ -= q_.front().n;
+            q_.pop_front();
+        }
+        if (q_.empty()) head_ = 0;
+    }
+    uint64_t used() const { return used_; }
+
+private:
+    struct A { uint64_t b, n; bool freed; uint64_t id; };
+    std::deque<A> q_;
+    uint64_t cap_ = 0, head_ = 0, next_ = 0, used_ = 0;
+};
+
+// First-fit allocator with coalescing, for inserted columns.
+class FreeList {
+public:
+    void init(uint64_t cap) { free_.clear(); if (cap) free_[0] = cap; used_ = 0; }
+    bool alloc(uint64_t n, uint64_t *off) {
+        n = up(n, kAlign);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                *off = it->first;
+                const uint64_t rest = it->second - n, at = it->first + n;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                used_ += n;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        n = up(n, kAlign);
+        used_ -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) { it->second += nx->second; free_.erase(nx); }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) { pv->second += it->second; free_.erase(it); }
+        }
+    }
+    uint64_t used() const { return used_; }
+
+private:
+    std::map<uint64_t, uint64_t> free_;
+    uint64_t used_ = 0;
+};
+
+struct Slot {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    uint64_t batch = 0;
+    std::vector<uint64_t> reqs;
+    std::vector<uint64_t> ring_ids;
+    uint64_t *h_idx = nullptr, *d_idx = nullptr;
+    leco::ServeItem *h_items = nullptr, *d_items = nullptr;
+};
+
+struct ColumnSlot {
+    leco::ServeColumn col{};
+    bool present = false, ready = false, evicting = false, owned = false;
+    uint64_t pool_off = 0, pool_len = 0, stream_bytes = 0, prefix = 0;
+    uint32_t refs = 0;
+    uint64_t insert_req = UINT64_MAX, evict_req = UINT64_MAX;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    uint64_t *h_total = nullptr;  // pinned: stream bytes, copied after encode
+};
+
+double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+}  // namespace
+
+struct leco_fix_server {
+    leco_fix_server_config cfg{};
+    std::vector<ColumnSlot> cols;
+    leco::ServeColumn *d_cols = nullptr;
+    uint8_t *result_pool = nullptr;
+    uint64_t result_cap = 0;
+    Ring ring;
+    uint8_t *column_pool = nullptr;
+    uint64_t column_cap = 0;
+    FreeList colfree;
+    std::vector<Slot> slots;
+    hipStream_t enc = nullptr;
+};
+
+namespace {
+
+int upload_column(leco_fix_server *s, uint32_t c, hipStream_t st) {
+    return hipMemcpyAsync(&s->d_cols[c], &s->cols[c].col, sizeof(leco::ServeColumn), hipMemcpyHostToDevice, st) ==
+                   hipSuccess
+               ? LECO_FIX_OK
+               : LECO_FIX_EHIP;
+}
+
+void destroy_server(leco_fix_server *s) {
+    (void)hipDeviceSynchronize();
+    for (Slot &sl : s->slots) {
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.st) (void)hipStreamDestroy(sl.st);
+        if (sl.h_idx) (void)hipHostFree(sl.h_idx);
+        if (sl.h_items) (void)hipHostFree(sl.h_items);
+        (void)hipFree(sl.d_idx);
+        (void)hipFree(sl.d_items);
+    }
+    for (ColumnSlot &c : s->cols) {
+        if (c.e0) (void)hipEventDestroy(c.e0);
+        if (c.e1) (void)hipEventDestroy(c.e1);
+        if (c.h_total) (void)hipHostFree(c.h_total);
+    }
+    if (s->enc) (void)hipStreamDestroy(s->enc);
+    (void)hipFree(s->d_cols);
+    (void)hipFree(s->result_pool);
+    (void)hipFree(s->column_pool);
+    delete s;
+}
+
+uint64_t result_bytes(const leco_fix_server *s, const leco_fix_request &r) {
+    return r.count * (s->cols[r.column].col.width / 8);
+}
+
+// Storage of an inserted column: [params][offsets][stream + tail pad].
+struct InsertLayout {
+    uint64_t nb, prm, offs, stream, total;
+};
+InsertLayout insert_layout(const leco_fix_request &r) {
+    InsertLayout L;
+    L.nb = (r.count + r.block_size - 1) / r.block_size;
+    L.prm = 0;
+    L.offs = up(L.nb * sizeof(leco_fix_block_param), kAlign);
+    L.stream = L.offs + up((L.nb + 1) * sizeof(uint64_t), kAlign);
+    L.total = L.stream + leco_fix_stream_bound(r.count, r.block_size, r.width_bits);
+    return L;
+}
+
+bool valid_insert(const leco_fix_request &r) {
+    return r.d_values && r.count > 0 && r.block_size >= 1 && r.block_size <= leco::kMaxBlockSize &&
+           (r.width_bits == 32 || r.width_bits == 64);
+}
+
+// Starts encoding an inserted column; false if the column pool is full.
+bool start_insert(leco_fix_server *s, const leco_fix_request &r, uint64_t ri) {
+    ColumnSlot &c = s->cols[r.column];
+    const InsertLayout L = insert_layout(r);
+    uint64_t off;
+    if (!s->colfree.alloc(L.total, &off)) return false;
+    uint8_t *base = s->column_pool + off;
+    c.col = leco::ServeColumn{base + L.stream, (const uint64_t *)(base + L.offs),
+                              (const leco_fix_block_param *)(base + L.prm), r.count, r.block_size, r.width_bits};
+    c.present = true;
+    c.ready = false;
+    c.evicting = false;
+    c.owned = true;
+    c.pool_off = off;
+    c.pool_len = L.total;
+    c.prefix = L.stream;
+    c.insert_req = ri;
+    upload_column(s, r.column, s->enc);
+    (void)hipEventRecord(c.e0, s->enc);
+    uint8_t *st = base + L.stream;
+    uint64_t *offs = (uint64_t *)(base + L.offs);
+    leco_fix_block_param *prm = (leco_fix_block_param *)(base + L.prm);
+    const uint64_t cap = L.total - L.stream;
+    if (r.width_bits == 64)
+        (void)leco::launch_encode<uint64_t>((const uint64_t *)r.d_values, r.count, r.block_size, st, offs, prm, s->enc);
+    else
+        (void)leco::launch_encode<uint32_t>((const uint32_t *)r.d_values, r.count, r.block_size, st, offs, prm, s->enc);
+    (void)cap;
+    (void)hipMemcpyAsync(c.h_total, offs + L.nb, sizeof(uint64_t), hipMemcpyDeviceToHost, s->enc);
+    (void)hipEventRecord(c.e1, s->enc);
+    return true;
+}
+
+void free_column(leco_fix_server *s, uint32_t ci) {
+    ColumnSlot &c = s->cols[ci];
+    if (c.owned) s->colfree.release(c.pool_off, c.pool_len);
+    c.present = c.ready = c.evicting = c.owned = false;
+    c.col = leco::ServeColumn{};
+    c.pool_len = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int leco_fix_server_create(const leco_fix_server_config *cfg_in, leco_fix_server **out) {
+    if (!out) return LECO_FIX_EINVAL;
+    leco_fix_server *s = new (std::nothrow) leco_fix_server;
+    if (!s) return LECO_FIX_ENOMEM;
+    if (cfg_in) s->cfg = *cfg_in;
+    leco_fix_server_config &cfg = s->cfg;
+    if (!cfg.slots) cfg.slots = 4;
+    if (!cfg.max_batch_requests) cfg.max_batch_requests = 8192;
+    if (!cfg.max_batch_values) cfg.max_batch_values = 1ull << 26;
+    size_t freeb = 0, totalb = 0;
+    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) { destroy_server(s); return LECO_FIX_EHIP; }
+    const uint64_t staging = (uint64_t)cfg.slots * (kIdxCap * 8 + kItemCap * sizeof(leco::ServeItem)) + (64ull << 20);
+    if (freeb < cfg.reserve_bytes + staging) { destroy_server(s); return LECO_FIX_ENOMEM; }
+    const uint64_t avail = freeb - cfg.reserve_bytes - staging;
+    if (!cfg.result_pool_bytes) cfg.result_pool_bytes = cfg.column_pool_bytes ? avail - std::min(avail, cfg.column_pool_bytes) : avail / 2;
+    if (!cfg.column_pool_bytes) cfg.column_pool_bytes = avail - std::min(avail, cfg.result_pool_bytes);
+    if (cfg.result_pool_bytes + cfg.column_pool_bytes > avail) { destroy_server(s); return LECO_FIX_ENOMEM; }
+    s->result_cap = cfg.result_pool_bytes;
+    s->column_cap = cfg.column_pool_bytes;
+    s->cols.resize(kMaxColumns);
+    bool ok = hipMalloc(&s->d_cols, kMaxColumns * sizeof(leco::ServeColumn)) == hipSuccess &&
+              (s->result_cap == 0 || hipMalloc(&s->result_pool, s->result_cap) == hipSuccess) &&
+              (s->column_cap == 0 || hipMalloc(&s->column_pool, s->column_cap) == hipSuccess) &&
+              hipStreamCreateWithFlags(&s->enc, hipStreamNonBlocking) == hipSuccess;
+    s->slots.resize(cfg.slots);
+    for (Slot &sl : s->slots) {
+        ok = ok && hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
+             hipHostMalloc((void **)&sl.h_idx, kIdxCap * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void **)&sl.h_items, kItemCap * sizeof(leco::ServeItem), hipHostMallocDefault) ==
+                 hipSuccess &&
+             hipMalloc(&sl.d_idx, kIdxCap * sizeof(uint64_t)) == hipSuccess &&
+             hipMalloc(&sl.d_items, kItemCap * sizeof(leco::ServeItem)) == hipSuccess;
+    }
+    if (!ok) { destroy_server(s); return LECO_FIX_ENOMEM; }
+    s->ring.init(s->result_cap);
+    s->colfree.init(s->column_cap);
+    *out = s;
+    return LECO_FIX_OK;
+}
+
+int leco_fix_server_destroy(leco_fix_server *s) {
+    if (!s) return LECO_FIX_EINVAL;
+    destroy_server(s);
+    return LECO_FIX_OK;
+}
+
+int leco_fix_server_attach_column(leco_fix_server *s, uint32_t c, const uint8_t *st, const uint64_t *offs,
+                                  const leco_fix_block_param *prm, uint64_t n, uint32_t bs, uint32_t w) {
+    if (!s || c >= kMaxColumns || !st || !offs || !prm || n == 0 || bs == 0 || bs > leco::kMaxBlockSize ||
+        (w != 32 && w != 64) || s->cols[c].present)
+        return LECO_FIX_EINVAL;
+    ColumnSlot &cs = s->cols[c];
+    cs.col = leco::ServeColumn{st, offs, prm, n, bs, w};
+    cs.present = cs.ready = true;
+    cs.owned = false;
+    int rc = upload_column(s, c, s->enc);
+    if (rc) return rc;
+    return hipStreamSynchronize(s->enc) == hipSuccess ? LECO_FIX_OK : LECO_FIX_EHIP;
+}
+
+int leco_fix_server_column(const leco_fix_server *s, uint32_t c, const uint8_t **st, const uint64_t **offs,
+                           const leco_fix_block_param **prm, uint64_t *n, uint32_t *bs, uint32_t *w,
+                           uint64_t *bytes) {
+    if (!s || c >= kMaxColumns || !s->cols[c].present || !s->cols[c].ready) return LECO_FIX_EINVAL;
+    const ColumnSlot &cs = s->cols[c];
+    if (st) *st = cs.col.stream;
+    if (offs) *offs = cs.col.offsets;
+    if (prm) *prm = cs.col.params;
+    if (n) *n = cs.col.n;
+    if (bs) *bs = cs.col.bs;
+    if (w) *w = cs.col.width;
+    if (bytes) *bytes = cs.stream_bytes;
+    return LECO_FIX_OK;
+}
+
+int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64_t nreq, int mode,
+                        leco_fix_request_result *res, leco_fix_server_stats *stats) {
+    if (!s || (!reqs && nreq) || !res || (mode != LECO_FIX_SERVE_BATCHED && mode != LECO_FIX_SERVE_PLAIN))
+        return LECO_FIX_EINVAL;
+    for (uint64_t i = 0; i < nreq; ++i) {
+        const leco_fix_request &r = reqs[i];
+        if (r.column >= kMaxColumns || r.kind > LECO_FIX_REQ_EVICT || (i && r.arrival_s < reqs[i - 1].arrival_s))
+            return LECO_FIX_EINVAL;
+        if (r.kind == LECO_FIX_REQ_LOOKUP && (!r.h_idx || r.count == 0 || r.count > kIdxCap)) return LECO_FIX_EINVAL;
+        if (r.kind == LECO_FIX_REQ_RANGE && (r.count == 0 || r.count > s->cfg.max_batch_values)) return LECO_FIX_EINVAL;
+        if (r.kind == LECO_FIX_REQ_INSERT && !valid_insert(r)) return LECO_FIX_EINVAL;
+        res[i] = leco_fix_request_result{0, 0, 0, -1, 0};
+    }
+    leco_fix_server_stats st{};
+    const Clock::time_point t0 = Clock::now();
+    auto now_s = [&]() { return secs(t0, Clock::now()); };
+    uint64_t done = 0, next = 0, batch_id = 0;
+    double last_done = 0;
+
+    auto complete = [&](uint64_t ri, double t, int status) {
+        res[ri].latency_s = t - reqs[ri].arrival_s;
+        res[ri].status = status;
+        if (status == LECO_FIX_OK) st.completed++;
+        else st.failed++;
+        last_done = std::max(last_done, t);
+        ++done;
+    };
+    auto ensure_events = [&](ColumnSlot &c) -> bool {
+        if (!c.e0 && hipEventCreate(&c.e0) != hipSuccess) return false;
+        if (!c.e1 && hipEventCreate(&c.e1) != hipSuccess) return false;
+        if (!c.h_total && hipHostMalloc((void **)&c.h_total, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
+            return false;
+        return true;
+    };
+    auto finish_insert = [&](uint32_t ci, double t) {
+        ColumnSlot &c = s->cols[ci];
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c.e0, c.e1);
+        st.encode_gpu_s += ms * 1e-3;
+        st.encoded_values += c.col.n;
+        st.encoded_bytes += c.col.n * (c.col.width / 8);
+        c.stream_bytes = *c.h_total;
+        // shrink the column's storage to what the stream needs
+        const uint64_t keep = up(c.prefix + c.stream_bytes + LECO_FIX_TAIL_PAD, kAlign);
+        if (keep < c.pool_len) {
+            s->colfree.release(c.pool_off + keep, c.pool_len - keep);
+            c.pool_len = keep;
+        }
+        c.ready = true;
+        complete(c.insert_req, t, LECO_FIX_OK);
+        c.insert_req = UINT64_MAX;
+    };
+    auto try_evict = [&](uint32_t ci, double t) {
+        ColumnSlot &c = s->cols[ci];
+        if (c.evicting && c.refs == 0 && c.ready) {
+            const uint64_t ri = c.evict_req;
+            free_column(s, ci);
+            c.evict_req = UINT64_MAX;
+            complete(ri, t, LECO_FIX_OK);
+        }
+    };
+
+    if (mode == LECO_FIX_SERVE_PLAIN) {
+        Slot &sl = s->slots[0];
+        for (uint64_t i = 0; i < nreq; ++i) {
+            const leco_fix_request &r = reqs[i];
+            while (now_s() < r.arrival_s) {
+            }
+            const double ts = now_s();
+            res[i].queued_s = ts - r.arrival_s;
+            res[i].batch = batch_id++;
+            ColumnSlot &c = s->cols[r.column];
+            int status = LECO_FIX_OK;
+            if (r.kind == LECO_FIX_REQ_INSERT) {
+                if (c.present || !ensure_events(c) || !start_insert(s, r, i)) { complete(i, now_s(), LECO_FIX_ENOMEM); continue; }
+                (void)hipStreamSynchronize(s->enc);
+                finish_insert(r.column, now_s());
+                continue;
+            }
+            if (r.kind == LECO_FIX_REQ_EVICT) {
+                if (!c.present) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
+                c.evicting = true;
+                c.evict_req = i;
+                try_evict(r.column, now_s());
+                continue;
+            }
+            if (!c.present || !c.ready || c.evicting) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
+            const uint64_t bytes = result_bytes(s, r);
+            if (bytes > s->result_cap) { complete(i, now_s(), LECO_FIX_ENOMEM); continue; }
+            void *out = s->result_pool;
+            hipError_t e;
+            if (r.kind == LECO_FIX_REQ_LOOKUP) {
+                std::memcpy(sl.h_idx, r.h_idx, r.count * sizeof(uint64_t));
+                e = hipMemcpyAsync(sl.d_idx, sl.h_idx, r.count * sizeof(uint64_t), hipMemcpyHostToDevice, sl.st);
+                if (e == hipSuccess)
+                    e = c.col.width == 64
+                            ? leco::launch_lookup<uint64_t>(c.col.stream, c.col.params, c.col.n, c.col.bs, sl.d_idx,
+                                                            r.count, (uint64_t *)out, sl.st)
+                            : leco::launch_lookup<uint32_t>(c.col.stream, c.col.params, c.col.n, c.col.bs, sl.d_idx,
+                                                            r.count, (uint32_t *)out, sl.st);
+            } else {
+                if (r.begin + r.count > c.col.n) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
+                e = c.col.width == 64
+                        ? leco::launch_decode<uint64_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
+                                                        r.begin + r.count, (uint64_t *)out, sl.st)
+                        : leco::launch_decode<uint32_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
+                                                        r.begin + r.count, (uint32_t *)out, sl.st);
+            }
+            if (e == hipSuccess && r.h_out) e = hipMemcpyAsync(r.h_out, out, bytes, hipMemcpyDeviceToHost, sl.st);
+            if (e == hipSuccess) e = hipStreamSynchronize(sl.st);
+            if (e != hipSuccess) status = LECO_FIX_EHIP;
+            st.values_out += r.count;
+            st.bytes_out += bytes;
+            st.batches++;
+            st.max_batch_requests = 1;
+            complete(i, now_s(), status);
+        }
+    } else {
+        std::deque<uint64_t> queue;  // arrived, not yet admitted, in arrival order
+        const size_t window = 4 * (size_t)s->cfg.max_batch_requests;  // queue entries examined per pass
+        std::vector<uint32_t> pending_inserts;
+        while (done < nreq) {
+            bool progressed = false;
+            double t = now_s();
+            while (next < nreq && reqs[next].arrival_s <= t) queue.push_back(next++);
+
+            // completions
+            for (Slot &sl : s->slots) {
+                if (!sl.busy || hipEventQuery(sl.done) != hipSuccess) continue;
+                t = now_s();
+                for (uint64_t ri : sl.reqs) {
+                    s->cols[reqs[ri].column].refs--;
+                    complete(ri, t, LECO_FIX_OK);
+                }
+                for (uint64_t id : sl.ring_ids) s->ring.release(id);
+                for (uint64_t ri : sl.reqs) try_evict(reqs[ri].column, t);
+                sl.reqs.clear();
+                sl.ring_ids.clear();
+                sl.busy = false;
+                progressed = true;
+            }
+            for (size_t k = 0; k < pending_inserts.size();) {
+                ColumnSlot &c = s->cols[pending_inserts[k]];
+                if (hipEventQuery(c.e1) == hipSuccess) {
+                    t = now_s();
+                    finish_insert(pending_inserts[k], t);
+                    try_evict(pending_inserts[k], t);
+                    pending_inserts[k] = pending_inserts.back();
+                    pending_inserts.pop_back();
+                    progressed = true;
+                } else {
+                    ++k;
+                }
+            }
+
+            // admission: control requests, then one batch into a free slot
+            Slot *free_slot = nullptr;
+            for (Slot &sl : s->slots)
+                if (!sl.busy) { free_slot = &sl; break; }
+            if (!queue.empty()) {
+                std::vector<uint64_t> keep;
+                const size_t scan = std::min(queue.size(), window);
+                uint64_t nitems = 0, nidx = 0, nvals = 0;
+                bool full = free_slot == nullptr;
+                std::vector<uint64_t> admitted;
+                for (size_t qi = 0; qi < scan; ++qi) {
+                    const uint64_t ri = queue[qi];
+                    const leco_fix_request &r = reqs[ri];
+                    ColumnSlot &c = s->cols[r.column];
+                    t = now_s();
+                    if (r.kind == LECO_FIX_REQ_INSERT) {
+                        if (c.present || !ensure_events(c)) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
+                        if (insert_layout(r).total > s->column_cap) { complete(ri, t, LECO_FIX_ENOMEM); progressed = true; continue; }
+                        if (start_insert(s, r, ri)) {
+                            res[ri].queued_s = t - r.arrival_s;
+                            pending_inserts.push_back(r.column);
+                            progressed = true;
+                        } else {
+                            keep.push_back(ri);  // waits for column memory
+                        }
+                        continue;
+                    }
+                    if (r.kind == LECO_FIX_REQ_EVICT) {
+                        bool earlier = false;
+                        for (uint64_t k : keep)
+                            if (reqs[k].column == r.column && reqs[k].kind <= LECO_FIX_REQ_RANGE) earlier = true;
+                        if (!c.present) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
+                        if (earlier || c.evicting) { keep.push_back(ri); continue; }
+                        c.evicting = true;
+                        c.evict_req = ri;
+                        res[ri].queued_s = t - r.arrival_s;
+                        try_evict(r.column, t);
+                        progressed = true;
+                        continue;
+                    }
+                    if (!c.present || c.evicting) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
+                    if (full || !c.ready) { keep.push_back(ri); continue; }
+                    if ((r.kind == LECO_FIX_REQ_RANGE && r.begin + r.count > c.col.n) ||
+                        up(result_bytes(s, r), kAlign) > s->result_cap) {
+                        complete(ri, t, r.kind == LECO_FIX_REQ_RANGE && r.begin + r.count > c.col.n ? LECO_FIX_EINVAL
+                                                                                                  : LECO_FIX_ENOMEM);
+                        progressed = true;
+                        continue;
+                    }
+                    const uint64_t items = r.kind == LECO_FIX_REQ_LOOKUP
+                                               ? (r.count + kLookupChunk - 1) / kLookupChunk
+                                               : (r.begin + r.count - 1) / c.col.bs - r.begin / c.col.bs + 1;
+                    const uint64_t idx = r.kind == LECO_FIX_REQ_LOOKUP ? r.count : 0;
+                    if (admitted.size() >= s->cfg.max_batch_requests || nitems + items > kItemCap ||
+                        nidx + idx > kIdxCap || (nvals + r.count > s->cfg.max_batch_values && !admitted.empty())) {
+                        full = true;
+                        keep.push_back(ri);
+                        continue;
+                    }
+                    uint64_t off, id;
+                    if (!s->ring.alloc(result_bytes(s, r), &off, &id)) {  // memory admission
+                        full = true;
+                        keep.push_back(ri);
+                        continue;
+                    }
+                    // stage the request's work items
+                    Slot &sl = *free_slot;
+                    uint8_t *out = s->result_pool + off;
+                    const uint32_t bw = c.col.width / 8;
+                    if (r.kind == LECO_FIX_REQ_LOOKUP) {
+                        std::memcpy(sl.h_idx + nidx, r.h_idx, r.count * sizeof(uint64_t));
+                        for (uint64_t q0 = 0; q0 < r.count; q0 += kLookupChunk) {
+                            leco::ServeItem &w = sl.h_items[nitems++];
+                            w = leco::ServeItem{leco::kServeLookup, r.column,
+                                                (uint32_t)std::min<uint64_t>(kLookupChunk, r.count - q0), 0,
+                                                sl.d_idx + nidx + q0, out + q0 * bw, 0, 0, 0};
+                        }
+                        nidx += r.count;
+                    } else {
+                        const uint64_t hi = r.begin + r.count;
+                        for (uint64_t b = r.begin / c.col.bs; b <= (hi - 1) / c.col.bs; ++b) {
+                            leco::ServeItem &w = sl.h_items[nitems++];
+                            w = leco::ServeItem{leco::kServeRange, r.column, 0, 0, nullptr, out, b, r.begin, hi};
+                        }
+                    }
+                    nvals += r.count;
+                    c.refs++;
+                    admitted.push_back(ri);
+                    sl.ring_ids.push_back(id);
+                    res[ri].queued_s = t - r.arrival_s;
+                    res[ri].batch = batch_id;
+                    if (r.h_out) {
+                        // recorded now, issued after the launch below
+                        sl.reqs.push_back(ri);
+                        sl.reqs.push_back(off);
+                    } else {
+                        sl.reqs.push_back(ri);
+                        sl.reqs.push_back(UINT64_MAX);
+                    }
+                }
+                for (size_t k = 0; k < scan; ++k) queue.pop_front();
+                for (size_t k = keep.size(); k-- > 0;) queue.push_front(keep[k]);
+                if (!admitted.empty()) {
+                    Slot &sl = *free_slot;
+                    hipError_t e = hipMemcpyAsync(sl.d_items, sl.h_items, nitems * sizeof(leco::ServeItem),
+                                                  hipMemcpyHostToDevice, sl.st);
+                    if (e == hipSuccess && nidx)
+                        e = hipMemcpyAsync(sl.d_idx, sl.h_idx, nidx * sizeof(uint64_t), hipMemcpyHostToDevice, sl.st);
+                    if (e == hipSuccess)
+                        e = leco::launch_serve(s->d_cols, sl.d_items, (uint32_t)nitems,
+                                               (uint32_t)std::min<uint64_t>(nitems, kServeGrid), sl.st);
+                    std::vector<uint64_t> ids;
+                    for (size_t k = 0; k < sl.reqs.size(); k += 2) {
+                        const uint64_t ri = sl.reqs[k], off = sl.reqs[k + 1];
+                        ids.push_back(ri);
+                        const uint64_t bytes = result_bytes(s, reqs[ri]);
+                        st.values_out += reqs[ri].count;
+                        st.bytes_out += bytes;
+                        if (off != UINT64_MAX && e == hipSuccess)
+                            e = hipMemcpyAsync(reqs[ri].h_out, s->result_pool + off, bytes, hipMemcpyDeviceToHost,
+                                               sl.st);
+                    }
+                    sl.reqs.swap(ids);
+                    if (e == hipSuccess) e = hipEventRecord(sl.done, sl.st);
+                    if (e != hipSuccess) return LECO_FIX_EHIP;
+                    sl.busy = true;
+                    sl.batch = batch_id++;
+                    st.batches++;
+                    st.max_batch_requests = std::max<uint64_t>(st.max_batch_requests, admitted.size());
+                    progressed = true;
+                }
+                st.peak_result_bytes = std::max(st.peak_result_bytes, s->ring.used());
+                st.peak_column_bytes = std::max(st.peak_column_bytes, s->colfree.used());
+            }
+            // Nothing in flight, nothing arriving, nothing admitted: what is
+            // queued waits for memory that nothing will free, so it fails.
+            if (!progressed && next == nreq && pending_inserts.empty() && !queue.empty()) {
+                bool busy = false;
+                for (Slot &sl : s->slots) busy = busy || sl.busy;
+                if (!busy) {
+                    t = now_s();
+                    for (uint64_t ri : queue) {
+                        if (reqs[ri].kind == LECO_FIX_REQ_EVICT && s->cols[reqs[ri].column].evicting) continue;
+                        complete(ri, t, LECO_FIX_ENOMEM);
+                    }
+                    queue.clear();
+                }
+            }
+        }
+    }
+    st.wall_s = nreq ? last_done - reqs[0].arrival_s : 0;
+    st.peak_column_bytes = std::max(st.peak_column_bytes, s->colfree.used());
+    if (stats) *stats = st;
+    return LECO_FIX_OK;
+}
+
+}  // extern "C"

@@ -75,6 +75,15 @@ SIGNATURES.update({
                                               c_ptr, c_ptr]),
 })
 
+SIGNATURES.update({
+    "leco_fix_server_create": (c_int, [c_ptr, _pp]),
+    "leco_fix_server_destroy": (c_int, [c_ptr]),
+    "leco_fix_server_attach_column": (c_int, [c_ptr, c_u32, c_ptr, c_ptr, c_ptr, c_u64, c_u32, c_u32]),
+    "leco_fix_server_run": (c_int, [c_ptr, c_ptr, c_u64, c_int, c_ptr, c_ptr]),
+    "leco_fix_server_column": (c_int, [c_ptr, c_u32, _pp, _pp, _pp, ctypes.POINTER(c_u64), ctypes.POINTER(c_u32),
+                                       ctypes.POINTER(c_u32), ctypes.POINTER(c_u64)]),
+})
+
 _lib = None
 
 

@@ -1,0 +1,157 @@
+"""GPU: the request server.  Every served result must equal the column's
+values (ground truth), in the batched mix and when the same request runs
+alone on the plain per-request path; columns inserted while traffic runs
+must encode byte-identically to the oracle; memory admission must hold
+requests back, not fail them, when the result pool is small."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GB = 1 << 30
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", file=sys.stderr, flush=True)
+
+
+def to_dev(torch, a):
+    st = np.int32 if a.dtype == np.uint32 else np.int64
+    return torch.from_numpy(np.ascontiguousarray(a).view(st)).to("cuda")
+
+
+def expected(req, values, idx):
+    v = values[int(req["column"])]
+    if req["kind"] == 0:
+        start = (int(req["h_idx"]) - idx.ctypes.data) // 8
+        return v[idx[start:start + int(req["count"])]]
+    b = int(req["begin"])
+    return v[b:b + int(req["count"])]
+
+
+@pytest.fixture(scope="module")
+def served_columns():
+    import torch
+    from leco_fix import Column, datagen
+    a = datagen.sorted_pwl_u64(2_000_000, seed=21)
+    b = datagen.linear_noise_u32(1_000_000, seed=22)
+    cols = {0: Column.encode(to_dev(torch, a), 4096), 1: Column.encode(to_dev(torch, b), 1000)}
+    return cols, {0: a, 1: b}
+
+
+def test_served_results_equal_alone_and_truth(served_columns):
+    from leco_fix.server import SERVE_BATCHED, SERVE_PLAIN, Server, make_trace
+    cols, values = served_columns
+    srv = Server(slots=4, reserve_bytes=0, result_pool_bytes=1 * GB, column_pool_bytes=64 << 20)
+    for s, c in cols.items():
+        srv.attach(s, c)
+    rng = np.random.default_rng(3)
+    specs = [(0, len(values[0]), 64), (1, len(values[1]), 32)]
+    reqs, idx, outs = make_trace(rng, specs, rate=2e5, nreq=3000, range_hi=2e5, keep_outputs=True)
+    reqs["arrival_s"][:800] = 0.0  # a burst, so batches hold many requests
+    say("batched run")
+    res, stats = srv.run(reqs)
+    assert stats["completed"] == len(reqs) and stats["failed"] == 0
+    assert stats["max_batch_requests"] > 1 and stats["batches"] < len(reqs)
+    mixed = [o.copy() for o in outs]
+    for i in range(len(reqs)):
+        assert np.array_equal(mixed[i], expected(reqs[i], values, idx)), i
+    say("plain run of the same requests, one at a time")
+    for o in outs:
+        o[:] = 0
+    res2, stats2 = srv.run(reqs, SERVE_PLAIN)
+    assert stats2["completed"] == len(reqs) and stats2["batches"] == len(reqs)
+    for i in range(len(reqs)):
+        assert np.array_equal(outs[i], mixed[i]), i
+    assert (res["status"] == 0).all() and (res2["status"] == 0).all()
+    srv.close()
+
+
+def test_admission_holds_requests_when_results_pool_is_small(served_columns):
+    from leco_fix.server import REQUEST_DTYPE, Server
+    cols, values = served_columns
+    pool = 4 << 20  # 4 MiB of results: at most a few ranges in flight
+    srv = Server(slots=4, reserve_bytes=0, result_pool_bytes=pool, column_pool_bytes=64 << 20)
+    srv.attach(0, cols[0])
+    n = len(values[0])
+    reqs = np.zeros(41, REQUEST_DTYPE)
+    reqs["kind"] = 1
+    reqs["column"] = 0
+    reqs["count"] = 300_000          # 2.4 MB each: one fits in the pool at a time
+    reqs["begin"] = np.linspace(0, n - 300_000, 41).astype(np.uint64)
+    reqs["count"][-1] = 600_000      # 4.8 MB: larger than the whole pool
+    reqs["begin"][-1] = 0
+    outs = [np.zeros(int(c), np.uint64) for c in reqs["count"]]
+    for i, o in enumerate(outs):
+        reqs["h_out"][i] = o.ctypes.data
+    res, stats = srv.run(reqs)
+    assert (res["status"][:-1] == 0).all() and res["status"][-1] == 3  # ENOMEM for the oversized one
+    assert stats["peak_result_bytes"] <= pool
+    for i in range(40):
+        b = int(reqs["begin"][i])
+        assert np.array_equal(outs[i], values[0][b:b + 300_000]), i
+    srv.close()
+
+
+def test_churn_insert_evict_while_serving(oracle_mod):
+    import torch
+    from leco_fix import datagen
+    from leco_fix.batch import _view
+    from leco_fix.server import REQ_EVICT, REQ_INSERT, REQ_LOOKUP, REQ_RANGE, REQUEST_DTYPE, Server
+    srv = Server(slots=3, reserve_bytes=0, result_pool_bytes=256 << 20, column_pool_bytes=1 * GB)
+    rng = np.random.default_rng(9)
+    data = {s: datagen.drift_u64(int(rng.integers(20_000, 400_000)), seed=100 + s) for s in range(4)}
+    dev = {s: to_dev(torch, v) for s, v in data.items()}
+    bss = {0: 1024, 1: 2048, 2: 4096, 3: 8192}
+    rows, outs, t = [], [], 0.0
+    idx_keep = []
+
+    def add(kind, col, count=0, begin=0, **kw):
+        nonlocal t
+        r = np.zeros(1, REQUEST_DTYPE)[0]
+        r["kind"], r["column"], r["count"], r["begin"], r["arrival_s"] = kind, col, count, begin, t
+        for k, v in kw.items():
+            r[k] = v
+        rows.append(r)
+        t += 2e-5
+
+    # insert 0 and 1, traffic on both; evict 0 and insert 2; traffic on 1 and 2
+    for c in (0, 1):
+        add(REQ_INSERT, c, len(data[c]), d_values=dev[c].data_ptr(), block_size=bss[c], width_bits=64)
+    for k in range(60):
+        c = k % 2 if k <= 30 else 1 + k % 2
+        n = len(data[c])
+        if k % 3:
+            ii = rng.integers(0, n, 700, dtype=np.uint64)
+            idx_keep.append(ii)
+            o = np.zeros(700, np.uint64)
+            outs.append((len(rows), o, data[c][ii]))
+            add(REQ_LOOKUP, c, 700, h_idx=ii.ctypes.data, h_out=o.ctypes.data)
+        else:
+            b = int(rng.integers(0, n - 5000))
+            o = np.zeros(5000, np.uint64)
+            outs.append((len(rows), o, data[c][b:b + 5000]))
+            add(REQ_RANGE, c, 5000, b, h_out=o.ctypes.data)
+        if k == 30:
+            add(REQ_EVICT, 0)
+            add(REQ_INSERT, 2, len(data[2]), d_values=dev[2].data_ptr(), block_size=bss[2], width_bits=64)
+    late = len(rows)
+    add(REQ_LOOKUP, 0, 10, h_idx=idx_keep[0].ctypes.data)   # slot 0 is gone by now: must fail
+    reqs = np.array(rows, REQUEST_DTYPE)
+    say(f"churn trace of {len(reqs)} requests")
+    res, stats = srv.run(reqs)
+    for i, o, want in outs:
+        assert res["status"][i] == 0, (i, res["status"][i])
+        assert np.array_equal(o, want), i
+    assert res["status"][late] != 0
+    assert (res["status"][np.array(reqs["kind"]) >= REQ_INSERT] == 0).all()
+    assert stats["encoded_values"] == sum(len(data[s]) for s in (0, 1, 2))
+    # the inserted columns are byte-identical to the oracle
+    for s in (1, 2):
+        info = srv.column_info(s)
+        got = _view(info["stream"], info["stream_bytes"], "|u1").cpu().numpy().tobytes()
+        want, _ = oracle_mod.encode_column(data[s], bss[s])
+        assert got == want.tobytes(), s
+    srv.close()
