@@ -21,8 +21,11 @@ struct leco_fix_batch {
     uint64_t total = 0;
     std::vector<BatchColumn> h_cols;
     std::vector<uint64_t> h_first;
+    std::vector<uint64_t> h_fit_first;
+    uint64_t fit_groups = 0;
     BatchColumn *d_cols = nullptr;
     uint64_t *d_first = nullptr;
+    uint64_t *d_fit_first = nullptr;
     uint8_t *d_arena = nullptr;
     uint64_t arena_cap = 0;
     uint64_t *d_offs = nullptr;
@@ -43,6 +46,7 @@ void free_batch(leco_fix_batch *b) {
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
     (void)hipFree(b->d_cols);
     (void)hipFree(b->d_first);
+    (void)hipFree(b->d_fit_first);
     (void)hipFree(b->d_arena);
     (void)hipFree(b->d_offs);
     (void)hipFree(b->d_prm);
@@ -57,8 +61,8 @@ bool valid_desc(const leco_fix_column_desc &c) {
 
 int enqueue(leco_fix_batch *b, int op, hipStream_t st) {
     if (op == 0)
-        return hip_status(leco::launch_batch_encode(b->d_cols, b->d_first, b->ncols, b->total, b->d_arena, b->d_offs,
-                                                    b->d_prm, b->d_widths, st));
+        return hip_status(leco::launch_batch_encode(b->d_cols, b->d_first, b->d_fit_first, b->ncols, b->total,
+                                                    b->fit_groups, b->d_arena, b->d_offs, b->d_prm, b->d_widths, st));
     return hip_status(leco::launch_batch_decode(b->d_cols, b->d_first, b->ncols, b->total, b->d_arena, b->d_offs,
                                                 b->decode_grid, st));
 }
@@ -84,22 +88,27 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
     b->ncols = ncols;
     b->h_cols.resize(ncols);
     b->h_first.resize(ncols + 1);
+    b->h_fit_first.resize(ncols + 1);
     uint64_t arena = 0;
     for (uint32_t c = 0; c < ncols; ++c) {
         const leco_fix_column_desc &d = cols[c];
         b->h_cols[c] = BatchColumn{d.d_in, d.d_out, d.n, d.block_size, d.width_bits};
         b->h_first[c] = b->total;
+        b->h_fit_first[c] = b->fit_groups;
         const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
+        b->fit_groups += (nb + 63) / 64;
         const uint64_t last = d.n - (nb - 1) * d.block_size;
         arena += (nb - 1) * leco::block_bound(d.width_bits, d.block_size) +
                  leco::block_bound(d.width_bits, (uint32_t)last);
         b->total += nb;
     }
     b->h_first[ncols] = b->total;
+    b->h_fit_first[ncols] = b->fit_groups;
     b->arena_cap = arena + LECO_FIX_TAIL_PAD;
     b->decode_grid = default_decode_grid();
     bool ok = hipMalloc(&b->d_cols, ncols * sizeof(BatchColumn)) == hipSuccess &&
               hipMalloc(&b->d_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
+              hipMalloc(&b->d_fit_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
               hipMalloc(&b->d_arena, b->arena_cap) == hipSuccess &&
               hipMalloc(&b->d_offs, (b->total + 1) * sizeof(uint64_t)) == hipSuccess &&
               hipMalloc(&b->d_prm, b->total * sizeof(leco_fix_block_param)) == hipSuccess &&
@@ -110,6 +119,8 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
     }
     if (hipMemcpy(b->d_cols, b->h_cols.data(), ncols * sizeof(BatchColumn), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(b->d_first, b->h_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemcpy(b->d_fit_first, b->h_fit_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
             hipSuccess ||
         hipMemset(b->d_arena + arena, 0, LECO_FIX_TAIL_PAD) != hipSuccess) {
         free_batch(b);

@@ -18,7 +18,14 @@ constexpr int kWave = 64;
 // x86 CVTTSD2SI (r64): truncation toward zero; NaN or out of [-2^63, 2^63)
 // gives INT64_MIN.  The reference converts predictions with a plain
 // (int64_t)/(long long) cast, which is this instruction on x86.
+// Fast path: for an integer t with |t| < 2^51, t + 1.5*2^52 is exact and its
+// low mantissa bits hold t + 2^51, so bits(t + 1.5*2^52) - 0x4338000000000000
+// is t (two's complement).  That is one add instead of the ~8 ops of the
+// generic f64 -> i64 lowering; NaN and large values take the generic path.
 __device__ __forceinline__ int64_t cvtt64(double x) {
+    const double t = __builtin_trunc(x);
+    if (__builtin_expect(__builtin_fabs(t) < 0x1p51, 1))
+        return (int64_t)((uint64_t)__double_as_longlong(t + 0x1.8p52) - 0x4338000000000000ull);
     if (!(x >= -9223372036854775808.0 && x < 9223372036854775808.0)) return INT64_MIN;
     return (int64_t)x;
 }
@@ -42,6 +49,16 @@ __device__ __forceinline__ uint64_t residual_field(uint64_t y, int64_t p, int l,
     *mag_out = mag;
     if (l <= 0) return 0;
     return (mag & ((1ull << (l - 1)) - 1)) | ((uint64_t)pos << (l - 1));
+}
+
+// Sign (true: y > p) and magnitude of the residual, as residual_field.
+template <int W>
+__device__ __forceinline__ bool residual_sign(uint64_t y, int64_t p, uint64_t *mag_out) {
+    const bool pos = (W == 64) ? (y > (uint64_t)p) : ((int64_t)y > p);
+    uint64_t mag = pos ? y - (uint64_t)p : (uint64_t)p - y;
+    if (W == 32) mag &= 0xFFFFFFFFull;
+    *mag_out = mag;
+    return pos;
 }
 
 // Value from a prediction and an l-bit field (bit_read.h:238-254).

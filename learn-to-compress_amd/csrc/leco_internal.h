@@ -78,9 +78,11 @@ hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint
 // total+1 arena offsets (offs[first[c] + b] for block b of column c), prm the
 // total per-block params, widths total uint32 of scratch.  first[c] is the
 // global index of column c's block 0; first[ncols] = total.
-hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
-                               uint8_t *arena, uint64_t *offs, leco_fix_block_param *prm, uint32_t *widths,
-                               hipStream_t st);
+// fit_first[c] = index of column c's first 64-block fit group (fit_first[ncols]
+// = fit_groups).
+hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, const uint64_t *fit_first,
+                               uint32_t ncols, uint64_t total, uint64_t fit_groups, uint8_t *arena, uint64_t *offs,
+                               leco_fix_block_param *prm, uint32_t *widths, hipStream_t st);
 // Batched decode: a persistent grid of `grid` workgroups strides over the blocks.
 hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
                                const uint8_t *arena, const uint64_t *offs, uint32_t grid, hipStream_t st);

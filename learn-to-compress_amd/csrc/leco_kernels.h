@@ -11,12 +11,17 @@ constexpr int kThreads = 256;                       // workgroup of the per-bloc
 constexpr uint32_t kChunk = 2048;                   // values staged through LDS at a time
 constexpr int kLdsBytes = kChunk * 8 + 64;          // one chunk of raw uint64 + alignment slack
 constexpr int kVPT = 16;                            // values per lane kept in registers (4096 per block)
-constexpr int kFitRows = 64;                        // blocks per fit wave (one lane each)
-constexpr int kFitTK = 32;                          // values per block row in one fit tile
-constexpr int kFitPer = kFitRows * kFitTK / kWave;  // tile values loaded per lane
+constexpr int kFitRows = 64;                        // blocks per fit workgroup (one lane of wave 0 each)
+constexpr int kFitTK = 128;                         // values per block row in one fit tile
+constexpr int kFitThreads = 256;                    // fit workgroup: four waves load, wave 0 also fits
+constexpr int kFitPer = kFitRows * kFitTK / kFitThreads;  // tile values loaded per thread
+constexpr int kLookPer = 4;                         // predecessors read per lane per look-back step
+constexpr int kLookWin = kWave * kLookPer;          // predecessors per look-back step (wave 0)
 
-// Look-back status word (in leco_fix_block_param::payload_offset while the
-// encoder runs): 2 flag bits + 62-bit byte count.
+// Look-back status word (offs[b+1] while the encoder runs; the fix-up clears
+// the flags and leaves the block's end offset): 2 flag bits + 62-bit count.  The word carries its own
+// value, so relaxed agent-scope atomics (which bypass the non-coherent L1)
+// are enough; nothing else is published through it.
 constexpr uint64_t kStatusAggregate = 1ull << 62;   // value = this block's size
 constexpr uint64_t kStatusInclusive = 2ull << 62;   // value = bytes up to and including this block
 constexpr uint64_t kStatusMask = 3ull << 62;
@@ -24,10 +29,10 @@ constexpr uint64_t kStatusValue = (1ull << 62) - 1;
 constexpr uint32_t kSpinLimit = 1u << 26;              // look-back wait bound (s_sleep 1 each)
 
 __device__ __forceinline__ void store_status(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t load_status(const uint64_t *p) {
-    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 struct EncodeShared {
@@ -60,6 +65,26 @@ struct PlainRows {
     }
 };
 
+// 64 consecutive blocks of one column, the unit of the fit kernel.
+struct FitGroup {
+    const uint8_t *base;  // the column's values
+    uint64_t n;
+    uint32_t bs;
+    uint32_t width;
+    uint64_t b0;          // first block of the group, within the column
+    uint64_t gb0;         // its index in the param table / offsets
+};
+
+struct PlainFit {
+    const uint8_t *in;
+    uint64_t n;
+    uint32_t bs;
+    uint32_t width;
+    __device__ __forceinline__ FitGroup group(uint32_t wg) const {
+        return {in, n, bs, width, (uint64_t)wg * kFitRows, (uint64_t)wg * kFitRows};
+    }
+};
+
 // Largest c with first[c] <= it (first[0] = 0 <= it < first[ncols]).
 __device__ __forceinline__ uint32_t find_column(const uint64_t *__restrict__ first, uint32_t ncols, uint64_t it) {
     uint32_t lo = 0, hi = ncols;
@@ -70,6 +95,20 @@ __device__ __forceinline__ uint32_t find_column(const uint64_t *__restrict__ fir
     }
     return lo;
 }
+
+// Fit groups of many columns: fit_first[c] is column c's first group.
+struct BatchFit {
+    const BatchColumn *cols;
+    const uint64_t *first;
+    const uint64_t *fit_first;
+    uint32_t ncols;
+    __device__ __forceinline__ FitGroup group(uint32_t wg) const {
+        const uint32_t c = find_column(fit_first, ncols, wg);
+        const BatchColumn col = cols[c];
+        const uint64_t b0 = ((uint64_t)wg - fit_first[c]) * kFitRows;
+        return {(const uint8_t *)col.in, col.n, col.bs, col.width, b0, first[c] + b0};
+    }
+};
 
 // Blocks of many columns, in column order.
 struct BatchRows {

@@ -16,48 +16,44 @@ namespace leco {
 
 // ---------------------------------------------------------------------------
 // K1 fit: lr_int_T<T>::caltheta (lr.h:66-85).  The sums over y and i*y are
-// sequential FP64 chains in the reference, so each lane owns one block and
-// replays its chain in index order, 64 blocks per wave.  Tiles of 64 blocks x
-// kFitTK values are loaded row by row (coalesced), converted and parked in
+// sequential FP64 chains in the reference, so each block's chain is replayed
+// in index order by one lane: a workgroup takes 64 consecutive blocks of one
+// column and its wave 0 owns one block per lane.  Tiles of 64 blocks x kFitTK
+// values are loaded by all four waves (16 rows each, 64 consecutive values
+// per instruction, no branches around the loads), converted and parked in
 // LDS, and read back one block per lane; the next tile's loads are in flight
 // while the current tile's chains run.  sum(i) and sum(i*i) are exact and
-// formed in closed form.  Also clears each block's look-back status word
-// (payload_offset) and, in workgroup 0, the ticket counter.
+// formed in closed form.  Also clears the look-back status words (offs[b+1])
+// and, in workgroup 0, the ticket counter (offs[0]).
 // ---------------------------------------------------------------------------
-template <class Rows>
-__global__ __launch_bounds__(64) void k_fit(Rows rows, leco_fix_block_param *__restrict__ prm,
-                                            unsigned long long *__restrict__ ticket) {
-    __shared__ double tile[kFitRows][kFitTK + 1];
-    __shared__ const uint8_t *rbase[kFitRows];
-    __shared__ uint32_t rlen[kFitRows];
-    __shared__ int rwid[kFitRows];
-    const int lane = threadIdx.x;
-    const uint64_t b = (uint64_t)blockIdx.x * kFitRows + lane;
-    if (blockIdx.x == 0 && lane == 0) *ticket = 0;
+template <typename T, class Src>
+__device__ __forceinline__ void fit_group(const FitGroup &g, leco_fix_block_param *__restrict__ prm,
+                                          uint64_t *__restrict__ offs, double (*tile)[kFitTK + 1]) {
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    const T *y = (const T *)g.base;
+    const uint64_t nb = (g.n + g.bs - 1) / g.bs;
+    auto row_len = [&](uint64_t r) -> uint32_t {
+        const uint64_t b = g.b0 + r;
+        if (b >= nb) return 0;
+        const uint64_t s = b * g.bs;
+        return (uint32_t)(g.n - s < g.bs ? g.n - s : g.bs);
+    };
+    const uint32_t m = row_len(lane);                 // wave 0: this lane's block
+    const uint32_t maxm = row_len(0);                 // the first row is never shorter than the others
 
-    RowRef me = rows.row(b);
-    rbase[lane] = me.base;
-    rlen[lane] = me.m;
-    rwid[lane] = me.width;
-    uint32_t maxm = me.m;
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        uint32_t u = (uint32_t)__shfl_xor((int)maxm, o, kWave);
-        maxm = u > maxm ? u : maxm;
-    }
-    __syncthreads();
-
+    // this thread's rows: wave w loads rows 16w .. 16w+15, two halves each
+    constexpr int kRowsPerWave = kFitRows / (kFitThreads / kWave);
     uint64_t reg[kFitPer];
     auto load = [&](uint32_t k0) {
 #pragma unroll
         for (int it = 0; it < kFitPer; ++it) {
-            const int r = it * (kWave / kFitTK) + lane / kFitTK;
-            const uint32_t k = k0 + (uint32_t)(lane % kFitTK);
-            uint64_t v = 0;
-            if (k < rlen[r]) {
-                const uint8_t *p = rbase[r];
-                v = rwid[r] == 64 ? ((const uint64_t *)p)[k] : (uint64_t)((const uint32_t *)p)[k];
-            }
-            reg[it] = v;
+            const int r = wid * kRowsPerWave + it / (kFitTK / kWave);
+            const uint32_t k = k0 + (uint32_t)((it % (kFitTK / kWave)) * kWave + lane);
+            const uint32_t mr = row_len(r);
+            const T *rp = y + (mr ? (g.b0 + r) * g.bs : 0);
+            const uint32_t kk = mr ? (k < mr ? k : mr - 1) : 0;
+            const uint64_t v = (uint64_t)rp[kk];
+            reg[it] = k < mr ? v : 0;
         }
     };
 
@@ -66,28 +62,31 @@ __global__ __launch_bounds__(64) void k_fit(Rows rows, leco_fix_block_param *__r
     for (uint32_t k0 = 0; k0 < maxm; k0 += kFitTK) {
 #pragma unroll
         for (int it = 0; it < kFitPer; ++it)
-            tile[it * (kWave / kFitTK) + lane / kFitTK][lane % kFitTK] = (double)reg[it];
+            tile[wid * kRowsPerWave + it / (kFitTK / kWave)][(it % (kFitTK / kWave)) * kWave + lane] =
+                (double)reg[it];
         __syncthreads();
-        if (k0 + kFitTK < maxm) load(k0 + kFitTK);
-        if (k0 + kFitTK <= me.m) {
-#pragma unroll
-            for (int c = 0; c < kFitTK; ++c) {
-                const double yi = tile[lane][c];
-                sy = sy + yi;
-                sxy = __builtin_fma((double)(k0 + c), yi, sxy);
-            }
-        } else if (k0 < me.m) {
-            for (uint32_t c = 0; c < me.m - k0; ++c) {
-                const double yi = tile[lane][c];
-                sy = sy + yi;
-                sxy = __builtin_fma((double)(k0 + c), yi, sxy);
+        if (k0 + kFitTK < maxm) load(k0 + kFitTK);  // in flight while wave 0 runs the chains
+        if (wid == 0) {
+            if (k0 + kFitTK <= m) {
+#pragma unroll 16
+                for (int c = 0; c < kFitTK; ++c) {
+                    const double yi = tile[lane][c];
+                    sy = sy + yi;
+                    sxy = __builtin_fma((double)(k0 + c), yi, sxy);
+                }
+            } else if (k0 < m) {
+                for (uint32_t c = 0; c < m - k0; ++c) {
+                    const double yi = tile[lane][c];
+                    sy = sy + yi;
+                    sxy = __builtin_fma((double)(k0 + c), yi, sxy);
+                }
             }
         }
         __syncthreads();
     }
-    if (!me.m) return;
-    const uint64_t mm = me.m;
-    const double md = (double)me.m;
+    if (wid != 0 || !m) return;
+    const uint64_t mm = m;
+    const double md = (double)m;
     const double sx = (double)(mm * (mm - 1) / 2);
     const double sxx = (double)((mm - 1) * mm * (2 * mm - 1) / 6);
     const double ccc = __builtin_fma(sxy, md, -(sx * sy));
@@ -100,11 +99,22 @@ __global__ __launch_bounds__(64) void k_fit(Rows rows, leco_fix_block_param *__r
         t1 = ccc / xxx;
         t0 = __builtin_fma(-t1, sx, sy) / md;
     }
-    leco_fix_block_param *p = prm + b;
+    const uint64_t gb = g.gb0 + lane;
+    leco_fix_block_param *p = prm + gb;
     p->theta0 = t0;
     p->theta1 = t1;
-    p->payload_offset = 0;  // look-back status: not ready
-    p->length = me.m;
+    p->length = m;
+    offs[gb + 1] = 0;  // look-back status: not ready
+}
+
+template <class Src>
+__global__ __launch_bounds__(kFitThreads) void k_fit(Src src, leco_fix_block_param *__restrict__ prm,
+                                                     uint64_t *__restrict__ offs) {
+    __shared__ double tile[kFitRows][kFitTK + 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) offs[0] = 0;  // ticket
+    const FitGroup g = src.group(blockIdx.x);
+    if (g.width == 64) fit_group<uint64_t, Src>(g, prm, offs, tile);
+    else fit_group<uint32_t, Src>(g, prm, offs, tile);
 }
 
 // ---------------------------------------------------------------------------
@@ -134,16 +144,14 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
         v[k] = j < m ? (uint64_t)y[j] : 0;
     }
 
-    // pass 1
+    // pass 1 (lanes past the block's end are masked by selects, not branches)
     int64_t mx = INT64_MIN, mn = INT64_MAX;
 #pragma unroll
     for (int k = 0; k < kVPT; ++k) {
         const uint32_t j = tid + k * kThreads;
-        if (j < m) {
-            const int64_t d = (int64_t)(v[k] - (uint64_t)predict(t0, t1, j));
-            mx = d > mx ? d : mx;
-            mn = d < mn ? d : mn;
-        }
+        const int64_t d = (int64_t)(v[k] - (uint64_t)predict(t0, t1, j));
+        mx = (j < m && d > mx) ? d : mx;
+        mn = (j < m && d < mn) ? d : mn;
     }
     for (uint32_t j = tid + kVPT * kThreads; j < m; j += kThreads) {
         const int64_t d = (int64_t)((uint64_t)y[j] - (uint64_t)predict(t0, t1, j));
@@ -164,15 +172,18 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
     // :46 theta0 += (max + min) / 2.  (int64 sum wraps; x * 0.5 is exact)
     if (!__builtin_isnan(t0)) t0 = t0 + (double)(int64_t)((uint64_t)mx + (uint64_t)mn) * 0.5;
 
-    // pass 2
+    // pass 2: the register-cached values are replaced by their magnitudes
+    // and signs, which the pack phase turns into fields without predicting
+    // again (raw blocks re-read their values)
     uint64_t mg = 0, mag;
+    uint32_t signs = 0;
 #pragma unroll
     for (int k = 0; k < kVPT; ++k) {
         const uint32_t j = tid + k * kThreads;
-        if (j < m) {
-            residual_field<W>(v[k], predict(t0, t1, j), 0, &mag);
-            mg = mag > mg ? mag : mg;
-        }
+        const bool pos = residual_sign<W>(v[k], predict(t0, t1, j), &mag);
+        v[k] = mag;
+        signs |= (uint32_t)pos << k;
+        mg = (j < m && mag > mg) ? mag : mg;
     }
     for (uint32_t j = tid + kVPT * kThreads; j < m; j += kThreads) {
         residual_field<W>((uint64_t)y[j], predict(t0, t1, j), 0, &mag);
@@ -182,40 +193,68 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
     if (lane == 0) sh.rmag[wid] = mg;
     __syncthreads();
 
-    // bit width, size and look-back (one lane)
-    if (tid == 0) {
+    // bit width and size (every thread, from the four wave maxima)
 #pragma unroll
-        for (int w = 0; w < kThreads / kWave; ++w) mg = sh.rmag[w] > mg ? sh.rmag[w] : mg;
-        int l = mg ? bitlen64(mg) + 1 : 0;
-        if (l > W) l = W;
-        const uint64_t size = block_bytes((uint32_t)l, W, m);
-        uint64_t *status = &prm[b].payload_offset;
-        uint64_t excl = 0;
+    for (int w = 0; w < kThreads / kWave; ++w) mg = sh.rmag[w] > mg ? sh.rmag[w] : mg;
+    int lb = mg ? bitlen64(mg) + 1 : 0;
+    if (lb > W) lb = W;
+    const uint64_t size = block_bytes((uint32_t)lb, W, m);
+
+    // Decoupled look-back over the status words offs[p+1] (wave 0): publish
+    // this block's size, then read kLookWin predecessors per step (kLookPer
+    // per lane, nearest first) until the nearest one that carries an
+    // inclusive prefix; the exclusive prefix is its value plus the sizes of
+    // the blocks in between.  Every lower block belongs to a workgroup that is
+    // already running and publishes its size without waiting, so the waits
+    // end; the spin bound (~seconds) only keeps a broken invariant from
+    // hanging the GPU.
+    uint64_t excl = 0;
+    if (wid == 0) {
         if (b == 0) {
-            store_status(status, kStatusInclusive | size);
+            if (lane == 0) store_status(&offs[1], kStatusInclusive | size);
         } else {
-            store_status(status, kStatusAggregate | size);
-            // Every lower block belongs to a workgroup that is already running
-            // and publishes its size without waiting, so this ends; the spin
-            // bound (~seconds) only keeps a broken invariant from hanging the GPU.
-            uint32_t spins = 0;
-            for (uint64_t p = b - 1;; --p) {
-                uint64_t s;
-                while (((s = load_status(&prm[p].payload_offset)) & kStatusMask) == 0 && ++spins < kSpinLimit)
-                    __builtin_amdgcn_s_sleep(1);
-                excl += s & kStatusValue;
-                if ((s & kStatusMask) == kStatusInclusive || spins >= kSpinLimit || p == 0) break;
+            if (lane == 0) store_status(&offs[b + 1], kStatusAggregate | size);
+            for (int64_t top = (int64_t)b - 1;; top -= kLookWin) {
+                uint64_t stw[kLookPer];
+#pragma unroll
+                for (int k = 0; k < kLookPer; ++k) {
+                    const int64_t p = top - (k * kWave + lane);
+                    stw[k] = p >= 0 ? load_status(&offs[p + 1]) : kStatusInclusive;
+                }
+#pragma unroll
+                for (int k = 0; k < kLookPer; ++k) {
+                    const int64_t p = top - (k * kWave + lane);
+                    uint32_t spins = 0;
+                    while ((stw[k] & kStatusMask) == 0 && ++spins < kSpinLimit) {
+                        __builtin_amdgcn_s_sleep(1);
+                        stw[k] = load_status(&offs[p + 1]);
+                    }
+                }
+                int stop = kLookWin;  // nearest predecessor (q = k*64 + lane) with an inclusive prefix
+#pragma unroll
+                for (int k = kLookPer - 1; k >= 0; --k) {
+                    const uint64_t incl = __ballot((stw[k] & kStatusMask) == kStatusInclusive);
+                    if (incl) stop = k * kWave + __ffsll((long long)incl) - 1;
+                }
+                uint64_t part = 0;
+#pragma unroll
+                for (int k = 0; k < kLookPer; ++k)
+                    if (k * kWave + lane <= stop) part += stw[k] & kStatusValue;
+                for (int o = kWave / 2; o > 0; o >>= 1) part += (uint64_t)__shfl_xor((long long)part, o, kWave);
+                excl += part;
+                if (stop < kLookWin) break;
             }
-            store_status(status, kStatusInclusive | (excl + size));
+            if (lane == 0) store_status(&offs[b + 1], kStatusInclusive | (excl + size));
         }
-        offs[b + 1] = excl + size;
+    }
+    if (tid == 0) {
         sh.off = excl;
         sh.end = excl + size;
-        sh.l = l;
-        sh.t0 = l == W ? 0.0 : t0;
+        sh.l = lb;
+        sh.t0 = lb == W ? 0.0 : t0;
         prm[b].theta0 = sh.t0;
-        if (l == W) prm[b].theta1 = 0.0;
-        prm[b].bits = (uint32_t)l;
+        if (lb == W) prm[b].theta1 = 0.0;
+        prm[b].bits = (uint32_t)lb;
     }
     __syncthreads();
     const int l = sh.l;
@@ -229,8 +268,8 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
     const uint64_t cbytes = (uint64_t)(kChunk / 8) * l;
     const uint64_t t0b = double_to_bits(t0), t1b = double_to_bits(t1);
     uint32_t *buf = sh.buf;
-    // value(kk) gives the value at chunk position tid + kk*kThreads; kk is a
-    // compile-time index so that register-cached values stay in registers
+    // value(kk, i) gives the field of value i = chunk position tid + kk*kThreads;
+    // kk is a compile-time index so that register-cached values stay in registers
     auto pack_chunk = [&](uint32_t c, auto value) {
         const uint64_t g0 = c == 0 ? off : off + hdr + c * cbytes;
         const uint64_t g1 = c + 1 == nchunks ? end : off + hdr + (c + 1) * cbytes;
@@ -256,10 +295,7 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
                 const uint32_t j = tid + kk * kThreads;  // position inside the chunk
                 if (j < jn) {
                     const uint32_t i = j0 + j;
-                    const uint64_t yv = value(kk, i);
-                    uint64_t field, mg2;
-                    if (raw) field = yv;
-                    else field = residual_field<W>(yv, predict(t0, t1, i), l, &mg2);
+                    const uint64_t field = value(kk, i);
                     const uint64_t q = pbit + (uint64_t)j * l;
                     const uint32_t dw = (uint32_t)(q >> 5);
                     const int shb = (int)(q & 31);
@@ -288,20 +324,29 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
         __syncthreads();
     };
     static_assert(kVPT == 2 * (kChunk / kThreads), "two chunks of values are register-cached");
-    pack_chunk(0, [&](int kk, uint32_t) { return v[kk]; });
-    if (nchunks > 1) pack_chunk(1, [&](int kk, uint32_t) { return v[kChunk / kThreads + kk]; });
-    for (uint32_t c = 2; c < nchunks; ++c) pack_chunk(c, [&](int, uint32_t i) { return (uint64_t)y[i]; });
+    const uint64_t low = l > 0 ? (1ull << (l - 1)) - 1 : 0;
+    auto cached = [&](int k, uint32_t i) -> uint64_t {
+        if (raw) return (uint64_t)y[i];
+        return (v[k] & low) | ((uint64_t)((signs >> k) & 1) << (l - 1));
+    };
+    auto fresh = [&](uint32_t i) -> uint64_t {
+        if (raw) return (uint64_t)y[i];
+        uint64_t mg2;
+        return residual_field<W>((uint64_t)y[i], predict(t0, t1, i), l, &mg2);
+    };
+    pack_chunk(0, [&](int kk, uint32_t i) { return cached(kk, i); });
+    if (nchunks > 1) pack_chunk(1, [&](int kk, uint32_t i) { return cached(kChunk / kThreads + kk, i); });
+    for (uint32_t c = 2; c < nchunks; ++c) pack_chunk(c, [&](int, uint32_t i) { return fresh(i); });
 }
 
 // One workgroup per block, in ticket order (the look-back needs every lower
 // block to be owned by a running workgroup).
 template <class Rows>
 __global__ __launch_bounds__(kThreads) void k_encode(Rows rows, leco_fix_block_param *__restrict__ prm,
-                                                     uint64_t *__restrict__ offs, uint8_t *__restrict__ out,
-                                                     unsigned long long *__restrict__ ticket) {
+                                                     uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
     __shared__ EncodeShared sh;
     __shared__ uint64_t s_b;
-    if (threadIdx.x == 0) s_b = atomicAdd(ticket, 1ull);
+    if (threadIdx.x == 0) s_b = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
     __syncthreads();
     const uint64_t b = s_b;
     const RowRef r = rows.row(b);
@@ -311,16 +356,18 @@ __global__ __launch_bounds__(kThreads) void k_encode(Rows rows, leco_fix_block_p
         encode_block_body<uint32_t, 32>((const uint32_t *)r.base, r.m, prm, b, offs, out, sh);
 }
 
-// K3: payload offsets for the param table, offs[0] = 0 (it held the ticket).
+// K3: clears the status flags from offs[1..nb] (leaving the inclusive
+// prefixes, i.e. block ends), resets offs[0] (the ticket) and sets each
+// block's payload offset in the param table.
 __global__ __launch_bounds__(kThreads) void k_encode_fixup(leco_fix_block_param *__restrict__ prm,
                                                            uint64_t *__restrict__ offs, uint64_t nb,
                                                            const uint32_t *__restrict__ widths, uint32_t width) {
     const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (b >= nb) return;
     const uint32_t w = widths ? widths[b] : width;
-    const uint64_t excl = (load_status(&prm[b].payload_offset) & kStatusValue) -
-                          block_bytes(prm[b].bits, w, prm[b].length);
+    const uint64_t excl = b == 0 ? 0 : (offs[b] & kStatusValue);
     prm[b].payload_offset = excl + (prm[b].bits >= w ? 1 : 17);
+    offs[b + 1] &= kStatusValue;
     if (b == 0) offs[0] = 0;
 }
 
@@ -332,6 +379,29 @@ __global__ __launch_bounds__(kThreads) void k_encode_fixup(leco_fix_block_param 
 // to o[j].  A block with l == W-1 is decoded, not copied as raw (the
 // reference's :171-177 treats it as raw and returns wrong values).
 // ---------------------------------------------------------------------------
+// Values [ja, jb) of a staged chunk that starts at value j0: field at LDS
+// bit lbit + (j - j0) * lw, read as two aligned words and funnel-shifted
+// ((w1 << 1) << (63 - sh) is w1 << (64 - sh) and also right for sh = 0).
+template <typename T, bool RAW>
+__device__ __forceinline__ void decode_run(const uint64_t *__restrict__ buf, uint64_t lbit, int lw, int l, double t0,
+                                           double t1, uint32_t j0, uint32_t ja, uint32_t jb, T *__restrict__ o) {
+    const int tid = threadIdx.x;
+    const uint64_t mask = low_mask(lw);
+    auto one = [&](uint32_t j) {
+        const uint64_t q = lbit + (uint64_t)(j - j0) * lw;
+        const uint32_t w = (uint32_t)(q >> 6);
+        const int sh = (int)(q & 63);
+        const uint64_t field = ((buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh))) & mask;
+        o[j] = RAW ? (T)field : (T)apply_field(predict(t0, t1, j), field, l);
+    };
+    if (ja == j0 && jb == j0 + kChunk) {
+#pragma unroll
+        for (int kk = 0; kk < (int)(kChunk / kThreads); ++kk) one(j0 + tid + kk * kThreads);
+    } else {
+        for (uint32_t j = ja + tid; j < jb; j += kThreads) one(j);
+    }
+}
+
 template <typename T, int W>
 __device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st, uint64_t off, uint32_t m,
                                                   uint32_t jlo, uint32_t jhi, T *__restrict__ o,
@@ -370,14 +440,8 @@ __device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st
         const uint64_t lbit = 8ull * (g0 - a0);
         const uint32_t ja = jlo > j0 ? jlo : j0;
         const uint32_t jb = jhi < j0 + jn ? jhi : j0 + jn;
-        for (uint32_t j = ja + tid; j < jb; j += kThreads) {
-            const uint64_t q = lbit + (uint64_t)(j - j0) * lw;
-            const uint32_t w = (uint32_t)(q >> 6);
-            const int sh = (int)(q & 63);
-            const uint64_t vv = sh ? (buf[w] >> sh) | (buf[w + 1] << (64 - sh)) : buf[w];
-            const uint64_t field = vv & low_mask(lw);
-            o[j] = raw ? (T)field : (T)apply_field(predict(t0, t1, j), field, l);
-        }
+        if (raw) decode_run<T, true>(buf, lbit, lw, l, t0, t1, j0, ja, jb, o);
+        else decode_run<T, false>(buf, lbit, lw, l, t0, t1, j0, ja, jb, o);
         __syncthreads();
     }
 }
@@ -534,9 +598,9 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
     constexpr int W = 8 * sizeof(T);
     const uint64_t nb = (n + bs - 1) / bs;
     PlainRows rows{(const uint8_t *)in, n, bs, W, nb};
-    unsigned long long *ticket = (unsigned long long *)offs;  // offs[0] is the ticket until the fix-up
-    k_fit<PlainRows><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(64), 0, st>>>(rows, prm, ticket);
-    k_encode<PlainRows><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream, ticket);
+    PlainFit fit{(const uint8_t *)in, n, bs, W};
+    k_fit<PlainFit><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
+    k_encode<PlainRows><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     k_encode_fixup<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, nb,
                                                                                                 nullptr, W);
     return hipGetLastError();
@@ -568,15 +632,15 @@ hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint
     return hipGetLastError();
 }
 
-hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
-                               uint8_t *arena, uint64_t *offs, leco_fix_block_param *prm, uint32_t *widths,
-                               hipStream_t st) {
+hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, const uint64_t *fit_first,
+                               uint32_t ncols, uint64_t total, uint64_t fit_groups, uint8_t *arena, uint64_t *offs,
+                               leco_fix_block_param *prm, uint32_t *widths, hipStream_t st) {
     BatchRows rows{cols, first, ncols, total};
-    unsigned long long *ticket = (unsigned long long *)offs;
+    BatchFit fit{cols, first, fit_first, ncols};
     k_batch_widths<<<dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(cols, first, ncols,
                                                                                                     total, widths);
-    k_fit<BatchRows><<<dim3((unsigned)((total + kFitRows - 1) / kFitRows)), dim3(64), 0, st>>>(rows, prm, ticket);
-    k_encode<BatchRows><<<dim3((unsigned)total), dim3(kThreads), 0, st>>>(rows, prm, offs, arena, ticket);
+    k_fit<BatchFit><<<dim3((unsigned)fit_groups), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
+    k_encode<BatchRows><<<dim3((unsigned)total), dim3(kThreads), 0, st>>>(rows, prm, offs, arena);
     k_encode_fixup<<<dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, total,
                                                                                                     widths, 0);
     return hipGetLastError();
