@@ -91,6 +91,13 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-reps", type=int, default=3,
                     help="plain device copies of the decoded column (a known-size HBM reference kernel)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched many-column section")
+    ap.add_argument("--batched-cols", type=int, default=800)
+    ap.add_argument("--batched-max", type=float, default=1e8, help="largest column length (log-uniform from 1e3)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the request-server section")
+    ap.add_argument("--serve-nreq", type=int, default=60_000)
+    ap.add_argument("--no-churn", action="store_true", help="skip the insert/evict-under-traffic section")
+    ap.add_argument("--churn-cols", type=int, default=2000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -149,6 +156,25 @@ def main() -> int:
         ms_cp, _ = timed(torch, lambda: dst.copy_(out), args.copy_reps, 1)
         copy_gbs = 2 * raw_bytes / (ms_cp * 1e-3) / 1e9
         del dst
+
+    del lk, idx
+    torch.cuda.empty_cache()
+    from leco_fix import workloads
+
+    batched = serving = churn = None
+    if not args.no_batched:
+        cols = workloads.batched_columns(args.batched_cols, 1e3, args.batched_max)
+        batched = workloads.run_batched(cols, log=log)
+        del cols
+        torch.cuda.empty_cache()
+        log(f"batched: {batched}")
+    if not args.no_serving:
+        serving = workloads.run_serving(col, nreq=args.serve_nreq, log=log)
+        log(f"serving: {serving}")
+    if not args.no_churn:
+        churn = workloads.run_churn(ncols=args.churn_cols, log=log)
+        torch.cuda.empty_cache()
+        log(f"churn: {churn}")
 
     cpu = None
     if not args.no_cpu:
@@ -209,6 +235,9 @@ def main() -> int:
             "bytes_per_launch": comp + raw_bytes,
         },
         "cpu_baseline": cpu,
+        "batched": batched,
+        "serving": serving,
+        "churn": churn,
     }
     print(json.dumps(line), flush=True)
     return 0 if verified else 1

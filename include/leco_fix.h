@@ -161,6 +161,12 @@ int leco_fix_batch_decode(leco_fix_batch *batch, void *stream);
 /* Same as encode (op 0) / decode (op 1), replayed from a hipGraph captured on
  * first use. */
 int leco_fix_batch_graph_launch(leco_fix_batch *batch, int op, void *stream);
+/* The plain per-column path over the same columns, for comparison: one
+ * decode launch per column (leco_fix_decode_column_*), or one column encode
+ * per column (leco_fix_encode_column_*) into a scratch stream the batch
+ * allocates on first use; launched from a host loop, asynchronous on `stream`. */
+int leco_fix_batch_decode_per_column(leco_fix_batch *batch, void *stream);
+int leco_fix_batch_encode_per_column(leco_fix_batch *batch, void *stream);
 /* Changes where decode writes column c (the graph picks it up). */
 int leco_fix_batch_set_output(leco_fix_batch *batch, uint32_t c, void *d_out);
 /* Persistent decode grid, in workgroups (0 = occupancy-derived default). */

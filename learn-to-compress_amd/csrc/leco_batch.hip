@@ -32,6 +32,8 @@ struct leco_fix_batch {
     leco_fix_block_param *d_prm = nullptr;
     uint32_t *d_widths = nullptr;
     uint32_t decode_grid = 0;
+    uint8_t *d_scratch = nullptr;  // per-column encode: stream, offsets, params of the largest column
+    uint64_t scratch_cap = 0;
     hipStream_t cap_stream = nullptr;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
 };
@@ -51,6 +53,7 @@ void free_batch(leco_fix_batch *b) {
     (void)hipFree(b->d_offs);
     (void)hipFree(b->d_prm);
     (void)hipFree(b->d_widths);
+    (void)hipFree(b->d_scratch);
     delete b;
 }
 
@@ -173,6 +176,52 @@ int leco_fix_batch_graph_launch(leco_fix_batch *b, int op, void *stream) {
         }
     }
     return hip_status(hipGraphLaunch(b->graph[op], (hipStream_t)stream));
+}
+
+int leco_fix_batch_decode_per_column(leco_fix_batch *b, void *stream) {
+    if (!b) return LECO_FIX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    for (uint32_t c = 0; c < b->ncols; ++c) {
+        const BatchColumn &col = b->h_cols[c];
+        if (!col.out) return LECO_FIX_EINVAL;
+        const uint64_t *offs = b->d_offs + b->h_first[c];
+        hipError_t e = col.width == 64
+                           ? leco::launch_decode<uint64_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
+                                                           (uint64_t *)col.out, st)
+                           : leco::launch_decode<uint32_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
+                                                           (uint32_t *)col.out, st);
+        if (e != hipSuccess) return LECO_FIX_EHIP;
+    }
+    return LECO_FIX_OK;
+}
+
+int leco_fix_batch_encode_per_column(leco_fix_batch *b, void *stream) {
+    if (!b) return LECO_FIX_EINVAL;
+    uint64_t need = 0, nbmax = 0;
+    for (const BatchColumn &c : b->h_cols) {
+        if (!c.in) return LECO_FIX_EINVAL;
+        need = std::max(need, leco_fix_stream_bound(c.n, c.bs, c.width));
+        nbmax = std::max(nbmax, (c.n + c.bs - 1) / c.bs);
+    }
+    const uint64_t offs_at = (need + 255) / 256 * 256, prm_at = offs_at + ((nbmax + 1) * 8 + 255) / 256 * 256;
+    const uint64_t total = prm_at + nbmax * sizeof(leco_fix_block_param);
+    if (total > b->scratch_cap) {
+        (void)hipFree(b->d_scratch);
+        b->d_scratch = nullptr;
+        b->scratch_cap = 0;
+        if (hipMalloc(&b->d_scratch, total) != hipSuccess) return LECO_FIX_ENOMEM;
+        b->scratch_cap = total;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    uint64_t *offs = (uint64_t *)(b->d_scratch + offs_at);
+    leco_fix_block_param *prm = (leco_fix_block_param *)(b->d_scratch + prm_at);
+    for (const BatchColumn &c : b->h_cols) {
+        hipError_t e = c.width == 64
+                           ? leco::launch_encode<uint64_t>((const uint64_t *)c.in, c.n, c.bs, b->d_scratch, offs, prm, st)
+                           : leco::launch_encode<uint32_t>((const uint32_t *)c.in, c.n, c.bs, b->d_scratch, offs, prm, st);
+        if (e != hipSuccess) return LECO_FIX_EHIP;
+    }
+    return LECO_FIX_OK;
 }
 
 int leco_fix_batch_set_output(leco_fix_batch *b, uint32_t c, void *d_out) {

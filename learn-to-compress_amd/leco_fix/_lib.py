@@ -68,6 +68,8 @@ SIGNATURES.update({
     "leco_fix_batch_decode": (c_int, [c_ptr, c_ptr]),
     "leco_fix_batch_graph_launch": (c_int, [c_ptr, c_int, c_ptr]),
     "leco_fix_batch_set_output": (c_int, [c_ptr, c_u32, c_ptr]),
+    "leco_fix_batch_decode_per_column": (c_int, [c_ptr, c_ptr]),
+    "leco_fix_batch_encode_per_column": (c_int, [c_ptr, c_ptr]),
     "leco_fix_batch_set_decode_grid": (c_int, [c_ptr, c_u32]),
     "leco_fix_batch_info": (c_int, [c_ptr, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64), _pp, _pp, _pp]),
     "leco_fix_batch_column": (c_int, [c_ptr, c_u32, ctypes.POINTER(c_u64), ctypes.POINTER(c_u64)]),

@@ -111,6 +111,15 @@ class Batch:
         check(rc, "batch_decode")
         return self.outs
 
+    def decode_per_column(self, stream=None) -> list:
+        """The plain path: one decode launch per column, from a host loop."""
+        check(lib().leco_fix_batch_decode_per_column(self._h, _stream_handle(stream)), "batch_decode_per_column")
+        return self.outs
+
+    def encode_per_column(self, stream=None) -> None:
+        """The plain path: one column encode per column into a scratch stream."""
+        check(lib().leco_fix_batch_encode_per_column(self._h, _stream_handle(stream)), "batch_encode_per_column")
+
     def set_decode_grid(self, workgroups: int) -> None:
         check(lib().leco_fix_batch_set_decode_grid(self._h, workgroups), "batch_set_decode_grid")
 
