@@ -133,6 +133,10 @@ def main() -> int:
     log(f"decode: warmup {args.warmup}, steps {args.steps}")
     ms_dec, wall_dec = timed(torch, lambda: col.decode(out=out), args.steps, args.warmup)
 
+    from leco_fix.column import DECODE_VALUES
+    ms_dec_values, _ = timed(torch, lambda: col.decode(out=out, shape=DECODE_VALUES), args.steps, 1)
+    verified = verified and bool(torch.equal(out, vals))
+
     log("encode timing")
     st_buf, off_buf, prm_buf = col.stream, col.offsets, col.params
 
@@ -221,6 +225,7 @@ def main() -> int:
             "lookups_per_s": round(args.lookups / (ms_lk * 1e-3), 0),
             "lookup_batch_ms": round(ms_lk, 3),
             "decode_wall_ms_per_step": round(wall_dec * 1e3 / args.steps, 4),
+            "decode_value_parallel_ms": round(ms_dec_values, 4),
             "device_copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
         },
         "roofline": {

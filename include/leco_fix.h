@@ -95,6 +95,20 @@ int leco_fix_decode_range_u64(const uint8_t *d_stream, const uint64_t *d_offsets
                               uint32_t block_size, uint64_t begin, uint64_t end,
                               uint64_t *d_out, void *stream);
 
+/* Range decode with the decode shape chosen explicitly: LECO_FIX_DECODE_AUTO
+ * (staged for block sizes >= 2048, value-parallel below), _STAGED (one
+ * workgroup per block through LDS; d_params may be NULL) or _VALUES (one lane
+ * per position from the param table; needs d_params). */
+#define LECO_FIX_DECODE_AUTO 0
+#define LECO_FIX_DECODE_STAGED 1
+#define LECO_FIX_DECODE_VALUES 2
+int leco_fix_decode_range_ex_u32(const uint8_t *d_stream, const uint64_t *d_offsets,
+                                 const leco_fix_block_param *d_params, uint64_t n, uint32_t block_size,
+                                 uint64_t begin, uint64_t end, int shape, uint32_t *d_out, void *stream);
+int leco_fix_decode_range_ex_u64(const uint8_t *d_stream, const uint64_t *d_offsets,
+                                 const leco_fix_block_param *d_params, uint64_t n, uint32_t block_size,
+                                 uint64_t begin, uint64_t end, int shape, uint64_t *d_out, void *stream);
+
 /* Point lookups: d_out[q] = value d_idx[q] of the column; per query
  * Leco_int<T>::randomdecodeArray8(block idx/bs, idx%bs)
  * (piecewise_fix_integer_template.h:305-337, leco_int_fix_template.cpp:189-193).
@@ -156,7 +170,8 @@ int leco_fix_batch_destroy(leco_fix_batch *batch);
 /* Encodes every column: fit, then one workgroup per block in ticket order
  * (arena offsets by decoupled look-back), then the param fix-up. */
 int leco_fix_batch_encode(leco_fix_batch *batch, void *stream);
-/* Decodes every column with one persistent launch over the block table. */
+/* Decodes every column with one launch over the batch's tile table (a tile
+ * is one large block, or a run of small blocks staged together). */
 int leco_fix_batch_decode(leco_fix_batch *batch, void *stream);
 /* Same as encode (op 0) / decode (op 1), replayed from a hipGraph captured on
  * first use. */
@@ -169,7 +184,8 @@ int leco_fix_batch_decode_per_column(leco_fix_batch *batch, void *stream);
 int leco_fix_batch_encode_per_column(leco_fix_batch *batch, void *stream);
 /* Changes where decode writes column c (the graph picks it up). */
 int leco_fix_batch_set_output(leco_fix_batch *batch, uint32_t c, void *d_out);
-/* Persistent decode grid, in workgroups (0 = occupancy-derived default). */
+/* Decode grid, in workgroups: 0 = one per tile (the default); fewer makes the
+ * launch persistent, each workgroup striding over the tiles. */
 int leco_fix_batch_set_decode_grid(leco_fix_batch *batch, uint32_t workgroups);
 int leco_fix_batch_info(const leco_fix_batch *batch, uint64_t *total_blocks, uint64_t *arena_capacity,
                         uint8_t **d_arena, uint64_t **d_offsets, leco_fix_block_param **d_params);

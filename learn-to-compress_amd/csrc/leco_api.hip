@@ -34,6 +34,19 @@ int decode_range(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32
 }
 
 template <typename T>
+int decode_range_ex(const uint8_t *stream, const uint64_t *offs, const leco_fix_block_param *prm, uint64_t n,
+                    uint32_t bs, uint64_t lo, uint64_t hi, int shape, T *out, void *st) {
+    if (!valid_bs(bs) || !stream || !out || lo >= hi || hi > n || shape < 0 || shape > 2) return LECO_FIX_EINVAL;
+    if (shape == LECO_FIX_DECODE_AUTO) shape = leco::staged_decode(bs) || !prm ? 1 : 2;
+    if (shape == LECO_FIX_DECODE_STAGED) {
+        if (!offs) return LECO_FIX_EINVAL;
+        return hip_status(leco::launch_decode<T>(stream, offs, n, bs, lo, hi, out, (hipStream_t)st));
+    }
+    if (!prm) return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_decode_values<T>(stream, prm, bs, lo, hi, out, (hipStream_t)st));
+}
+
+template <typename T>
 int lookup(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, uint32_t bs, const uint64_t *idx,
            uint64_t nq, T *out, void *st) {
     if (!valid_bs(bs) || n == 0 || !stream || !prm) return LECO_FIX_EINVAL;
@@ -225,6 +238,17 @@ int leco_fix_decode_range_u32(const uint8_t *d_stream, const uint64_t *d_offsets
 int leco_fix_decode_range_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
                               uint64_t begin, uint64_t end, uint64_t *d_out, void *stream) {
     return decode_range<uint64_t>(d_stream, d_offsets, n, bs, begin, end, d_out, stream);
+}
+
+int leco_fix_decode_range_ex_u32(const uint8_t *d_stream, const uint64_t *d_offsets,
+                                 const leco_fix_block_param *d_params, uint64_t n, uint32_t bs, uint64_t begin,
+                                 uint64_t end, int shape, uint32_t *d_out, void *stream) {
+    return decode_range_ex<uint32_t>(d_stream, d_offsets, d_params, n, bs, begin, end, shape, d_out, stream);
+}
+int leco_fix_decode_range_ex_u64(const uint8_t *d_stream, const uint64_t *d_offsets,
+                                 const leco_fix_block_param *d_params, uint64_t n, uint32_t bs, uint64_t begin,
+                                 uint64_t end, int shape, uint64_t *d_out, void *stream) {
+    return decode_range_ex<uint64_t>(d_stream, d_offsets, d_params, n, bs, begin, end, shape, d_out, stream);
 }
 
 int leco_fix_lookup_u32(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n, uint32_t bs,

@@ -26,11 +26,15 @@ struct leco_fix_batch {
     BatchColumn *d_cols = nullptr;
     uint64_t *d_first = nullptr;
     uint64_t *d_fit_first = nullptr;
+    uint64_t tiles = 0;            // decode tiles: one per block (staged) or per kValueTile positions
+    uint64_t *d_tile_first = nullptr;
+    uint32_t *d_tile_col = nullptr;
     uint8_t *d_arena = nullptr;
     uint64_t arena_cap = 0;
     uint64_t *d_offs = nullptr;
     leco_fix_block_param *d_prm = nullptr;
     uint32_t *d_widths = nullptr;
+    uint32_t *d_bcol = nullptr;
     uint32_t decode_grid = 0;
     uint8_t *d_scratch = nullptr;  // per-column encode: stream, offsets, params of the largest column
     uint64_t scratch_cap = 0;
@@ -49,10 +53,13 @@ void free_batch(leco_fix_batch *b) {
     (void)hipFree(b->d_cols);
     (void)hipFree(b->d_first);
     (void)hipFree(b->d_fit_first);
+    (void)hipFree(b->d_tile_first);
+    (void)hipFree(b->d_tile_col);
     (void)hipFree(b->d_arena);
     (void)hipFree(b->d_offs);
     (void)hipFree(b->d_prm);
     (void)hipFree(b->d_widths);
+    (void)hipFree(b->d_bcol);
     (void)hipFree(b->d_scratch);
     delete b;
 }
@@ -64,19 +71,16 @@ bool valid_desc(const leco_fix_column_desc &c) {
 
 int enqueue(leco_fix_batch *b, int op, hipStream_t st) {
     if (op == 0)
-        return hip_status(leco::launch_batch_encode(b->d_cols, b->d_first, b->d_fit_first, b->ncols, b->total,
-                                                    b->fit_groups, b->d_arena, b->d_offs, b->d_prm, b->d_widths, st));
-    return hip_status(leco::launch_batch_decode(b->d_cols, b->d_first, b->ncols, b->total, b->d_arena, b->d_offs,
-                                                b->decode_grid, st));
+        return hip_status(leco::launch_batch_encode(b->d_cols, b->d_first, b->d_fit_first, b->d_bcol, b->d_widths,
+                                                    b->ncols, b->total, b->fit_groups, b->d_arena, b->d_offs,
+                                                    b->d_prm, st));
+    return hip_status(leco::launch_batch_decode(b->d_cols, b->d_first, b->d_tile_first, b->d_tile_col, b->tiles,
+                                                b->d_arena, b->d_offs, b->d_prm, b->decode_grid, st));
 }
 
-uint32_t default_decode_grid() {
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 1024;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-        return 1024;
-    return (uint32_t)cus * 8;  // 8 workgroups (32 waves) per CU
-}
+// One workgroup per tile (capped at the launch limit; the kernel strides
+// over any tiles beyond the grid).
+uint32_t default_decode_grid(uint64_t tiles) { return (uint32_t)std::min<uint64_t>(tiles, 0x7FFFFFFFull); }
 
 }  // namespace
 
@@ -92,6 +96,7 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
     b->h_cols.resize(ncols);
     b->h_first.resize(ncols + 1);
     b->h_fit_first.resize(ncols + 1);
+    std::vector<uint64_t> tile_first(ncols + 1);
     uint64_t arena = 0;
     for (uint32_t c = 0; c < ncols; ++c) {
         const leco_fix_column_desc &d = cols[c];
@@ -100,6 +105,9 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
         b->h_fit_first[c] = b->fit_groups;
         const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
         b->fit_groups += (nb + 63) / 64;
+        tile_first[c] = b->tiles;
+        const uint32_t kb = leco::multi_blocks(d.block_size, d.width_bits);
+        b->tiles += leco::staged_decode(d.block_size) ? nb : (nb + kb - 1) / kb;
         const uint64_t last = d.n - (nb - 1) * d.block_size;
         arena += (nb - 1) * leco::block_bound(d.width_bits, d.block_size) +
                  leco::block_bound(d.width_bits, (uint32_t)last);
@@ -107,15 +115,19 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
     }
     b->h_first[ncols] = b->total;
     b->h_fit_first[ncols] = b->fit_groups;
+    tile_first[ncols] = b->tiles;
     b->arena_cap = arena + LECO_FIX_TAIL_PAD;
-    b->decode_grid = default_decode_grid();
+    b->decode_grid = default_decode_grid(b->tiles);
     bool ok = hipMalloc(&b->d_cols, ncols * sizeof(BatchColumn)) == hipSuccess &&
               hipMalloc(&b->d_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
               hipMalloc(&b->d_fit_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
+              hipMalloc(&b->d_tile_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
+              hipMalloc(&b->d_tile_col, b->tiles * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&b->d_arena, b->arena_cap) == hipSuccess &&
               hipMalloc(&b->d_offs, (b->total + 1) * sizeof(uint64_t)) == hipSuccess &&
               hipMalloc(&b->d_prm, b->total * sizeof(leco_fix_block_param)) == hipSuccess &&
-              hipMalloc(&b->d_widths, b->total * sizeof(uint32_t)) == hipSuccess;
+              hipMalloc(&b->d_widths, b->total * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&b->d_bcol, b->total * sizeof(uint32_t)) == hipSuccess;
     if (!ok) {
         free_batch(b);
         return LECO_FIX_ENOMEM;
@@ -125,7 +137,13 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
             hipSuccess ||
         hipMemcpy(b->d_fit_first, b->h_fit_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
             hipSuccess ||
-        hipMemset(b->d_arena + arena, 0, LECO_FIX_TAIL_PAD) != hipSuccess) {
+        hipMemcpy(b->d_tile_first, tile_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        hipMemset(b->d_arena + arena, 0, LECO_FIX_TAIL_PAD) != hipSuccess ||
+        leco::launch_tile_columns(b->d_tile_first, ncols, b->tiles, b->d_tile_col, nullptr) != hipSuccess ||
+        leco::launch_batch_blocks(b->d_cols, b->d_first, ncols, b->total, b->d_bcol, b->d_widths, nullptr) !=
+            hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
         free_batch(b);
         return LECO_FIX_EHIP;
     }
@@ -185,11 +203,13 @@ int leco_fix_batch_decode_per_column(leco_fix_batch *b, void *stream) {
         const BatchColumn &col = b->h_cols[c];
         if (!col.out) return LECO_FIX_EINVAL;
         const uint64_t *offs = b->d_offs + b->h_first[c];
-        hipError_t e = col.width == 64
-                           ? leco::launch_decode<uint64_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
-                                                           (uint64_t *)col.out, st)
-                           : leco::launch_decode<uint32_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
-                                                           (uint32_t *)col.out, st);
+        const leco_fix_block_param *prm = b->d_prm + b->h_first[c];
+        hipError_t e;
+        (void)prm;
+        e = col.width == 64 ? leco::launch_decode<uint64_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
+                                                             (uint64_t *)col.out, st)
+                            : leco::launch_decode<uint32_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
+                                                             (uint32_t *)col.out, st);
         if (e != hipSuccess) return LECO_FIX_EHIP;
     }
     return LECO_FIX_OK;
@@ -232,7 +252,7 @@ int leco_fix_batch_set_output(leco_fix_batch *b, uint32_t c, void *d_out) {
 
 int leco_fix_batch_set_decode_grid(leco_fix_batch *b, uint32_t wgs) {
     if (!b) return LECO_FIX_EINVAL;
-    b->decode_grid = wgs ? wgs : default_decode_grid();
+    b->decode_grid = wgs ? wgs : default_decode_grid(b->tiles);
     if (b->graph[1]) {  // the captured launch has the old grid baked in
         (void)hipGraphExecDestroy(b->graph[1]);
         b->graph[1] = nullptr;

@@ -46,10 +46,12 @@ struct ServeColumn {
     uint32_t width;
 };
 
-enum : uint32_t { kServeLookup = 0, kServeRange = 1 };
+enum : uint32_t { kServeLookup = 0, kServeRange = 1, kServeRangeValues = 2 };
 
 // One workgroup's share of a served batch: a chunk of a point-lookup request,
-// or one block of a range-decode request.
+// one block of a range-decode request (staged columns), or kValueTile
+// positions of a range-decode request (value-parallel columns: positions
+// [lo, lo + count) go to out[0..count)).
 struct ServeItem {
     uint32_t kind;
     uint32_t column;
@@ -61,6 +63,26 @@ struct ServeItem {
     uint64_t lo, hi;       // range: [lo, hi) in column positions
 };
 
+// Decode shape for a block size: blocks of at least this many values are
+// decoded staged (one workgroup per block through LDS), smaller ones
+// value-parallel (one lane per position, no per-block workgroup).
+constexpr uint32_t kStagedMinBlock = 2048;
+__host__ __device__ inline bool staged_decode(uint32_t bs) { return bs >= kStagedMinBlock; }
+// Positions per value-parallel tile (one workgroup).
+constexpr uint32_t kValueTile = 4096;
+// Small blocks are decoded multi_blocks() at a time by one workgroup: as many
+// consecutive blocks as keep the tile within 2048 positions, the staged bytes
+// within the LDS chunk buffer (largest block of each) and at most kMultiMax.
+constexpr uint32_t kMultiMax = 256;
+constexpr uint32_t kMultiLds = 2048 * 8 + 32;
+__host__ __device__ inline uint32_t multi_blocks(uint32_t bs, uint32_t w) {
+    uint32_t k = 2048 / bs;
+    const uint32_t by_lds = (uint32_t)(kMultiLds / block_bound(w, bs));
+    if (by_lds < k) k = by_lds;
+    if (k > kMultiMax) k = kMultiMax;
+    return k ? k : 1;
+}
+
 // Each launcher enqueues on `st` and returns the hipGetLastError() of the launch.
 template <typename T>
 hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t *offs,
@@ -68,6 +90,9 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
 template <typename T>
 hipError_t launch_decode(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
                          uint64_t lo, uint64_t hi, T *out, hipStream_t st);
+template <typename T>
+hipError_t launch_decode_values(const uint8_t *stream, const leco_fix_block_param *prm, uint32_t bs, uint64_t lo,
+                                uint64_t hi, T *out, hipStream_t st);
 template <typename T>
 hipError_t launch_lookup(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, uint32_t bs,
                          const uint64_t *idx, uint64_t nq, T *out, hipStream_t st);
@@ -80,12 +105,21 @@ hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint
 // global index of column c's block 0; first[ncols] = total.
 // fit_first[c] = index of column c's first 64-block fit group (fit_first[ncols]
 // = fit_groups).
+// bcol[b] / widths[b]: column index and width of global block b (see launch_batch_blocks).
 hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, const uint64_t *fit_first,
-                               uint32_t ncols, uint64_t total, uint64_t fit_groups, uint8_t *arena, uint64_t *offs,
-                               leco_fix_block_param *prm, uint32_t *widths, hipStream_t st);
-// Batched decode: a persistent grid of `grid` workgroups strides over the blocks.
-hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
-                               const uint8_t *arena, const uint64_t *offs, uint32_t grid, hipStream_t st);
+                               const uint32_t *bcol, const uint32_t *widths, uint32_t ncols, uint64_t total,
+                               uint64_t fit_groups, uint8_t *arena, uint64_t *offs, leco_fix_block_param *prm,
+                               hipStream_t st);
+// Fills bcol and widths for every block of a batch (once, at creation).
+hipError_t launch_batch_blocks(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
+                               uint32_t *bcol, uint32_t *widths, hipStream_t st);
+// Batched decode: `grid` workgroups (one per tile, or fewer striding over them) cover the tile
+// table (tile_first[c]: column c's first tile; tile_col[t]: column of tile t).
+hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, const uint64_t *tile_first,
+                               const uint32_t *tile_col, uint64_t tiles, const uint8_t *arena, const uint64_t *offs,
+                               const leco_fix_block_param *prm, uint32_t grid, hipStream_t st);
+hipError_t launch_tile_columns(const uint64_t *tile_first, uint32_t ncols, uint64_t tiles, uint32_t *tile_col,
+                               hipStream_t st);
 // Served batch: `grid` workgroups stride over nitems items.
 hipError_t launch_serve(const ServeColumn *cols, const ServeItem *items, uint32_t nitems, uint32_t grid,
                         hipStream_t st);

@@ -11,6 +11,7 @@ constexpr int kThreads = 256;                       // workgroup of the per-bloc
 constexpr uint32_t kChunk = 2048;                   // values staged through LDS at a time
 constexpr int kLdsBytes = kChunk * 8 + 64;          // one chunk of raw uint64 + alignment slack
 constexpr int kVPT = 16;                            // values per lane kept in registers (4096 per block)
+constexpr uint32_t kValTile = kValueTile;           // positions per workgroup in value-parallel decode
 constexpr int kFitRows = 64;                        // blocks per fit workgroup (one lane of wave 0 each)
 constexpr int kFitTK = 128;                         // values per block row in one fit tile
 constexpr int kFitThreads = 256;                    // fit workgroup: four waves load, wave 0 also fits
@@ -110,15 +111,15 @@ struct BatchFit {
     }
 };
 
-// Blocks of many columns, in column order.
+// Blocks of many columns, in column order; bcol[it] is block it's column.
 struct BatchRows {
     const BatchColumn *cols;
     const uint64_t *first;
-    uint32_t ncols;
+    const uint32_t *bcol;
     uint64_t total;
     __device__ __forceinline__ RowRef row(uint64_t it) const {
         if (it >= total) return {nullptr, 0, 64};
-        const uint32_t c = find_column(first, ncols, it);
+        const uint32_t c = bcol[it];
         const BatchColumn col = cols[c];
         const uint64_t s = (it - first[c]) * col.bs;
         return {(const uint8_t *)col.in + s * (col.width / 8), (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs),

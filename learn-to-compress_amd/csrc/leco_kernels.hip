@@ -446,6 +446,75 @@ __device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st
     }
 }
 
+// Little-endian 8 bytes at LDS byte offset `byte` (two aligned words).
+__device__ __forceinline__ uint64_t lds_u64_at(const uint64_t *__restrict__ buf, uint32_t byte) {
+    const uint32_t w = byte >> 3;
+    const int sh = (int)(byte & 7) * 8;
+    return (buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh));
+}
+
+// Multi-block staged decode for small blocks: blocks [b0, b0 + kb) of one
+// column (kb * bs <= kChunk values) are staged into LDS in one sweep of
+// aligned 16-byte loads, with their offsets; each lane then decodes positions
+// from its block's header and field, both read from LDS.  Position p of the
+// tile goes to o[p].
+template <typename T, int W>
+__device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st, const uint64_t *__restrict__ offs,
+                                                  uint64_t n, uint32_t bs, uint64_t b0, uint32_t kb,
+                                                  T *__restrict__ o, uint64_t *__restrict__ buf,
+                                                  uint64_t *__restrict__ soffs) {
+    const int tid = threadIdx.x;
+    const uint64_t s0 = b0 * bs;
+    const uint64_t nb = (n + bs - 1) / bs;
+    if (b0 + kb > nb) kb = (uint32_t)(nb - b0);
+    const uint32_t cnt = (uint32_t)(n - s0 < (uint64_t)kb * bs ? n - s0 : (uint64_t)kb * bs);
+    for (uint32_t k = tid; k <= kb; k += kThreads) soffs[k] = offs[b0 + k];
+    __syncthreads();
+    const uint64_t g0 = soffs[0], g1 = soffs[kb];
+    const uint64_t a0 = g0 & ~15ull;
+    const uint32_t n16 = (uint32_t)((g1 - a0 + 15) / 16);
+    for (uint32_t u = tid; u < n16; u += kThreads)
+        *(uint4 *)(buf + 2 * u) = *(const uint4 *)(st + a0 + 16ull * u);
+    __syncthreads();
+    const bool pow2 = (bs & (bs - 1)) == 0;
+    const int lsh = __ffs((int)bs) - 1;
+    for (uint32_t p = tid; p < cnt; p += kThreads) {
+        const uint32_t k = pow2 ? p >> lsh : p / bs;
+        const uint32_t j = p - k * bs;
+        const uint32_t hb = (uint32_t)(soffs[k] - a0);
+        const int l = (int)(lds_u64_at(buf, hb) & 0xFF);
+        const bool raw = l >= W;
+        const int lw = raw ? W : l;
+        const uint64_t q = 8ull * (hb + (raw ? 1 : 17)) + (uint64_t)j * lw;
+        const uint32_t w = (uint32_t)(q >> 6);
+        const int sh = (int)(q & 63);
+        const uint64_t field = ((buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh))) & low_mask(lw);
+        if (raw) {
+            o[p] = (T)field;
+        } else {
+            const double t0 = bits_to_double(lds_u64_at(buf, hb + 1));
+            const double t1 = bits_to_double(lds_u64_at(buf, hb + 9));
+            const int64_t pr = predict(t0, t1, j);
+            const int lc = l > 0 ? l : 1;
+            const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
+            const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
+            o[p] = (T)(l ? v : (uint64_t)pr);
+        }
+    }
+    __syncthreads();
+}
+
+// One workgroup per tile of kb blocks of one column (full decode).
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_decode_multi(const uint8_t *__restrict__ st,
+                                                           const uint64_t *__restrict__ offs, uint64_t n,
+                                                           uint32_t bs, uint32_t kb, T *__restrict__ out) {
+    __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
+    __shared__ uint64_t soffs[kMultiMax + 1];
+    const uint64_t b0 = (uint64_t)blockIdx.x * kb;
+    decode_multi_body<T, W>(st, offs, n, bs, b0, kb, out + b0 * bs, buf, soffs);
+}
+
 // One workgroup per block of one column; values [lo, hi) go to out[global - lo].
 template <typename T, int W>
 __global__ __launch_bounds__(kThreads) void k_decode(const uint8_t *__restrict__ st,
@@ -467,6 +536,23 @@ __global__ __launch_bounds__(kThreads) void k_decode(const uint8_t *__restrict__
 // block's parameters come from the param table, the field from an unaligned
 // bit window read as two aligned 8-byte words.
 // ---------------------------------------------------------------------------
+// Value at position j of the block described by p, branch-free over the
+// block kinds (lanes of one wave may sit in blocks of different widths):
+// raw (l >= W), constant (l == 0) and packed.
+template <int W>
+__device__ __forceinline__ uint64_t decode_at(const uint8_t *__restrict__ st, const leco_fix_block_param &p, uint32_t j) {
+    const int l = (int)p.bits;
+    const bool raw = l >= W;
+    const int lw = raw ? W : l;
+    // an l == 0 block has no payload: its window lies in the next block or the tail pad, and is masked away
+    const uint64_t field = load_bits(st + p.payload_offset, (uint64_t)j * lw, lw);
+    const int64_t pr = predict(p.theta0, p.theta1, j);
+    const int lc = l > 0 ? l : 1;
+    const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
+    const uint64_t packed = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
+    return raw ? field : (l ? packed : (uint64_t)pr);
+}
+
 template <int W>
 __device__ __forceinline__ uint64_t lookup_one(const uint8_t *__restrict__ st,
                                                const leco_fix_block_param *__restrict__ prm, uint32_t bs,
@@ -481,11 +567,39 @@ __device__ __forceinline__ uint64_t lookup_one(const uint8_t *__restrict__ st,
         b = i / bs;
         j = (uint32_t)(i - b * bs);
     }
-    const leco_fix_block_param p = prm[b];
-    const int l = (int)p.bits;
-    if (l >= W) return load_bits(st + p.payload_offset, (uint64_t)j * W, W);
-    const int64_t pr = predict(p.theta0, p.theta1, j);
-    return l ? apply_field(pr, load_bits(st + p.payload_offset, (uint64_t)j * l, l), l) : (uint64_t)pr;
+    return decode_at<W>(st, prm[b], j);
+}
+
+// Value-parallel decode of column positions [v0, v0 + cnt) into o[0..cnt):
+// lane k decodes position v0 + k from its block's param entry (shared by the
+// lanes of that block) and a bit window read as two aligned words.  Unlike
+// the staged block decode its cost does not depend on the block size, which
+// makes it the better shape for small blocks.
+template <typename T, int W>
+__device__ __forceinline__ void decode_values(const uint8_t *__restrict__ st,
+                                              const leco_fix_block_param *__restrict__ prm, uint32_t bs,
+                                              uint64_t v0, uint32_t cnt, T *__restrict__ o) {
+    const uint64_t b0 = v0 / bs;
+    const uint32_t j0 = (uint32_t)(v0 - b0 * bs);
+    const bool pow2 = (bs & (bs - 1)) == 0;
+    const int sh = __ffs((int)bs) - 1;
+#pragma unroll 4
+    for (uint32_t k = threadIdx.x; k < cnt; k += kThreads) {
+        const uint32_t jj = j0 + k;
+        const uint32_t db = pow2 ? jj >> sh : jj / bs;
+        o[k] = (T)decode_at<W>(st, prm[b0 + db], jj - db * bs);
+    }
+}
+
+// One workgroup per kValTile positions of one column; positions [lo, hi).
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_decode_values(const uint8_t *__restrict__ st,
+                                                            const leco_fix_block_param *__restrict__ prm,
+                                                            uint32_t bs, uint64_t lo, uint64_t hi,
+                                                            T *__restrict__ out) {
+    const uint64_t v0 = lo + (uint64_t)blockIdx.x * kValTile;
+    const uint32_t cnt = (uint32_t)(hi - v0 < kValTile ? hi - v0 : kValTile);
+    decode_values<T, W>(st, prm, bs, v0, cnt, out + (v0 - lo));
 }
 
 template <typename T, int W>
@@ -527,30 +641,60 @@ __global__ __launch_bounds__(kThreads) void k_build_params(const uint8_t *__rest
 // one global list, items in column order.  Encode writes one arena; decode is
 // a persistent kernel striding over the items.
 // ---------------------------------------------------------------------------
+// Persistent batched decode over the tile table: a tile is one block of a
+// column decoded staged (large blocks) or multi_blocks() consecutive small
+// blocks decoded staged together; tile_col[t] names its column, so no search.
 __global__ __launch_bounds__(kThreads) void k_batch_decode(const BatchColumn *__restrict__ cols,
-                                                           const uint64_t *__restrict__ first, uint32_t ncols,
-                                                           uint64_t total, const uint8_t *__restrict__ arena,
-                                                           const uint64_t *__restrict__ offs) {
+                                                           const uint64_t *__restrict__ first,
+                                                           const uint64_t *__restrict__ tile_first,
+                                                           const uint32_t *__restrict__ tile_col, uint64_t tiles,
+                                                           const uint8_t *__restrict__ arena,
+                                                           const uint64_t *__restrict__ offs,
+                                                           const leco_fix_block_param *__restrict__ prm) {
     __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
-    for (uint64_t it = blockIdx.x; it < total; it += gridDim.x) {
-        const uint32_t c = find_column(first, ncols, it);
+    __shared__ uint64_t soffs[kMultiMax + 1];
+    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        const uint32_t c = tile_col[t];
         const BatchColumn col = cols[c];
-        const uint64_t b = it - first[c];
-        const uint64_t s = b * col.bs;
-        const uint32_t m = (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs);
-        if (col.width == 64)
-            decode_block_body<uint64_t, 64>(arena, offs[it], m, 0, m, (uint64_t *)col.out + s, buf);
-        else
-            decode_block_body<uint32_t, 32>(arena, offs[it], m, 0, m, (uint32_t *)col.out + s, buf);
+        const uint64_t local = t - tile_first[c];
+        if (staged_decode(col.bs)) {
+            const uint64_t s = local * col.bs;
+            const uint32_t m = (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs);
+            const uint64_t off = offs[first[c] + local];
+            if (col.width == 64)
+                decode_block_body<uint64_t, 64>(arena, off, m, 0, m, (uint64_t *)col.out + s, buf);
+            else
+                decode_block_body<uint32_t, 32>(arena, off, m, 0, m, (uint32_t *)col.out + s, buf);
+        } else {
+            const uint32_t kb = multi_blocks(col.bs, col.width);
+            const uint64_t b0 = local * kb;
+            if (col.width == 64)
+                decode_multi_body<uint64_t, 64>(arena, offs + first[c], col.n, col.bs, b0, kb,
+                                                (uint64_t *)col.out + b0 * col.bs, buf, soffs);
+            else
+                decode_multi_body<uint32_t, 32>(arena, offs + first[c], col.n, col.bs, b0, kb,
+                                                (uint32_t *)col.out + b0 * col.bs, buf, soffs);
+        }
     }
 }
 
-__global__ __launch_bounds__(kThreads) void k_batch_widths(const BatchColumn *__restrict__ cols,
+// tile_col[t] = column of tile t (tile_first is the per-column prefix).
+__global__ __launch_bounds__(kThreads) void k_tile_columns(const uint64_t *__restrict__ tile_first, uint32_t ncols,
+                                                           uint64_t tiles, uint32_t *__restrict__ tile_col) {
+    const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (t < tiles) tile_col[t] = find_column(tile_first, ncols, t);
+}
+
+// Per-block column index and width of a batch (one binary search per block, once).
+__global__ __launch_bounds__(kThreads) void k_batch_blocks(const BatchColumn *__restrict__ cols,
                                                            const uint64_t *__restrict__ first, uint32_t ncols,
-                                                           uint64_t total, uint32_t *__restrict__ widths) {
+                                                           uint64_t total, uint32_t *__restrict__ bcol,
+                                                           uint32_t *__restrict__ widths) {
     const uint64_t it = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (it >= total) return;
-    widths[it] = cols[find_column(first, ncols, it)].width;
+    const uint32_t c = find_column(first, ncols, it);
+    bcol[it] = c;
+    widths[it] = cols[c].width;
 }
 
 // ---------------------------------------------------------------------------
@@ -572,6 +716,11 @@ __global__ __launch_bounds__(kThreads) void k_serve(const ServeColumn *__restric
                     ((uint32_t *)w.out)[q] =
                         i < col.n ? (uint32_t)lookup_one<32>(col.stream, col.params, col.bs, i) : 0u;
             }
+        } else if (w.kind == kServeRangeValues) {
+            if (col.width == 64)
+                decode_values<uint64_t, 64>(col.stream, col.params, col.bs, w.lo, w.count, (uint64_t *)w.out);
+            else
+                decode_values<uint32_t, 32>(col.stream, col.params, col.bs, w.lo, w.count, (uint32_t *)w.out);
         } else {
             // one block of a range [lo, hi): w.block, output base w.out = value lo
             const uint64_t s = w.block * col.bs;
@@ -610,8 +759,24 @@ template <typename T>
 hipError_t launch_decode(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, uint64_t lo,
                          uint64_t hi, T *out, hipStream_t st) {
     constexpr int W = 8 * sizeof(T);
+    if (!staged_decode(bs) && lo == 0 && hi == n) {
+        const uint32_t kb = multi_blocks(bs, W);
+        const uint64_t nb = (n + bs - 1) / bs;
+        k_decode_multi<T, W><<<dim3((unsigned)((nb + kb - 1) / kb)), dim3(kThreads), 0, st>>>(stream, offs, n, bs, kb,
+                                                                                                out);
+        return hipGetLastError();
+    }
     const uint64_t bf = lo / bs, bl = (hi - 1) / bs;
     k_decode<T, W><<<dim3((unsigned)(bl - bf + 1)), dim3(kThreads), 0, st>>>(stream, offs, n, bs, bf, lo, hi, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_decode_values(const uint8_t *stream, const leco_fix_block_param *prm, uint32_t bs, uint64_t lo,
+                                uint64_t hi, T *out, hipStream_t st) {
+    constexpr int W = 8 * sizeof(T);
+    k_decode_values<T, W><<<dim3((unsigned)((hi - lo + kValTile - 1) / kValTile)), dim3(kThreads), 0, st>>>(
+        stream, prm, bs, lo, hi, out);
     return hipGetLastError();
 }
 
@@ -632,13 +797,19 @@ hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint
     return hipGetLastError();
 }
 
+hipError_t launch_batch_blocks(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
+                               uint32_t *bcol, uint32_t *widths, hipStream_t st) {
+    k_batch_blocks<<<dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(cols, first, ncols,
+                                                                                                    total, bcol, widths);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, const uint64_t *fit_first,
-                               uint32_t ncols, uint64_t total, uint64_t fit_groups, uint8_t *arena, uint64_t *offs,
-                               leco_fix_block_param *prm, uint32_t *widths, hipStream_t st) {
-    BatchRows rows{cols, first, ncols, total};
+                               const uint32_t *bcol, const uint32_t *widths, uint32_t ncols, uint64_t total,
+                               uint64_t fit_groups, uint8_t *arena, uint64_t *offs, leco_fix_block_param *prm,
+                               hipStream_t st) {
+    BatchRows rows{cols, first, bcol, total};
     BatchFit fit{cols, first, fit_first, ncols};
-    k_batch_widths<<<dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(cols, first, ncols,
-                                                                                                    total, widths);
     k_fit<BatchFit><<<dim3((unsigned)fit_groups), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
     k_encode<BatchRows><<<dim3((unsigned)total), dim3(kThreads), 0, st>>>(rows, prm, offs, arena);
     k_encode_fixup<<<dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, total,
@@ -646,9 +817,17 @@ hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, c
     return hipGetLastError();
 }
 
-hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
-                               const uint8_t *arena, const uint64_t *offs, uint32_t grid, hipStream_t st) {
-    k_batch_decode<<<dim3(grid), dim3(kThreads), 0, st>>>(cols, first, ncols, total, arena, offs);
+hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, const uint64_t *tile_first,
+                               const uint32_t *tile_col, uint64_t tiles, const uint8_t *arena, const uint64_t *offs,
+                               const leco_fix_block_param *prm, uint32_t grid, hipStream_t st) {
+    k_batch_decode<<<dim3(grid), dim3(kThreads), 0, st>>>(cols, first, tile_first, tile_col, tiles, arena, offs, prm);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_columns(const uint64_t *tile_first, uint32_t ncols, uint64_t tiles, uint32_t *tile_col,
+                               hipStream_t st) {
+    k_tile_columns<<<dim3((unsigned)((tiles + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(tile_first, ncols,
+                                                                                                    tiles, tile_col);
     return hipGetLastError();
 }
 
@@ -666,6 +845,10 @@ template hipError_t launch_decode<uint32_t>(const uint8_t *, const uint64_t *, u
                                             uint64_t, uint32_t *, hipStream_t);
 template hipError_t launch_decode<uint64_t>(const uint8_t *, const uint64_t *, uint64_t, uint32_t, uint64_t,
                                             uint64_t, uint64_t *, hipStream_t);
+template hipError_t launch_decode_values<uint32_t>(const uint8_t *, const leco_fix_block_param *, uint32_t, uint64_t,
+                                                   uint64_t, uint32_t *, hipStream_t);
+template hipError_t launch_decode_values<uint64_t>(const uint8_t *, const leco_fix_block_param *, uint32_t, uint64_t,
+                                                   uint64_t, uint64_t *, hipStream_t);
 template hipError_t launch_lookup<uint32_t>(const uint8_t *, const leco_fix_block_param *, uint64_t, uint32_t,
                                             const uint64_t *, uint64_t, uint32_t *, hipStream_t);
 template hipError_t launch_lookup<uint64_t>(const uint8_t *, const leco_fix_block_param *, uint64_t, uint32_t,

@@ -434,11 +434,18 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                                                             r.count, (uint32_t *)out, sl.st);
             } else {
                 if (r.begin + r.count > c.col.n) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
-                e = c.col.width == 64
-                        ? leco::launch_decode<uint64_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
-                                                        r.begin + r.count, (uint64_t *)out, sl.st)
-                        : leco::launch_decode<uint32_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
-                                                        r.begin + r.count, (uint32_t *)out, sl.st);
+                if (leco::staged_decode(c.col.bs))
+                    e = c.col.width == 64
+                            ? leco::launch_decode<uint64_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
+                                                            r.begin + r.count, (uint64_t *)out, sl.st)
+                            : leco::launch_decode<uint32_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
+                                                            r.begin + r.count, (uint32_t *)out, sl.st);
+                else
+                    e = c.col.width == 64
+                            ? leco::launch_decode_values<uint64_t>(c.col.stream, c.col.params, c.col.bs, r.begin,
+                                                                   r.begin + r.count, (uint64_t *)out, sl.st)
+                            : leco::launch_decode_values<uint32_t>(c.col.stream, c.col.params, c.col.bs, r.begin,
+                                                                   r.begin + r.count, (uint32_t *)out, sl.st);
             }
             if (e == hipSuccess && r.h_out) e = hipMemcpyAsync(r.h_out, out, bytes, hipMemcpyDeviceToHost, sl.st);
             if (e == hipSuccess) e = hipStreamSynchronize(sl.st);
@@ -536,9 +543,11 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                         progressed = true;
                         continue;
                     }
+                    const bool staged = leco::staged_decode(c.col.bs);
                     const uint64_t items = r.kind == LECO_FIX_REQ_LOOKUP
                                                ? (r.count + kLookupChunk - 1) / kLookupChunk
-                                               : (r.begin + r.count - 1) / c.col.bs - r.begin / c.col.bs + 1;
+                                           : staged ? (r.begin + r.count - 1) / c.col.bs - r.begin / c.col.bs + 1
+                                                    : (r.count + leco::kValueTile - 1) / leco::kValueTile;
                     const uint64_t idx = r.kind == LECO_FIX_REQ_LOOKUP ? r.count : 0;
                     if (admitted.size() >= s->cfg.max_batch_requests || nitems + items > kItemCap ||
                         nidx + idx > kIdxCap || (nvals + r.count > s->cfg.max_batch_values && !admitted.empty())) {
@@ -565,11 +574,18 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                                                 sl.d_idx + nidx + q0, out + q0 * bw, 0, 0, 0};
                         }
                         nidx += r.count;
-                    } else {
+                    } else if (staged) {
                         const uint64_t hi = r.begin + r.count;
                         for (uint64_t b = r.begin / c.col.bs; b <= (hi - 1) / c.col.bs; ++b) {
                             leco::ServeItem &w = sl.h_items[nitems++];
                             w = leco::ServeItem{leco::kServeRange, r.column, 0, 0, nullptr, out, b, r.begin, hi};
+                        }
+                    } else {
+                        for (uint64_t q0 = 0; q0 < r.count; q0 += leco::kValueTile) {
+                            leco::ServeItem &w = sl.h_items[nitems++];
+                            w = leco::ServeItem{leco::kServeRangeValues, r.column,
+                                                (uint32_t)std::min<uint64_t>(leco::kValueTile, r.count - q0), 0,
+                                                nullptr, out + q0 * bw, 0, r.begin + q0, 0};
                         }
                     }
                     nvals += r.count;

@@ -46,6 +46,8 @@ for _w in ("u32", "u64"):
         f"leco_fix_encode_column_{_w}": (c_int, [c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr, c_ptr]),
         f"leco_fix_decode_column_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_ptr]),
         f"leco_fix_decode_range_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_u64, c_u64, c_ptr, c_ptr]),
+        f"leco_fix_decode_range_ex_{_w}": (c_int, [c_ptr, c_ptr, c_ptr, c_u64, c_u32, c_u64, c_u64, c_int, c_ptr,
+                                                   c_ptr]),
         f"leco_fix_lookup_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr]),
         f"leco_fix_encode_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr, c_u64, ctypes.POINTER(c_u64)]),
         f"leco_fix_decode_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr]),

@@ -18,6 +18,7 @@ import torch
 from ._lib import LECO_FIX_TAIL_PAD, BlockParam, check, lib
 
 _STORAGE = {32: torch.int32, 64: torch.int64}
+DECODE_AUTO, DECODE_STAGED, DECODE_VALUES = 0, 1, 2
 PARAM_BYTES = ctypes.sizeof(BlockParam)
 
 
@@ -95,17 +96,19 @@ class Column:
     def stream_bytes(self) -> bytes:
         return self.stream[self.base:self.base + self.nbytes].cpu().numpy().tobytes()
 
-    def decode(self, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    def decode(self, out: torch.Tensor | None = None, stream=None, shape: int = DECODE_AUTO) -> torch.Tensor:
         """Leco_int<T>::decodeArray8 over every block."""
-        return self.decode_range(0, self.n, out=out, stream=stream)
+        return self.decode_range(0, self.n, out=out, stream=stream, shape=shape)
 
-    def decode_range(self, begin: int, end: int, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
+    def decode_range(self, begin: int, end: int, out: torch.Tensor | None = None, stream=None,
+                     shape: int = DECODE_AUTO) -> torch.Tensor:
+        """Values [begin, end); shape picks the decode kernel (auto, staged or value-parallel)."""
         if out is None:
             out = torch.empty(end - begin, dtype=_STORAGE[self.width], device=self.stream.device)
         L = lib()
-        fn = L.leco_fix_decode_range_u32 if self.width == 32 else L.leco_fix_decode_range_u64
-        check(fn(self.stream.data_ptr(), self.offsets.data_ptr(), self.n, self.block_size, begin, end,
-                 out.data_ptr(), _stream_handle(stream)), "decode_range")
+        fn = L.leco_fix_decode_range_ex_u32 if self.width == 32 else L.leco_fix_decode_range_ex_u64
+        check(fn(self.stream.data_ptr(), self.offsets.data_ptr(), self.params.data_ptr(), self.n, self.block_size,
+                 begin, end, shape, out.data_ptr(), _stream_handle(stream)), "decode_range")
         return out
 
     def lookup(self, idx: torch.Tensor, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:

@@ -130,6 +130,11 @@ def run_batched(cols, reps: int = 5, log=print) -> dict:
     comp = batch.compressed_bytes()
     t_benc = _timed(lambda: batch.encode(graph=True), reps)
     t_bdec = _timed(lambda: batch.decode(graph=True), reps)
+    # the same launch as a persistent grid (8 workgroups per CU striding over the tiles)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    batch.set_decode_grid(8 * cus)
+    t_pers = _timed(lambda: batch.decode(graph=True), reps)
+    batch.set_decode_grid(0)
     # plain path: one decode launch / one column encode per column, from a host loop
     for o in batch.outs:
         o.zero_()
@@ -141,6 +146,7 @@ def run_batched(cols, reps: int = 5, log=print) -> dict:
         "columns": len(values), "raw_bytes": raw, "compressed_bytes": comp, "compression_ratio": round(comp / raw, 5),
         "block_sizes": hist, "autotune_s": round(t_tune, 3), "verified_bit_exact": ok,
         "batched_decode_gbs": round(raw / t_bdec / 1e9, 1), "plain_decode_gbs": round(raw / t_pdec / 1e9, 1),
+        "persistent_decode_gbs": round(raw / t_pers / 1e9, 1),
         "decode_speedup": round(t_pdec / t_bdec, 3),
         "batched_encode_gbs": round(raw / t_benc / 1e9, 1), "plain_encode_gbs": round(raw / t_penc / 1e9, 1),
         "encode_speedup": round(t_penc / t_benc, 3),
@@ -193,6 +199,20 @@ def run_serving(col, rates=(1e5, 1e6, 1e7), nreq: int = 60_000, plain_nreq: int 
     out["plain_same_trace"] = plain
     out["batched_same_trace"] = same
     out["gbs_ratio_same_trace"] = round(same["gbs"] / plain["gbs"], 2) if plain["gbs"] else None
+    # saturation: every request arrives at t = 0, so each path runs at its maximum rate
+    sat = reqs.copy()
+    sat["arrival_s"] = 0.0
+    log(f"serving: saturation, {plain_nreq} requests at t=0, plain then batched")
+    _, st_p = srv.run(sat, srvmod.SERVE_PLAIN)
+    _, st_b = srv.run(sat, srvmod.SERVE_BATCHED)
+    out["saturation"] = {
+        "requests": plain_nreq,
+        "plain_req_s": round(st_p["completed"] / st_p["wall_s"], 1),
+        "plain_gbs": round(st_p["bytes_out"] / st_p["wall_s"] / 1e9, 1),
+        "batched_req_s": round(st_b["completed"] / st_b["wall_s"], 1),
+        "batched_gbs": round(st_b["bytes_out"] / st_b["wall_s"] / 1e9, 1),
+        "gbs_ratio": round((st_b["bytes_out"] / st_b["wall_s"]) / (st_p["bytes_out"] / st_p["wall_s"]), 2),
+    }
     srv.close()
     return out
 

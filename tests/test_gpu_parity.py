@@ -181,3 +181,25 @@ def test_cpp_leco_int_driver(torch_dev, width):
     r = subprocess.run([exe, "60000", "7", str(width)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     assert r.stdout.strip().endswith("OK")
+
+
+@pytest.mark.parametrize("bs", [1, 7, 256, 1000, 2048, 4096, 16384])
+def test_decode_shapes_agree(torch_dev, bs):
+    # staged (one workgroup per block) and value-parallel (one lane per
+    # position) decode give the same values on every kind of block
+    from leco_fix import Column, datagen
+    from leco_fix.column import DECODE_STAGED, DECODE_VALUES
+    torch = torch_dev
+    rng = np.random.default_rng(bs)
+    for width in (32, 64):
+        top = (1 << width) - 1
+        parts = [datagen.sorted_uniform(5 * bs + 3, width, bs).astype(np.uint64),
+                 rng.integers(0, top, 2 * bs + 1, dtype=np.uint64, endpoint=True),       # raw blocks
+                 np.full(bs + 5, 12345, np.uint64)]                                        # constant blocks
+        vals = np.concatenate(parts).astype(np.uint32 if width == 32 else np.uint64)
+        say(f"decode shapes width={width} bs={bs} n={len(vals)}")
+        col = Column.encode(to_dev(torch, vals), bs)
+        for shape in (DECODE_STAGED, DECODE_VALUES):
+            assert np.array_equal(to_host(col.decode(shape=shape), vals.dtype), vals), shape
+            a, b = len(vals) // 3, len(vals) - 2
+            assert np.array_equal(to_host(col.decode_range(a, b, shape=shape), vals.dtype), vals[a:b]), shape
