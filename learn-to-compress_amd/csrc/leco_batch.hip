@@ -34,7 +34,9 @@ struct leco_fix_batch {
     uint64_t *d_offs = nullptr;
     leco_fix_block_param *d_prm = nullptr;
     uint32_t *d_widths = nullptr;
-    uint32_t *d_bcol = nullptr;
+    uint64_t units = 0;            // encode units (one ticket each)
+    uint64_t *d_unit_first = nullptr;
+    uint32_t *d_unit_col = nullptr;
     uint32_t decode_grid = 0;
     uint8_t *d_scratch = nullptr;  // per-column encode: stream, offsets, params of the largest column
     uint64_t scratch_cap = 0;
@@ -59,7 +61,8 @@ void free_batch(leco_fix_batch *b) {
     (void)hipFree(b->d_offs);
     (void)hipFree(b->d_prm);
     (void)hipFree(b->d_widths);
-    (void)hipFree(b->d_bcol);
+    (void)hipFree(b->d_unit_first);
+    (void)hipFree(b->d_unit_col);
     (void)hipFree(b->d_scratch);
     delete b;
 }
@@ -71,9 +74,9 @@ bool valid_desc(const leco_fix_column_desc &c) {
 
 int enqueue(leco_fix_batch *b, int op, hipStream_t st) {
     if (op == 0)
-        return hip_status(leco::launch_batch_encode(b->d_cols, b->d_first, b->d_fit_first, b->d_bcol, b->d_widths,
-                                                    b->ncols, b->total, b->fit_groups, b->d_arena, b->d_offs,
-                                                    b->d_prm, st));
+        return hip_status(leco::launch_batch_encode(b->d_cols, b->d_first, b->d_fit_first, b->d_unit_first,
+                                                    b->d_unit_col, b->d_widths, b->ncols, b->total, b->fit_groups,
+                                                    b->units, b->d_arena, b->d_offs, b->d_prm, st));
     return hip_status(leco::launch_batch_decode(b->d_cols, b->d_first, b->d_tile_first, b->d_tile_col, b->tiles,
                                                 b->d_arena, b->d_offs, b->d_prm, b->decode_grid, st));
 }
@@ -96,7 +99,7 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
     b->h_cols.resize(ncols);
     b->h_first.resize(ncols + 1);
     b->h_fit_first.resize(ncols + 1);
-    std::vector<uint64_t> tile_first(ncols + 1);
+    std::vector<uint64_t> tile_first(ncols + 1), unit_first(ncols + 1);
     uint64_t arena = 0;
     for (uint32_t c = 0; c < ncols; ++c) {
         const leco_fix_column_desc &d = cols[c];
@@ -105,6 +108,8 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
         b->h_fit_first[c] = b->fit_groups;
         const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
         b->fit_groups += (nb + 63) / 64;
+        unit_first[c] = b->units;
+        b->units += leco::encode_units(nb, d.block_size);
         tile_first[c] = b->tiles;
         const uint32_t kb = leco::multi_blocks(d.block_size, d.width_bits);
         b->tiles += leco::staged_decode(d.block_size) ? nb : (nb + kb - 1) / kb;
@@ -116,6 +121,7 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
     b->h_first[ncols] = b->total;
     b->h_fit_first[ncols] = b->fit_groups;
     tile_first[ncols] = b->tiles;
+    unit_first[ncols] = b->units;
     b->arena_cap = arena + LECO_FIX_TAIL_PAD;
     b->decode_grid = default_decode_grid(b->tiles);
     bool ok = hipMalloc(&b->d_cols, ncols * sizeof(BatchColumn)) == hipSuccess &&
@@ -127,7 +133,8 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
               hipMalloc(&b->d_offs, (b->total + 1) * sizeof(uint64_t)) == hipSuccess &&
               hipMalloc(&b->d_prm, b->total * sizeof(leco_fix_block_param)) == hipSuccess &&
               hipMalloc(&b->d_widths, b->total * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc(&b->d_bcol, b->total * sizeof(uint32_t)) == hipSuccess;
+              hipMalloc(&b->d_unit_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
+              hipMalloc(&b->d_unit_col, b->units * sizeof(uint32_t)) == hipSuccess;
     if (!ok) {
         free_batch(b);
         return LECO_FIX_ENOMEM;
@@ -139,10 +146,12 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
             hipSuccess ||
         hipMemcpy(b->d_tile_first, tile_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
             hipSuccess ||
+        hipMemcpy(b->d_unit_first, unit_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+            hipSuccess ||
         hipMemset(b->d_arena + arena, 0, LECO_FIX_TAIL_PAD) != hipSuccess ||
         leco::launch_tile_columns(b->d_tile_first, ncols, b->tiles, b->d_tile_col, nullptr) != hipSuccess ||
-        leco::launch_batch_blocks(b->d_cols, b->d_first, ncols, b->total, b->d_bcol, b->d_widths, nullptr) !=
-            hipSuccess ||
+        leco::launch_tile_columns(b->d_unit_first, ncols, b->units, b->d_unit_col, nullptr) != hipSuccess ||
+        leco::launch_batch_blocks(b->d_cols, b->d_first, ncols, b->total, b->d_widths, nullptr) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
         free_batch(b);
         return LECO_FIX_EHIP;

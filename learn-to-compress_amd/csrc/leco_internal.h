@@ -27,6 +27,22 @@ __host__ __device__ inline uint64_t block_bound(uint32_t w, uint32_t n) {
     return a > b ? a : b;
 }
 
+// Blocks of at most kWaveBlock values are encoded one per wave, a workgroup
+// taking encode_unit_blocks() consecutive blocks of a column per ticket (about
+// 4 * kWaveBlock values, at most kUnitMaxBlocks blocks); larger blocks one per
+// workgroup.
+constexpr uint32_t kWaveBlock = 1024;
+constexpr uint32_t kUnitMaxBlocks = 64;
+__host__ __device__ inline uint32_t encode_unit_blocks(uint32_t bs) {
+    if (bs > kWaveBlock) return 1;
+    const uint32_t per = 4 * (kWaveBlock / bs);
+    return per < kUnitMaxBlocks ? per : kUnitMaxBlocks;
+}
+__host__ __device__ inline uint64_t encode_units(uint64_t nb, uint32_t bs) {
+    const uint32_t per = encode_unit_blocks(bs);
+    return (nb + per - 1) / per;
+}
+
 // One column of a batch, as the device sees it.
 struct BatchColumn {
     const void *in;   // values to encode (device)
@@ -105,14 +121,16 @@ hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint
 // global index of column c's block 0; first[ncols] = total.
 // fit_first[c] = index of column c's first 64-block fit group (fit_first[ncols]
 // = fit_groups).
-// bcol[b] / widths[b]: column index and width of global block b (see launch_batch_blocks).
+// unit_first[c] = column c's first encode unit (encode_units() per column;
+// unit_first[ncols] = units), unit_col[u] = unit u's column (see
+// launch_tile_columns).  widths[b]: width of global block b (see launch_batch_blocks).
 hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, const uint64_t *fit_first,
-                               const uint32_t *bcol, const uint32_t *widths, uint32_t ncols, uint64_t total,
-                               uint64_t fit_groups, uint8_t *arena, uint64_t *offs, leco_fix_block_param *prm,
-                               hipStream_t st);
-// Fills bcol and widths for every block of a batch (once, at creation).
+                               const uint64_t *unit_first, const uint32_t *unit_col, const uint32_t *widths,
+                               uint32_t ncols, uint64_t total, uint64_t fit_groups, uint64_t units, uint8_t *arena,
+                               uint64_t *offs, leco_fix_block_param *prm, hipStream_t st);
+// Fills widths for every block of a batch (once, at creation).
 hipError_t launch_batch_blocks(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
-                               uint32_t *bcol, uint32_t *widths, hipStream_t st);
+                               uint32_t *widths, hipStream_t st);
 // Batched decode: `grid` workgroups (one per tile, or fewer striding over them) cover the tile
 // table (tile_first[c]: column c's first tile; tile_col[t]: column of tile t).
 hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, const uint64_t *tile_first,

@@ -16,6 +16,9 @@ constexpr int kFitRows = 64;                        // blocks per fit workgroup 
 constexpr int kFitTK = 128;                         // values per block row in one fit tile
 constexpr int kFitThreads = 256;                    // fit workgroup: four waves load, wave 0 also fits
 constexpr int kFitPer = kFitRows * kFitTK / kFitThreads;  // tile values loaded per thread
+constexpr int kWaveVPL = 8;                         // values per lane register-cached by a wave-encoded block
+constexpr uint32_t kWaveChunk = kWaveVPL * kWave;   // values a wave packs through LDS at a time
+constexpr int kWaveLdsBytes = kLdsBytes / (kThreads / kWave);  // one wave's share of the LDS image
 constexpr int kLookPer = 4;                         // predecessors read per lane per look-back step
 constexpr int kLookWin = kWave * kLookPer;          // predecessors per look-back step (wave 0)
 
@@ -43,13 +46,24 @@ struct EncodeShared {
     uint64_t off, end;
     double t0;
     int l;
+    // wave-encoded unit: per block thetas, width and byte size (then offset)
+    double ut0[kUnitMaxBlocks], ut1[kUnitMaxBlocks];
+    uint64_t usz[kUnitMaxBlocks];
+    int ul[kUnitMaxBlocks];
 };
 
-// Where block b's values live.
-struct RowRef {
-    const uint8_t *base;
-    uint32_t m;  // 0 when b is past the last block
-    int width;
+// One ticket's worth of encode work: `count` consecutive blocks of one
+// column, encoded one per wave (`wave`) or a single block by the whole
+// workgroup.
+struct EncodeUnit {
+    const uint8_t *base;  // the column's values
+    uint64_t n;
+    uint32_t bs;
+    uint32_t width;
+    uint64_t cb0;         // first block of the unit, within the column
+    uint64_t gb0;         // its index in the param table / offsets
+    uint32_t count;       // blocks in the unit (0 past the last unit)
+    bool wave;
 };
 
 // Blocks of one column.
@@ -57,12 +71,13 @@ struct PlainRows {
     const uint8_t *in;
     uint64_t n;
     uint32_t bs;
-    int width;
+    uint32_t width;
     uint64_t nb;
-    __device__ __forceinline__ RowRef row(uint64_t b) const {
-        if (b >= nb) return {nullptr, 0, width};
-        const uint64_t s = b * bs;
-        return {in + s * (width / 8), (uint32_t)(n - s < bs ? n - s : bs), width};
+    __device__ __forceinline__ EncodeUnit unit(uint64_t u) const {
+        const uint32_t per = encode_unit_blocks(bs);
+        const uint64_t cb0 = u * per;
+        const uint32_t count = cb0 < nb ? (uint32_t)(nb - cb0 < per ? nb - cb0 : per) : 0;
+        return {in, n, bs, width, cb0, cb0, count, bs <= kWaveBlock};
     }
 };
 
@@ -111,19 +126,23 @@ struct BatchFit {
     }
 };
 
-// Blocks of many columns, in column order; bcol[it] is block it's column.
+// Encode units of many columns, in column order: unit_first[c] is column c's
+// first unit, unit_col[u] unit u's column, first[c] column c's first block.
 struct BatchRows {
     const BatchColumn *cols;
     const uint64_t *first;
-    const uint32_t *bcol;
-    uint64_t total;
-    __device__ __forceinline__ RowRef row(uint64_t it) const {
-        if (it >= total) return {nullptr, 0, 64};
-        const uint32_t c = bcol[it];
+    const uint64_t *unit_first;
+    const uint32_t *unit_col;
+    uint64_t units;
+    __device__ __forceinline__ EncodeUnit unit(uint64_t u) const {
+        if (u >= units) return {nullptr, 0, 1, 64, 0, 0, 0, false};
+        const uint32_t c = unit_col[u];
         const BatchColumn col = cols[c];
-        const uint64_t s = (it - first[c]) * col.bs;
-        return {(const uint8_t *)col.in + s * (col.width / 8), (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs),
-                (int)col.width};
+        const uint32_t per = encode_unit_blocks(col.bs);
+        const uint64_t nb = first[c + 1] - first[c];
+        const uint64_t cb0 = (u - unit_first[c]) * per;
+        const uint32_t count = (uint32_t)(nb - cb0 < per ? nb - cb0 : per);
+        return {(const uint8_t *)col.in, col.n, col.bs, col.width, cb0, first[c] + cb0, count, col.bs <= kWaveBlock};
     }
 };
 

@@ -6,8 +6,8 @@
 // plus a 32-byte leco_fix_block_param per block for point lookups.
 //
 // Encode = K1 fit (one lane per block) -> K2 residuals + bit width + offset
-// (decoupled look-back) + pack, one workgroup per block, values kept in
-// registers -> K3 fix-up of payload offsets.  Decode = one workgroup per
+// (decoupled look-back) + pack, one workgroup per large block or one wave per
+// small block, values kept in registers -> K3 fix-up of payload offsets.  Decode = one workgroup per
 // block; lookup = one lane per query.  The batched variants run the same
 // per-block bodies over a device table of blocks from many columns.
 #include "leco_kernels.h"
@@ -118,20 +118,164 @@ __global__ __launch_bounds__(kFitThreads) void k_fit(Src src, leco_fix_block_par
 }
 
 // ---------------------------------------------------------------------------
-// K2 encode body: piecewise_fix_integer_template.h:36-137.
+// K2 encode: piecewise_fix_integer_template.h:36-137.
 //   pass 1: min/max int64 residual under the fitted line; theta0 += (max+min)/2
 //   pass 2: largest sign/magnitude residual -> bit width l (lr.h:116-130)
 //   offset: decoupled look-back over the preceding blocks' sizes
 //   pack:   header [l][theta0][theta1] (or [l] + raw values when l == W), then
 //           field k = (mag & (2^(l-1)-1)) | sign<<(l-1) LSB-first, then zero
-//           padding (bit_write.h:257-323).  kChunk values at a time are OR-ed
-//           into an LDS image aligned like their HBM destination and written
-//           with 16-byte stores; byte stores only at the block's ragged ends.
-// The block's values stay in registers (kVPT per lane) across all passes.
+//           padding (bit_write.h:257-323).  A chunk of values at a time is
+//           OR-ed into an LDS image aligned like its HBM destination and
+//           written with 16-byte stores; byte stores only at the block's
+//           ragged ends.
+// A block of more than kWaveBlock values is encoded by a whole workgroup;
+// smaller blocks by one wave each, four blocks side by side per workgroup.
+// Either way the block's values stay in registers across all passes.
 // ---------------------------------------------------------------------------
+
+// Decoupled look-back of block b (one wave; every lane returns the same
+// exclusive byte prefix).  Publishes the block's size, then reads kLookWin
+// predecessors' status words offs[p+1] per step (kLookPer per lane, nearest
+// first) until the nearest one that carries an inclusive prefix; the prefix
+// is its value plus the sizes of the blocks in between; finally publishes
+// the block's inclusive prefix.  Every lower block belongs to a wave that is
+// already running and publishes its size before it waits, so the waits end;
+// the spin bound (~seconds) only keeps a broken invariant from hanging the GPU.
+__device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_t *__restrict__ offs) {
+    const int lane = threadIdx.x % kWave;
+    if (b == 0) {
+        if (lane == 0) store_status(&offs[1], kStatusInclusive | size);
+        return 0;
+    }
+    if (lane == 0) store_status(&offs[b + 1], kStatusAggregate | size);
+    uint64_t excl = 0;
+    for (int64_t top = (int64_t)b - 1;; top -= kLookWin) {
+        uint64_t stw[kLookPer];
+#pragma unroll
+        for (int k = 0; k < kLookPer; ++k) {
+            const int64_t p = top - (k * kWave + lane);
+            stw[k] = p >= 0 ? load_status(&offs[p + 1]) : kStatusInclusive;
+        }
+#pragma unroll
+        for (int k = 0; k < kLookPer; ++k) {
+            const int64_t p = top - (k * kWave + lane);
+            uint32_t spins = 0;
+            while ((stw[k] & kStatusMask) == 0 && ++spins < kSpinLimit) {
+                __builtin_amdgcn_s_sleep(1);
+                stw[k] = load_status(&offs[p + 1]);
+            }
+        }
+        int stop = kLookWin;  // nearest predecessor (q = k*64 + lane) with an inclusive prefix
+#pragma unroll
+        for (int k = kLookPer - 1; k >= 0; --k) {
+            const uint64_t incl = __ballot((stw[k] & kStatusMask) == kStatusInclusive);
+            if (incl) stop = k * kWave + __ffsll((long long)incl) - 1;
+        }
+        uint64_t part = 0;
+#pragma unroll
+        for (int k = 0; k < kLookPer; ++k)
+            if (k * kWave + lane <= stop) part += stw[k] & kStatusValue;
+        for (int o = kWave / 2; o > 0; o >>= 1) part += (uint64_t)__shfl_xor((long long)part, o, kWave);
+        excl += part;
+        if (stop < kLookWin) break;
+    }
+    if (lane == 0) store_status(&offs[b + 1], kStatusInclusive | (excl + size));
+    return excl;
+}
+
+// Orders one wave's LDS accesses around the point (LDS executes a wave's
+// instructions in issue order; the fences keep the compiler from moving
+// accesses across it).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Where a block's bytes go and what its header holds.
+struct PackedBlock {
+    uint64_t off, end;     // byte range of the block in the stream
+    uint64_t t0b, t1b;     // header theta bits
+    uint32_t m;            // values
+    int l;                 // bit width
+    uint32_t hdr;          // header bytes (1 raw, 17 otherwise)
+    uint32_t nchunks;
+};
+
+// Chunk c (CH values) of a block, packed by NT threads (thread `tid`) through
+// the LDS image `buf`; sync() orders the group's LDS accesses.  value(kk, i)
+// is the field of block value i = chunk position tid + kk*NT.  CACHED: kk is
+// a compile-time index (fully unrolled) so that register-cached values stay
+// in registers; otherwise value() recomputes the field and the loop is kept
+// rolled, which keeps register pressure down.
+template <int NT, uint32_t CH, bool CACHED, class Sync, class Value>
+__device__ __forceinline__ void pack_chunk(const PackedBlock &pb, uint32_t c, int tid, uint32_t *buf,
+                                           uint8_t *__restrict__ out, Sync sync, Value value) {
+    const int l = pb.l;
+    const uint64_t cbytes = (uint64_t)(CH / 8) * l;
+    const uint64_t g0 = c == 0 ? pb.off : pb.off + pb.hdr + c * cbytes;
+    const uint64_t g1 = c + 1 == pb.nchunks ? pb.end : pb.off + pb.hdr + (c + 1) * cbytes;
+    const uint64_t a0 = g0 & ~15ull;
+    const uint32_t span = (uint32_t)(((g1 + 15) & ~15ull) - a0);
+    const uint32_t lead = (uint32_t)(g0 - a0);
+    for (uint32_t w = tid; w < span / 4; w += NT) buf[w] = 0;
+    sync();
+    if (c == 0 && (uint32_t)tid < pb.hdr) {
+        uint32_t byte;
+        if (tid == 0) byte = (uint32_t)l;
+        else if (tid < 9) byte = (uint32_t)(pb.t0b >> (8 * (tid - 1))) & 0xFF;
+        else byte = (uint32_t)(pb.t1b >> (8 * (tid - 9))) & 0xFF;
+        const uint32_t pos = lead + tid;
+        atomicOr(&buf[pos >> 2], byte << (8 * (pos & 3)));
+    }
+    if (l > 0) {
+        const uint64_t pbit = 8ull * (lead + (c == 0 ? pb.hdr : 0));
+        const uint32_t j0 = c * CH;
+        const uint32_t jn = pb.m - j0 < CH ? pb.m - j0 : CH;
+        auto put = [&](int kk) {
+            const uint32_t j = tid + kk * NT;  // position inside the chunk
+            if (j < jn) {
+                const uint64_t field = value(kk, j0 + j);
+                const uint64_t q = pbit + (uint64_t)j * l;
+                const uint32_t dw = (uint32_t)(q >> 5);
+                const int shb = (int)(q & 31);
+                const uint32_t d0 = (uint32_t)(field << shb);
+                const uint32_t d1 = (uint32_t)(shb ? field >> (32 - shb) : field >> 32);
+                const uint32_t d2 = shb ? (uint32_t)(field >> (64 - shb)) : 0u;
+                if (d0) atomicOr(&buf[dw], d0);
+                if (shb + l > 32 && d1) atomicOr(&buf[dw + 1], d1);
+                if (shb + l > 64 && d2) atomicOr(&buf[dw + 2], d2);
+            }
+        };
+        if constexpr (CACHED) {
+#pragma unroll
+            for (int kk = 0; kk < (int)(CH / NT); ++kk) put(kk);
+        } else {
+#pragma unroll 2
+            for (int kk = 0; kk < (int)(CH / NT); ++kk) put(kk);
+        }
+    }
+    sync();
+    const uint8_t *bb = (const uint8_t *)buf;
+    for (uint32_t u = tid; u < span / 16; u += NT) {
+        const uint64_t ga = a0 + 16ull * u;
+        if (ga >= g0 && ga + 16 <= g1) {
+            *(uint4 *)(out + ga) = *(const uint4 *)(bb + 16 * u);
+        } else {
+            for (int k = 0; k < 16; ++k) {
+                const uint64_t gb = ga + k;
+                if (gb >= g0 && gb < g1) out[gb] = bb[16 * u + k];
+            }
+        }
+    }
+    sync();
+}
+
+// Workgroup body: a block of any size up to kMaxBlockSize; kVPT values per
+// thread are register-cached, the rest re-read in the passes.
 template <typename T, int W>
 __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint32_t m, leco_fix_block_param *prm,
-                                                  uint64_t b, uint64_t *__restrict__ offs,
+                                                  uint64_t b, uint64_t u, uint64_t *__restrict__ offs,
                                                   uint8_t *__restrict__ out, EncodeShared &sh) {
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     double t0 = prm[b].theta0;
@@ -200,129 +344,26 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
     if (lb > W) lb = W;
     const uint64_t size = block_bytes((uint32_t)lb, W, m);
 
-    // Decoupled look-back over the status words offs[p+1] (wave 0): publish
-    // this block's size, then read kLookWin predecessors per step (kLookPer
-    // per lane, nearest first) until the nearest one that carries an
-    // inclusive prefix; the exclusive prefix is its value plus the sizes of
-    // the blocks in between.  Every lower block belongs to a workgroup that is
-    // already running and publishes its size without waiting, so the waits
-    // end; the spin bound (~seconds) only keeps a broken invariant from
-    // hanging the GPU.
-    uint64_t excl = 0;
     if (wid == 0) {
-        if (b == 0) {
-            if (lane == 0) store_status(&offs[1], kStatusInclusive | size);
-        } else {
-            if (lane == 0) store_status(&offs[b + 1], kStatusAggregate | size);
-            for (int64_t top = (int64_t)b - 1;; top -= kLookWin) {
-                uint64_t stw[kLookPer];
-#pragma unroll
-                for (int k = 0; k < kLookPer; ++k) {
-                    const int64_t p = top - (k * kWave + lane);
-                    stw[k] = p >= 0 ? load_status(&offs[p + 1]) : kStatusInclusive;
-                }
-#pragma unroll
-                for (int k = 0; k < kLookPer; ++k) {
-                    const int64_t p = top - (k * kWave + lane);
-                    uint32_t spins = 0;
-                    while ((stw[k] & kStatusMask) == 0 && ++spins < kSpinLimit) {
-                        __builtin_amdgcn_s_sleep(1);
-                        stw[k] = load_status(&offs[p + 1]);
-                    }
-                }
-                int stop = kLookWin;  // nearest predecessor (q = k*64 + lane) with an inclusive prefix
-#pragma unroll
-                for (int k = kLookPer - 1; k >= 0; --k) {
-                    const uint64_t incl = __ballot((stw[k] & kStatusMask) == kStatusInclusive);
-                    if (incl) stop = k * kWave + __ffsll((long long)incl) - 1;
-                }
-                uint64_t part = 0;
-#pragma unroll
-                for (int k = 0; k < kLookPer; ++k)
-                    if (k * kWave + lane <= stop) part += stw[k] & kStatusValue;
-                for (int o = kWave / 2; o > 0; o >>= 1) part += (uint64_t)__shfl_xor((long long)part, o, kWave);
-                excl += part;
-                if (stop < kLookWin) break;
-            }
-            if (lane == 0) store_status(&offs[b + 1], kStatusInclusive | (excl + size));
+        const uint64_t excl = look_back(u, size, offs);
+        if (lane == 0) {
+            sh.off = excl;
+            sh.end = excl + size;
+            sh.l = lb;
+            sh.t0 = lb == W ? 0.0 : t0;
+            prm[b].theta0 = sh.t0;
+            if (lb == W) prm[b].theta1 = 0.0;
+            prm[b].bits = (uint32_t)lb;
+            prm[b].payload_offset = excl + (lb == W ? 1 : 17);
         }
-    }
-    if (tid == 0) {
-        sh.off = excl;
-        sh.end = excl + size;
-        sh.l = lb;
-        sh.t0 = lb == W ? 0.0 : t0;
-        prm[b].theta0 = sh.t0;
-        if (lb == W) prm[b].theta1 = 0.0;
-        prm[b].bits = (uint32_t)lb;
     }
     __syncthreads();
     const int l = sh.l;
-    const uint64_t off = sh.off, end = sh.end;
     t0 = sh.t0;
-
-    // pack
     const bool raw = (l == W);
-    const uint32_t hdr = raw ? 1 : 17;
-    const uint32_t nchunks = l == 0 ? 1 : (m + kChunk - 1) / kChunk;
-    const uint64_t cbytes = (uint64_t)(kChunk / 8) * l;
-    const uint64_t t0b = double_to_bits(t0), t1b = double_to_bits(t1);
-    uint32_t *buf = sh.buf;
-    // value(kk, i) gives the field of value i = chunk position tid + kk*kThreads;
-    // kk is a compile-time index so that register-cached values stay in registers
-    auto pack_chunk = [&](uint32_t c, auto value) {
-        const uint64_t g0 = c == 0 ? off : off + hdr + c * cbytes;
-        const uint64_t g1 = c + 1 == nchunks ? end : off + hdr + (c + 1) * cbytes;
-        const uint64_t a0 = g0 & ~15ull;
-        const uint32_t span = (uint32_t)(((g1 + 15) & ~15ull) - a0);
-        const uint32_t lb = (uint32_t)(g0 - a0);
-        for (uint32_t w = tid; w < span / 4; w += kThreads) buf[w] = 0;
-        __syncthreads();
-        if (c == 0 && (uint32_t)tid < hdr) {
-            uint32_t byte;
-            if (tid == 0) byte = (uint32_t)l;
-            else if (tid < 9) byte = (uint32_t)(t0b >> (8 * (tid - 1))) & 0xFF;
-            else byte = (uint32_t)(t1b >> (8 * (tid - 9))) & 0xFF;
-            const uint32_t pos = lb + tid;
-            atomicOr(&buf[pos >> 2], byte << (8 * (pos & 3)));
-        }
-        if (l > 0) {
-            const uint64_t pbit = 8ull * (lb + (c == 0 ? hdr : 0));
-            const uint32_t j0 = c * kChunk;
-            const uint32_t jn = m - j0 < kChunk ? m - j0 : kChunk;
-#pragma unroll
-            for (int kk = 0; kk < (int)(kChunk / kThreads); ++kk) {
-                const uint32_t j = tid + kk * kThreads;  // position inside the chunk
-                if (j < jn) {
-                    const uint32_t i = j0 + j;
-                    const uint64_t field = value(kk, i);
-                    const uint64_t q = pbit + (uint64_t)j * l;
-                    const uint32_t dw = (uint32_t)(q >> 5);
-                    const int shb = (int)(q & 31);
-                    const uint32_t d0 = (uint32_t)(field << shb);
-                    const uint32_t d1 = (uint32_t)(shb ? field >> (32 - shb) : field >> 32);
-                    const uint32_t d2 = shb ? (uint32_t)(field >> (64 - shb)) : 0u;
-                    if (d0) atomicOr(&buf[dw], d0);
-                    if (shb + l > 32 && d1) atomicOr(&buf[dw + 1], d1);
-                    if (shb + l > 64 && d2) atomicOr(&buf[dw + 2], d2);
-                }
-            }
-        }
-        __syncthreads();
-        const uint8_t *bb = (const uint8_t *)buf;
-        for (uint32_t u = tid; u < span / 16; u += kThreads) {
-            const uint64_t ga = a0 + 16ull * u;
-            if (ga >= g0 && ga + 16 <= g1) {
-                *(uint4 *)(out + ga) = *(const uint4 *)(bb + 16 * u);
-            } else {
-                for (int k = 0; k < 16; ++k) {
-                    const uint64_t gb = ga + k;
-                    if (gb >= g0 && gb < g1) out[gb] = bb[16 * u + k];
-                }
-            }
-        }
-        __syncthreads();
-    };
+    const PackedBlock pb{sh.off, sh.end, double_to_bits(t0), double_to_bits(t1), m, l, raw ? 1u : 17u,
+                         l == 0 ? 1 : (m + kChunk - 1) / kChunk};
+
     static_assert(kVPT == 2 * (kChunk / kThreads), "two chunks of values are register-cached");
     const uint64_t low = l > 0 ? (1ull << (l - 1)) - 1 : 0;
     auto cached = [&](int k, uint32_t i) -> uint64_t {
@@ -334,40 +375,187 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
         uint64_t mg2;
         return residual_field<W>((uint64_t)y[i], predict(t0, t1, i), l, &mg2);
     };
-    pack_chunk(0, [&](int kk, uint32_t i) { return cached(kk, i); });
-    if (nchunks > 1) pack_chunk(1, [&](int kk, uint32_t i) { return cached(kChunk / kThreads + kk, i); });
-    for (uint32_t c = 2; c < nchunks; ++c) pack_chunk(c, [&](int, uint32_t i) { return fresh(i); });
+    auto sync = [] { __syncthreads(); };
+    pack_chunk<kThreads, kChunk, true>(pb, 0, tid, sh.buf, out, sync,
+                                       [&](int kk, uint32_t i) { return cached(kk, i); });
+    if (pb.nchunks > 1)
+        pack_chunk<kThreads, kChunk, true>(pb, 1, tid, sh.buf, out, sync,
+                                           [&](int kk, uint32_t i) { return cached(kChunk / kThreads + kk, i); });
+    for (uint32_t c = 2; c < pb.nchunks; ++c)
+        pack_chunk<kThreads, kChunk, false>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) { return fresh(i); });
 }
 
-// One workgroup per block, in ticket order (the look-back needs every lower
-// block to be owned by a running workgroup).
-template <class Rows>
-__global__ __launch_bounds__(kThreads) void k_encode(Rows rows, leco_fix_block_param *__restrict__ prm,
-                                                     uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
-    __shared__ EncodeShared sh;
-    __shared__ uint64_t s_b;
-    if (threadIdx.x == 0) s_b = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+// Wave-encoded unit: `count` consecutive blocks of at most kWaveBlock values,
+// wave w taking blocks w, w+4, w+8, ... of the unit.
+//   phase A: each wave fits the sign/magnitude widths of its blocks (values
+//            register-cached kWaveVPL per lane, the tail of a longer block
+//            re-read) and parks l, thetas and byte size in LDS;
+//   unit:    wave 0 scans the sizes, publishes the unit's total and does one
+//            look-back over the unit status words;
+//   phase B: each wave packs its blocks at their offsets, kWaveChunk values
+//            at a time through its own kWaveLdsBytes of LDS (values re-read,
+//            mostly from cache).
+// Publishing one size per unit as soon as phase A ends keeps the look-back
+// from waiting on blocks that a wave has not reached yet.
+template <typename T, int W>
+__device__ __forceinline__ void encode_unit_waves(const EncodeUnit &un, uint64_t u, leco_fix_block_param *prm,
+                                                  uint64_t *__restrict__ offs, uint8_t *__restrict__ out,
+                                                  EncodeShared &sh) {
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    constexpr uint32_t kWaves = kThreads / kWave;
+    const T *col = (const T *)un.base;
+
+    // phase A
+    for (uint32_t k = wid; k < un.count; k += kWaves) {
+        const uint64_t b = un.gb0 + k;
+        const T *y = col + (un.cb0 + k) * un.bs;
+        const uint64_t s = (un.cb0 + k) * un.bs;
+        const uint32_t m = (uint32_t)(un.n - s < un.bs ? un.n - s : un.bs);
+        double t0 = prm[b].theta0;
+        const double t1 = prm[b].theta1;
+        uint64_t v[kWaveVPL];
+#pragma unroll
+        for (int q = 0; q < kWaveVPL; ++q) {
+            const uint32_t j = lane + q * kWave;
+            v[q] = j < m ? (uint64_t)y[j] : 0;
+        }
+        int64_t mx = INT64_MIN, mn = INT64_MAX;
+#pragma unroll
+        for (int q = 0; q < kWaveVPL; ++q) {
+            const uint32_t j = lane + q * kWave;
+            const int64_t d = (int64_t)(v[q] - (uint64_t)predict(t0, t1, j));
+            mx = (j < m && d > mx) ? d : mx;
+            mn = (j < m && d < mn) ? d : mn;
+        }
+        for (uint32_t j = lane + kWaveChunk; j < m; j += kWave) {
+            const int64_t d = (int64_t)((uint64_t)y[j] - (uint64_t)predict(t0, t1, j));
+            mx = d > mx ? d : mx;
+            mn = d < mn ? d : mn;
+        }
+        mx = wave_max_i64(mx);
+        mn = wave_min_i64(mn);
+        if (!__builtin_isnan(t0)) t0 = t0 + (double)(int64_t)((uint64_t)mx + (uint64_t)mn) * 0.5;
+        uint64_t mg = 0, mag;
+#pragma unroll
+        for (int q = 0; q < kWaveVPL; ++q) {
+            const uint32_t j = lane + q * kWave;
+            residual_sign<W>(v[q], predict(t0, t1, j), &mag);
+            mg = (j < m && mag > mg) ? mag : mg;
+        }
+        for (uint32_t j = lane + kWaveChunk; j < m; j += kWave) {
+            residual_sign<W>((uint64_t)y[j], predict(t0, t1, j), &mag);
+            mg = mag > mg ? mag : mg;
+        }
+        mg = wave_max_u64(mg);
+        int l = mg ? bitlen64(mg) + 1 : 0;
+        if (l > W) l = W;
+        if (l == W) t0 = 0.0;
+        if (lane == 0) {
+            sh.ut0[k] = t0;
+            sh.ut1[k] = l == W ? 0.0 : t1;
+            sh.ul[k] = l;
+            sh.usz[k] = block_bytes((uint32_t)l, W, m);
+        }
+    }
     __syncthreads();
-    const uint64_t b = s_b;
-    const RowRef r = rows.row(b);
-    if (r.width == 64)
-        encode_block_body<uint64_t, 64>((const uint64_t *)r.base, r.m, prm, b, offs, out, sh);
-    else
-        encode_block_body<uint32_t, 32>((const uint32_t *)r.base, r.m, prm, b, offs, out, sh);
+
+    // unit offset: exclusive scan of the sizes (one per lane) + look-back
+    if (wid == 0) {
+        const uint64_t mine = (uint32_t)lane < un.count ? sh.usz[lane] : 0;
+        uint64_t inc = mine;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint64_t t = (uint64_t)__shfl_up((long long)inc, o, kWave);
+            if (lane >= o) inc += t;
+        }
+        const uint64_t total = (uint64_t)__shfl((long long)inc, kWave - 1, kWave);
+        const uint64_t excl = look_back(u, total, offs);
+        if ((uint32_t)lane < un.count) sh.usz[lane] = excl + inc - mine;  // now: the block's offset
+    }
+    __syncthreads();
+
+    // phase B
+    uint32_t *buf = sh.buf + wid * (kWaveLdsBytes / 4);
+    for (uint32_t k = wid; k < un.count; k += kWaves) {
+        const uint64_t b = un.gb0 + k;
+        const T *y = col + (un.cb0 + k) * un.bs;
+        const uint64_t s = (un.cb0 + k) * un.bs;
+        const uint32_t m = (uint32_t)(un.n - s < un.bs ? un.n - s : un.bs);
+        const int l = __builtin_amdgcn_readfirstlane(sh.ul[k]);
+        const double t0 = sh.ut0[k], t1 = sh.ut1[k];
+        const uint64_t off = sh.usz[k];
+        const bool raw = (l == W);
+        const uint32_t hdr = raw ? 1u : 17u;
+        if (lane == 0) {
+            prm[b].theta0 = t0;
+            if (raw) prm[b].theta1 = 0.0;
+            prm[b].bits = (uint32_t)l;
+            prm[b].payload_offset = off + hdr;
+        }
+        const PackedBlock pb{off, off + block_bytes((uint32_t)l, W, m), double_to_bits(t0), double_to_bits(t1), m, l,
+                             hdr, l == 0 ? 1 : (m + kWaveChunk - 1) / kWaveChunk};
+        static_assert(kWaveBlock <= 2 * kWaveChunk, "a wave-encoded block packs in at most two chunks");
+        auto sync = [] { wave_lds_sync(); };
+        for (uint32_t c = 0; c < pb.nchunks; ++c) {
+            uint64_t f[kWaveVPL];
+#pragma unroll
+            for (int q = 0; q < kWaveVPL; ++q) {
+                const uint32_t i = c * kWaveChunk + lane + q * kWave;
+                f[q] = i < m ? (uint64_t)y[i] : 0;
+            }
+            if (!raw && l > 0) {
+#pragma unroll
+                for (int q = 0; q < kWaveVPL; ++q) {
+                    uint64_t mg2;
+                    const uint32_t i = c * kWaveChunk + lane + q * kWave;
+                    f[q] = residual_field<W>(f[q], predict(t0, t1, i), l, &mg2);
+                }
+            }
+            pack_chunk<kWave, kWaveChunk, true>(pb, c, lane, buf, out, sync,
+                                                [&](int q, uint32_t) -> uint64_t { return f[q]; });
+        }
+    }
 }
 
-// K3: clears the status flags from offs[1..nb] (leaving the inclusive
-// prefixes, i.e. block ends), resets offs[0] (the ticket) and sets each
-// block's payload offset in the param table.
-__global__ __launch_bounds__(kThreads) void k_encode_fixup(leco_fix_block_param *__restrict__ prm,
+// One workgroup per encode unit, in ticket order (the look-back needs every
+// lower unit to be owned by a running workgroup): a unit is one block of
+// more than kWaveBlock values (whole workgroup) or encode_unit_blocks()
+// blocks of one column encoded one per wave.  The look-back runs over unit
+// status words offs[u+1] (u < units <= blocks); the fix-up rebuilds the block
+// offsets from the param table.
+template <class Rows>
+__global__ __launch_bounds__(kThreads, 4) void k_encode(Rows rows, leco_fix_block_param *__restrict__ prm,
+                                                        uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
+    __shared__ EncodeShared sh;
+    __shared__ uint64_t s_u;
+    if (threadIdx.x == 0) s_u = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+    __syncthreads();
+    const uint64_t u = s_u;
+    const EncodeUnit un = rows.unit(u);
+    if (!un.count) return;
+    if (un.wave) {
+        if (un.width == 64) encode_unit_waves<uint64_t, 64>(un, u, prm, offs, out, sh);
+        else encode_unit_waves<uint32_t, 32>(un, u, prm, offs, out, sh);
+        return;
+    }
+    const uint64_t s = un.cb0 * un.bs;
+    const uint32_t m = (uint32_t)(un.n - s < un.bs ? un.n - s : un.bs);
+    if (un.width == 64)
+        encode_block_body<uint64_t, 64>((const uint64_t *)un.base + s, m, prm, un.gb0, u, offs, out, sh);
+    else
+        encode_block_body<uint32_t, 32>((const uint32_t *)un.base + s, m, prm, un.gb0, u, offs, out, sh);
+}
+
+// K3: block end offsets offs[b+1] from the param table (the unit status
+// words that shared offs during the encode are overwritten), and offs[0]
+// (the ticket) back to 0.
+__global__ __launch_bounds__(kThreads) void k_encode_fixup(const leco_fix_block_param *__restrict__ prm,
                                                            uint64_t *__restrict__ offs, uint64_t nb,
                                                            const uint32_t *__restrict__ widths, uint32_t width) {
     const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (b >= nb) return;
     const uint32_t w = widths ? widths[b] : width;
-    const uint64_t excl = b == 0 ? 0 : (offs[b] & kStatusValue);
-    prm[b].payload_offset = excl + (prm[b].bits >= w ? 1 : 17);
-    offs[b + 1] &= kStatusValue;
+    const leco_fix_block_param p = prm[b];
+    offs[b + 1] = p.payload_offset - (p.bits >= w ? 1 : 17) + block_bytes(p.bits, w, p.length);
     if (b == 0) offs[0] = 0;
 }
 
@@ -685,16 +873,13 @@ __global__ __launch_bounds__(kThreads) void k_tile_columns(const uint64_t *__res
     if (t < tiles) tile_col[t] = find_column(tile_first, ncols, t);
 }
 
-// Per-block column index and width of a batch (one binary search per block, once).
+// Per-block width of a batch (one binary search per block, once).
 __global__ __launch_bounds__(kThreads) void k_batch_blocks(const BatchColumn *__restrict__ cols,
                                                            const uint64_t *__restrict__ first, uint32_t ncols,
-                                                           uint64_t total, uint32_t *__restrict__ bcol,
-                                                           uint32_t *__restrict__ widths) {
+                                                           uint64_t total, uint32_t *__restrict__ widths) {
     const uint64_t it = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (it >= total) return;
-    const uint32_t c = find_column(first, ncols, it);
-    bcol[it] = c;
-    widths[it] = cols[c].width;
+    widths[it] = cols[find_column(first, ncols, it)].width;
 }
 
 // ---------------------------------------------------------------------------
@@ -749,7 +934,7 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
     PlainRows rows{(const uint8_t *)in, n, bs, W, nb};
     PlainFit fit{(const uint8_t *)in, n, bs, W};
     k_fit<PlainFit><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
-    k_encode<PlainRows><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
+    k_encode<PlainRows><<<dim3((unsigned)encode_units(nb, bs)), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     k_encode_fixup<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, nb,
                                                                                                 nullptr, W);
     return hipGetLastError();
@@ -798,20 +983,20 @@ hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint
 }
 
 hipError_t launch_batch_blocks(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
-                               uint32_t *bcol, uint32_t *widths, hipStream_t st) {
+                               uint32_t *widths, hipStream_t st) {
     k_batch_blocks<<<dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(cols, first, ncols,
-                                                                                                    total, bcol, widths);
+                                                                                                    total, widths);
     return hipGetLastError();
 }
 
 hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, const uint64_t *fit_first,
-                               const uint32_t *bcol, const uint32_t *widths, uint32_t ncols, uint64_t total,
-                               uint64_t fit_groups, uint8_t *arena, uint64_t *offs, leco_fix_block_param *prm,
-                               hipStream_t st) {
-    BatchRows rows{cols, first, bcol, total};
+                               const uint64_t *unit_first, const uint32_t *unit_col, const uint32_t *widths,
+                               uint32_t ncols, uint64_t total, uint64_t fit_groups, uint64_t units, uint8_t *arena,
+                               uint64_t *offs, leco_fix_block_param *prm, hipStream_t st) {
+    BatchRows rows{cols, first, unit_first, unit_col, units};
     BatchFit fit{cols, first, fit_first, ncols};
     k_fit<BatchFit><<<dim3((unsigned)fit_groups), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
-    k_encode<BatchRows><<<dim3((unsigned)total), dim3(kThreads), 0, st>>>(rows, prm, offs, arena);
+    k_encode<BatchRows><<<dim3((unsigned)units), dim3(kThreads), 0, st>>>(rows, prm, offs, arena);
     k_encode_fixup<<<dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, total,
                                                                                                     widths, 0);
     return hipGetLastError();

@@ -25,7 +25,7 @@ def mixed_columns(seed, count=12, max_n=40_000):
     for c in range(count):
         width = 32 if c % 2 else 64
         n = int(np.exp(rng.uniform(np.log(50), np.log(max_n))))
-        bs = int(rng.choice([1, 3, 256, 1000, 2048, 4096, 5000, 16384]))
+        bs = int(rng.choice([1, 3, 256, 512, 700, 1000, 1024, 2048, 4096, 5000, 16384]))
         cols.append((dists[c % 4](n, width, int(rng.integers(1 << 30))), bs))
     return cols
 

@@ -49,7 +49,7 @@ def test_golden_stream_decode_lookup(torch_dev, name):
     assert torch.equal(col.rebuild_params(), col.params)
 
 
-BS_LIST = [1, 2, 3, 17, 255, 256, 1000, 2047, 2048, 2049, 4096, 5000, 10000, 16384]
+BS_LIST = [1, 2, 3, 17, 255, 256, 512, 513, 1000, 1024, 1025, 2047, 2048, 2049, 4096, 5000, 10000, 16384]
 
 
 def test_random_columns_match_oracle(torch_dev, oracle_mod):
@@ -82,7 +82,7 @@ def test_raw_and_edge_distributions_match_oracle(torch_dev, oracle_mod):
     from leco_fix import Column
     torch = torch_dev
     rng = np.random.default_rng(3)
-    for width, bs in [(64, 4096), (32, 4096), (64, 3000), (32, 2500)]:
+    for width, bs in [(64, 4096), (32, 4096), (64, 3000), (32, 2500), (64, 512), (32, 1000), (64, 1024)]:
         top = (1 << width) - 1
         parts = [rng.integers(0, top, 3 * bs, dtype=np.uint64, endpoint=True),          # raw blocks
                  np.sort(rng.integers(top >> 1, top, 2 * bs, dtype=np.uint64)),          # >= 2^(w-1)
@@ -93,6 +93,36 @@ def test_raw_and_edge_distributions_match_oracle(torch_dev, oracle_mod):
         col = Column.encode(to_dev(torch, vals), bs)
         assert col.stream_bytes() == want_st.tobytes()
         assert np.array_equal(to_host(col.decode(), vals.dtype), vals)
+
+
+def test_every_bit_width_small_and_large_blocks(torch_dev, oracle_mod):
+    # Block k's residuals span about (k mod (W+1)) bits on top of a line, so
+    # consecutive blocks take every width l = 0..W (raw included), for block
+    # sizes encoded one per wave (<= 1024, one or two LDS chunks) and one per
+    # workgroup.
+    from leco_fix import Column
+    torch = torch_dev
+    rng = np.random.default_rng(11)
+    for width in (32, 64):
+        for bs in (64, 300, 512, 777, 1024, 1500, 4096):
+            nb = width + 3
+            n = nb * bs - bs // 3
+            i = np.arange(n, dtype=np.uint64)
+            bits = (i // np.uint64(bs)) % np.uint64(width + 1)
+            noise = rng.integers(0, 1 << 62, n, dtype=np.uint64) | (rng.integers(0, 4, n, dtype=np.uint64) << np.uint64(62))
+            noise = np.where(bits == 0, np.uint64(0),
+                             noise >> (np.uint64(64) - np.maximum(bits, np.uint64(1))))
+            vals = (np.uint64(12345) + np.uint64(977) * i + noise)
+            vals = vals.astype(np.uint32) if width == 32 else vals
+            say(f"every width: width={width} bs={bs} n={n}")
+            want_st, want_off = oracle_mod.encode_column(vals, bs, threads=8)
+            col = Column.encode(to_dev(torch, vals), bs)
+            assert np.array_equal(to_host(col.offsets, np.uint64), want_off), (width, bs)
+            assert col.stream_bytes() == want_st.tobytes(), (width, bs)
+            assert np.array_equal(to_host(col.decode(), vals.dtype), vals), (width, bs)
+            idx = rng.integers(0, n, 4000, dtype=np.uint64)
+            got = col.lookup(torch.from_numpy(idx.view(np.int64)).to("cuda"))
+            assert np.array_equal(to_host(got, vals.dtype), vals[idx]), (width, bs)
 
 
 def test_reference_cpu_workload_on_gpu(torch_dev, oracle_mod):
