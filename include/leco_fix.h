@@ -96,9 +96,9 @@ int leco_fix_decode_range_u64(const uint8_t *d_stream, const uint64_t *d_offsets
                               uint64_t *d_out, void *stream);
 
 /* Range decode with the decode shape chosen explicitly: LECO_FIX_DECODE_AUTO
- * (staged for block sizes >= 2048, value-parallel below), _STAGED (one
- * workgroup per block through LDS; d_params may be NULL) or _VALUES (one lane
- * per position from the param table; needs d_params). */
+ * (= _STAGED), _STAGED (through LDS: one workgroup per block for block sizes
+ * >= 2048, tiles of several blocks per workgroup below; d_params may be NULL)
+ * or _VALUES (one lane per position from the param table; needs d_params). */
 #define LECO_FIX_DECODE_AUTO 0
 #define LECO_FIX_DECODE_STAGED 1
 #define LECO_FIX_DECODE_VALUES 2

@@ -37,7 +37,7 @@ template <typename T>
 int decode_range_ex(const uint8_t *stream, const uint64_t *offs, const leco_fix_block_param *prm, uint64_t n,
                     uint32_t bs, uint64_t lo, uint64_t hi, int shape, T *out, void *st) {
     if (!valid_bs(bs) || !stream || !out || lo >= hi || hi > n || shape < 0 || shape > 2) return LECO_FIX_EINVAL;
-    if (shape == LECO_FIX_DECODE_AUTO) shape = leco::staged_decode(bs) || !prm ? 1 : 2;
+    if (shape == LECO_FIX_DECODE_AUTO) shape = LECO_FIX_DECODE_STAGED;
     if (shape == LECO_FIX_DECODE_STAGED) {
         if (!offs) return LECO_FIX_EINVAL;
         return hip_status(leco::launch_decode<T>(stream, offs, n, bs, lo, hi, out, (hipStream_t)st));

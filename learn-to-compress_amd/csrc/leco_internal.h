@@ -79,22 +79,23 @@ struct ServeItem {
     uint64_t lo, hi;       // range: [lo, hi) in column positions
 };
 
-// Decode shape for a block size: blocks of at least this many values are
-// decoded staged (one workgroup per block through LDS), smaller ones
-// value-parallel (one lane per position, no per-block workgroup).
+// Decode shape for a block size: full decodes of blocks of at least this many
+// values are staged one workgroup per block through LDS; smaller blocks are
+// decoded multi_blocks() at a time per workgroup.
 constexpr uint32_t kStagedMinBlock = 2048;
 __host__ __device__ inline bool staged_decode(uint32_t bs) { return bs >= kStagedMinBlock; }
 // Positions per value-parallel tile (one workgroup).
 constexpr uint32_t kValueTile = 4096;
-// Small blocks are decoded multi_blocks() at a time by one workgroup: as many
-// consecutive blocks as keep the tile within 2048 positions, the staged bytes
-// within the LDS chunk buffer (largest block of each) and at most kMultiMax.
+// Small blocks: as many consecutive blocks per workgroup as keep the tile
+// within kMultiPositions positions, at most kMultiMax.  Their bytes are staged
+// through kMultiLds bytes of LDS, whole blocks at a time (one block of fewer
+// than kStagedMinBlock values always fits).
+constexpr uint32_t kMultiPositions = 8192;
 constexpr uint32_t kMultiMax = 256;
 constexpr uint32_t kMultiLds = 2048 * 8 + 32;
-__host__ __device__ inline uint32_t multi_blocks(uint32_t bs, uint32_t w) {
-    uint32_t k = 2048 / bs;
-    const uint32_t by_lds = (uint32_t)(kMultiLds / block_bound(w, bs));
-    if (by_lds < k) k = by_lds;
+static_assert(kMultiLds >= 1 + (kStagedMinBlock - 1) * 8 + 15, "a small block fits the staging buffer");
+__host__ __device__ inline uint32_t multi_blocks(uint32_t bs, uint32_t /*w*/) {
+    uint32_t k = kMultiPositions / bs;
     if (k > kMultiMax) k = kMultiMax;
     return k ? k : 1;
 }

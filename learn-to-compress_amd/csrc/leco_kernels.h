@@ -19,6 +19,8 @@ constexpr int kFitPer = kFitRows * kFitTK / kFitThreads;  // tile values loaded 
 constexpr int kWaveVPL = 8;                         // values per lane register-cached by a wave-encoded block
 constexpr uint32_t kWaveChunk = kWaveVPL * kWave;   // values a wave packs through LDS at a time
 constexpr int kWaveLdsBytes = kLdsBytes / (kThreads / kWave);  // one wave's share of the LDS image
+static_assert(kMultiLds + 16 <= kLdsBytes, "staged small-block bytes plus one word of field over-read fit the LDS buffer");
+constexpr int kMultiUnroll = 4;                     // positions per thread in flight in small-block decode
 constexpr int kLookPer = 4;                         // predecessors read per lane per look-back step
 constexpr int kLookWin = kWave * kLookPer;          // predecessors per look-back step (wave 0)
 
@@ -50,6 +52,15 @@ struct EncodeShared {
     double ut0[kUnitMaxBlocks], ut1[kUnitMaxBlocks];
     uint64_t usz[kUnitMaxBlocks];
     int ul[kUnitMaxBlocks];
+};
+
+// Small-block decode staging: the bytes of whole blocks, their offsets and
+// per-block header tables (width | first-field bit << 7, thetas).
+struct MultiShared {
+    alignas(16) uint64_t buf[kLdsBytes / 8];
+    uint64_t offs[kMultiMax + 1];
+    double2 ht[kMultiMax];
+    uint32_t hlb[kMultiMax];
 };
 
 // One ticket's worth of encode work: `count` consecutive blocks of one

@@ -642,65 +642,119 @@ __device__ __forceinline__ uint64_t lds_u64_at(const uint64_t *__restrict__ buf,
 }
 
 // Multi-block staged decode for small blocks: blocks [b0, b0 + kb) of one
-// column (kb * bs <= kChunk values) are staged into LDS in one sweep of
-// aligned 16-byte loads, with their offsets; each lane then decodes positions
-// from its block's header and field, both read from LDS.  Position p of the
-// tile goes to o[p].
+// column (kb * bs <= kMultiPositions values) with their offsets.  The bytes
+// are staged into LDS with aligned 16-byte loads, as many whole blocks at a
+// time as fit in kMultiLds bytes (all of them unless the data is nearly
+// incompressible); one thread per block then parses its header into small
+// tables (width | first-field bit, thetas), and every position is decoded
+// from its block's table entry and field.  Tile positions p in [plo, phi)
+// are decoded to o[p]; the others are skipped.  Widths 127 and 126 are the
+// reference decoder's one- and two-value raw blocks (:152-170): only their
+// first one or two positions are written.
 template <typename T, int W>
 __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st, const uint64_t *__restrict__ offs,
-                                                  uint64_t n, uint32_t bs, uint64_t b0, uint32_t kb,
-                                                  T *__restrict__ o, uint64_t *__restrict__ buf,
-                                                  uint64_t *__restrict__ soffs) {
+                                                  uint64_t n, uint32_t bs, uint64_t b0, uint32_t kb, uint32_t plo,
+                                                  uint32_t phi, T *__restrict__ o, MultiShared &sm) {
     const int tid = threadIdx.x;
     const uint64_t s0 = b0 * bs;
     const uint64_t nb = (n + bs - 1) / bs;
     if (b0 + kb > nb) kb = (uint32_t)(nb - b0);
-    const uint32_t cnt = (uint32_t)(n - s0 < (uint64_t)kb * bs ? n - s0 : (uint64_t)kb * bs);
-    for (uint32_t k = tid; k <= kb; k += kThreads) soffs[k] = offs[b0 + k];
-    __syncthreads();
-    const uint64_t g0 = soffs[0], g1 = soffs[kb];
-    const uint64_t a0 = g0 & ~15ull;
-    const uint32_t n16 = (uint32_t)((g1 - a0 + 15) / 16);
-    for (uint32_t u = tid; u < n16; u += kThreads)
-        *(uint4 *)(buf + 2 * u) = *(const uint4 *)(st + a0 + 16ull * u);
+    uint32_t cnt = (uint32_t)(n - s0 < (uint64_t)kb * bs ? n - s0 : (uint64_t)kb * bs);
+    if (phi < cnt) cnt = phi;
+    for (uint32_t k = tid; k <= kb; k += kThreads) sm.offs[k] = offs[b0 + k];
     __syncthreads();
     const bool pow2 = (bs & (bs - 1)) == 0;
     const int lsh = __ffs((int)bs) - 1;
-    for (uint32_t p = tid; p < cnt; p += kThreads) {
-        const uint32_t k = pow2 ? p >> lsh : p / bs;
-        const uint32_t j = p - k * bs;
-        const uint32_t hb = (uint32_t)(soffs[k] - a0);
-        const int l = (int)(lds_u64_at(buf, hb) & 0xFF);
-        const bool raw = l >= W;
-        const int lw = raw ? W : l;
-        const uint64_t q = 8ull * (hb + (raw ? 1 : 17)) + (uint64_t)j * lw;
-        const uint32_t w = (uint32_t)(q >> 6);
-        const int sh = (int)(q & 63);
-        const uint64_t field = ((buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh))) & low_mask(lw);
-        if (raw) {
-            o[p] = (T)field;
-        } else {
-            const double t0 = bits_to_double(lds_u64_at(buf, hb + 1));
-            const double t1 = bits_to_double(lds_u64_at(buf, hb + 9));
-            const int64_t pr = predict(t0, t1, j);
-            const int lc = l > 0 ? l : 1;
-            const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
-            const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
-            o[p] = (T)(l ? v : (uint64_t)pr);
+    const uint8_t *bytes = (const uint8_t *)sm.buf;
+    for (uint32_t ks = 0; ks < kb;) {
+        const uint64_t a0 = sm.offs[ks] & ~15ull;
+        // largest ke in (ks, kb] whose bytes [a0, offs[ke]) fit (one block always does)
+        uint32_t ke = kb;
+        if (sm.offs[kb] - a0 > kMultiLds) {
+            uint32_t lo = ks + 1, hi = kb;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) / 2;
+                if (sm.offs[mid] - a0 <= kMultiLds) lo = mid;
+                else hi = mid - 1;
+            }
+            ke = lo;
         }
+        const uint32_t n16 = (uint32_t)((sm.offs[ke] - a0 + 15) / 16);
+        for (uint32_t u = tid; u < n16; u += kThreads)
+            *(uint4 *)(sm.buf + 2 * u) = *(const uint4 *)(st + a0 + 16ull * u);
+        __syncthreads();
+        for (uint32_t k = ks + tid; k < ke; k += kThreads) {
+            const uint32_t hb = (uint32_t)(sm.offs[k] - a0);
+            const uint32_t l = bytes[hb];
+            const bool raw = l >= (uint32_t)W;
+            sm.hlb[k - ks] = ((8 * (hb + (raw ? 1 : 17))) << 7) | l;
+            if (!raw) {
+                sm.ht[k - ks] = make_double2(bits_to_double(lds_u64_at(sm.buf, hb + 1)),
+                                             bits_to_double(lds_u64_at(sm.buf, hb + 9)));
+            }
+        }
+        __syncthreads();
+        const uint32_t p0 = ks * bs > plo ? ks * bs : plo;
+        const uint32_t pend = ke * bs < cnt ? ke * bs : cnt;
+        // kMultiUnroll positions per thread per step: table reads, then field
+        // reads, then the arithmetic, so that the LDS latencies overlap
+        for (uint32_t pb = p0; pb < pend; pb += kMultiUnroll * kThreads) {
+            uint32_t lb[kMultiUnroll], jj[kMultiUnroll];
+            double2 th[kMultiUnroll];
+            uint64_t w0[kMultiUnroll], w1[kMultiUnroll];
+#pragma unroll
+            for (int u = 0; u < kMultiUnroll; ++u) {
+                uint32_t p = pb + tid + u * kThreads;
+                p = p < pend ? p : pend - 1;
+                const uint32_t k = pow2 ? p >> lsh : p / bs;
+                jj[u] = p - k * bs;
+                lb[u] = sm.hlb[k - ks];
+                th[u] = sm.ht[k - ks];
+            }
+#pragma unroll
+            for (int u = 0; u < kMultiUnroll; ++u) {
+                const int l = (int)(lb[u] & 127);
+                const int lw = l >= W ? W : l;
+                const uint64_t q = (uint64_t)(lb[u] >> 7) + (uint64_t)jj[u] * lw;
+                w0[u] = sm.buf[q >> 6];
+                w1[u] = sm.buf[(q >> 6) + 1];
+            }
+#pragma unroll
+            for (int u = 0; u < kMultiUnroll; ++u) {
+                const uint32_t p = pb + tid + u * kThreads;
+                const uint32_t j = jj[u];
+                const int l = (int)(lb[u] & 127);
+                const bool raw = l >= W;
+                const int lw = raw ? W : l;
+                const int sh = (int)(((lb[u] >> 7) + (uint64_t)j * lw) & 63);
+                const uint64_t field = ((w0[u] >> sh) | ((w1[u] << 1) << (63 - sh))) & low_mask(lw);
+                const int64_t pr = predict(th[u].x, th[u].y, j);
+                const int lc = l > 0 ? l : 1;
+                const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
+                const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
+                const uint64_t val = raw ? field : (l ? v : (uint64_t)pr);
+                const bool wr = p < pend && (l < 126 || j < (uint32_t)(128 - l));
+                if (wr) o[p] = (T)val;
+            }
+        }
+        __syncthreads();
+        ks = ke;
     }
-    __syncthreads();
 }
 
-// One workgroup per tile of kb blocks of one column (full decode).
+// One workgroup per tile of kb blocks of one column, tiles starting at block
+// b_first; values [lo, hi) go to out[global - lo].
 template <typename T, int W>
 __global__ __launch_bounds__(kThreads) void k_decode_multi(const uint8_t *__restrict__ st,
                                                            const uint64_t *__restrict__ offs, uint64_t n,
-                                                           uint32_t bs, uint32_t kb, T *__restrict__ out) {
-    __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
-    __shared__ uint64_t soffs[kMultiMax + 1];
-    const uint64_t b0 = (uint64_t)blockIdx.x * kb;
-    decode_multi_body<T, W>(st, offs, n, bs, b0, kb, out + b0 * bs, buf, soffs);
+                                                           uint32_t bs, uint32_t kb, uint64_t b_first, uint64_t lo,
+                                                           uint64_t hi, T *__restrict__ out) {
+    __shared__ MultiShared sm;
+    const uint64_t b0 = b_first + (uint64_t)blockIdx.x * kb;
+    const uint64_t s0 = b0 * bs;
+    const uint32_t plo = lo > s0 ? (uint32_t)(lo - s0) : 0;
+    const uint32_t phi = hi - s0 < (uint64_t)kb * bs ? (uint32_t)(hi - s0) : kb * bs;
+    decode_multi_body<T, W>(st, offs, n, bs, b0, kb, plo, phi, out - lo + s0, sm);
 }
 
 // One workgroup per block of one column; values [lo, hi) go to out[global - lo].
@@ -839,8 +893,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_decode(const BatchColumn *__
                                                            const uint8_t *__restrict__ arena,
                                                            const uint64_t *__restrict__ offs,
                                                            const leco_fix_block_param *__restrict__ prm) {
-    __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
-    __shared__ uint64_t soffs[kMultiMax + 1];
+    __shared__ MultiShared sm;
     for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
         const uint32_t c = tile_col[t];
         const BatchColumn col = cols[c];
@@ -850,18 +903,18 @@ __global__ __launch_bounds__(kThreads) void k_batch_decode(const BatchColumn *__
             const uint32_t m = (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs);
             const uint64_t off = offs[first[c] + local];
             if (col.width == 64)
-                decode_block_body<uint64_t, 64>(arena, off, m, 0, m, (uint64_t *)col.out + s, buf);
+                decode_block_body<uint64_t, 64>(arena, off, m, 0, m, (uint64_t *)col.out + s, sm.buf);
             else
-                decode_block_body<uint32_t, 32>(arena, off, m, 0, m, (uint32_t *)col.out + s, buf);
+                decode_block_body<uint32_t, 32>(arena, off, m, 0, m, (uint32_t *)col.out + s, sm.buf);
         } else {
             const uint32_t kb = multi_blocks(col.bs, col.width);
             const uint64_t b0 = local * kb;
             if (col.width == 64)
-                decode_multi_body<uint64_t, 64>(arena, offs + first[c], col.n, col.bs, b0, kb,
-                                                (uint64_t *)col.out + b0 * col.bs, buf, soffs);
+                decode_multi_body<uint64_t, 64>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
+                                                (uint64_t *)col.out + b0 * col.bs, sm);
             else
-                decode_multi_body<uint32_t, 32>(arena, offs + first[c], col.n, col.bs, b0, kb,
-                                                (uint32_t *)col.out + b0 * col.bs, buf, soffs);
+                decode_multi_body<uint32_t, 32>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
+                                                (uint32_t *)col.out + b0 * col.bs, sm);
         }
     }
 }
@@ -944,14 +997,14 @@ template <typename T>
 hipError_t launch_decode(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, uint64_t lo,
                          uint64_t hi, T *out, hipStream_t st) {
     constexpr int W = 8 * sizeof(T);
-    if (!staged_decode(bs) && lo == 0 && hi == n) {
+    const uint64_t bf = lo / bs, bl = (hi - 1) / bs;
+    if (!staged_decode(bs)) {
         const uint32_t kb = multi_blocks(bs, W);
-        const uint64_t nb = (n + bs - 1) / bs;
-        k_decode_multi<T, W><<<dim3((unsigned)((nb + kb - 1) / kb)), dim3(kThreads), 0, st>>>(stream, offs, n, bs, kb,
-                                                                                                out);
+        const uint64_t tiles = (bl - bf + 1 + kb - 1) / kb;
+        k_decode_multi<T, W><<<dim3((unsigned)tiles), dim3(kThreads), 0, st>>>(stream, offs, n, bs, kb, bf, lo, hi,
+                                                                                 out);
         return hipGetLastError();
     }
-    const uint64_t bf = lo / bs, bl = (hi - 1) / bs;
     k_decode<T, W><<<dim3((unsigned)(bl - bf + 1)), dim3(kThreads), 0, st>>>(stream, offs, n, bs, bf, lo, hi, out);
     return hipGetLastError();
 }

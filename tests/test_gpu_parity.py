@@ -233,3 +233,33 @@ def test_decode_shapes_agree(torch_dev, bs):
             assert np.array_equal(to_host(col.decode(shape=shape), vals.dtype), vals), shape
             a, b = len(vals) // 3, len(vals) - 2
             assert np.array_equal(to_host(col.decode_range(a, b, shape=shape), vals.dtype), vals[a:b]), shape
+
+
+@pytest.mark.parametrize("bs", [1, 7, 100, 256, 1000, 2047])
+def test_small_block_ranges_cross_tiles(torch_dev, bs):
+    # small blocks are decoded several per workgroup (tiles of up to 8192
+    # positions): ranges that start and end inside tiles and blocks, span
+    # several tiles, or sit inside one block, against the values and the
+    # value-parallel shape
+    from leco_fix import Column, datagen
+    from leco_fix.column import DECODE_STAGED, DECODE_VALUES
+    torch = torch_dev
+    rng = np.random.default_rng(1000 + bs)
+    n = 50_000
+    for width in (32, 64):
+        top = (1 << width) - 1
+        vals = datagen.sorted_uniform(n, width, bs).astype(np.uint64)
+        raw_at = int(rng.integers(0, n - 3000))
+        vals[raw_at:raw_at + 3000] = rng.integers(0, top, 3000, dtype=np.uint64, endpoint=True)
+        vals = vals.astype(np.uint32 if width == 32 else np.uint64)
+        say(f"small-block ranges width={width} bs={bs} n={n}")
+        col = Column.encode(to_dev(torch, vals), bs)
+        assert np.array_equal(to_host(col.decode(), vals.dtype), vals)
+        ranges = [(0, 1), (n - 1, n), (8191, 8193), (0, n), (bs // 2, bs // 2 + 1)]
+        for _ in range(12):
+            a = int(rng.integers(0, n))
+            ranges.append((a, int(rng.integers(a + 1, min(n, a + 30_000) + 1))))
+        for a, b in ranges:
+            for shape in (DECODE_STAGED, DECODE_VALUES):
+                got = to_host(col.decode_range(a, b, shape=shape), vals.dtype)
+                assert np.array_equal(got, vals[a:b]), (width, a, b, shape)
