@@ -34,6 +34,7 @@ extern "C" {
 #define LECO_FIX_EINVAL 1   /* bad argument (size, width, null pointer, capacity) */
 #define LECO_FIX_EHIP 2     /* a HIP runtime call or kernel launch failed */
 #define LECO_FIX_ENOMEM 3   /* device or host allocation failed */
+#define LECO_FIX_ELIMIT 4   /* a filter_range_close block needed more than 65536 periods */
 
 #define LECO_FIX_TAIL_PAD 64
 
@@ -144,6 +145,56 @@ int leco_fix_decode_block_u64(const uint8_t *h_in, uint64_t length, uint64_t *h_
  * to_find of one block. */
 int leco_fix_random_decode_u32(const uint8_t *h_in, uint32_t to_find, uint32_t *h_value);
 int leco_fix_random_decode_u64(const uint8_t *h_in, uint32_t to_find, uint64_t *h_value);
+
+/* ---- predicate pushdown on compressed columns -------------------------- */
+
+/* Leco_int<T>::filter_range (piecewise_fix_integer_template.h:206-249) over
+ * every block of a column: per block, the positions i >= the block's model
+ * threshold whose value is above `filter`, as uint32 column positions, block
+ * by block.  Values compare as the reference compares them (unsigned for
+ * u64, signed int64 for u32).  At most out_capacity positions go to d_out;
+ * d_result[0] receives the total count and d_result[1] flags (bit 0: a block
+ * of filter_range_close needed more than 65536 periods and was cut short).
+ * d_work: leco_fix_filter_workspace_bytes(n, block_size) bytes of device
+ * memory (look-back status words).  Asynchronous on `stream`; n <= 2^32.
+ * Raw blocks (bits >= width) are filtered by their values: the reference reads
+ * theta from their raw bytes, which is undefined. */
+uint64_t leco_fix_filter_workspace_bytes(uint64_t n, uint32_t block_size);
+int leco_fix_filter_range_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t block_size,
+                              uint32_t filter, uint32_t *d_out, uint64_t out_capacity, uint64_t *d_result,
+                              void *d_work, uint64_t work_bytes, void *stream);
+int leco_fix_filter_range_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t block_size,
+                              uint64_t filter, uint32_t *d_out, uint64_t out_capacity, uint64_t *d_result,
+                              void *d_work, uint64_t work_bytes, void *stream);
+/* Leco_int<T>::filter_range_close (:251-303) over every block: block b starts
+ * at the filters d_filter1[b], d_filter2[b] (the reference's driver derives
+ * them from a zonemap) and reports values in (f1, f2), then (f1+base,
+ * f2+base), ... while the block's model reaches them; positions period by
+ * period (the reference overwrites earlier periods' positions, :274; its
+ * count is this count). */
+int leco_fix_filter_range_close_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                    uint32_t block_size, const uint32_t *d_filter1, const uint32_t *d_filter2,
+                                    int32_t base, uint32_t *d_out, uint64_t out_capacity, uint64_t *d_result,
+                                    void *d_work, uint64_t work_bytes, void *stream);
+int leco_fix_filter_range_close_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                    uint32_t block_size, const uint64_t *d_filter1, const uint64_t *d_filter2,
+                                    int32_t base, uint32_t *d_out, uint64_t out_capacity, uint64_t *d_result,
+                                    void *d_work, uint64_t work_bytes, void *stream);
+/* The two member functions on one block in host memory (filter_range(in,
+ * length, filter, out, block_id) and filter_range_close(in, length, out,
+ * block_id, filter1, filter2, base)): positions block_id * block_size + i.
+ * *count receives the number of matches; at most out_capacity are written.
+ * Returns LECO_FIX_ELIMIT when the period limit cut a close filter short. */
+int leco_fix_filter_block_u32(const uint8_t *h_in, uint64_t length, uint32_t filter, uint32_t *h_out,
+                              uint64_t out_capacity, int32_t block_id, uint32_t block_size, uint64_t *count);
+int leco_fix_filter_block_u64(const uint8_t *h_in, uint64_t length, uint64_t filter, uint32_t *h_out,
+                              uint64_t out_capacity, int32_t block_id, uint32_t block_size, uint64_t *count);
+int leco_fix_filter_close_block_u32(const uint8_t *h_in, uint64_t length, uint32_t *h_out, uint64_t out_capacity,
+                                    int32_t block_id, uint32_t block_size, uint32_t filter1, uint32_t filter2,
+                                    int32_t base, uint64_t *count);
+int leco_fix_filter_close_block_u64(const uint8_t *h_in, uint64_t length, uint32_t *h_out, uint64_t out_capacity,
+                                    int32_t block_id, uint32_t block_size, uint64_t filter1, uint64_t filter2,
+                                    int32_t base, uint64_t *count);
 
 /* ---- batches: many columns in one launch ------------------------------ */
 

@@ -10,6 +10,9 @@
 //  - a failing call throws leco_fix_error (the reference reports nothing);
 //  - decodeArray8 decodes a block of bit width 8*sizeof(T)-1 correctly (the
 //    reference copies it as raw bytes, :171-177);
+//  - filter_range / filter_range_close filter raw blocks by their values
+//    (the reference reads theta from the raw bytes, which is undefined), and
+//    filter_range_close writes every period's positions one after another;
 //  - encodeArray8 and get_block_nums (names of the reference's IntegerCODEC,
 //    headers/codecs.h:35,60) are provided: encodeArray8 is encodeArray8_int,
 //    get_block_nums returns the block count given to init().
@@ -94,6 +97,41 @@ public:
         else
             check(leco_fix_random_decode_u64(in, (uint32_t)to_find, &v), "randomdecodeArray8");
         return v;
+    }
+
+    // Positions block_id*block_size + i of the block's values above `filter`
+    // from its model threshold on (:206-249); out needs room for `length`.
+    int filter_range(uint8_t *in, const size_t length, T filter, uint32_t *out, int block_id) {
+        uint64_t count = 0;
+        if constexpr (W == 32)
+            check(leco_fix_filter_block_u32(in, length, filter, out, length, block_id, (uint32_t)block_size,
+                                            &count),
+                  "filter_range");
+        else
+            check(leco_fix_filter_block_u64(in, length, filter, out, length, block_id, (uint32_t)block_size,
+                                            &count),
+                  "filter_range");
+        return (int)count;
+    }
+
+    // Values in (filter1, filter2), (filter1+base, filter2+base), ... within
+    // the block's model reach (:251-303).  Every period's positions are
+    // written one after another (the reference overwrites them at `out`);
+    // out needs room for the returned count (at most length + 1 per period).
+    int filter_range_close(uint8_t *in, const size_t length, uint32_t *out, int block_id, T filter1, T filter2,
+                           int base) {
+        uint64_t count = 0;
+        auto call = [&](uint32_t *dst, uint64_t cap) {
+            if constexpr (W == 32)
+                return leco_fix_filter_close_block_u32(in, length, dst, cap, block_id, (uint32_t)block_size,
+                                                       filter1, filter2, base, &count);
+            else
+                return leco_fix_filter_close_block_u64(in, length, dst, cap, block_id, (uint32_t)block_size,
+                                                       filter1, filter2, base, &count);
+        };
+        check(call(out, length + 1), "filter_range_close");
+        if (count > length + 1) check(call(out, count), "filter_range_close");
+        return (int)count;
     }
 
     uint32_t get_block_nums() { return (uint32_t)blocks; }

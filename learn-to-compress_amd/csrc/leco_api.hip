@@ -181,9 +181,119 @@ int random_decode_host(const uint8_t *h_in, uint32_t to_find, T *h_value) {
     return hip_status(hipStreamSynchronize(s));
 }
 
+template <typename T>
+int filter_column(const uint8_t *d_stream, const uint64_t *d_offs, uint64_t n, uint32_t bs, bool close,
+                  uint64_t filter, const T *f1, const T *f2, int32_t base, uint32_t *d_out, uint64_t cap,
+                  uint64_t *d_result, void *d_work, uint64_t work_bytes, void *st) {
+    if (n == 0 || n > (1ull << 32) || !valid_bs(bs) || !d_stream || !d_offs || !d_result || !d_work ||
+        (cap && !d_out) || (close && (!f1 || !f2)))
+        return LECO_FIX_EINVAL;
+    if (work_bytes < leco_fix_filter_workspace_bytes(n, bs)) return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_filter<T>(d_stream, d_offs, n, bs, close, filter, f1, f2, base, 0, d_out, cap,
+                                             d_result, (uint64_t *)d_work, (hipStream_t)st));
+}
+
+// One block in host memory through the GPU path: staged, filtered as a
+// one-block column with positions from block_id * block_size, copied back.
+template <typename T>
+int filter_block_host(const uint8_t *h_in, uint64_t len, bool close, uint64_t filter, T f1, T f2, int32_t base,
+                      uint32_t *h_out, uint64_t cap, int32_t block_id, uint32_t block_size, uint64_t *count) {
+    if (!h_in || !count || len == 0 || len > kMaxBlockSize || (cap && !h_out)) return LECO_FIX_EINVAL;
+    const uint64_t bytes = host_block_bytes<T>(h_in, len);
+    const uint32_t pos_base = (uint32_t)block_id * block_size;
+    uint64_t dcap = cap < len + 1 ? cap : len + 1;
+    for (;;) {
+        const size_t st_b = up256(bytes + LECO_FIX_TAIL_PAD);
+        const size_t small_b = up256(8 * sizeof(uint64_t));
+        const size_t out_b = up256((dcap ? dcap : 1) * sizeof(uint32_t));
+        int rc = g_stage.reserve(st_b + small_b + out_b);
+        if (rc) return rc;
+        uint8_t *base_p = (uint8_t *)g_stage.dev;
+        uint8_t *d_st = base_p;
+        uint64_t *d_small = (uint64_t *)(base_p + st_b);  // offsets[2], f1, f2, result[2], work[2]
+        uint32_t *d_out = (uint32_t *)(base_p + st_b + small_b);
+        hipStream_t s = g_stage.stream;
+        uint64_t small[8] = {0, bytes, 0, 0, 0, 0, 0, 0};
+        memcpy(&small[2], &f1, sizeof(T));
+        memcpy(&small[3], &f2, sizeof(T));
+        if (hipMemcpyAsync(d_st, h_in, bytes, hipMemcpyHostToDevice, s) != hipSuccess) return LECO_FIX_EHIP;
+        if (hipMemsetAsync(d_st + bytes, 0, st_b - bytes, s) != hipSuccess) return LECO_FIX_EHIP;
+        if (hipMemcpyAsync(d_small, small, sizeof(small), hipMemcpyHostToDevice, s) != hipSuccess)
+            return LECO_FIX_EHIP;
+        if (leco::launch_filter<T>(d_st, d_small, len, (uint32_t)len, close, filter, (const T *)&d_small[2],
+                                   (const T *)&d_small[3], base, pos_base, d_out, dcap, &d_small[4], &d_small[6],
+                                   s) != hipSuccess)
+            return LECO_FIX_EHIP;
+        uint64_t res[2];
+        if (hipMemcpyAsync(res, &d_small[4], sizeof(res), hipMemcpyDeviceToHost, s) != hipSuccess)
+            return LECO_FIX_EHIP;
+        if (hipStreamSynchronize(s) != hipSuccess) return LECO_FIX_EHIP;
+        if (res[0] > dcap && dcap < cap) {  // more matches than staged room: once more with room for all
+            dcap = res[0] < cap ? res[0] : cap;
+            continue;
+        }
+        const uint64_t ncopy = res[0] < cap ? res[0] : cap;
+        if (ncopy && hipMemcpyAsync(h_out, d_out, ncopy * sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+            return LECO_FIX_EHIP;
+        if (hipStreamSynchronize(s) != hipSuccess) return LECO_FIX_EHIP;
+        *count = res[0];
+        return (res[1] & 1) ? LECO_FIX_ELIMIT : LECO_FIX_OK;
+    }
+}
+
 }  // namespace
 
 extern "C" {
+
+uint64_t leco_fix_filter_workspace_bytes(uint64_t n, uint32_t block_size) {
+    if (n == 0 || block_size == 0) return 0;
+    return 8 * (leco::filter_tiles(n, block_size) + 1);
+}
+
+int leco_fix_filter_range_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                              uint32_t filter, uint32_t *d_out, uint64_t cap, uint64_t *d_result, void *d_work,
+                              uint64_t work_bytes, void *stream) {
+    return filter_column<uint32_t>(d_stream, d_offsets, n, bs, false, filter, nullptr, nullptr, 0, d_out, cap,
+                                   d_result, d_work, work_bytes, stream);
+}
+int leco_fix_filter_range_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                              uint64_t filter, uint32_t *d_out, uint64_t cap, uint64_t *d_result, void *d_work,
+                              uint64_t work_bytes, void *stream) {
+    return filter_column<uint64_t>(d_stream, d_offsets, n, bs, false, filter, nullptr, nullptr, 0, d_out, cap,
+                                   d_result, d_work, work_bytes, stream);
+}
+int leco_fix_filter_range_close_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                                    const uint32_t *f1, const uint32_t *f2, int32_t base, uint32_t *d_out,
+                                    uint64_t cap, uint64_t *d_result, void *d_work, uint64_t work_bytes,
+                                    void *stream) {
+    return filter_column<uint32_t>(d_stream, d_offsets, n, bs, true, 0, f1, f2, base, d_out, cap, d_result, d_work,
+                                   work_bytes, stream);
+}
+int leco_fix_filter_range_close_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                                    const uint64_t *f1, const uint64_t *f2, int32_t base, uint32_t *d_out,
+                                    uint64_t cap, uint64_t *d_result, void *d_work, uint64_t work_bytes,
+                                    void *stream) {
+    return filter_column<uint64_t>(d_stream, d_offsets, n, bs, true, 0, f1, f2, base, d_out, cap, d_result, d_work,
+                                   work_bytes, stream);
+}
+int leco_fix_filter_block_u32(const uint8_t *h_in, uint64_t len, uint32_t filter, uint32_t *h_out, uint64_t cap,
+                              int32_t block_id, uint32_t block_size, uint64_t *count) {
+    return filter_block_host<uint32_t>(h_in, len, false, filter, 0, 0, 0, h_out, cap, block_id, block_size, count);
+}
+int leco_fix_filter_block_u64(const uint8_t *h_in, uint64_t len, uint64_t filter, uint32_t *h_out, uint64_t cap,
+                              int32_t block_id, uint32_t block_size, uint64_t *count) {
+    return filter_block_host<uint64_t>(h_in, len, false, filter, 0, 0, 0, h_out, cap, block_id, block_size, count);
+}
+int leco_fix_filter_close_block_u32(const uint8_t *h_in, uint64_t len, uint32_t *h_out, uint64_t cap,
+                                    int32_t block_id, uint32_t block_size, uint32_t f1, uint32_t f2, int32_t base,
+                                    uint64_t *count) {
+    return filter_block_host<uint32_t>(h_in, len, true, 0, f1, f2, base, h_out, cap, block_id, block_size, count);
+}
+int leco_fix_filter_close_block_u64(const uint8_t *h_in, uint64_t len, uint32_t *h_out, uint64_t cap,
+                                    int32_t block_id, uint32_t block_size, uint64_t f1, uint64_t f2, int32_t base,
+                                    uint64_t *count) {
+    return filter_block_host<uint64_t>(h_in, len, true, 0, f1, f2, base, h_out, cap, block_id, block_size, count);
+}
 
 const char *leco_fix_strerror(int code) {
     switch (code) {
@@ -191,6 +301,7 @@ const char *leco_fix_strerror(int code) {
     case LECO_FIX_EINVAL: return "invalid argument";
     case LECO_FIX_EHIP: return "HIP runtime error";
     case LECO_FIX_ENOMEM: return "out of memory";
+    case LECO_FIX_ELIMIT: return "a filter_range_close block needed more than 65536 periods";
     default: return "unknown error";
     }
 }

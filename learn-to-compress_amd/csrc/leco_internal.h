@@ -100,6 +100,21 @@ __host__ __device__ inline uint32_t multi_blocks(uint32_t bs, uint32_t /*w*/) {
     return k ? k : 1;
 }
 
+// Filters (filter_range / filter_range_close) work on tiles of
+// filter_tile_blocks() consecutive blocks (about kFilterTile positions), one
+// workgroup each; a block stops after kFilterMaxPeriods periods of
+// filter_range_close (flag 1 of the result).
+constexpr uint32_t kFilterTile = 8192;
+constexpr uint32_t kFilterMaxPeriods = 1u << 16;
+__host__ __device__ inline uint32_t filter_tile_blocks(uint32_t bs) {
+    const uint32_t k = kFilterTile / bs;
+    return k < 1 ? 1 : (k > 256 ? 256 : k);
+}
+__host__ __device__ inline uint64_t filter_tiles(uint64_t n, uint32_t bs) {
+    const uint64_t nb = (n + bs - 1) / bs, kb = filter_tile_blocks(bs);
+    return (nb + kb - 1) / kb;
+}
+
 // Each launcher enqueues on `st` and returns the hipGetLastError() of the launch.
 template <typename T>
 hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t *offs,
@@ -139,6 +154,15 @@ hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, c
                                const leco_fix_block_param *prm, uint32_t grid, hipStream_t st);
 hipError_t launch_tile_columns(const uint64_t *tile_first, uint32_t ncols, uint64_t tiles, uint32_t *tile_col,
                                hipStream_t st);
+// Filter over a column: filter_range (close = false: value > filter) or
+// filter_range_close (per-block starting filters f1[b], f2[b], period base).
+// Positions pos_base + global index (uint32) go to out[0..cap) in the
+// reference's order; result[0] = count, result[1] = flags; work holds
+// filter_tiles(n, bs) + 1 words.
+template <typename T>
+hipError_t launch_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, bool close,
+                         uint64_t filter, const T *f1, const T *f2, int32_t base, uint32_t pos_base, uint32_t *out,
+                         uint64_t cap, uint64_t *result, uint64_t *work, hipStream_t st);
 // Served batch: `grid` workgroups stride over nitems items.
 hipError_t launch_serve(const ServeColumn *cols, const ServeItem *items, uint32_t nitems, uint32_t grid,
                         hipStream_t st);

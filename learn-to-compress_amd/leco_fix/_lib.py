@@ -18,6 +18,7 @@ c_ptr = ctypes.c_void_p
 c_int = ctypes.c_int
 
 LECO_FIX_OK = 0
+LECO_FIX_ELIMIT = 4
 LECO_FIX_TAIL_PAD = 64
 
 
@@ -52,7 +53,18 @@ for _w in ("u32", "u64"):
         f"leco_fix_encode_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr, c_u64, ctypes.POINTER(c_u64)]),
         f"leco_fix_decode_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr]),
         f"leco_fix_random_decode_{_w}": (c_int, [c_ptr, c_u32, c_ptr]),
+        f"leco_fix_filter_range_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_u32 if _w == "u32" else c_u64, c_ptr,
+                                                c_u64, c_ptr, c_ptr, c_u64, c_ptr]),
+        f"leco_fix_filter_range_close_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_ptr, ctypes.c_int32,
+                                                      c_ptr, c_u64, c_ptr, c_ptr, c_u64, c_ptr]),
+        f"leco_fix_filter_block_{_w}": (c_int, [c_ptr, c_u64, c_u32 if _w == "u32" else c_u64, c_ptr, c_u64,
+                                                ctypes.c_int32, c_u32, ctypes.POINTER(c_u64)]),
+        f"leco_fix_filter_close_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr, c_u64, ctypes.c_int32, c_u32,
+                                                      c_u32 if _w == "u32" else c_u64,
+                                                      c_u32 if _w == "u32" else c_u64, ctypes.c_int32,
+                                                      ctypes.POINTER(c_u64)]),
     })
+SIGNATURES["leco_fix_filter_workspace_bytes"] = (c_u64, [c_u64, c_u32])
 
 
 

@@ -124,6 +124,59 @@ class Column:
                  idx.numel(), out.data_ptr(), _stream_handle(stream)), "lookup")
         return out
 
+    def _filter(self, close: bool, filt, f1, f2, base: int, out, capacity, stream):
+        L = lib()
+        sfx = "u32" if self.width == 32 else "u64"
+        dev = self.stream.device
+        if out is None:
+            cap = self.n if capacity is None else capacity
+            out = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+        else:
+            cap = out.numel() if capacity is None else min(capacity, out.numel())
+        result = torch.zeros(2, dtype=torch.int64, device=dev)
+        wb = int(L.leco_fix_filter_workspace_bytes(self.n, self.block_size))
+        work = torch.empty(wb, dtype=torch.uint8, device=dev)
+        if close:
+            st = _STORAGE[self.width]
+            f1 = _as_storage(f1.to(device=dev).contiguous()) if f1.dtype != st else f1.contiguous()
+            f2 = _as_storage(f2.to(device=dev).contiguous()) if f2.dtype != st else f2.contiguous()
+            if f1.numel() != self.num_blocks or f2.numel() != self.num_blocks:
+                raise ValueError("filter_range_close takes one starting filter pair per block")
+            rc = getattr(L, f"leco_fix_filter_range_close_{sfx}")(
+                self.stream.data_ptr(), self.offsets.data_ptr(), self.n, self.block_size, f1.data_ptr(),
+                f2.data_ptr(), base, out.data_ptr(), cap, result.data_ptr(), work.data_ptr(), wb,
+                _stream_handle(stream))
+        else:
+            rc = getattr(L, f"leco_fix_filter_range_{sfx}")(
+                self.stream.data_ptr(), self.offsets.data_ptr(), self.n, self.block_size, int(filt), out.data_ptr(),
+                cap, result.data_ptr(), work.data_ptr(), wb, _stream_handle(stream))
+        check(rc, "filter_range_close" if close else "filter_range")
+        return out, result
+
+    def filter_range(self, filt: int, out: torch.Tensor | None = None, capacity: int | None = None,
+                     stream=None) -> torch.Tensor:
+        """Leco_int<T>::filter_range over every block: uint32 positions (as int32) of values above `filt`,
+        from each block's model threshold on.  Synchronizes to size the result."""
+        out, result = self._filter(False, filt, None, None, 0, out, capacity, stream)
+        count = int(result[0].item())
+        if count > out.numel():
+            raise ValueError(f"filter_range: {count} matches, room for {out.numel()}")
+        return out[:count]
+
+    def filter_range_close(self, filter1: torch.Tensor, filter2: torch.Tensor, base: int,
+                           out: torch.Tensor | None = None, capacity: int | None = None,
+                           stream=None) -> torch.Tensor:
+        """Leco_int<T>::filter_range_close over every block with per-block starting filters
+        (see close_filters_from_zonemap); positions block by block, period by period."""
+        from ._lib import LECO_FIX_ELIMIT, LecoError
+        out, result = self._filter(True, 0, filter1, filter2, int(base), out, capacity, stream)
+        count, flags = (int(v) for v in result.tolist())
+        if flags & 1:
+            raise LecoError(LECO_FIX_ELIMIT, "filter_range_close")
+        if count > out.numel():
+            out, result = self._filter(True, 0, filter1, filter2, int(base), None, count, stream)
+        return out[:count]
+
     def rebuild_params(self, stream=None) -> torch.Tensor:
         """Param table recomputed from the block headers (leco_fix_build_params)."""
         prm = torch.empty_like(self.params)
@@ -131,3 +184,39 @@ class Column:
                                           self.block_size, self.width, prm.data_ptr(), _stream_handle(stream)),
               "build_params")
         return prm
+
+
+def close_filters_from_zonemap(block_min: torch.Tensor, filter1: int, filter2: int, base: int, width: int):
+    """Per-block starting filters of the reference's close-filter driver
+    (experiments/leco_int_fix_template.cpp:293-298): from each block's minimum
+    m, k = ceil((double)(T)(m - f1) / base) periods on, one period earlier
+    when m % base lies inside (f1, f2).  Float64 arithmetic and T wrap as the
+    driver's C++ expressions; minimums, filters and base below 2^63, base > 0.
+    Returns (f1[], f2[]) in the column's storage dtype."""
+    if base <= 0:
+        raise ValueError("base must be positive")
+    mn = block_min.to(torch.int64)
+    if width == 64 and bool((mn < 0).any()):
+        raise ValueError("block minimums at or above 2^63 are not supported")
+    wmask = (1 << width) - 1
+    d = mn - filter1
+    if width == 32:
+        d = (d & wmask).to(torch.float64)
+    else:
+        d = d.to(torch.float64)
+        d = torch.where(d < 0, d + 2.0 ** 64, d)               # (uint64)(m - f1)
+    k = torch.ceil(d / float(base)) * float(base)
+    t1 = _to_uint(k + float(filter1))
+    t2 = _to_uint(k + float(filter2))
+    r = mn % base
+    inside = (r < filter2) & (r > filter1)
+    t1 = torch.where(inside, t1 - base, t1)
+    t2 = torch.where(inside, t2 - base, t2)
+    if width == 32:
+        return (t1 & wmask).to(torch.int64).to(torch.int32), (t2 & wmask).to(torch.int64).to(torch.int32)
+    return t1, t2
+
+
+def _to_uint(x: torch.Tensor) -> torch.Tensor:
+    """double -> unsigned integer by truncation, as int64 bits (values in [2^63, 2^64) wrap like uint64)."""
+    return torch.trunc(torch.where(x >= 2.0 ** 63, x - 2.0 ** 64, x)).to(torch.int64)

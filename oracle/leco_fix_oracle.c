@@ -382,3 +382,202 @@ ORACLE_API void leco_oracle_lookup_column(const uint8_t *stream, const uint64_t 
     p.idx = idx; p.nq = nq; p.out = out;
     run_jobs(ra_worker, p, nq, threads);
 }
+
+/* ---- predicate pushdown: filter_range / filter_range_close ----------------
+ * Leco_int<T>::filter_range (:206-249) and filter_range_close (:251-303),
+ * with read_all_bit_fix_range<T> / read_all_bit_fix_range_close<T>
+ * (bit_read.h:387-441, 443-497).  Each returns the positions block_start + i
+ * of matching values, in the reference's order, and their count.  The
+ * reference's types decide the arithmetic: thresholds are computed in the
+ * integer type of (filter +/- delta) -- uint64 (wrapping) for T = uint64,
+ * int64 for T = uint32 (uint32 + 1 wraps first in filter_range) -- then
+ * converted to double; `int thre = std::max(...)` is x86 CVTTSD2SI r32 (NaN
+ * or out of range -> INT_MIN); decoded values are the int64
+ * cvtt64(fma(i,t1,t0)) +/- magnitude, compared unsigned for T = uint64 and
+ * signed int64 for T = uint32.
+ * Deviations (the reference is undefined there): raw blocks (l >= w) are
+ * filtered by their stored values, in position order, instead of reading
+ * theta from the raw bytes and shifting by w; filter_range_close writes the
+ * positions of every period after the previous ones (the reference passes the
+ * same `out` to each period, :274, so only its count is defined) and gives up
+ * with status 1 after LECO_ORACLE_MAX_PERIODS periods in one block. */
+
+#define LECO_ORACLE_MAX_PERIODS (1u << 16)
+
+static int32_t cvtt32(double x)
+{
+    if (!(x >= -2147483648.0 && x < 2147483648.0)) return INT32_MIN;
+    return (int32_t)x;
+}
+static double std_max(double a, double b) { return a < b ? b : a; } /* std::max(a, b) */
+static double std_min(double a, double b) { return b < a ? b : a; } /* std::min(a, b) */
+
+static inline int pred_gt(int width, uint64_t dv, uint64_t f)
+{
+    return width == 64 ? dv > f : (int64_t)dv > (int64_t)f;
+}
+static inline int pred_lt(int width, uint64_t dv, uint64_t f)
+{
+    return width == 64 ? dv < f : (int64_t)dv < (int64_t)f;
+}
+
+/* Decoded value of position i of a non-raw block as the filter loops form it
+   (int64 bits): bit_read.h:411-420 for l > 0, :237 / :285 for l == 0. */
+static inline uint64_t filter_value(const uint8_t *in, int l, double t0, double t1, uint64_t i)
+{
+    int64_t p = predict(t0, t1, i);
+    return l ? apply_residual(p, read_field(in + 17, i * (uint64_t)l, l), l) : (uint64_t)p;
+}
+
+static inline uint64_t emit(uint32_t *out, uint64_t cap, uint64_t cnt, uint32_t pos)
+{
+    if (cnt < cap) out[cnt] = pos;
+    return cnt + 1;
+}
+
+/* Values the raw-block deviation looks at: m, or 1 / 2 for the one- and
+   two-value blocks l = 127 / 126 of decodeArray8 (:152-170). */
+static inline uint32_t raw_count(int l, uint32_t m)
+{
+    if (l == 127) return m < 1 ? m : 1;
+    if (l == 126) return m < 2 ? m : 2;
+    return m;
+}
+
+ORACLE_API uint64_t leco_oracle_filter_block(const uint8_t *in, int width, uint32_t m, uint64_t filter,
+                                             uint32_t block_start, uint32_t *out, uint64_t cap)
+{
+    uint64_t wmask = width == 32 ? 0xFFFFFFFFull : ~0ull;
+    filter &= wmask;
+    int l = in[0];
+    uint64_t cnt = 0;
+    if (l >= width) {
+        uint32_t mm = raw_count(l, m);
+        for (uint32_t i = 0; i < mm; i++) {
+            uint64_t v = 0;
+            memcpy(&v, in + 1 + (uint64_t)i * (width / 8), width / 8);
+            if (v > filter) cnt = emit(out, cap, cnt, block_start + i);
+        }
+        return cnt;
+    }
+    double t0, t1;
+    memcpy(&t0, in + 1, 8);
+    memcpy(&t1, in + 9, 8);
+    int64_t delta = l ? (int64_t)1 << (l - 1) : 0;
+    double x;
+    if (width == 64) x = (double)(filter + 1 - (uint64_t)delta) - t0;
+    else x = (double)((int64_t)(uint32_t)(filter + 1) - delta) - t0;
+    int32_t thre = cvtt32(std_max(x / t1, 0.0));
+    if ((uint64_t)(int64_t)thre >= m) return 0;
+    for (uint32_t i = (uint32_t)thre; i < m; i++)
+        if (pred_gt(width, filter_value(in, l, t0, t1, i), filter)) cnt = emit(out, cap, cnt, block_start + i);
+    return cnt;
+}
+
+/* Raw-block deviation of filter_range_close: position order, each position
+   once, if its value lies in (f1 + k*base, f2 + k*base) for some k >= 0
+   (k = 0 only when base <= 0). */
+static int raw_close_match(uint64_t v, uint64_t f1, uint64_t f2, int32_t base)
+{
+    if (v <= f1) return 0;
+    uint64_t d = v - f1, span = f2 > f1 ? f2 - f1 : 0;
+    if (base <= 0) return v < f2;
+    uint64_t r = (d - 1) % (uint64_t)base + 1; /* d - k*base for the largest k with k*base < d */
+    return r < span;
+}
+
+ORACLE_API int leco_oracle_filter_close_block(const uint8_t *in, int width, uint32_t m, uint64_t f1, uint64_t f2,
+                                              int32_t base, uint32_t block_start, uint32_t *out, uint64_t cap,
+                                              uint64_t *count)
+{
+    uint64_t wmask = width == 32 ? 0xFFFFFFFFull : ~0ull;
+    f1 &= wmask;
+    f2 &= wmask;
+    int l = in[0];
+    uint64_t cnt = 0;
+    *count = 0;
+    if (l >= width) {
+        uint32_t mm = raw_count(l, m);
+        for (uint32_t i = 0; i < mm; i++) {
+            uint64_t v = 0;
+            memcpy(&v, in + 1 + (uint64_t)i * (width / 8), width / 8);
+            if (raw_close_match(v, f1, f2, base)) cnt = emit(out, cap, cnt, block_start + i);
+        }
+        *count = cnt;
+        return 0;
+    }
+    double t0, t1;
+    memcpy(&t0, in + 1, 8);
+    memcpy(&t1, in + 9, 8);
+    int64_t delta = l ? (int64_t)1 << (l - 1) : 0;
+    uint64_t bstep = (uint64_t)(int64_t)base & wmask; /* T += int */
+    uint32_t periods = 0;
+    for (;;) {
+        double x1, x2;
+        if (width == 64) {
+            x1 = (double)(f1 - (uint64_t)delta) - t0;
+            x2 = (double)(f2 + (uint64_t)delta) - t0;
+        } else {
+            x1 = (double)((int64_t)f1 - delta) - t0;
+            x2 = (double)((int64_t)f2 + delta) - t0;
+        }
+        int32_t thre1 = cvtt32(std_max(x1 / t1 - 1, 0.0));
+        int32_t thre2 = cvtt32(std_min(x2 / t1 + 1, (double)m));
+        if (!((uint64_t)(int64_t)thre1 < m && thre2 > 0)) break;
+        if (++periods > LECO_ORACLE_MAX_PERIODS) { *count = cnt; return 1; }
+        if (l) {
+            for (int64_t i = thre1; i < thre2; i++) {
+                uint64_t dv = filter_value(in, l, t0, t1, (uint64_t)i);
+                if (pred_gt(width, dv, f1) && pred_lt(width, dv, f2))
+                    cnt = emit(out, cap, cnt, block_start + (uint32_t)i);
+            }
+        } else {
+            for (int64_t i = thre1; i <= thre2; i++) {
+                uint64_t dv = (uint64_t)predict(t0, t1, (uint64_t)i);
+                if (pred_gt(width, dv, f1) && pred_lt(width, dv, f2))
+                    cnt = emit(out, cap, cnt, block_start + (uint32_t)i);
+            }
+        }
+        f1 = (f1 + bstep) & wmask;
+        f2 = (f2 + bstep) & wmask;
+    }
+    *count = cnt;
+    return 0;
+}
+
+/* The driver's filter loops (experiments/leco_int_fix_template.cpp:238-252,
+   283-312) over every block of a column, block_start = b * bs as uint32.
+   Positions go to out[0..cap) in block order; *count gets the total. */
+ORACLE_API void leco_oracle_filter_column(const uint8_t *stream, const uint64_t *offsets, int width, uint64_t n,
+                                          uint32_t bs, uint64_t filter, uint32_t *out, uint64_t cap,
+                                          uint64_t *count)
+{
+    uint64_t nb = (n + bs - 1) / bs, total = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+        uint64_t s = b * bs, m = n - s < bs ? n - s : bs;
+        uint64_t room = total < cap ? cap - total : 0;
+        total += leco_oracle_filter_block(stream + offsets[b], width, (uint32_t)m, filter, (uint32_t)s,
+                                          out + (total < cap ? total : 0), room);
+    }
+    *count = total;
+}
+
+/* As above for filter_range_close with per-block starting filters f1[b],
+   f2[b] (the driver derives them from a zonemap, :293-298).  Returns 0, or 1
+   if a block needed more than LECO_ORACLE_MAX_PERIODS periods. */
+ORACLE_API int leco_oracle_filter_close_column(const uint8_t *stream, const uint64_t *offsets, int width,
+                                               uint64_t n, uint32_t bs, const uint64_t *f1, const uint64_t *f2,
+                                               int32_t base, uint32_t *out, uint64_t cap, uint64_t *count)
+{
+    uint64_t nb = (n + bs - 1) / bs, total = 0;
+    int st = 0;
+    for (uint64_t b = 0; b < nb && !st; b++) {
+        uint64_t s = b * bs, m = n - s < bs ? n - s : bs, c = 0;
+        uint64_t room = total < cap ? cap - total : 0;
+        st = leco_oracle_filter_close_block(stream + offsets[b], width, (uint32_t)m, f1[b], f2[b], base,
+                                            (uint32_t)s, out + (total < cap ? total : 0), room, &c);
+        total += c;
+    }
+    *count = total;
+    return st;
+}

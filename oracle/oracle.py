@@ -59,6 +59,23 @@ def lib() -> ctypes.CDLL:
                                                 ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
                                                 ctypes.c_void_p, ctypes.c_int]
         L.leco_oracle_lookup_column.restype = None
+        L.leco_oracle_filter_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64,
+                                               ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64]
+        L.leco_oracle_filter_block.restype = ctypes.c_uint64
+        L.leco_oracle_filter_close_block.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32,
+                                                     ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int32,
+                                                     ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint64,
+                                                     ctypes.POINTER(ctypes.c_uint64)]
+        L.leco_oracle_filter_close_block.restype = ctypes.c_int
+        L.leco_oracle_filter_column.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64,
+                                                ctypes.c_uint32, ctypes.c_uint64, ctypes.c_void_p,
+                                                ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+        L.leco_oracle_filter_column.restype = None
+        L.leco_oracle_filter_close_column.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                      ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
+                                                      ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                                      ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+        L.leco_oracle_filter_close_column.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -151,3 +168,67 @@ def lookup_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, idx: np
     lib().leco_oracle_lookup_column(_ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs, _ptr(idx), len(idx),
                                     _ptr(out), threads)
     return out
+
+
+def _padded(stream: np.ndarray, offs: np.ndarray) -> np.ndarray:
+    stream = np.ascontiguousarray(stream, np.uint8)
+    if len(stream) >= int(offs[-1]) + 64:
+        return stream
+    return np.concatenate([stream, np.zeros(64, np.uint8)])
+
+
+def filter_block(blk: bytes, n: int, dtype, filt: int, block_start: int = 0) -> np.ndarray:
+    """Leco_int<T>::filter_range(blk, n, filt, out, block_id) positions (block_start = block_id * bs)."""
+    dtype = np.dtype(dtype)
+    src = np.frombuffer(bytes(blk) + bytes(64), np.uint8)
+    out = np.zeros(max(n, 1), np.uint32)
+    c = lib().leco_oracle_filter_block(_ptr(src), 8 * dtype.itemsize, n, filt, block_start, _ptr(out), len(out))
+    return out[:c].copy()
+
+
+def filter_close_block(blk: bytes, n: int, dtype, f1: int, f2: int, base: int, block_start: int = 0) -> np.ndarray:
+    """Leco_int<T>::filter_range_close(blk, n, out, block_id, f1, f2, base) positions, every period's in order."""
+    dtype = np.dtype(dtype)
+    src = np.frombuffer(bytes(blk) + bytes(64), np.uint8)
+    cap = 4 * n + 16
+    while True:
+        out = np.zeros(cap, np.uint32)
+        cnt = ctypes.c_uint64()
+        st = lib().leco_oracle_filter_close_block(_ptr(src), 8 * dtype.itemsize, n, f1, f2, base, block_start,
+                                                  _ptr(out), cap, ctypes.byref(cnt))
+        if st:
+            raise RuntimeError("filter_close_block: too many periods in one block")
+        if cnt.value <= cap:
+            return out[:cnt.value].copy()
+        cap = cnt.value
+
+
+def filter_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, dtype, filt: int) -> np.ndarray:
+    """filter_range over every block of a column (positions in block order)."""
+    dtype = np.dtype(dtype)
+    padded = _padded(stream, offs)
+    cnt = ctypes.c_uint64()
+    out = np.zeros(n, np.uint32)
+    lib().leco_oracle_filter_column(_ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs, filt, _ptr(out), n,
+                                    ctypes.byref(cnt))
+    return out[:cnt.value].copy()
+
+
+def filter_close_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, dtype, f1: np.ndarray,
+                        f2: np.ndarray, base: int) -> np.ndarray:
+    """filter_range_close over every block with per-block starting filters f1[b], f2[b]."""
+    dtype = np.dtype(dtype)
+    padded = _padded(stream, offs)
+    f1 = np.ascontiguousarray(f1, np.uint64)
+    f2 = np.ascontiguousarray(f2, np.uint64)
+    cap = n + len(offs)
+    while True:
+        out = np.zeros(cap, np.uint32)
+        cnt = ctypes.c_uint64()
+        st = lib().leco_oracle_filter_close_column(_ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs, _ptr(f1),
+                                                   _ptr(f2), base, _ptr(out), cap, ctypes.byref(cnt))
+        if st:
+            raise RuntimeError("filter_close_column: too many periods in one block")
+        if cnt.value <= cap:
+            return out[:cnt.value].copy()
+        cap = cnt.value
