@@ -74,7 +74,75 @@ def cpu_baseline(vals_dev, bs: int, sample: int, repeat: int):
         oracle.decode_column(padded, offs, len(y), bs, np.uint64, threads=nthr, out=out)
     t1 = time.perf_counter()
     gbsn = len(y) * 8 * repeat / (t1 - t0) / 1e9
-    return gbs1, gbsn, nthr
+    # filter_range at the sample's median, 1 thread
+    f = int(y[len(y) // 2])
+    oracle.filter_column(padded, offs, len(y), bs, np.uint64, f)
+    t0 = time.perf_counter()
+    for _ in range(max(1, repeat // 2)):
+        got = oracle.filter_column(padded, offs, len(y), bs, np.uint64, f)
+    t1 = time.perf_counter()
+    assert len(got) == int(np.count_nonzero(y > np.uint64(f)))
+    gbsf = len(y) * 8 * max(1, repeat // 2) / (t1 - t0) / 1e9
+    return gbs1, gbsn, nthr, gbsf
+
+
+def filter_section(torch, vals, col, reps: int) -> dict:
+    """filter_range at the median key and filter_range_close with the reference
+    driver's zonemap-derived per-block filters, on the resident column; one
+    launch each, preallocated output and workspace, HIP events."""
+    from leco_fix._lib import lib
+    from leco_fix.column import _stream_handle, close_filters_from_zonemap
+
+    L = lib()
+    n, bs = col.n, col.block_size
+    out = torch.empty(n + col.num_blocks, dtype=torch.int32, device="cuda")
+    res = torch.zeros(2, dtype=torch.int64, device="cuda")
+    wb = int(L.leco_fix_filter_workspace_bytes(n, bs))
+    work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+    fmed = int(vals[n // 2].item())
+
+    def gt():
+        L.leco_fix_filter_range_u64(col.stream.data_ptr(), col.offsets.data_ptr(), n, bs, fmed, out.data_ptr(),
+                                    out.numel(), res.data_ptr(), work.data_ptr(), wb,
+                                    _stream_handle(None))
+    ms_gt, _ = timed(torch, gt, reps, 1)
+    cnt = int(res[0].item())
+    expect = int((vals > fmed).sum().item())
+    ok_gt = cnt == expect and (cnt == 0 or int(out[0].item()) == n - cnt)
+
+    # period 2^20 over keys up to ~2^45: a few hundred periods per block
+    base = 1 << 20
+    f1, f2 = base // 4, base // 2
+    nb = col.num_blocks
+    full = (n // bs) * bs
+    mins = vals[:full].view(-1, bs).amin(dim=1)
+    if full < n:
+        mins = torch.cat([mins, vals[full:].amin().view(1)])
+    t1, t2 = close_filters_from_zonemap(mins, f1, f2, base, 64)
+
+    def close():
+        L.leco_fix_filter_range_close_u64(col.stream.data_ptr(), col.offsets.data_ptr(), n, bs, t1.data_ptr(),
+                                          t2.data_ptr(), base, out.data_ptr(), out.numel(), res.data_ptr(),
+                                          work.data_ptr(), wb, _stream_handle(None))
+    ms_close, _ = timed(torch, close, reps, 1)
+    ccnt, flags = (int(v) for v in res.tolist())
+    r = vals % base
+    truecount = int(((r > f1) & (r < f2)).sum().item())
+    del r
+    raw = n * 8
+    comp = col.nbytes
+    return {
+        "filter_range": {"filter": fmed, "matches": cnt, "ms": round(ms_gt, 4),
+                         "gbs": round(raw / (ms_gt * 1e-3) / 1e9, 1),
+                         "algorithmic_gbs": round((comp + 4 * cnt) / (ms_gt * 1e-3) / 1e9, 1),
+                         "verified": ok_gt},
+        "filter_range_close": {"filter1": f1, "filter2": f2, "base": base, "matches": ccnt,
+                               "driver_truecount": truecount, "flags": flags, "ms": round(ms_close, 4),
+                               "gbs": round(raw / (ms_close * 1e-3) / 1e9, 1),
+                               "algorithmic_gbs": round((comp + 4 * ccnt) / (ms_close * 1e-3) / 1e9, 1)},
+        "note": "gbs = uncompressed bytes covered / time; algorithmic_gbs = (compressed stream + 4 B per match) / "
+                "time, an upper bound on the bytes the scanned windows need",
+    }
 
 
 def main() -> int:
@@ -91,6 +159,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-reps", type=int, default=3,
                     help="plain device copies of the decoded column (a known-size HBM reference kernel)")
+    ap.add_argument("--no-filter", action="store_true", help="skip the filter_range / filter_range_close section")
+    ap.add_argument("--filter-reps", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-column section")
     ap.add_argument("--batched-cols", type=int, default=800)
     ap.add_argument("--batched-max", type=float, default=1e8, help="largest column length (log-uniform from 1e3)")
@@ -153,6 +223,12 @@ def main() -> int:
     log("lookup timing")
     ms_lk, _ = timed(torch, lambda: col.lookup(idx, out=lk), 5, 1)
 
+    filt = None
+    if not args.no_filter:
+        log("filter timing")
+        filt = filter_section(torch, vals, col, args.filter_reps)
+        log(f"filter: {filt}")
+
     copy_gbs = None
     if args.copy_reps > 0:
         log("device copy reference")
@@ -184,11 +260,12 @@ def main() -> int:
     if not args.no_cpu:
         sample = min(args.cpu_sample, n)
         log(f"cpu baseline: oracle decode of the first {sample} values, {args.cpu_repeat} repeats")
-        gbs1, gbsn, nthr = cpu_baseline(vals, bs, sample, args.cpu_repeat)
+        gbs1, gbsn, nthr, gbsf = cpu_baseline(vals, bs, sample, args.cpu_repeat)
         cpu = {"value": round(gbs1, 3), "unit": "GB/s", "cores": 1, "kind": "port",
                "sample": f"first {sample} values of the column ({(sample + bs - 1) // bs} blocks), "
                          f"decoded {args.cpu_repeat}x by the C oracle, 1 thread",
-               "all_cores_value": round(gbsn, 3), "all_cores": nthr}
+               "all_cores_value": round(gbsn, 3), "all_cores": nthr,
+               "filter_range_value": round(gbsf, 3)}
 
     traffic, traffic_src = None, None
     pmc_file = os.path.join(ROOT, "profiles", f"decode_pwl{n // 1_000_000}M_u64_bs{bs}_pmc.json")
@@ -240,6 +317,7 @@ def main() -> int:
             "bytes_per_launch": comp + raw_bytes,
         },
         "cpu_baseline": cpu,
+        "filter": filt,
         "batched": batched,
         "serving": serving,
         "churn": churn,

@@ -38,11 +38,22 @@ __device__ __forceinline__ bool below(uint64_t v, uint64_t f) {
     return W == 64 ? v < f : (int64_t)v < (int64_t)f;
 }
 
+constexpr int kWaves = kThreads / kWave;
+
 struct FilterShared {
-    uint64_t wsum[kThreads / kWave];
-    uint32_t wcnt[kThreads / kWave];
+    uint64_t wsum[kWaves];
+    uint32_t wcnt[kWaves];
+    uint32_t wfail[kWaves];
+    uint32_t stop_passing;  // the period the close loop stopped at still passed its test (limit cut)
     uint64_t base;
     uint64_t tile;
+    // one chunk of kThreads periods of filter_range_close
+    uint64_t pf1[kThreads], pf2[kThreads];
+    uint32_t plo[kThreads];
+    uint32_t ppre[kThreads + 1];
+    // per block of the tile: the close loop's period count and whether pass 1 chose the scan
+    uint32_t bK[256];
+    uint8_t bscan[256];
 };
 
 // Arguments of one filter launch.
@@ -63,59 +74,7 @@ struct FilterArgs {
     uint64_t *work;         // [0] = ticket, [1 + t] = tile t's look-back status
 };
 
-// Visits the windows of block b: visit(lo, hi, raw, l, t0, t1, f1, f2) for
-// each window of positions [lo, hi) with its predicate bounds (filter_range:
-// value > f1; close: f1 < value < f2; raw blocks: see the raw test below).
-// Returns false if the block needed more than kFilterMaxPeriods periods.
-template <typename T, int W, bool CLOSE, class Visit>
-__device__ __forceinline__ bool block_windows(const FilterArgs &a, uint64_t b, uint32_t m, Visit visit) {
-    constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
-    const uint8_t *blk = a.stream + a.offs[b];
-    const int l = blk[0];
-    uint64_t f1 = CLOSE ? (uint64_t)((const T *)a.f1)[b] : (a.filter & wmask);
-    uint64_t f2 = CLOSE ? (uint64_t)((const T *)a.f2)[b] : 0;
-    if (l >= W) {
-        // raw blocks (and the one-/two-value blocks 127/126): every stored value
-        const uint32_t mm = l == 127 ? (m < 1 ? m : 1) : l == 126 ? (m < 2 ? m : 2) : m;
-        visit(0u, mm, true, l, 0.0, 0.0, f1, f2);
-        return true;
-    }
-    const double t0 = bits_to_double(load_u64_any(blk + 1)), t1 = bits_to_double(load_u64_any(blk + 9));
-    const int64_t delta = l ? (int64_t)1 << (l - 1) : 0;
-    if (!CLOSE) {
-        // :225-229
-        double x;
-        if (W == 64) x = (double)(f1 + 1 - (uint64_t)delta) - t0;
-        else x = (double)((int64_t)(uint32_t)(f1 + 1) - delta) - t0;
-        const int32_t thre = cvtt32(std_max(x / t1, 0.0));
-        if ((uint64_t)(int64_t)thre >= m) return true;
-        visit((uint32_t)thre, m, false, l, t0, t1, f1, f2);
-        return true;
-    }
-    // :269-300: windows [thre1, thre2) (l > 0) or [thre1, thre2] (l == 0),
-    // filters advanced by `base` after each
-    const uint64_t step = (uint64_t)(int64_t)a.base & wmask;
-    for (uint32_t periods = 0;; ++periods) {
-        double x1, x2;
-        if (W == 64) {
-            x1 = (double)(f1 - (uint64_t)delta) - t0;
-            x2 = (double)(f2 + (uint64_t)delta) - t0;
-        } else {
-            x1 = (double)((int64_t)f1 - delta) - t0;
-            x2 = (double)((int64_t)f2 + delta) - t0;
-        }
-        const int32_t thre1 = cvtt32(std_max(x1 / t1 - 1, 0.0));
-        const int32_t thre2 = cvtt32(std_min(x2 / t1 + 1, (double)m));
-        if (!((uint64_t)(int64_t)thre1 < m && thre2 > 0)) return true;
-        if (periods >= kFilterMaxPeriods) return false;
-        const uint32_t hi = l ? (uint32_t)thre2 : (uint32_t)thre2 + 1;
-        if ((uint32_t)thre1 < hi) visit((uint32_t)thre1, hi, false, l, t0, t1, f1, f2);
-        f1 = (f1 + step) & wmask;
-        f2 = (f2 + step) & wmask;
-    }
-}
-
-// Does position i of block b (header at blk) pass?
+// Does position i of the block at blk pass?
 template <int W, bool CLOSE>
 __device__ __forceinline__ bool passes(const uint8_t *blk, uint32_t i, bool raw, int l, double t0, double t1,
                                        uint64_t f1, uint64_t f2, int32_t base) {
@@ -131,6 +90,317 @@ __device__ __forceinline__ bool passes(const uint8_t *blk, uint32_t i, bool raw,
     const int64_t p = predict(t0, t1, i);
     const uint64_t v = l ? apply_field(p, load_bits(blk + 17, (uint64_t)i * l, l), l) : (uint64_t)p;
     return CLOSE ? (above<W>(v, f1) && below<W>(v, f2)) : above<W>(v, f1);
+}
+
+// One round of kThreads candidates in order: pass 1 (WRITE = false) adds
+// this thread's hit to acc; pass 2 writes the hits' positions at acc (the
+// workgroup's running output slot, the same in every thread) and advances it.
+template <bool WRITE>
+__device__ __forceinline__ void emit_round(const FilterArgs &a, FilterShared &sh, bool hit, uint32_t position,
+                                           uint64_t &acc) {
+    if (!WRITE) {
+        acc += hit;
+        return;
+    }
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    const uint64_t bal = __ballot(hit);
+    if (lane == 0) sh.wcnt[wid] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint64_t at = acc;
+    uint32_t round = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        if (w < wid) at += sh.wcnt[w];
+        round += sh.wcnt[w];
+    }
+    at += (uint64_t)__popcll(bal & ((1ull << lane) - 1));
+    if (hit && at < a.cap) a.out[at] = position;
+    acc += round;
+    __syncthreads();
+}
+
+// Positions [lo, hi) of one window against one predicate.
+template <int W, bool CLOSE, bool WRITE>
+__device__ __forceinline__ void scan_window(const FilterArgs &a, FilterShared &sh, const uint8_t *blk, uint32_t lo,
+                                            uint32_t hi, bool raw, int l, double t0, double t1, uint64_t f1,
+                                            uint64_t f2, uint32_t bstart, uint64_t &acc) {
+    const int tid = threadIdx.x;
+    if (!WRITE) {
+        for (uint32_t i = lo + tid; i < hi; i += kThreads)
+            acc += passes<W, CLOSE>(blk, i, raw, l, t0, t1, f1, f2, a.base);
+        return;
+    }
+    for (uint32_t r = lo; r < hi; r += kThreads) {
+        const uint32_t i = r + tid;
+        const bool hit = i < hi && passes<W, CLOSE>(blk, i, raw, l, t0, t1, f1, f2, a.base);
+        emit_round<true>(a, sh, hit, bstart + i, acc);
+    }
+}
+
+// Workgroup exclusive scan of one value per thread; returns this thread's
+// prefix and the total in *total (every thread).
+__device__ __forceinline__ uint32_t wg_exclusive_scan(FilterShared &sh, uint32_t v, uint32_t *total) {
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    uint32_t inc = v;
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t t = (uint32_t)__shfl_up((int)inc, o, kWave);
+        if (lane >= o) inc += t;
+    }
+    if (lane == kWave - 1) sh.wcnt[wid] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+        if (w < wid) before += sh.wcnt[w];
+        all += sh.wcnt[w];
+    }
+    __syncthreads();
+    *total = all;
+    return before + inc - v;
+}
+
+// filter_range (:206-249) on block b: one window from the model threshold.
+template <typename T, int W, bool WRITE>
+__device__ __forceinline__ void gt_block(const FilterArgs &a, FilterShared &sh, uint64_t b, uint32_t m,
+                                         uint32_t bstart, uint64_t &acc) {
+    constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
+    const uint8_t *blk = a.stream + a.offs[b];
+    const int l = blk[0];
+    const uint64_t f = a.filter & wmask;
+    if (l >= W) {
+        // raw blocks (and the one-/two-value blocks 127/126): every stored value
+        const uint32_t mm = l == 127 ? (m < 1 ? m : 1) : l == 126 ? (m < 2 ? m : 2) : m;
+        scan_window<W, false, WRITE>(a, sh, blk, 0, mm, true, l, 0.0, 0.0, f, 0, bstart, acc);
+        return;
+    }
+    const double t0 = bits_to_double(load_u64_any(blk + 1)), t1 = bits_to_double(load_u64_any(blk + 9));
+    const int64_t delta = l ? (int64_t)1 << (l - 1) : 0;
+    double x;
+    if (W == 64) x = (double)(f + 1 - (uint64_t)delta) - t0;
+    else x = (double)((int64_t)(uint32_t)(f + 1) - delta) - t0;
+    const int32_t thre = cvtt32(std_max(x / t1, 0.0));
+    if ((uint64_t)(int64_t)thre >= m) return;
+    scan_window<W, false, WRITE>(a, sh, blk, (uint32_t)thre, m, false, l, t0, t1, f, 0, bstart, acc);
+}
+
+// ---- filter_range_close (:251-303) ----------------------------------------
+// The reference's loop visits periods k = 0, 1, ... of block b with filters
+// (f1 + k*base, f2 + k*base) (T wrap; the same values as adding base k
+// times) and window [thre1_k, thre2_k) ([.., thre2_k] when l == 0), and stops
+// at the first period whose window test fails (K periods).  Two exact ways to
+// produce its output:
+//   walk:  the periods in order, kThreads at a time: window sizes scanned into
+//          one flat list of (period, position) candidates, walked in order;
+//   scan:  each position once.  When the periods' intervals are disjoint
+//          (f2 - f1 <= base) and do not wrap before period K, a value can pass
+//          only in its own period k*(v); when the block's values also do not
+//          decrease, k* does not decrease with the position, so period-major
+//          order is position order.  Position i is reported iff k*(v_i) < K,
+//          i lies in that period's window and the value passes.
+// Pass 1 finds K, picks scan when its conditions hold (the value order is
+// checked during the same sweep) and remembers the choice for pass 2.
+
+// Thresholds of period k (its filters f1, f2): thre1, thre2 as the reference's int conversions.
+template <int W>
+__device__ __forceinline__ void period_window(uint64_t f1, uint64_t f2, int64_t delta, double t0, double t1,
+                                              uint32_t m, int32_t *thre1, int32_t *thre2) {
+    double x1, x2;
+    if (W == 64) {
+        x1 = (double)(f1 - (uint64_t)delta) - t0;
+        x2 = (double)(f2 + (uint64_t)delta) - t0;
+    } else {
+        x1 = (double)((int64_t)f1 - delta) - t0;
+        x2 = (double)((int64_t)f2 + delta) - t0;
+    }
+    *thre1 = cvtt32(std_max(x1 / t1 - 1, 0.0));
+    *thre2 = cvtt32(std_min(x2 / t1 + 1, (double)m));
+}
+
+struct CloseBlock {
+    const uint8_t *blk;
+    int l;
+    double t0, t1;
+    int64_t delta;
+    uint64_t f1, f2, step;  // period 0's filters, base as T
+    uint32_t m;
+};
+
+// K: the periods the reference's loop runs (at most kFilterMaxPeriods);
+// *cut: the loop was stopped by the limit with the next period still passing.
+template <int W>
+__device__ __forceinline__ uint32_t close_periods(const CloseBlock &c, FilterShared &sh, bool *cut) {
+    constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    for (uint32_t k0 = 0;; k0 += kThreads) {
+        const uint32_t k = k0 + tid;
+        int32_t thre1, thre2;
+        period_window<W>((c.f1 + (uint64_t)k * c.step) & wmask, (c.f2 + (uint64_t)k * c.step) & wmask, c.delta,
+                         c.t0, c.t1, c.m, &thre1, &thre2);
+        const bool cond = (uint64_t)(int64_t)thre1 < c.m && thre2 > 0;
+        const uint64_t sb = __ballot(!cond || k >= kFilterMaxPeriods);
+        if (lane == 0) sh.wfail[wid] = sb ? (uint32_t)(wid * kWave + __ffsll((long long)sb) - 1) : kThreads;
+        __syncthreads();
+        uint32_t first = kThreads;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) first = sh.wfail[w] < first ? sh.wfail[w] : first;
+        if ((uint32_t)tid == first) sh.stop_passing = cond ? 1u : 0u;
+        __syncthreads();
+        if (first < (uint32_t)kThreads) {
+            *cut = sh.stop_passing != 0;
+            __syncthreads();
+            return k0 + first;
+        }
+    }
+}
+
+// Walk: periods [0, K) in order, kThreads at a time.
+template <int W, bool WRITE>
+__device__ __forceinline__ void close_walk(const FilterArgs &a, const CloseBlock &c, FilterShared &sh, uint32_t K,
+                                           uint32_t bstart, uint64_t &acc) {
+    constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
+    const int tid = threadIdx.x;
+    for (uint32_t k0 = 0; k0 < K; k0 += kThreads) {
+        const uint32_t k = k0 + tid;
+        const uint32_t live = K - k0 < (uint32_t)kThreads ? K - k0 : kThreads;
+        const uint64_t f1 = (c.f1 + (uint64_t)k * c.step) & wmask, f2 = (c.f2 + (uint64_t)k * c.step) & wmask;
+        int32_t thre1, thre2;
+        period_window<W>(f1, f2, c.delta, c.t0, c.t1, c.m, &thre1, &thre2);
+        const uint32_t hi = c.l ? (uint32_t)thre2 : (uint32_t)thre2 + 1;
+        const bool on = (uint32_t)tid < live;
+        const uint32_t sz = (on && (uint32_t)thre1 < hi) ? hi - (uint32_t)thre1 : 0;
+        uint32_t total;
+        const uint32_t pre = wg_exclusive_scan(sh, sz, &total);
+        sh.pf1[tid] = f1;
+        sh.pf2[tid] = f2;
+        sh.plo[tid] = on ? (uint32_t)thre1 : 0;
+        sh.ppre[tid] = pre;
+        __syncthreads();
+        for (uint32_t r = 0; r < total; r += kThreads) {
+            const uint32_t q = r + tid;
+            bool hit = false;
+            uint32_t position = 0;
+            if (q < total) {
+                uint32_t lo = 0, hi2 = live;  // the last period j < live with ppre[j] <= q
+                while (hi2 - lo > 1) {
+                    const uint32_t mid = (lo + hi2) / 2;
+                    if (sh.ppre[mid] <= q) lo = mid;
+                    else hi2 = mid;
+                }
+                const uint32_t i = sh.plo[lo] + (q - sh.ppre[lo]);
+                hit = passes<W, true>(c.blk, i, false, c.l, c.t0, c.t1, sh.pf1[lo], sh.pf2[lo], a.base);
+                position = bstart + i;
+            }
+            emit_round<WRITE>(a, sh, hit, position, acc);
+        }
+        __syncthreads();
+    }
+}
+
+// The value the close loop compares at position i (int64 bits); i == m is
+// the l == 0 loop's extra position (model only).
+__device__ __forceinline__ uint64_t close_value(const CloseBlock &c, uint32_t i) {
+    const int64_t p = predict(c.t0, c.t1, i);
+    return c.l ? apply_field(p, load_bits(c.blk + 17, (uint64_t)i * c.l, c.l), c.l) : (uint64_t)p;
+}
+
+// Scan: does position i (value v) pass in its own period?
+template <int W>
+__device__ __forceinline__ bool close_scan_hit(const CloseBlock &c, uint32_t K, uint32_t i, uint64_t v,
+                                               int32_t base) {
+    uint64_t d;
+    if (W == 64) {
+        if (v <= c.f1) return false;
+        d = v - c.f1;
+    } else {
+        if ((int64_t)v <= (int64_t)c.f1) return false;
+        d = (uint64_t)((int64_t)v - (int64_t)c.f1);
+    }
+    const uint64_t k = (d - 1) / (uint64_t)base;  // the last period whose f1 lies below v
+    if (k >= K) return false;
+    const uint64_t f1 = c.f1 + k * (uint64_t)base, f2 = c.f2 + k * (uint64_t)base;
+    if (!(W == 64 ? v < f2 : (int64_t)v < (int64_t)f2)) return false;
+    int32_t thre1, thre2;
+    period_window<W>(f1, f2, c.delta, c.t0, c.t1, c.m, &thre1, &thre2);
+    const uint32_t hi = c.l ? (uint32_t)thre2 : (uint32_t)thre2 + 1;
+    return i >= (uint32_t)thre1 && i < hi;
+}
+
+template <typename T, int W, bool WRITE>
+__device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &sh, uint64_t b, uint32_t m,
+                                            uint32_t slot, uint32_t bstart, uint64_t &acc) {
+    constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    CloseBlock c;
+    c.blk = a.stream + a.offs[b];
+    c.l = c.blk[0];
+    c.f1 = (uint64_t)((const T *)a.f1)[b];
+    c.f2 = (uint64_t)((const T *)a.f2)[b];
+    c.m = m;
+    if (c.l >= W) {
+        const uint32_t mm = c.l == 127 ? (m < 1 ? m : 1) : c.l == 126 ? (m < 2 ? m : 2) : m;
+        scan_window<W, true, WRITE>(a, sh, c.blk, 0, mm, true, c.l, 0.0, 0.0, c.f1, c.f2, bstart, acc);
+        return true;
+    }
+    c.t0 = bits_to_double(load_u64_any(c.blk + 1));
+    c.t1 = bits_to_double(load_u64_any(c.blk + 9));
+    c.delta = c.l ? (int64_t)1 << (c.l - 1) : 0;
+    c.step = (uint64_t)(int64_t)a.base & wmask;
+    bool cut = false;
+    uint32_t K;
+    bool scan;
+    if (!WRITE) {
+        K = close_periods<W>(c, sh, &cut);
+        // scan needs disjoint, non-wrapping intervals up to period K
+        const uint64_t reach = K ? (uint64_t)(K - 1) * c.step : 0;
+        scan = K > 0 && a.base > 0 && (c.f2 <= c.f1 || c.f2 - c.f1 <= (uint64_t)a.base) &&
+               c.f1 <= wmask - reach && c.f2 <= wmask - reach;
+    } else {
+        K = sh.bK[slot];
+        scan = sh.bscan[slot] != 0;
+    }
+    if (K == 0) {
+        if (!WRITE && tid == 0) { sh.bK[slot] = 0; sh.bscan[slot] = 0; }
+        return !cut;
+    }
+    const uint32_t mp = c.l ? m : m + 1;  // positions the windows can reach
+    if (!WRITE && scan) {
+        // one sweep: the scan's count and whether the values never decrease
+        uint64_t cnt = 0;
+        bool sorted = true;
+        for (uint32_t i = tid; i < mp; i += kThreads) {
+            const uint64_t v = close_value(c, i);
+            cnt += close_scan_hit<W>(c, K, i, v, a.base);
+            if (i + 1 < mp) {
+                const uint64_t u = close_value(c, i + 1);
+                sorted &= W == 64 ? v <= u : (int64_t)v <= (int64_t)u;
+            }
+        }
+        const uint64_t bad = __ballot(!sorted);
+        if (lane == 0) sh.wfail[wid] = bad ? 1u : 0u;
+        __syncthreads();
+        bool all_sorted = true;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) all_sorted &= sh.wfail[w] == 0;
+        __syncthreads();
+        scan = all_sorted;
+        if (scan) acc += cnt;
+    }
+    if (!WRITE && tid == 0) {
+        sh.bK[slot] = K;
+        sh.bscan[slot] = scan ? 1 : 0;
+    }
+    if (scan) {
+        if (WRITE) {
+            for (uint32_t r = 0; r < mp; r += kThreads) {
+                const uint32_t i = r + tid;
+                const bool hit = i < mp && close_scan_hit<W>(c, K, i, close_value(c, i), a.base);
+                emit_round<true>(a, sh, hit, bstart + i, acc);
+            }
+        }
+    } else {
+        close_walk<W, WRITE>(a, c, sh, K, bstart, acc);
+    }
+    return !cut;
 }
 
 template <typename T, int W, bool CLOSE>
@@ -153,13 +423,8 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     bool ok = true;
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
-        const uint8_t *blk = a.stream + a.offs[b];
-        ok &= block_windows<T, W, CLOSE>(a, b, block_len(b),
-                                         [&](uint32_t lo, uint32_t hi, bool raw, int l, double t0, double t1,
-                                             uint64_t f1, uint64_t f2) {
-                                             for (uint32_t i = lo + tid; i < hi; i += kThreads)
-                                                 mine += passes<W, CLOSE>(blk, i, raw, l, t0, t1, f1, f2, a.base);
-                                         });
+        if (CLOSE) ok &= close_block<T, W, false>(a, sh, b, block_len(b), k, 0, mine);
+        else gt_block<T, W, false>(a, sh, b, block_len(b), 0, mine);
     }
     for (int o = kWave / 2; o > 0; o >>= 1) mine += (uint64_t)__shfl_xor((long long)mine, o, kWave);
     if (lane == 0) sh.wsum[wid] = mine;
@@ -167,7 +432,7 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     if (wid == 0) {
         uint64_t total = 0;
 #pragma unroll
-        for (int w = 0; w < kThreads / kWave; ++w) total += sh.wsum[w];
+        for (int w = 0; w < kWaves; ++w) total += sh.wsum[w];
         const uint64_t excl = look_back(t, total, a.work);
         if (lane == 0) {
             sh.base = excl;
@@ -177,35 +442,13 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     }
     __syncthreads();
 
-    // pass 2: the same windows, positions written in order
+    // pass 2: the same candidates, positions written in order
     uint64_t pos = sh.base;
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
-        const uint8_t *blk = a.stream + a.offs[b];
         const uint32_t bstart = a.pos_base + (uint32_t)(b * a.bs);
-        block_windows<T, W, CLOSE>(a, b, block_len(b),
-                                   [&](uint32_t lo, uint32_t hi, bool raw, int l, double t0, double t1, uint64_t f1,
-                                       uint64_t f2) {
-                                       for (uint32_t r = lo; r < hi; r += kThreads) {
-                                           const uint32_t i = r + tid;
-                                           const bool hit =
-                                               i < hi && passes<W, CLOSE>(blk, i, raw, l, t0, t1, f1, f2, a.base);
-                                           const uint64_t bal = __ballot(hit);
-                                           if (lane == 0) sh.wcnt[wid] = (uint32_t)__popcll(bal);
-                                           __syncthreads();
-                                           uint64_t at = pos;
-                                           uint32_t round = 0;
-#pragma unroll
-                                           for (int w = 0; w < kThreads / kWave; ++w) {
-                                               if (w < wid) at += sh.wcnt[w];
-                                               round += sh.wcnt[w];
-                                           }
-                                           at += (uint64_t)__popcll(bal & ((1ull << lane) - 1));
-                                           if (hit && at < a.cap) a.out[at] = bstart + i;
-                                           pos += round;
-                                           __syncthreads();
-                                       }
-                                   });
+        if (CLOSE) close_block<T, W, true>(a, sh, b, block_len(b), k, bstart, pos);
+        else gt_block<T, W, true>(a, sh, b, block_len(b), bstart, pos);
     }
 }
 

@@ -72,9 +72,10 @@ def test_filter_range_close_matches_oracle(oracle_mod, bs):
         nb = col.num_blocks
         mins = [int(vals[b * bs:(b + 1) * bs].min()) for b in range(nb)]
         mask = (1 << width) - 1
-        for base in (1 << 28, 12345, (1 << 31) - 1):
-            say(f"filter_range_close width={width} bs={bs} n={n} base={base}")
-            g1, g2 = base // 6, base // 2
+        # (g2 - g1 > base: overlapping periods, which every block walks period by period)
+        for base, g1, g2 in ((1 << 28, (1 << 28) // 6, (1 << 28) // 2), (12345, 2057, 6172),
+                             ((1 << 31) - 1, 357913941, 1073741823), (40_000, 1000, 70_000)):
+            say(f"filter_range_close width={width} bs={bs} n={n} base={base} window=({g1}, {g2})")
             o1 = np.array([(m - m % base + g1) & mask for m in mins], np.uint64)
             o2 = np.array([(m - m % base + g2) & mask for m in mins], np.uint64)
             t1, t2 = to_dev(torch, o1.astype(vals.dtype)), to_dev(torch, o2.astype(vals.dtype))
