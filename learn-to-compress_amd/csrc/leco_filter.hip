@@ -39,10 +39,12 @@ __device__ __forceinline__ bool below(uint64_t v, uint64_t f) {
 }
 
 constexpr int kWaves = kThreads / kWave;
+constexpr int kEmitPer = 8;  // rounds of kThreads candidates per compaction step when writing positions
 
 struct FilterShared {
     uint64_t wsum[kWaves];
     uint32_t wcnt[kWaves];
+    uint32_t jc[kEmitPer][kWaves];
     uint32_t wfail[kWaves];
     uint32_t stop_passing;  // the period the close loop stopped at still passed its test (limit cut)
     uint64_t base;
@@ -92,31 +94,61 @@ __device__ __forceinline__ bool passes(const uint8_t *blk, uint32_t i, bool raw,
     return CLOSE ? (above<W>(v, f1) && below<W>(v, f2)) : above<W>(v, f1);
 }
 
-// One round of kThreads candidates in order: pass 1 (WRITE = false) adds
-// this thread's hit to acc; pass 2 writes the hits' positions at acc (the
-// workgroup's running output slot, the same in every thread) and advances it.
-template <bool WRITE>
-__device__ __forceinline__ void emit_round(const FilterArgs &a, FilterShared &sh, bool hit, uint32_t position,
-                                           uint64_t &acc) {
+// Workgroup exclusive scan of one value per thread; returns this thread's
+// prefix and the total in *total (every thread).
+__device__ __forceinline__ uint32_t wg_exclusive_scan(FilterShared &sh, uint32_t v, uint32_t *total);
+
+// Candidates q in [0, total) in order; cand(q, &position) says whether q
+// passes and where it is.  Pass 1 (WRITE = false) adds this thread's hits to
+// acc (any order); pass 2 takes kEmitPer rounds of kThreads candidates per
+// step (round j: q = r + j*kThreads + tid, so loads and stores stay
+// coalesced), keeps each wave's ballot per round, exchanges the per-(round,
+// wave) counts through LDS once, and writes every hit at acc + the number of
+// hits before it in (round, wave, lane) order -- candidate order.  acc is the
+// workgroup's running output slot (the same in every thread).
+template <bool WRITE, class Cand>
+__device__ __forceinline__ void emit_ordered(const FilterArgs &a, FilterShared &sh, uint32_t total, Cand cand,
+                                             uint64_t &acc) {
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     if (!WRITE) {
-        acc += hit;
+        uint32_t pos;
+        for (uint32_t q = tid; q < total; q += kThreads) acc += cand(q, &pos);
         return;
     }
-    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
-    const uint64_t bal = __ballot(hit);
-    if (lane == 0) sh.wcnt[wid] = (uint32_t)__popcll(bal);
-    __syncthreads();
-    uint64_t at = acc;
-    uint32_t round = 0;
+    const uint64_t below = (1ull << lane) - 1;
+    for (uint32_t r = 0; r < total; r += kThreads * kEmitPer) {
+        uint64_t bal[kEmitPer];
+        uint32_t where[kEmitPer];
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-        if (w < wid) at += sh.wcnt[w];
-        round += sh.wcnt[w];
+        for (int j = 0; j < kEmitPer; ++j) {
+            const uint32_t q = r + j * kThreads + tid;
+            where[j] = 0;
+            bal[j] = __ballot(q < total && cand(q, &where[j]));
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < kEmitPer; ++j) sh.jc[j][wid] = (uint32_t)__popcll(bal[j]);
+        }
+        __syncthreads();
+        uint64_t run = acc;
+#pragma unroll
+        for (int j = 0; j < kEmitPer; ++j) {
+            uint32_t before = 0, row = 0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) {
+                const uint32_t c = sh.jc[j][w];
+                before += w < wid ? c : 0;
+                row += c;
+            }
+            if ((bal[j] >> lane) & 1) {
+                const uint64_t at = run + before + (uint64_t)__popcll(bal[j] & below);
+                if (at < a.cap) a.out[at] = where[j];
+            }
+            run += row;
+        }
+        acc = run;
+        __syncthreads();
     }
-    at += (uint64_t)__popcll(bal & ((1ull << lane) - 1));
-    if (hit && at < a.cap) a.out[at] = position;
-    acc += round;
-    __syncthreads();
 }
 
 // Positions [lo, hi) of one window against one predicate.
@@ -124,21 +156,14 @@ template <int W, bool CLOSE, bool WRITE>
 __device__ __forceinline__ void scan_window(const FilterArgs &a, FilterShared &sh, const uint8_t *blk, uint32_t lo,
                                             uint32_t hi, bool raw, int l, double t0, double t1, uint64_t f1,
                                             uint64_t f2, uint32_t bstart, uint64_t &acc) {
-    const int tid = threadIdx.x;
-    if (!WRITE) {
-        for (uint32_t i = lo + tid; i < hi; i += kThreads)
-            acc += passes<W, CLOSE>(blk, i, raw, l, t0, t1, f1, f2, a.base);
-        return;
-    }
-    for (uint32_t r = lo; r < hi; r += kThreads) {
-        const uint32_t i = r + tid;
-        const bool hit = i < hi && passes<W, CLOSE>(blk, i, raw, l, t0, t1, f1, f2, a.base);
-        emit_round<true>(a, sh, hit, bstart + i, acc);
-    }
+    emit_ordered<WRITE>(a, sh, hi > lo ? hi - lo : 0,
+                        [&](uint32_t q, uint32_t *where) {
+                            *where = bstart + lo + q;
+                            return passes<W, CLOSE>(blk, lo + q, raw, l, t0, t1, f1, f2, a.base);
+                        },
+                        acc);
 }
 
-// Workgroup exclusive scan of one value per thread; returns this thread's
-// prefix and the total in *total (every thread).
 __device__ __forceinline__ uint32_t wg_exclusive_scan(FilterShared &sh, uint32_t v, uint32_t *total) {
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     uint32_t inc = v;
@@ -275,23 +300,20 @@ __device__ __forceinline__ void close_walk(const FilterArgs &a, const CloseBlock
         sh.plo[tid] = on ? (uint32_t)thre1 : 0;
         sh.ppre[tid] = pre;
         __syncthreads();
-        for (uint32_t r = 0; r < total; r += kThreads) {
-            const uint32_t q = r + tid;
-            bool hit = false;
-            uint32_t position = 0;
-            if (q < total) {
-                uint32_t lo = 0, hi2 = live;  // the last period j < live with ppre[j] <= q
-                while (hi2 - lo > 1) {
-                    const uint32_t mid = (lo + hi2) / 2;
-                    if (sh.ppre[mid] <= q) lo = mid;
-                    else hi2 = mid;
-                }
-                const uint32_t i = sh.plo[lo] + (q - sh.ppre[lo]);
-                hit = passes<W, true>(c.blk, i, false, c.l, c.t0, c.t1, sh.pf1[lo], sh.pf2[lo], a.base);
-                position = bstart + i;
-            }
-            emit_round<WRITE>(a, sh, hit, position, acc);
-        }
+        emit_ordered<WRITE>(a, sh, total,
+                            [&](uint32_t q, uint32_t *where) {
+                                uint32_t lo = 0, hi2 = live;  // the last period j < live with ppre[j] <= q
+                                while (hi2 - lo > 1) {
+                                    const uint32_t mid = (lo + hi2) / 2;
+                                    if (sh.ppre[mid] <= q) lo = mid;
+                                    else hi2 = mid;
+                                }
+                                const uint32_t i = sh.plo[lo] + (q - sh.ppre[lo]);
+                                *where = bstart + i;
+                                return passes<W, true>(c.blk, i, false, c.l, c.t0, c.t1, sh.pf1[lo], sh.pf2[lo],
+                                                       a.base);
+                            },
+                            acc);
         __syncthreads();
     }
 }
@@ -390,13 +412,13 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
         sh.bscan[slot] = scan ? 1 : 0;
     }
     if (scan) {
-        if (WRITE) {
-            for (uint32_t r = 0; r < mp; r += kThreads) {
-                const uint32_t i = r + tid;
-                const bool hit = i < mp && close_scan_hit<W>(c, K, i, close_value(c, i), a.base);
-                emit_round<true>(a, sh, hit, bstart + i, acc);
-            }
-        }
+        if (WRITE)
+            emit_ordered<true>(a, sh, mp,
+                               [&](uint32_t i, uint32_t *where) {
+                                   *where = bstart + i;
+                                   return close_scan_hit<W>(c, K, i, close_value(c, i), a.base);
+                               },
+                               acc);
     } else {
         close_walk<W, WRITE>(a, c, sh, K, bstart, acc);
     }
