@@ -164,6 +164,8 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-column section")
     ap.add_argument("--batched-cols", type=int, default=800)
     ap.add_argument("--batched-max", type=float, default=1e8, help="largest column length (log-uniform from 1e3)")
+    ap.add_argument("--batched-small-cols", type=int, default=20000,
+                    help="second batched workload: this many columns of 1e3..1e5 values (0 = skip)")
     ap.add_argument("--no-serving", action="store_true", help="skip the request-server section")
     ap.add_argument("--serve-nreq", type=int, default=60_000)
     ap.add_argument("--no-churn", action="store_true", help="skip the insert/evict-under-traffic section")
@@ -248,6 +250,15 @@ def main() -> int:
         del cols
         torch.cuda.empty_cache()
         log(f"batched: {batched}")
+        if args.batched_small_cols > 0:
+            # many short columns: where one launch per column from a host loop is launch-bound
+            cols = workloads.batched_columns(args.batched_small_cols, 1e3, 1e5, seed=7)
+            small = workloads.run_batched(cols, log=log)
+            small["workload"] = f"{args.batched_small_cols} columns, lengths log-uniform in [1e3, 1e5]"
+            batched["small_columns"] = small
+            del cols
+            torch.cuda.empty_cache()
+            log(f"batched small columns: {small}")
     if not args.no_serving:
         serving = workloads.run_serving(col, nreq=args.serve_nreq, log=log)
         log(f"serving: {serving}")
