@@ -86,6 +86,41 @@ def cpu_baseline(vals_dev, bs: int, sample: int, repeat: int):
     return gbs1, gbsn, nthr, gbsf
 
 
+def io_section(torch, keys, bs: int) -> dict:
+    """SOSD binary -> HBM -> encode -> column file -> HBM, in a temporary
+    directory (the host page cache serves the reads)."""
+    import tempfile
+
+    from leco_fix import Column, io
+
+    raw = keys.numel() * 8
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR")) as d:
+        sosd, colf = os.path.join(d, "keys.sosd"), os.path.join(d, "keys.leco")
+        t0 = time.perf_counter()
+        io.save_sosd(sosd, keys)
+        t1 = time.perf_counter()
+        loaded = io.load_sosd(sosd, 64)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        col = Column.encode(loaded, bs)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        io.save_column(colf, col)
+        t4 = time.perf_counter()
+        back = io.load_column(colf)
+        torch.cuda.synchronize()
+        t5 = time.perf_counter()
+        ok = bool(torch.equal(loaded, keys)) and back.stream_bytes() == col.stream_bytes()
+        comp = col.nbytes
+    return {"keys": keys.numel(), "raw_bytes": raw, "column_bytes": comp, "verified": ok,
+            "sosd_save_gbs": round(raw / (t1 - t0) / 1e9, 2), "sosd_load_gbs": round(raw / (t2 - t1) / 1e9, 2),
+            "encode_gbs": round(raw / (t3 - t2) / 1e9, 1),
+            "column_save_gbs": round(comp / (t4 - t3) / 1e9, 2),
+            "column_load_gbs": round(comp / (t5 - t4) / 1e9, 2),
+            "column_load_raw_equiv_gbs": round(raw / (t5 - t4) / 1e9, 2),
+            "note": "wall clock; file bytes through the host page cache and PCIe; encode includes allocation"}
+
+
 def filter_section(torch, vals, col, reps: int) -> dict:
     """filter_range at the median key and filter_range_close with the reference
     driver's zonemap-derived per-block filters, on the resident column; one
@@ -160,6 +195,8 @@ def main() -> int:
     ap.add_argument("--copy-reps", type=int, default=3,
                     help="plain device copies of the decoded column (a known-size HBM reference kernel)")
     ap.add_argument("--no-filter", action="store_true", help="skip the filter_range / filter_range_close section")
+    ap.add_argument("--io-values", type=int, default=128_000_000,
+                    help="keys written / read through SOSD and column files (0 = skip)")
     ap.add_argument("--filter-reps", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true", help="skip the batched many-column section")
     ap.add_argument("--batched-cols", type=int, default=800)
@@ -224,6 +261,12 @@ def main() -> int:
 
     log("lookup timing")
     ms_lk, _ = timed(torch, lambda: col.lookup(idx, out=lk), 5, 1)
+
+    files = None
+    if args.io_values > 0:
+        log("file timing")
+        files = io_section(torch, vals[:args.io_values], bs)
+        log(f"files: {files}")
 
     filt = None
     if not args.no_filter:
@@ -329,6 +372,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "filter": filt,
+        "files": files,
         "batched": batched,
         "serving": serving,
         "churn": churn,

@@ -196,6 +196,39 @@ int leco_fix_filter_close_block_u64(const uint8_t *h_in, uint64_t length, uint32
                                     int32_t block_id, uint32_t block_size, uint64_t filter1, uint64_t filter2,
                                     int32_t base, uint64_t *count);
 
+/* ---- files ------------------------------------------------------------- */
+
+/* SOSD binary, the reference's input format (experiments/leco_int_fix_template.cpp:21-40,
+ * load_data_binary): a little-endian uint64 count, then count values of width_bits.
+ * _info checks the file and returns the count; _load streams the values into
+ * device memory (d_out holds capacity values) through pinned staging on `stream`
+ * and returns when they are there; _save writes count device values.  Host
+ * threads may call these concurrently (staging is per thread). */
+int leco_fix_sosd_info(const char *path, uint32_t width_bits, uint64_t *count);
+int leco_fix_sosd_load(const char *path, uint32_t width_bits, void *d_out, uint64_t capacity, uint64_t *count,
+                       void *stream);
+int leco_fix_sosd_save(const char *path, uint32_t width_bits, const void *d_in, uint64_t count, void *stream);
+
+/* Column file: an encoded column as it lives in HBM (the reference keeps it as
+ * block_start_vec, experiments/leco_int_fix_template.cpp:108,125): 64-byte header
+ * ("LECOCOL1", version, width, n, block size, block count, stream bytes,
+ * FNV-1a check), the block offsets from 0, and the stream.  _save accepts
+ * absolute offsets (a column inside a batch arena).  _load needs
+ * stream_capacity >= stream_bytes + LECO_FIX_TAIL_PAD and num_blocks + 1
+ * offsets; when d_params is not NULL it rebuilds the param table on the GPU. */
+typedef struct leco_fix_file_info {
+    uint64_t n;
+    uint64_t num_blocks;
+    uint64_t stream_bytes;
+    uint32_t block_size;
+    uint32_t width_bits;
+} leco_fix_file_info;
+int leco_fix_column_file_info(const char *path, leco_fix_file_info *info);
+int leco_fix_column_save(const char *path, const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                         uint32_t block_size, uint32_t width_bits, void *stream);
+int leco_fix_column_load(const char *path, uint8_t *d_stream, uint64_t stream_capacity, uint64_t *d_offsets,
+                         leco_fix_block_param *d_params, void *stream);
+
 /* ---- batches: many columns in one launch ------------------------------ */
 
 /* One column of a batch. d_in is read by encode, d_out written by decode;

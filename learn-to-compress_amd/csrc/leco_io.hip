@@ -1,0 +1,292 @@
+// Column files and the reference's binary input format (C ABI in include/leco_fix.h).
+//
+// SOSD binary (experiments/leco_int_fix_template.cpp:21-40 of the reference,
+// load_data_binary): a little-endian uint64 count, then count values of T.
+// Loaded straight into HBM: the file is read in kChunk pieces into two
+// pinned host buffers (hipHostMalloc) while the previous piece is copied to
+// the device on the caller's stream.
+//
+// Column file: the compressed column as it lives in HBM -- what the
+// reference keeps in block_start_vec (:108,125) plus the stream -- so a
+// column is encoded once and reloaded without re-encoding:
+//   [0, 64)      header: "LECOCOL1", version 1, width, n, block size, flags,
+//                number of blocks, stream bytes, FNV-1a of bytes [0, 48)
+//   [64, ...)    block offsets, (num_blocks + 1) little-endian uint64, from 0
+//   64-aligned   the stream: the blocks' bytes back to back
+// The param table is not stored; loading rebuilds it on the GPU from the
+// block headers.
+#include <errno.h>
+#include <fcntl.h>
+#include <hip/hip_runtime.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <vector>
+
+#include "leco_internal.h"
+
+namespace {
+
+constexpr size_t kChunk = 64ull << 20;  // bytes per staged piece
+constexpr uint32_t kFileVersion = 1;
+constexpr char kMagic[8] = {'L', 'E', 'C', 'O', 'C', 'O', 'L', '1'};
+
+struct FileHeader {
+    char magic[8];
+    uint32_t version;
+    uint32_t width_bits;
+    uint64_t n;
+    uint32_t block_size;
+    uint32_t flags;
+    uint64_t num_blocks;
+    uint64_t stream_bytes;
+    uint64_t check;     // FNV-1a 64 of bytes [0, 48)
+    uint64_t reserved;
+};
+static_assert(sizeof(FileHeader) == 64, "the column file header is 64 bytes");
+
+uint64_t fnv1a(const void *p, size_t len) {
+    const uint8_t *b = (const uint8_t *)p;
+    uint64_t h = 0xcbf29ce484222325ull;
+    for (size_t i = 0; i < len; ++i) h = (h ^ b[i]) * 0x100000001b3ull;
+    return h;
+}
+
+uint64_t stream_start(uint64_t num_blocks) { return (64 + 8 * (num_blocks + 1) + 63) & ~63ull; }
+
+int hip_status(hipError_t e) { return e == hipSuccess ? LECO_FIX_OK : LECO_FIX_EHIP; }
+
+struct Fd {
+    int fd = -1;
+    ~Fd() {
+        if (fd >= 0) close(fd);
+    }
+};
+
+bool read_full(int fd, void *dst, size_t len, uint64_t off) {
+    uint8_t *p = (uint8_t *)dst;
+    while (len) {
+        const ssize_t r = pread(fd, p, len, (off_t)off);
+        if (r < 0 && errno == EINTR) continue;
+        if (r <= 0) return false;
+        p += r;
+        off += (uint64_t)r;
+        len -= (size_t)r;
+    }
+    return true;
+}
+
+bool write_full(int fd, const void *src, size_t len, uint64_t off) {
+    const uint8_t *p = (const uint8_t *)src;
+    while (len) {
+        const ssize_t r = pwrite(fd, p, len, (off_t)off);
+        if (r < 0 && errno == EINTR) continue;
+        if (r <= 0) return false;
+        p += r;
+        off += (uint64_t)r;
+        len -= (size_t)r;
+    }
+    return true;
+}
+
+// Two pinned staging buffers and their events, kept per host thread.
+struct PinnedPair {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool ready = false;
+    ~PinnedPair() {
+        for (int i = 0; i < 2; ++i) {
+            if (buf[i]) (void)hipHostFree(buf[i]);
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+        }
+    }
+    int init() {
+        if (ready) return LECO_FIX_OK;
+        for (int i = 0; i < 2; ++i) {
+            if (hipHostMalloc(&buf[i], kChunk, hipHostMallocDefault) != hipSuccess) return LECO_FIX_ENOMEM;
+            if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) return LECO_FIX_EHIP;
+        }
+        ready = true;
+        return LECO_FIX_OK;
+    }
+};
+thread_local PinnedPair g_pinned;
+
+// File bytes [off, off + len) -> device dst; piece i+1 is read while piece i is copied.
+int file_to_device(int fd, uint64_t off, uint8_t *dst, uint64_t len, hipStream_t st) {
+    int rc = g_pinned.init();
+    if (rc) return rc;
+    for (uint64_t pos = 0, i = 0; pos < len; pos += kChunk, ++i) {
+        const int k = (int)(i & 1);
+        const size_t piece = (size_t)(len - pos < kChunk ? len - pos : kChunk);
+        if (i >= 2 && hipEventSynchronize(g_pinned.ev[k]) != hipSuccess) return LECO_FIX_EHIP;
+        if (!read_full(fd, g_pinned.buf[k], piece, off + pos)) return LECO_FIX_EINVAL;
+        if (hipMemcpyAsync(dst + pos, g_pinned.buf[k], piece, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipEventRecord(g_pinned.ev[k], st) != hipSuccess)
+            return LECO_FIX_EHIP;
+    }
+    return hip_status(hipStreamSynchronize(st));
+}
+
+// Device src[0, len) -> file bytes [off, off + len); piece i is written while piece i+1 is copied.
+int device_to_file(const uint8_t *src, uint64_t len, int fd, uint64_t off, hipStream_t st) {
+    int rc = g_pinned.init();
+    if (rc) return rc;
+    const uint64_t pieces = (len + kChunk - 1) / kChunk;
+    auto piece_len = [&](uint64_t i) { return (size_t)(len - i * kChunk < kChunk ? len - i * kChunk : kChunk); };
+    for (uint64_t i = 0; i <= pieces; ++i) {
+        if (i < pieces) {
+            const int k = (int)(i & 1);
+            if (hipMemcpyAsync(g_pinned.buf[k], src + i * kChunk, piece_len(i), hipMemcpyDeviceToHost, st) !=
+                    hipSuccess ||
+                hipEventRecord(g_pinned.ev[k], st) != hipSuccess)
+                return LECO_FIX_EHIP;
+        }
+        if (i > 0) {
+            const int k = (int)((i - 1) & 1);
+            if (hipEventSynchronize(g_pinned.ev[k]) != hipSuccess) return LECO_FIX_EHIP;
+            if (!write_full(fd, g_pinned.buf[k], piece_len(i - 1), off + (i - 1) * kChunk)) return LECO_FIX_EINVAL;
+        }
+    }
+    return LECO_FIX_OK;
+}
+
+int read_header(int fd, FileHeader *h) {
+    if (!read_full(fd, h, sizeof(*h), 0)) return LECO_FIX_EINVAL;
+    if (memcmp(h->magic, kMagic, 8) != 0 || h->version != kFileVersion) return LECO_FIX_EINVAL;
+    if (fnv1a(h, 48) != h->check) return LECO_FIX_EINVAL;
+    if ((h->width_bits != 32 && h->width_bits != 64) || h->n == 0 || h->block_size == 0 ||
+        h->block_size > leco::kMaxBlockSize || h->num_blocks != (h->n + h->block_size - 1) / h->block_size)
+        return LECO_FIX_EINVAL;
+    struct stat sb;
+    if (fstat(fd, &sb) != 0 || (uint64_t)sb.st_size < stream_start(h->num_blocks) + h->stream_bytes)
+        return LECO_FIX_EINVAL;
+    return LECO_FIX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int leco_fix_sosd_info(const char *path, uint32_t width_bits, uint64_t *count) {
+    if (!path || !count || (width_bits != 32 && width_bits != 64)) return LECO_FIX_EINVAL;
+    Fd f;
+    f.fd = open(path, O_RDONLY | O_CLOEXEC);
+    if (f.fd < 0) return LECO_FIX_EINVAL;
+    uint64_t c = 0;
+    struct stat sb;
+    if (!read_full(f.fd, &c, 8, 0) || fstat(f.fd, &sb) != 0) return LECO_FIX_EINVAL;
+    if ((uint64_t)sb.st_size < 8 || ((uint64_t)sb.st_size - 8) / (width_bits / 8) < c) return LECO_FIX_EINVAL;
+    *count = c;
+    return LECO_FIX_OK;
+}
+
+int leco_fix_sosd_load(const char *path, uint32_t width_bits, void *d_out, uint64_t capacity, uint64_t *count,
+                       void *stream) {
+    uint64_t c = 0;
+    int rc = leco_fix_sosd_info(path, width_bits, &c);
+    if (rc) return rc;
+    if (!d_out || c > capacity) return LECO_FIX_EINVAL;
+    Fd f;
+    f.fd = open(path, O_RDONLY | O_CLOEXEC);
+    if (f.fd < 0) return LECO_FIX_EINVAL;
+    rc = file_to_device(f.fd, 8, (uint8_t *)d_out, c * (width_bits / 8), (hipStream_t)stream);
+    if (rc == LECO_FIX_OK && count) *count = c;
+    return rc;
+}
+
+int leco_fix_sosd_save(const char *path, uint32_t width_bits, const void *d_in, uint64_t count, void *stream) {
+    if (!path || (width_bits != 32 && width_bits != 64) || (count && !d_in)) return LECO_FIX_EINVAL;
+    Fd f;
+    f.fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+    if (f.fd < 0) return LECO_FIX_EINVAL;
+    if (!write_full(f.fd, &count, 8, 0)) return LECO_FIX_EINVAL;
+    return device_to_file((const uint8_t *)d_in, count * (width_bits / 8), f.fd, 8, (hipStream_t)stream);
+}
+
+int leco_fix_column_file_info(const char *path, leco_fix_file_info *info) {
+    if (!path || !info) return LECO_FIX_EINVAL;
+    Fd f;
+    f.fd = open(path, O_RDONLY | O_CLOEXEC);
+    if (f.fd < 0) return LECO_FIX_EINVAL;
+    FileHeader h;
+    int rc = read_header(f.fd, &h);
+    if (rc) return rc;
+    info->n = h.n;
+    info->num_blocks = h.num_blocks;
+    info->stream_bytes = h.stream_bytes;
+    info->block_size = h.block_size;
+    info->width_bits = h.width_bits;
+    return LECO_FIX_OK;
+}
+
+int leco_fix_column_save(const char *path, const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                         uint32_t block_size, uint32_t width_bits, void *stream) {
+    if (!path || !d_stream || !d_offsets || n == 0 || block_size == 0 || block_size > leco::kMaxBlockSize ||
+        (width_bits != 32 && width_bits != 64))
+        return LECO_FIX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const uint64_t nb = (n + block_size - 1) / block_size;
+    std::vector<uint64_t> offs(nb + 1);
+    if (hipMemcpyAsync(offs.data(), d_offsets, (nb + 1) * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return LECO_FIX_EHIP;
+    // offsets may be absolute (a column inside a batch arena): store them from 0
+    const uint64_t base = offs[0];
+    for (uint64_t b = 0; b <= nb; ++b) {
+        if (offs[b] < base || (b && offs[b] < offs[b - 1])) return LECO_FIX_EINVAL;
+        offs[b] -= base;
+    }
+    FileHeader h{};
+    memcpy(h.magic, kMagic, 8);
+    h.version = kFileVersion;
+    h.width_bits = width_bits;
+    h.n = n;
+    h.block_size = block_size;
+    h.num_blocks = nb;
+    h.stream_bytes = offs[nb];
+    h.check = fnv1a(&h, 48);
+    Fd f;
+    f.fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+    if (f.fd < 0) return LECO_FIX_EINVAL;
+    const uint64_t start = stream_start(nb);
+    std::vector<uint8_t> pad(start - 64 - 8 * (nb + 1), 0);
+    if (!write_full(f.fd, &h, sizeof(h), 0) || !write_full(f.fd, offs.data(), 8 * (nb + 1), 64) ||
+        (!pad.empty() && !write_full(f.fd, pad.data(), pad.size(), 64 + 8 * (nb + 1))))
+        return LECO_FIX_EINVAL;
+    return device_to_file(d_stream + base, h.stream_bytes, f.fd, start, st);
+}
+
+int leco_fix_column_load(const char *path, uint8_t *d_stream, uint64_t stream_capacity, uint64_t *d_offsets,
+                         leco_fix_block_param *d_params, void *stream) {
+    if (!path || !d_stream || !d_offsets) return LECO_FIX_EINVAL;
+    Fd f;
+    f.fd = open(path, O_RDONLY | O_CLOEXEC);
+    if (f.fd < 0) return LECO_FIX_EINVAL;
+    FileHeader h;
+    int rc = read_header(f.fd, &h);
+    if (rc) return rc;
+    if (stream_capacity < h.stream_bytes + LECO_FIX_TAIL_PAD) return LECO_FIX_EINVAL;
+    const uint64_t nb = h.num_blocks;
+    std::vector<uint64_t> offs(nb + 1);
+    if (!read_full(f.fd, offs.data(), 8 * (nb + 1), 64)) return LECO_FIX_EINVAL;
+    if (offs[0] != 0 || offs[nb] != h.stream_bytes) return LECO_FIX_EINVAL;
+    for (uint64_t b = 1; b <= nb; ++b)
+        if (offs[b] < offs[b - 1]) return LECO_FIX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (hipMemcpyAsync(d_offsets, offs.data(), 8 * (nb + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemsetAsync(d_stream + h.stream_bytes, 0, LECO_FIX_TAIL_PAD, st) != hipSuccess)
+        return LECO_FIX_EHIP;
+    rc = file_to_device(f.fd, stream_start(nb), d_stream, h.stream_bytes, st);
+    if (rc) return rc;
+    if (d_params) {
+        if (leco::launch_build_params(d_stream, d_offsets, h.n, h.block_size, h.width_bits, d_params, st) !=
+            hipSuccess)
+            return LECO_FIX_EHIP;
+        rc = hip_status(hipStreamSynchronize(st));
+    }
+    return rc;
+}
+
+}  // extern "C"

@@ -67,6 +67,22 @@ for _w in ("u32", "u64"):
 SIGNATURES["leco_fix_filter_workspace_bytes"] = (c_u64, [c_u64, c_u32])
 
 
+class ColumnFileInfo(ctypes.Structure):
+    """leco_fix_file_info (include/leco_fix.h)."""
+    _fields_ = [("n", ctypes.c_uint64), ("num_blocks", ctypes.c_uint64), ("stream_bytes", ctypes.c_uint64),
+                ("block_size", ctypes.c_uint32), ("width_bits", ctypes.c_uint32)]
+
+
+SIGNATURES.update({
+    "leco_fix_sosd_info": (c_int, [ctypes.c_char_p, c_u32, ctypes.POINTER(c_u64)]),
+    "leco_fix_sosd_load": (c_int, [ctypes.c_char_p, c_u32, c_ptr, c_u64, ctypes.POINTER(c_u64), c_ptr]),
+    "leco_fix_sosd_save": (c_int, [ctypes.c_char_p, c_u32, c_ptr, c_u64, c_ptr]),
+    "leco_fix_column_file_info": (c_int, [ctypes.c_char_p, ctypes.POINTER(ColumnFileInfo)]),
+    "leco_fix_column_save": (c_int, [ctypes.c_char_p, c_ptr, c_ptr, c_u64, c_u32, c_u32, c_ptr]),
+    "leco_fix_column_load": (c_int, [ctypes.c_char_p, c_ptr, c_u64, c_ptr, c_ptr, c_ptr]),
+})
+
+
 
 class ColumnDesc(ctypes.Structure):
     """leco_fix_column_desc (include/leco_fix.h)."""

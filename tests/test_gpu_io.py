@@ -1,0 +1,100 @@
+"""GPU: SOSD files into HBM and column files through save / load, against
+numpy-written files, the original column and the oracle."""
+import struct
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", file=sys.stderr, flush=True)
+
+
+def to_dev(torch, a):
+    st = np.int32 if a.dtype == np.uint32 else np.int64
+    return torch.from_numpy(np.ascontiguousarray(a).view(st)).to("cuda")
+
+
+@pytest.mark.parametrize("width", [32, 64])
+def test_sosd_load_save_round_trip(tmp_path, oracle_mod, width):
+    import torch
+    from leco_fix import Column, datagen, io
+    # 20M values: more than one 64 MB staging piece for both widths
+    n = 20_000_000
+    vals = datagen.sorted_uniform(n, width, 5)
+    p = tmp_path / "keys.bin"
+    p.write_bytes(struct.pack("<Q", n) + vals.tobytes())
+    say(f"sosd load width={width} n={n}")
+    dev = io.load_sosd(p, width)
+    assert dev.numel() == n
+    assert np.array_equal(dev.cpu().numpy().view(vals.dtype), vals)
+    q = tmp_path / "again.bin"
+    io.save_sosd(q, dev)
+    assert q.read_bytes() == p.read_bytes()
+    # the reference's flow: load, encode, compare with the oracle
+    small = vals[:300_000]
+    p2 = tmp_path / "small.bin"
+    p2.write_bytes(struct.pack("<Q", len(small)) + small.tobytes())
+    col = Column.encode(io.load_sosd(p2, width), 1000)
+    st, _ = oracle_mod.encode_column(small, 1000, threads=8)
+    assert col.stream_bytes() == st.tobytes()
+
+
+@pytest.mark.parametrize("width,bs", [(64, 4096), (32, 256), (64, 7)])
+def test_column_file_round_trip(tmp_path, width, bs):
+    import torch
+    from leco_fix import Column, datagen, io
+    from leco_fix.column import PARAM_BYTES
+    n = 9_000_000 if bs == 4096 else 600_000
+    vals = datagen.sorted_uniform(n, width, bs) if bs != 7 else datagen.stepwise(n, width, 3)
+    col = Column.encode(to_dev(torch, vals), bs)
+    p = tmp_path / "col.leco"
+    say(f"column file width={width} bs={bs} n={n}")
+    io.save_column(p, col)
+    info = io.column_file_info(p)
+    assert info == {"n": n, "num_blocks": col.num_blocks, "stream_bytes": col.nbytes, "block_size": bs,
+                    "width": width}
+    back = io.load_column(p)
+    assert back.stream_bytes() == col.stream_bytes()
+    assert torch.equal(back.offsets, col.offsets)
+    assert torch.equal(back.params, col.params[:col.num_blocks * PARAM_BYTES])
+    assert np.array_equal(back.decode().cpu().numpy().view(vals.dtype), vals)
+    idx = torch.randint(0, n, (100_000,), device="cuda")
+    assert torch.equal(back.lookup(idx), col.lookup(idx))
+
+
+def test_column_file_from_batch_arena(tmp_path):
+    import torch
+    from leco_fix import datagen, io
+    from leco_fix.batch import Batch
+    cols = [datagen.sorted_uniform(50_000, 64, 1), datagen.stepwise(70_000, 32, 2)]
+    b = Batch([to_dev(torch, v) for v in cols], [1000, 512])
+    b.encode()
+    torch.cuda.synchronize()
+    for c, v in enumerate(cols):
+        col = b.column(c)
+        p = tmp_path / f"c{c}.leco"
+        say(f"column file from batch arena column {c} (base {col.base})")
+        io.save_column(p, col)
+        back = io.load_column(p)
+        assert back.stream_bytes() == b.column_bytes(c)
+        assert np.array_equal(back.decode().cpu().numpy().view(v.dtype), v)
+
+
+def test_column_load_rejects_small_buffers(tmp_path):
+    import torch
+    from leco_fix import Column, datagen, io
+    from leco_fix._lib import LecoError, lib
+    vals = datagen.sorted_uniform(100_000, 64, 9)
+    col = Column.encode(to_dev(torch, vals), 1000)
+    p = tmp_path / "col.leco"
+    io.save_column(p, col)
+    st = torch.empty(col.nbytes, dtype=torch.uint8, device="cuda")      # no room for the tail pad
+    offs = torch.empty(col.num_blocks + 1, dtype=torch.int64, device="cuda")
+    rc = lib().leco_fix_column_load(bytes(p), st.data_ptr(), st.numel(), offs.data_ptr(), None, None)
+    assert rc == 1
+    with pytest.raises(LecoError):
+        io.load_sosd(tmp_path / "missing.bin", 64)
