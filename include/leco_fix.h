@@ -196,6 +196,74 @@ int leco_fix_filter_close_block_u64(const uint8_t *h_in, uint64_t length, uint32
                                     int32_t block_id, uint32_t block_size, uint64_t filter1, uint64_t filter2,
                                     int32_t base, uint64_t *count);
 
+/* ---- fixed-partition baselines: FOR_int<T> and Delta_int<T> ----------- */
+
+/* The reference's FOR_int<T> (headers/FOR_integer_template.h:14-250) and
+ * Delta_int<T> (headers/delta_integer_template.h:14-160): codec is
+ * LECO_FIX_CODEC_FOR or LECO_FIX_CODEC_DELTA.  Columns use the LeCo layout
+ * (blocks back to back, d_offsets[b] = block b's first byte, nb+1 offsets;
+ * d_offsets[nb] = stream size after encode); block b's bytes are exactly what
+ * the reference's encodeArray8_int writes for it, except that Delta's last
+ * packed field (read past the reference's vector of n-1 deltas,
+ * delta_integer_template.h:81) is zero.  Decoding differs from the reference
+ * where the reference is wrong: FOR blocks of width 8*sizeof(T)-1 are packed
+ * (its decodeArray8 copies them as raw, :88) and Delta raw blocks are raw (its
+ * decodeArray8 reads them as packed, :98-126).  The one-block host calls stage
+ * through HBM like the LeCo ones; random_decode takes the block length. */
+#define LECO_FIX_CODEC_FOR 1
+#define LECO_FIX_CODEC_DELTA 2
+uint64_t leco_fix_base_stream_bound(int codec, uint64_t n, uint32_t block_size, uint32_t width_bits);
+int leco_fix_base_encode_column_u32(int codec, const uint32_t *d_in, uint64_t n, uint32_t block_size,
+                                    uint8_t *d_stream, uint64_t stream_capacity, uint64_t *d_offsets, void *stream);
+int leco_fix_base_encode_column_u64(int codec, const uint64_t *d_in, uint64_t n, uint32_t block_size,
+                                    uint8_t *d_stream, uint64_t stream_capacity, uint64_t *d_offsets, void *stream);
+int leco_fix_base_decode_column_u32(int codec, const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                    uint32_t block_size, uint32_t *d_out, void *stream);
+int leco_fix_base_decode_column_u64(int codec, const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                    uint32_t block_size, uint64_t *d_out, void *stream);
+int leco_fix_base_lookup_u32(int codec, const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                             uint32_t block_size, const uint64_t *d_idx, uint64_t nq, uint32_t *d_out, void *stream);
+int leco_fix_base_lookup_u64(int codec, const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                             uint32_t block_size, const uint64_t *d_idx, uint64_t nq, uint64_t *d_out, void *stream);
+/* FOR_int<T>::filter_range (:111-158: every position, value > filter; a block
+ * whose max <= filter is skipped) and filter_range_close (:160-216: value %
+ * base in (filter1, filter2), base > 0), over every block; results and
+ * workspace as leco_fix_filter_range_*. */
+int leco_fix_for_filter_range_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                  uint32_t block_size, uint32_t filter, uint32_t *d_out, uint64_t out_capacity,
+                                  uint64_t *d_result, void *d_work, uint64_t work_bytes, void *stream);
+int leco_fix_for_filter_range_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                  uint32_t block_size, uint64_t filter, uint32_t *d_out, uint64_t out_capacity,
+                                  uint64_t *d_result, void *d_work, uint64_t work_bytes, void *stream);
+int leco_fix_for_filter_range_close_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                        uint32_t block_size, uint32_t filter1, uint32_t filter2, int32_t base,
+                                        uint32_t *d_out, uint64_t out_capacity, uint64_t *d_result, void *d_work,
+                                        uint64_t work_bytes, void *stream);
+int leco_fix_for_filter_range_close_u64(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                        uint32_t block_size, uint64_t filter1, uint64_t filter2, int32_t base,
+                                        uint32_t *d_out, uint64_t out_capacity, uint64_t *d_result, void *d_work,
+                                        uint64_t work_bytes, void *stream);
+int leco_fix_base_encode_block_u32(int codec, const uint32_t *h_in, uint64_t length, uint8_t *h_res,
+                                   uint64_t res_capacity, uint64_t *written);
+int leco_fix_base_encode_block_u64(int codec, const uint64_t *h_in, uint64_t length, uint8_t *h_res,
+                                   uint64_t res_capacity, uint64_t *written);
+int leco_fix_base_decode_block_u32(int codec, const uint8_t *h_in, uint64_t length, uint32_t *h_out);
+int leco_fix_base_decode_block_u64(int codec, const uint8_t *h_in, uint64_t length, uint64_t *h_out);
+int leco_fix_base_random_decode_u32(int codec, const uint8_t *h_in, uint64_t length, uint32_t to_find,
+                                    uint32_t *h_value);
+int leco_fix_base_random_decode_u64(int codec, const uint8_t *h_in, uint64_t length, uint32_t to_find,
+                                    uint64_t *h_value);
+int leco_fix_for_filter_block_u32(const uint8_t *h_in, uint64_t length, uint32_t filter, uint32_t *h_out,
+                                  uint64_t out_capacity, int32_t block_id, uint32_t block_size, uint64_t *count);
+int leco_fix_for_filter_block_u64(const uint8_t *h_in, uint64_t length, uint64_t filter, uint32_t *h_out,
+                                  uint64_t out_capacity, int32_t block_id, uint32_t block_size, uint64_t *count);
+int leco_fix_for_filter_close_block_u32(const uint8_t *h_in, uint64_t length, uint32_t *h_out,
+                                        uint64_t out_capacity, int32_t block_id, uint32_t block_size,
+                                        uint32_t filter1, uint32_t filter2, int32_t base, uint64_t *count);
+int leco_fix_for_filter_close_block_u64(const uint8_t *h_in, uint64_t length, uint32_t *h_out,
+                                        uint64_t out_capacity, int32_t block_id, uint32_t block_size,
+                                        uint64_t filter1, uint64_t filter2, int32_t base, uint64_t *count);
+
 /* ---- files ------------------------------------------------------------- */
 
 /* SOSD binary, the reference's input format (experiments/leco_int_fix_template.cpp:21-40,

@@ -23,7 +23,8 @@ LIB = os.path.join(LIBDIR, "libleco_fix.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["leco_kernels.hip", "leco_filter.hip", "leco_api.hip", "leco_batch.hip", "leco_serve.hip", "leco_io.hip"]
+SOURCES = ["leco_kernels.hip", "leco_filter.hip", "leco_api.hip", "leco_batch.hip", "leco_serve.hip", "leco_io.hip",
+           "leco_base.hip"]
 CFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -64,10 +65,16 @@ DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "leco_int_driver.cpp")
 DRIVER = os.path.join(ROOT, "tests", "cpp", "leco_int_driver")
 
 
+BASE_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "base_driver.cpp")
+BASE_DRIVER = os.path.join(ROOT, "tests", "cpp", "base_driver")
+
+
 def build_driver() -> str:
-    """The C++ driver over include/leco_int.hpp (host code: g++ links the library)."""
-    _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), DRIVER_SRC,
-          "-o", DRIVER, "-L", LIBDIR, "-lleco_fix", "-Wl,-rpath,$ORIGIN/../../learn-to-compress_amd/lib"])
+    """The C++ drivers over include/leco_int.hpp and include/leco_baselines.hpp
+    (host code: g++ links the library)."""
+    for src, exe in ((DRIVER_SRC, DRIVER), (BASE_DRIVER_SRC, BASE_DRIVER)):
+        _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src,
+              "-o", exe, "-L", LIBDIR, "-lleco_fix", "-Wl,-rpath,$ORIGIN/../../learn-to-compress_amd/lib"])
     return DRIVER
 
 

@@ -241,9 +241,222 @@ int filter_block_host(const uint8_t *h_in, uint64_t len, bool close, uint64_t fi
     }
 }
 
+// ---- FOR_int / Delta_int baselines ----
+
+bool valid_codec(int codec) { return codec == leco::kCodecFor || codec == leco::kCodecDelta; }
+
+uint64_t base_block_bound(int codec, uint32_t w, uint32_t m) {
+    uint64_t worst = 0;
+    for (uint32_t l = 0; l <= w; ++l) {
+        const uint64_t b = codec == leco::kCodecFor ? leco::for_block_bytes(l, w, m) : leco::delta_block_bytes(l, w, m);
+        worst = b > worst ? b : worst;
+    }
+    return worst;
+}
+
+// Bytes of a stored FOR / Delta block of `len` values, from its header byte.
+template <typename T>
+uint64_t base_host_block_bytes(int codec, const uint8_t *h_in, uint64_t len) {
+    constexpr uint32_t W = 8 * sizeof(T);
+    const uint32_t l = h_in[0];
+    if (codec == leco::kCodecDelta && (l == 126 || l == 127)) return 1 + (128 - l) * sizeof(T);
+    return codec == leco::kCodecFor ? leco::for_block_bytes(l, W, (uint32_t)len)
+                                    : leco::delta_block_bytes(l, W, (uint32_t)len);
+}
+
+template <typename T>
+int base_encode_block_host(int codec, const T *h, uint64_t len, uint8_t *res, uint64_t rcap, uint64_t *written) {
+    if (!valid_codec(codec) || !h || !res || !written || len == 0 || len > kMaxBlockSize) return LECO_FIX_EINVAL;
+    constexpr uint32_t W = 8 * sizeof(T);
+    const size_t in_b = up256(len * sizeof(T));
+    const size_t st_b = up256(base_block_bound(codec, W, (uint32_t)len) + LECO_FIX_TAIL_PAD);
+    const size_t off_b = up256(2 * sizeof(uint64_t));
+    int rc = g_stage.reserve(in_b + st_b + off_b);
+    if (rc) return rc;
+    uint8_t *base = (uint8_t *)g_stage.dev;
+    T *d_in = (T *)base;
+    uint8_t *d_st = base + in_b;
+    uint64_t *d_off = (uint64_t *)(base + in_b + st_b);
+    hipStream_t s = g_stage.stream;
+    if (hipMemcpyAsync(d_in, h, len * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (leco::launch_base_encode<T>(codec, d_in, len, (uint32_t)len, d_st, d_off, s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    uint64_t offs[2];
+    if (hipMemcpyAsync(offs, d_off, sizeof(offs), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    if (offs[1] > rcap) return LECO_FIX_EINVAL;
+    if (hipMemcpyAsync(res, d_st, offs[1], hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    *written = offs[1];
+    return LECO_FIX_OK;
+}
+
+// The block staged to the device with a 2-entry offsets table; returns the staged pointers.
+template <typename T>
+int stage_block(int codec, const uint8_t *h_in, uint64_t len, size_t extra, uint8_t **d_st, uint64_t **d_off,
+                uint8_t **d_extra) {
+    const uint64_t bytes = base_host_block_bytes<T>(codec, h_in, len);
+    const size_t st_b = up256(bytes + LECO_FIX_TAIL_PAD);
+    const size_t off_b = up256(2 * sizeof(uint64_t));
+    int rc = g_stage.reserve(st_b + off_b + up256(extra));
+    if (rc) return rc;
+    uint8_t *base = (uint8_t *)g_stage.dev;
+    *d_st = base;
+    *d_off = (uint64_t *)(base + st_b);
+    *d_extra = base + st_b + off_b;
+    hipStream_t s = g_stage.stream;
+    const uint64_t offs[2] = {0, bytes};
+    if (hipMemcpyAsync(*d_st, h_in, bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(*d_st + bytes, 0, st_b - bytes, s) != hipSuccess ||
+        hipMemcpyAsync(*d_off, offs, sizeof(offs), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    return LECO_FIX_OK;
+}
+
+template <typename T>
+int base_decode_block_host(int codec, const uint8_t *h_in, uint64_t len, T *h_out) {
+    if (!valid_codec(codec) || !h_in || !h_out || len == 0 || len > kMaxBlockSize) return LECO_FIX_EINVAL;
+    uint8_t *d_st, *d_x;
+    uint64_t *d_off;
+    int rc = stage_block<T>(codec, h_in, len, len * sizeof(T), &d_st, &d_off, &d_x);
+    if (rc) return rc;
+    hipStream_t s = g_stage.stream;
+    if (leco::launch_base_decode<T>(codec, d_st, d_off, len, (uint32_t)len, (T *)d_x, s) != hipSuccess ||
+        hipMemcpyAsync(h_out, d_x, len * sizeof(T), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    return hip_status(hipStreamSynchronize(s));
+}
+
+template <typename T>
+int base_random_decode_host(int codec, const uint8_t *h_in, uint64_t len, uint32_t to_find, T *h_value) {
+    if (!valid_codec(codec) || !h_in || !h_value || len == 0 || len > kMaxBlockSize || to_find >= len)
+        return LECO_FIX_EINVAL;
+    uint8_t *d_st, *d_x;
+    uint64_t *d_off;
+    int rc = stage_block<T>(codec, h_in, len, 16, &d_st, &d_off, &d_x);
+    if (rc) return rc;
+    hipStream_t s = g_stage.stream;
+    const uint64_t q = to_find;
+    if (hipMemcpyAsync(d_x, &q, sizeof(q), hipMemcpyHostToDevice, s) != hipSuccess ||
+        leco::launch_base_lookup<T>(codec, d_st, d_off, len, (uint32_t)len, (const uint64_t *)d_x, 1, (T *)(d_x + 8),
+                                    s) != hipSuccess ||
+        hipMemcpyAsync(h_value, d_x + 8, sizeof(T), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    return hip_status(hipStreamSynchronize(s));
+}
+
+template <typename T>
+int for_filter_block_host(const uint8_t *h_in, uint64_t len, bool close, uint64_t f1, uint64_t f2, int32_t base,
+                          uint32_t *h_out, uint64_t cap, int32_t block_id, uint32_t block_size, uint64_t *count) {
+    if (!h_in || !count || len == 0 || len > kMaxBlockSize || (cap && !h_out) || (close && base <= 0))
+        return LECO_FIX_EINVAL;
+    const uint64_t dcap = cap < len ? cap : len;
+    uint8_t *d_st, *d_x;
+    uint64_t *d_off;
+    int rc = stage_block<T>(leco::kCodecFor, h_in, len, 64 + (dcap ? dcap : 1) * 4, &d_st, &d_off, &d_x);
+    if (rc) return rc;
+    hipStream_t s = g_stage.stream;
+    uint64_t *d_res = (uint64_t *)d_x, *d_work = d_res + 2;
+    uint32_t *d_out = (uint32_t *)(d_x + 64);
+    uint64_t res[2];
+    if (leco::launch_for_filter<T>(d_st, d_off, len, (uint32_t)len, close, f1, f2, base,
+                                   (uint32_t)block_id * block_size, d_out, dcap, d_res, d_work, s) != hipSuccess ||
+        hipMemcpyAsync(res, d_res, sizeof(res), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    const uint64_t ncopy = res[0] < cap ? res[0] : cap;
+    if (ncopy && (hipMemcpyAsync(h_out, d_out, ncopy * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                  hipStreamSynchronize(s) != hipSuccess))
+        return LECO_FIX_EHIP;
+    *count = res[0];
+    return LECO_FIX_OK;
+}
+
+template <typename T>
+int for_filter_column(const uint8_t *d_stream, const uint64_t *d_offs, uint64_t n, uint32_t bs, bool close,
+                      uint64_t f1, uint64_t f2, int32_t base, uint32_t *d_out, uint64_t cap, uint64_t *d_result,
+                      void *d_work, uint64_t work_bytes, void *st) {
+    if (n == 0 || n > (1ull << 32) || !valid_bs(bs) || !d_stream || !d_offs || !d_result || !d_work ||
+        (cap && !d_out) || (close && base <= 0))
+        return LECO_FIX_EINVAL;
+    if (work_bytes < leco_fix_filter_workspace_bytes(n, bs)) return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_for_filter<T>(d_stream, d_offs, n, bs, close, f1, f2, base, 0, d_out, cap,
+                                                 d_result, (uint64_t *)d_work, (hipStream_t)st));
+}
+
 }  // namespace
 
 extern "C" {
+
+uint64_t leco_fix_base_stream_bound(int codec, uint64_t n, uint32_t bs, uint32_t w) {
+    if (!valid_codec(codec) || n == 0 || bs == 0 || (w != 32 && w != 64)) return 0;
+    const uint64_t nb = (n + bs - 1) / bs, last = n - (nb - 1) * bs;
+    return (nb - 1) * base_block_bound(codec, w, bs) + base_block_bound(codec, w, (uint32_t)last) +
+           LECO_FIX_TAIL_PAD;
+}
+
+#define LECO_BASE_ABI(SFX, T)                                                                                      \
+    int leco_fix_base_encode_column_##SFX(int codec, const T *d_in, uint64_t n, uint32_t bs, uint8_t *d_stream,    \
+                                          uint64_t cap, uint64_t *d_offsets, void *stream) {                       \
+        if (!valid_codec(codec) || n == 0 || !valid_bs(bs) || !d_in || !d_stream || !d_offsets ||                  \
+            cap < leco_fix_base_stream_bound(codec, n, bs, 8 * sizeof(T)))                                         \
+            return LECO_FIX_EINVAL;                                                                                \
+        return hip_status(leco::launch_base_encode<T>(codec, d_in, n, bs, d_stream, d_offsets, (hipStream_t)stream)); \
+    }                                                                                                              \
+    int leco_fix_base_decode_column_##SFX(int codec, const uint8_t *d_stream, const uint64_t *d_offsets,           \
+                                          uint64_t n, uint32_t bs, T *d_out, void *stream) {                       \
+        if (!valid_codec(codec) || n == 0 || !valid_bs(bs) || !d_stream || !d_offsets || !d_out)                   \
+            return LECO_FIX_EINVAL;                                                                                \
+        return hip_status(leco::launch_base_decode<T>(codec, d_stream, d_offsets, n, bs, d_out, (hipStream_t)stream)); \
+    }                                                                                                              \
+    int leco_fix_base_lookup_##SFX(int codec, const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,      \
+                                   uint32_t bs, const uint64_t *d_idx, uint64_t nq, T *d_out, void *stream) {      \
+        if (!valid_codec(codec) || n == 0 || !valid_bs(bs) || !d_stream || !d_offsets) return LECO_FIX_EINVAL;     \
+        if (nq == 0) return LECO_FIX_OK;                                                                           \
+        if (!d_idx || !d_out) return LECO_FIX_EINVAL;                                                              \
+        return hip_status(                                                                                         \
+            leco::launch_base_lookup<T>(codec, d_stream, d_offsets, n, bs, d_idx, nq, d_out, (hipStream_t)stream)); \
+    }                                                                                                              \
+    int leco_fix_for_filter_range_##SFX(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,            \
+                                        uint32_t bs, T filter, uint32_t *d_out, uint64_t cap, uint64_t *d_result,  \
+                                        void *d_work, uint64_t work_bytes, void *stream) {                         \
+        return for_filter_column<T>(d_stream, d_offsets, n, bs, false, filter, 0, 1, d_out, cap, d_result, d_work, \
+                                    work_bytes, stream);                                                           \
+    }                                                                                                              \
+    int leco_fix_for_filter_range_close_##SFX(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,      \
+                                              uint32_t bs, T filter1, T filter2, int32_t base, uint32_t *d_out,    \
+                                              uint64_t cap, uint64_t *d_result, void *d_work, uint64_t work_bytes, \
+                                              void *stream) {                                                      \
+        return for_filter_column<T>(d_stream, d_offsets, n, bs, true, filter1, filter2, base, d_out, cap, d_result, \
+                                    d_work, work_bytes, stream);                                                   \
+    }                                                                                                              \
+    int leco_fix_base_encode_block_##SFX(int codec, const T *h, uint64_t len, uint8_t *res, uint64_t cap,          \
+                                         uint64_t *written) {                                                      \
+        return base_encode_block_host<T>(codec, h, len, res, cap, written);                                        \
+    }                                                                                                              \
+    int leco_fix_base_decode_block_##SFX(int codec, const uint8_t *h_in, uint64_t len, T *h_out) {                 \
+        return base_decode_block_host<T>(codec, h_in, len, h_out);                                                 \
+    }                                                                                                              \
+    int leco_fix_base_random_decode_##SFX(int codec, const uint8_t *h_in, uint64_t len, uint32_t to_find,          \
+                                          T *h_value) {                                                            \
+        return base_random_decode_host<T>(codec, h_in, len, to_find, h_value);                                     \
+    }                                                                                                              \
+    int leco_fix_for_filter_block_##SFX(const uint8_t *h_in, uint64_t len, T filter, uint32_t *h_out, uint64_t cap, \
+                                        int32_t block_id, uint32_t block_size, uint64_t *count) {                  \
+        return for_filter_block_host<T>(h_in, len, false, filter, 0, 1, h_out, cap, block_id, block_size, count);  \
+    }                                                                                                              \
+    int leco_fix_for_filter_close_block_##SFX(const uint8_t *h_in, uint64_t len, uint32_t *h_out, uint64_t cap,    \
+                                              int32_t block_id, uint32_t block_size, T filter1, T filter2,         \
+                                              int32_t base, uint64_t *count) {                                     \
+        return for_filter_block_host<T>(h_in, len, true, filter1, filter2, base, h_out, cap, block_id, block_size, \
+                                        count);                                                                    \
+    }
+LECO_BASE_ABI(u32, uint32_t)
+LECO_BASE_ABI(u64, uint64_t)
+#undef LECO_BASE_ABI
 
 uint64_t leco_fix_filter_workspace_bytes(uint64_t n, uint32_t block_size) {
     if (n == 0 || block_size == 0) return 0;

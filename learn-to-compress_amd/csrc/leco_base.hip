@@ -1,0 +1,319 @@
+// gfx950 kernels of the fixed-partition baselines FOR_int<T> and Delta_int<T>
+// (headers/FOR_integer_template.h:14-250, delta_integer_template.h:14-160 of
+// the reference), on the same column layout as the LeCo codec: the blocks'
+// byte strings back to back, block b at offs[b].
+//
+// Encode: one workgroup per block in ticket order -- block statistics (FOR:
+// min and max; Delta: the largest |y[i+1] - y[i]|), bit width and size, the
+// decoupled look-back for the block's offset, then the LDS-image packer shared
+// with the LeCo encoder.  Decode: one workgroup per block; Delta decodes by a
+// workgroup prefix sum of the signed deltas (mod 2^w, so any summation order
+// gives the reference's sequential result).  Lookups: one lane per query (FOR)
+// or one wave per query summing the deltas before it (Delta).
+#include "leco_kernels.h"
+
+namespace leco {
+
+namespace {
+
+__device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) { return ~wave_max_u64(~v); }
+
+template <int W>
+__device__ __forceinline__ uint64_t wmask() {
+    return W == 32 ? 0xFFFFFFFFull : ~0ull;
+}
+
+// Signed delta field of y[i+1] - y[i] on l bits (write_delta_int_T, bit_write.h:258-300).
+template <int W>
+__device__ __forceinline__ uint64_t delta_field(uint64_t a, uint64_t b, int l) {
+    const bool up = b > a;
+    const uint64_t mag = up ? b - a : a - b;
+    return (mag & ((1ull << (l - 1)) - 1)) | ((uint64_t)up << (l - 1));
+}
+
+template <int CODEC, typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_base_encode(const T *__restrict__ y_all, uint64_t n, uint32_t bs,
+                                                          uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
+    __shared__ EncodeShared sh;
+    __shared__ uint64_t s_b;
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    if (tid == 0) s_b = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+    __syncthreads();
+    const uint64_t b = s_b;
+    const uint64_t s = b * bs;
+    const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
+    const T *y = y_all + s;
+
+    // statistics
+    uint64_t lo = ~0ull, hi = 0;
+    for (uint32_t j = tid; j < m; j += kThreads) {
+        const uint64_t v = (uint64_t)y[j];
+        if (CODEC == kCodecFor) {
+            lo = v < lo ? v : lo;
+            hi = v > hi ? v : hi;
+        } else if (j + 1 < m) {
+            const uint64_t u = (uint64_t)y[j + 1];
+            const uint64_t d = (u > v ? u - v : v - u) & wmask<W>();
+            hi = d > hi ? d : hi;
+        }
+    }
+    hi = wave_max_u64(hi);
+    if (CODEC == kCodecFor) lo = wave_min_u64(lo);
+    if (lane == 0) {
+        sh.rmag[wid] = hi;
+        sh.rmax[wid] = (int64_t)lo;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kThreads / kWave; ++w) {
+        hi = sh.rmag[w] > hi ? sh.rmag[w] : hi;
+        if (CODEC == kCodecFor) lo = (uint64_t)sh.rmax[w] < lo ? (uint64_t)sh.rmax[w] : lo;
+    }
+    int l;
+    if (CODEC == kCodecFor) l = hi - lo ? bitlen64(hi - lo) : 0;
+    else l = hi ? bitlen64(hi) + 1 : 0;
+    if (l > W) l = W;
+    const uint64_t size = CODEC == kCodecFor ? for_block_bytes((uint32_t)l, W, m) : delta_block_bytes((uint32_t)l, W, m);
+
+    if (wid == 0) {
+        const uint64_t excl = look_back(b, size, offs);
+        if (lane == 0) {
+            sh.off = excl;
+            sh.end = excl + size;
+        }
+    }
+    __syncthreads();
+
+    const bool raw = l == W;
+    constexpr uint32_t tb = W / 8;
+    PackedBlock pb;
+    pb.off = sh.off;
+    pb.end = sh.end;
+    pb.m = m;
+    pb.l = l;
+    pb.nchunks = l == 0 ? 1 : (m + kChunk - 1) / kChunk;
+    if (raw) {
+        pb.hdr = 1;
+        pb.t0b = pb.t1b = 0;
+    } else if (CODEC == kCodecFor) {
+        pb.hdr = 1 + 2 * tb;
+        pb.t0b = W == 64 ? lo : (lo | (hi << 32));
+        pb.t1b = W == 64 ? hi : 0;
+    } else {
+        pb.hdr = 1 + tb;
+        pb.t0b = (uint64_t)y[0];
+        pb.t1b = 0;
+    }
+    auto sync = [] { __syncthreads(); };
+    for (uint32_t c = 0; c < pb.nchunks; ++c)
+        pack_chunk<kThreads, kChunk, false>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) -> uint64_t {
+            const uint64_t v = (uint64_t)y[i];
+            if (raw) return v;
+            if (CODEC == kCodecFor) return v - lo;
+            // n fields: the last one, past the reference's vector of n-1 deltas, is zero
+            return i + 1 < m ? delta_field<W>(v, (uint64_t)y[i + 1], l) : 0;
+        });
+}
+
+__global__ __launch_bounds__(kThreads) void k_base_clear(uint64_t *__restrict__ offs, uint64_t words) {
+    const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (i < words) offs[i] = 0;
+}
+
+// Block end offsets from the look-back words (flags cleared), the ticket back to 0.
+__global__ __launch_bounds__(kThreads) void k_base_fixup(uint64_t *__restrict__ offs, uint64_t nb) {
+    const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (b >= nb) return;
+    offs[b + 1] &= kStatusValue;
+    if (b == 0) offs[0] = 0;
+}
+
+// FOR_int<T>::decodeArray8 (:79-108) / Delta_int<T>::decodeArray8 (:85-126).
+template <int CODEC, typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_base_decode(const uint8_t *__restrict__ st,
+                                                          const uint64_t *__restrict__ offs, uint64_t n, uint32_t bs,
+                                                          T *__restrict__ out) {
+    __shared__ uint64_t wsum[kThreads / kWave];
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    const uint64_t b = blockIdx.x;
+    const uint64_t s = b * bs;
+    const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
+    const uint8_t *blk = st + offs[b];
+    const int l = blk[0];
+    T *o = out + s;
+    constexpr uint32_t tb = W / 8;
+    if (CODEC == kCodecDelta && (l == 127 || l == 126)) {  // :98-113, one- and two-value raw blocks
+        const uint32_t cnt = l == 127 ? 1 : 2;
+        if ((uint32_t)tid < cnt && (uint32_t)tid < m) o[tid] = (T)load_bits(blk + 1, (uint64_t)tid * W, W);
+        return;
+    }
+    if (l >= W) {
+        for (uint32_t j = tid; j < m; j += kThreads) o[j] = (T)load_bits(blk + 1, (uint64_t)j * W, W);
+        return;
+    }
+    const uint64_t base = load_u64_any(blk + 1) & wmask<W>();
+    if (CODEC == kCodecFor) {
+        const uint8_t *pay = blk + 1 + 2 * tb;
+        for (uint32_t j = tid; j < m; j += kThreads) o[j] = (T)(l ? base + load_bits(pay, (uint64_t)j * l, l) : base);
+        return;
+    }
+    // Delta: o[0] = y_0, o[i] = y_0 + (signed deltas 0 .. i-1)
+    if (tid == 0) o[0] = (T)base;
+    if (!l) {
+        for (uint32_t j = 1 + tid; j < m; j += kThreads) o[j] = (T)base;
+        return;
+    }
+    const uint8_t *pay = blk + 1 + tb;
+    constexpr uint32_t K = 8;  // consecutive deltas per thread per step
+    uint64_t carry = base;
+    for (uint32_t c0 = 0; c0 + 1 < m; c0 += kThreads * K) {
+        const uint32_t d0 = c0 + tid * K;
+        uint64_t d[K], run = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k) {
+            d[k] = 0;
+            if (d0 + k + 1 < m) {
+                const uint64_t f = load_bits(pay, (uint64_t)(d0 + k) * l, l);
+                const uint64_t mag = f & ((1ull << (l - 1)) - 1);
+                d[k] = ((f >> (l - 1)) & 1) ? mag : 0 - mag;
+            }
+            run += d[k];
+        }
+        // exclusive prefix of the threads' sums (wrapping uint64)
+        uint64_t inc = run;
+        for (int sft = 1; sft < kWave; sft <<= 1) {
+            const uint64_t t = (uint64_t)__shfl_up((long long)inc, sft, kWave);
+            if (lane >= sft) inc += t;
+        }
+        if (lane == kWave - 1) wsum[wid] = inc;
+        __syncthreads();
+        uint64_t before = 0, all = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / kWave; ++w) {
+            before += w < wid ? wsum[w] : 0;
+            all += wsum[w];
+        }
+        __syncthreads();
+        uint64_t v = carry + before + inc - run;
+#pragma unroll
+        for (uint32_t k = 0; k < K; ++k) {
+            v += d[k];
+            if (d0 + k + 1 < m) o[d0 + k + 1] = (T)v;
+        }
+        carry += all;
+    }
+}
+
+// FOR_int<T>::randomdecodeArray8 (:219-246): one lane per query.
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_for_lookup(const uint8_t *__restrict__ st,
+                                                         const uint64_t *__restrict__ offs, uint64_t n, uint32_t bs,
+                                                         const uint64_t *__restrict__ idx, uint64_t nq,
+                                                         T *__restrict__ out) {
+    const uint64_t q = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (q >= nq) return;
+    const uint64_t i = idx[q];
+    if (i >= n) {
+        out[q] = 0;
+        return;
+    }
+    const uint64_t b = i / bs;
+    const uint32_t j = (uint32_t)(i - b * bs);
+    const uint8_t *blk = st + offs[b];
+    const int l = blk[0];
+    if (l == W) {
+        out[q] = (T)load_bits(blk + 1, (uint64_t)j * W, W);
+        return;
+    }
+    const uint64_t base = load_u64_any(blk + 1) & wmask<W>();
+    out[q] = (T)(l ? base + load_bits(blk + 1 + 2 * (W / 8), (uint64_t)j * l, l) : base);
+}
+
+// Delta_int<T>::randomdecodeArray8 (:131-153) with read_Delta_int: one wave per
+// query sums the signed deltas before position j.
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_delta_lookup(const uint8_t *__restrict__ st,
+                                                           const uint64_t *__restrict__ offs, uint64_t n,
+                                                           uint32_t bs, const uint64_t *__restrict__ idx, uint64_t nq,
+                                                           T *__restrict__ out) {
+    const int lane = threadIdx.x % kWave;
+    const uint64_t q = (uint64_t)blockIdx.x * (kThreads / kWave) + threadIdx.x / kWave;
+    if (q >= nq) return;
+    const uint64_t i = idx[q];
+    if (i >= n) {
+        if (lane == 0) out[q] = 0;
+        return;
+    }
+    const uint64_t b = i / bs;
+    const uint32_t j = (uint32_t)(i - b * bs);
+    const uint8_t *blk = st + offs[b];
+    const int l = blk[0];
+    if (l == W) {
+        if (lane == 0) out[q] = (T)load_bits(blk + 1, (uint64_t)j * W, W);
+        return;
+    }
+    const uint64_t base = load_u64_any(blk + 1) & wmask<W>();
+    uint64_t sum = 0;
+    if (l)
+        for (uint32_t k = lane; k < j; k += kWave) {
+            const uint64_t f = load_bits(blk + 1 + W / 8, (uint64_t)k * l, l);
+            const uint64_t mag = f & ((1ull << (l - 1)) - 1);
+            sum += ((f >> (l - 1)) & 1) ? mag : 0 - mag;
+        }
+    for (int o = kWave / 2; o > 0; o >>= 1) sum += (uint64_t)__shfl_xor((long long)sum, o, kWave);
+    if (lane == 0) out[q] = (T)(base + sum);
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_base_encode(int codec, const T *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t *offs,
+                              hipStream_t st) {
+    constexpr int W = 8 * sizeof(T);
+    const uint64_t nb = (n + bs - 1) / bs;
+    k_base_clear<<<dim3((unsigned)((nb + 1 + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(offs, nb + 1);
+    if (codec == kCodecFor)
+        k_base_encode<kCodecFor, T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(in, n, bs, offs, stream);
+    else
+        k_base_encode<kCodecDelta, T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(in, n, bs, offs, stream);
+    k_base_fixup<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(offs, nb);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_base_decode(int codec, const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
+                              T *out, hipStream_t st) {
+    constexpr int W = 8 * sizeof(T);
+    const uint64_t nb = (n + bs - 1) / bs;
+    if (codec == kCodecFor)
+        k_base_decode<kCodecFor, T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(stream, offs, n, bs, out);
+    else
+        k_base_decode<kCodecDelta, T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(stream, offs, n, bs, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_base_lookup(int codec, const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
+                              const uint64_t *idx, uint64_t nq, T *out, hipStream_t st) {
+    constexpr int W = 8 * sizeof(T);
+    if (codec == kCodecFor)
+        k_for_lookup<T, W><<<dim3((unsigned)((nq + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(
+            stream, offs, n, bs, idx, nq, out);
+    else
+        k_delta_lookup<T, W><<<dim3((unsigned)((nq + kThreads / kWave - 1) / (kThreads / kWave))), dim3(kThreads), 0,
+                               st>>>(stream, offs, n, bs, idx, nq, out);
+    return hipGetLastError();
+}
+
+#define LECO_BASE_INST(T)                                                                                          \
+    template hipError_t launch_base_encode<T>(int, const T *, uint64_t, uint32_t, uint8_t *, uint64_t *,           \
+                                              hipStream_t);                                                        \
+    template hipError_t launch_base_decode<T>(int, const uint8_t *, const uint64_t *, uint64_t, uint32_t, T *,     \
+                                              hipStream_t);                                                        \
+    template hipError_t launch_base_lookup<T>(int, const uint8_t *, const uint64_t *, uint64_t, uint32_t,          \
+                                              const uint64_t *, uint64_t, T *, hipStream_t);
+LECO_BASE_INST(uint32_t)
+LECO_BASE_INST(uint64_t)
+#undef LECO_BASE_INST
+
+}  // namespace leco

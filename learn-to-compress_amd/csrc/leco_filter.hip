@@ -66,8 +66,9 @@ struct FilterArgs {
     uint32_t bs;
     uint32_t kb;            // blocks per tile
     uint64_t tiles;
-    uint64_t filter;        // filter_range
-    const void *f1, *f2;    // filter_range_close: per-block starting filters (T)
+    uint64_t filter;        // filter_range; FOR filter_range_close: filter1
+    uint64_t filter2;       // FOR filter_range_close: filter2
+    const void *f1, *f2;    // LeCo filter_range_close: per-block starting filters (T)
     int32_t base;           // filter_range_close: period
     uint32_t pos_base;      // added to every reported position (uint32 arithmetic)
     uint32_t *out;
@@ -425,7 +426,42 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
     return !cut;
 }
 
-template <typename T, int W, bool CLOSE>
+// FOR_int<T>::filter_range (:111-158) and filter_range_close (:160-216) on
+// block b: every position, value = min + field (or the raw value); a packed
+// block whose max <= filter reports nothing (filter_range).  Blocks of width
+// w-1 are packed, not raw (the reference's `maxerror >= w-1` raw branch reads
+// them as raw values).
+template <typename T, int W, bool CLOSE, bool WRITE>
+__device__ __forceinline__ void for_block(const FilterArgs &a, FilterShared &sh, uint64_t b, uint32_t m,
+                                          uint32_t bstart, uint64_t &acc) {
+    constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
+    constexpr uint32_t tb = W / 8;
+    const uint8_t *blk = a.stream + a.offs[b];
+    const int l = blk[0];
+    const bool raw = l >= W;
+    const uint64_t f1 = a.filter & wmask, f2 = a.filter2 & wmask;
+    const uint64_t base_v = (uint64_t)(int64_t)a.base & wmask;
+    uint64_t lo = 0;
+    if (!raw) {
+        lo = load_u64_any(blk + 1) & wmask;
+        const uint64_t hi = load_u64_any(blk + 1 + tb) & wmask;
+        if (!CLOSE && hi <= f1) return;
+    }
+    emit_ordered<WRITE>(a, sh, m,
+                        [&](uint32_t i, uint32_t *where) {
+                            *where = bstart + i;
+                            uint64_t v;
+                            if (raw) v = load_bits(blk + 1, (uint64_t)i * W, W);
+                            else v = l ? (lo + load_bits(blk + 1 + 2 * tb, (uint64_t)i * l, l)) & wmask : lo;
+                            if (!CLOSE) return v > f1;
+                            const uint64_t r = v % base_v;
+                            return r > f1 && r < f2;
+                        },
+                        acc);
+}
+
+// MODE: 0 LeCo filter_range, 1 LeCo filter_range_close, 2 FOR filter_range, 3 FOR filter_range_close.
+template <typename T, int W, int MODE>
 __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     __shared__ FilterShared sh;
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
@@ -445,8 +481,9 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     bool ok = true;
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
-        if (CLOSE) ok &= close_block<T, W, false>(a, sh, b, block_len(b), k, 0, mine);
-        else gt_block<T, W, false>(a, sh, b, block_len(b), 0, mine);
+        if (MODE == 1) ok &= close_block<T, W, false>(a, sh, b, block_len(b), k, 0, mine);
+        else if (MODE == 0) gt_block<T, W, false>(a, sh, b, block_len(b), 0, mine);
+        else for_block<T, W, MODE == 3, false>(a, sh, b, block_len(b), 0, mine);
     }
     for (int o = kWave / 2; o > 0; o >>= 1) mine += (uint64_t)__shfl_xor((long long)mine, o, kWave);
     if (lane == 0) sh.wsum[wid] = mine;
@@ -469,8 +506,9 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
         const uint32_t bstart = a.pos_base + (uint32_t)(b * a.bs);
-        if (CLOSE) close_block<T, W, true>(a, sh, b, block_len(b), k, bstart, pos);
-        else gt_block<T, W, true>(a, sh, b, block_len(b), bstart, pos);
+        if (MODE == 1) close_block<T, W, true>(a, sh, b, block_len(b), k, bstart, pos);
+        else if (MODE == 0) gt_block<T, W, true>(a, sh, b, block_len(b), bstart, pos);
+        else for_block<T, W, MODE == 3, true>(a, sh, b, block_len(b), bstart, pos);
     }
 }
 
@@ -488,15 +526,37 @@ hipError_t launch_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n
                          uint64_t filter, const T *f1, const T *f2, int32_t base, uint32_t pos_base, uint32_t *out,
                          uint64_t cap, uint64_t *result, uint64_t *work, hipStream_t st) {
     constexpr int W = 8 * sizeof(T);
-    FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), filter_tiles(n, bs), filter, f1, f2, base, pos_base,
-                 out, cap, result, work};
+    FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), filter_tiles(n, bs), filter, 0, f1, f2, base,
+                 pos_base, out, cap, result, work};
     const uint64_t words = a.tiles + 1;
     k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
                                                                                                     result);
-    if (close) k_filter<T, W, true><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
-    else k_filter<T, W, false><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    if (close) k_filter<T, W, 1><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    else k_filter<T, W, 0><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_for_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, bool close,
+                             uint64_t f1, uint64_t f2, int32_t base, uint32_t pos_base, uint32_t *out, uint64_t cap,
+                             uint64_t *result, uint64_t *work, hipStream_t st) {
+    constexpr int W = 8 * sizeof(T);
+    FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), filter_tiles(n, bs), f1, f2, nullptr, nullptr, base,
+                 pos_base, out, cap, result, work};
+    const uint64_t words = a.tiles + 1;
+    k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
+                                                                                                    result);
+    if (close) k_filter<T, W, 3><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    else k_filter<T, W, 2><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_for_filter<uint32_t>(const uint8_t *, const uint64_t *, uint64_t, uint32_t, bool, uint64_t,
+                                                uint64_t, int32_t, uint32_t, uint32_t *, uint64_t, uint64_t *,
+                                                uint64_t *, hipStream_t);
+template hipError_t launch_for_filter<uint64_t>(const uint8_t *, const uint64_t *, uint64_t, uint32_t, bool, uint64_t,
+                                                uint64_t, int32_t, uint32_t, uint32_t *, uint64_t, uint64_t *,
+                                                uint64_t *, hipStream_t);
 
 template hipError_t launch_filter<uint32_t>(const uint8_t *, const uint64_t *, uint64_t, uint32_t, bool, uint64_t,
                                             const uint32_t *, const uint32_t *, int32_t, uint32_t, uint32_t *,

@@ -43,6 +43,25 @@ __host__ __device__ inline uint64_t encode_units(uint64_t nb, uint32_t bs) {
     return (nb + per - 1) / per;
 }
 
+// Fixed-partition baselines (FOR_integer_template.h, delta_integer_template.h):
+// block sizes of write_FOR_int_T (E+1 bytes, then (32 - l)/8 zero bytes in C
+// integer division) and write_delta_int_T (n fields: E+1 bytes, then 8 - E%8).
+constexpr int kCodecFor = 1;
+constexpr int kCodecDelta = 2;
+__host__ __device__ inline uint64_t for_block_bytes(uint32_t l, uint32_t w, uint32_t n) {
+    if (l >= w) return 1 + (uint64_t)n * (w / 8);
+    if (l == 0) return 1 + 2 * (w / 8);
+    const uint64_t e = ((uint64_t)l * n + 7) / 8;
+    const int pad = (32 - (int)l) / 8;
+    return 1 + 2 * (w / 8) + e + 1 + (pad > 0 ? pad : 0);
+}
+__host__ __device__ inline uint64_t delta_block_bytes(uint32_t l, uint32_t w, uint32_t n) {
+    if (l >= w) return 1 + (uint64_t)n * (w / 8);
+    if (l == 0) return 1 + w / 8;
+    const uint64_t e = ((uint64_t)l * n + 7) / 8;
+    return 1 + w / 8 + e + 1 + (8 - e % 8);
+}
+
 // One column of a batch, as the device sees it.
 struct BatchColumn {
     const void *in;   // values to encode (device)
@@ -163,6 +182,23 @@ template <typename T>
 hipError_t launch_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, bool close,
                          uint64_t filter, const T *f1, const T *f2, int32_t base, uint32_t pos_base, uint32_t *out,
                          uint64_t cap, uint64_t *result, uint64_t *work, hipStream_t st);
+// FOR / Delta columns (codec kCodecFor or kCodecDelta): offs holds nb+1 words
+// (offs[nb] = stream size after encode).
+template <typename T>
+hipError_t launch_base_encode(int codec, const T *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t *offs,
+                              hipStream_t st);
+template <typename T>
+hipError_t launch_base_decode(int codec, const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
+                              T *out, hipStream_t st);
+template <typename T>
+hipError_t launch_base_lookup(int codec, const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
+                              const uint64_t *idx, uint64_t nq, T *out, hipStream_t st);
+// FOR_int filters: filter_range (close = false: values > f1) or
+// filter_range_close (v % base in (f1, f2)) over every block.
+template <typename T>
+hipError_t launch_for_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, bool close,
+                             uint64_t f1, uint64_t f2, int32_t base, uint32_t pos_base, uint32_t *out, uint64_t cap,
+                             uint64_t *result, uint64_t *work, hipStream_t st);
 // Served batch: `grid` workgroups stride over nitems items.
 hipError_t launch_serve(const ServeColumn *cols, const ServeItem *items, uint32_t nitems, uint32_t grid,
                         hipStream_t st);

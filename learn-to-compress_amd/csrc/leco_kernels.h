@@ -115,6 +115,85 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Where a block's bytes go and what its header holds.
+struct PackedBlock {
+    uint64_t off, end;     // byte range of the block in the stream
+    uint64_t t0b, t1b;     // header theta bits
+    uint32_t m;            // values
+    int l;                 // bit width
+    uint32_t hdr;          // header bytes (1 raw, 17 otherwise)
+    uint32_t nchunks;
+};
+
+// Chunk c (CH values) of a block, packed by NT threads (thread `tid`) through
+// the LDS image `buf`; sync() orders the group's LDS accesses.  value(kk, i)
+// is the field of block value i = chunk position tid + kk*NT.  CACHED: kk is
+// a compile-time index (fully unrolled) so that register-cached values stay
+// in registers; otherwise value() recomputes the field and the loop is kept
+// rolled, which keeps register pressure down.
+template <int NT, uint32_t CH, bool CACHED, class Sync, class Value>
+__device__ __forceinline__ void pack_chunk(const PackedBlock &pb, uint32_t c, int tid, uint32_t *buf,
+                                           uint8_t *__restrict__ out, Sync sync, Value value) {
+    const int l = pb.l;
+    const uint64_t cbytes = (uint64_t)(CH / 8) * l;
+    const uint64_t g0 = c == 0 ? pb.off : pb.off + pb.hdr + c * cbytes;
+    const uint64_t g1 = c + 1 == pb.nchunks ? pb.end : pb.off + pb.hdr + (c + 1) * cbytes;
+    const uint64_t a0 = g0 & ~15ull;
+    const uint32_t span = (uint32_t)(((g1 + 15) & ~15ull) - a0);
+    const uint32_t lead = (uint32_t)(g0 - a0);
+    for (uint32_t w = tid; w < span / 4; w += NT) buf[w] = 0;
+    sync();
+    if (c == 0 && (uint32_t)tid < pb.hdr) {
+        uint32_t byte;
+        if (tid == 0) byte = (uint32_t)l;
+        else if (tid < 9) byte = (uint32_t)(pb.t0b >> (8 * (tid - 1))) & 0xFF;
+        else byte = (uint32_t)(pb.t1b >> (8 * (tid - 9))) & 0xFF;
+        const uint32_t pos = lead + tid;
+        atomicOr(&buf[pos >> 2], byte << (8 * (pos & 3)));
+    }
+    if (l > 0) {
+        const uint64_t pbit = 8ull * (lead + (c == 0 ? pb.hdr : 0));
+        const uint32_t j0 = c * CH;
+        const uint32_t jn = pb.m - j0 < CH ? pb.m - j0 : CH;
+        auto put = [&](int kk) {
+            const uint32_t j = tid + kk * NT;  // position inside the chunk
+            if (j < jn) {
+                const uint64_t field = value(kk, j0 + j);
+                const uint64_t q = pbit + (uint64_t)j * l;
+                const uint32_t dw = (uint32_t)(q >> 5);
+                const int shb = (int)(q & 31);
+                const uint32_t d0 = (uint32_t)(field << shb);
+                const uint32_t d1 = (uint32_t)(shb ? field >> (32 - shb) : field >> 32);
+                const uint32_t d2 = shb ? (uint32_t)(field >> (64 - shb)) : 0u;
+                if (d0) atomicOr(&buf[dw], d0);
+                if (shb + l > 32 && d1) atomicOr(&buf[dw + 1], d1);
+                if (shb + l > 64 && d2) atomicOr(&buf[dw + 2], d2);
+            }
+        };
+        if constexpr (CACHED) {
+#pragma unroll
+            for (int kk = 0; kk < (int)(CH / NT); ++kk) put(kk);
+        } else {
+#pragma unroll 2
+            for (int kk = 0; kk < (int)(CH / NT); ++kk) put(kk);
+        }
+    }
+    sync();
+    const uint8_t *bb = (const uint8_t *)buf;
+    for (uint32_t u = tid; u < span / 16; u += NT) {
+        const uint64_t ga = a0 + 16ull * u;
+        if (ga >= g0 && ga + 16 <= g1) {
+            *(uint4 *)(out + ga) = *(const uint4 *)(bb + 16 * u);
+        } else {
+            for (int k = 0; k < 16; ++k) {
+                const uint64_t gb = ga + k;
+                if (gb >= g0 && gb < g1) out[gb] = bb[16 * u + k];
+            }
+        }
+    }
+    sync();
+}
+
 // Small-block decode staging: the bytes of whole blocks, their offsets and
 // per-block header tables (width | first-field bit << 7, thetas).
 struct MultiShared {

@@ -67,6 +67,28 @@ for _w in ("u32", "u64"):
 SIGNATURES["leco_fix_filter_workspace_bytes"] = (c_u64, [c_u64, c_u32])
 
 
+LECO_FIX_CODEC_FOR, LECO_FIX_CODEC_DELTA = 1, 2
+SIGNATURES["leco_fix_base_stream_bound"] = (c_u64, [c_int, c_u64, c_u32, c_u32])
+for _w in ("u32", "u64"):
+    _T = c_u32 if _w == "u32" else c_u64
+    SIGNATURES.update({
+        f"leco_fix_base_encode_column_{_w}": (c_int, [c_int, c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr]),
+        f"leco_fix_base_decode_column_{_w}": (c_int, [c_int, c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_ptr]),
+        f"leco_fix_base_lookup_{_w}": (c_int, [c_int, c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr]),
+        f"leco_fix_for_filter_range_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, _T, c_ptr, c_u64, c_ptr, c_ptr, c_u64,
+                                                    c_ptr]),
+        f"leco_fix_for_filter_range_close_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, _T, _T, ctypes.c_int32, c_ptr,
+                                                          c_u64, c_ptr, c_ptr, c_u64, c_ptr]),
+        f"leco_fix_base_encode_block_{_w}": (c_int, [c_int, c_ptr, c_u64, c_ptr, c_u64, ctypes.POINTER(c_u64)]),
+        f"leco_fix_base_decode_block_{_w}": (c_int, [c_int, c_ptr, c_u64, c_ptr]),
+        f"leco_fix_base_random_decode_{_w}": (c_int, [c_int, c_ptr, c_u64, c_u32, c_ptr]),
+        f"leco_fix_for_filter_block_{_w}": (c_int, [c_ptr, c_u64, _T, c_ptr, c_u64, ctypes.c_int32, c_u32,
+                                                    ctypes.POINTER(c_u64)]),
+        f"leco_fix_for_filter_close_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr, c_u64, ctypes.c_int32, c_u32, _T, _T,
+                                                          ctypes.c_int32, ctypes.POINTER(c_u64)]),
+    })
+
+
 class ColumnFileInfo(ctypes.Structure):
     """leco_fix_file_info (include/leco_fix.h)."""
     _fields_ = [("n", ctypes.c_uint64), ("num_blocks", ctypes.c_uint64), ("stream_bytes", ctypes.c_uint64),

@@ -581,3 +581,315 @@ ORACLE_API int leco_oracle_filter_close_column(const uint8_t *stream, const uint
     *count = total;
     return st;
 }
+
+/* ---- FOR_int<T> / Delta_int<T>: the fixed-partition baselines ------------
+ * FOR_integer_template.h:26-250 and delta_integer_template.h:26-160, with
+ * write_FOR_int_T / write_delta_int_T (bit_write.h:258-390) and the readers
+ * read_all_bit_FOR, read_FOR_int, read_all_bit_Delta, read_Delta_int,
+ * filter_read_all_bit_FOR[_close] (bit_read.h:1270-1505).
+ *
+ * FOR block:   [u8 l][T min][T max][fields], field i = y_i - min on l = bitlen(max - min)
+ *              bits LSB-first: E+1 bytes (E = ceil(l*n/8)), then (32 - l)/8 zero bytes
+ *              (C integer division, none for l > 32); [u8 l][T min][T max] alone for
+ *              l = 0; [u8 w][raw values] when l = w.
+ * Delta block: [u8 l][T y_0][fields], field i = the sign/magnitude of y_{i+1} - y_i on
+ *              l = bitlen(max |delta|) + 1 bits; n fields are packed (the reference hands
+ *              the writer `length` for a vector of length-1 deltas, delta_integer_template.h:81),
+ *              the last one reads past that vector -- undefined there, zero here; E+1 bytes
+ *              then 8 - E%8 zero bytes; [u8 0][T y_0] for l = 0; raw values when l = w.
+ * Deviations (each tested): FOR decode and filters treat l = w-1 as packed (the reference
+ * copies such a block as raw values, FOR_integer_template.h:88, 123); Delta decode handles
+ * raw blocks (the reference reads a raw block as [y_0][fields], :98-126). */
+
+static inline uint64_t wmask_of(int width) { return width == 32 ? 0xFFFFFFFFull : ~0ull; }
+
+static void pack_fields(uint8_t *out, const uint64_t *f, uint64_t count, int l)
+{
+    /* bit k*l+b of the region = bit b of field k; bytes beyond the last field stay zero */
+    for (uint64_t k = 0; k < count; k++) {
+        uint64_t bit = k * (uint64_t)l;
+        for (int b = 0; b < l; b++, bit++)
+            if ((f[k] >> b) & 1) out[bit >> 3] |= (uint8_t)(1u << (bit & 7));
+    }
+}
+
+ORACLE_API uint64_t leco_oracle_for_block_bytes(int l, int width, uint32_t n)
+{
+    uint32_t tb = width / 8;
+    if (l >= width) return 1 + (uint64_t)n * tb;
+    if (l == 0) return 1 + 2 * tb;
+    uint64_t e = ((uint64_t)l * n + 7) / 8;
+    int pad = (32 - l) / 8;
+    return 1 + 2 * tb + e + 1 + (pad > 0 ? pad : 0);
+}
+
+ORACLE_API uint64_t leco_oracle_delta_block_bytes(int l, int width, uint32_t n)
+{
+    uint32_t tb = width / 8;
+    if (l >= width) return 1 + (uint64_t)n * tb;
+    if (l == 0) return 1 + tb;
+    uint64_t e = ((uint64_t)l * n + 7) / 8;
+    return 1 + tb + e + 1 + (8 - e % 8);
+}
+
+/* FOR_int<T>::encodeArray8_int (:26-77). */
+ORACLE_API uint64_t leco_oracle_for_encode_block(const void *y, int width, uint32_t n, uint8_t *out)
+{
+    uint32_t tb = width / 8;
+    uint64_t mn = load_val(y, width, 0), mx = mn;
+    for (uint32_t i = 1; i < n; i++) {
+        uint64_t v = load_val(y, width, i);
+        if (v > mx) mx = v;
+        if (v < mn) mn = v;
+    }
+    int l = (mx - mn) ? bitlen64(mx - mn) : 0;
+    if (l > width) l = width;
+    uint64_t size = leco_oracle_for_block_bytes(l, width, n);
+    memset(out, 0, size);
+    out[0] = (uint8_t)l;
+    if (l == width) {
+        memcpy(out + 1, y, (uint64_t)n * tb);
+        return size;
+    }
+    memcpy(out + 1, &mn, tb);
+    memcpy(out + 1 + tb, &mx, tb);
+    if (l) {
+        uint64_t *f = (uint64_t *)malloc((uint64_t)n * 8);
+        for (uint32_t i = 0; i < n; i++) f[i] = (load_val(y, width, i) - mn) & wmask_of(width);
+        pack_fields(out + 1 + 2 * tb, f, n, l);
+        free(f);
+    }
+    return size;
+}
+
+/* FOR_int<T>::decodeArray8 (:79-108) with read_all_bit_FOR; ref_compat = 1 also copies
+   the reference's l = w-1 -> raw handling (:88). */
+ORACLE_API void leco_oracle_for_decode_block(const uint8_t *in, int width, uint32_t m, void *out, int ref_compat)
+{
+    int l = in[0];
+    uint32_t tb = width / 8;
+    if (l >= width || (ref_compat && l == width - 1)) {
+        memcpy(out, in + 1, (uint64_t)m * tb);
+        return;
+    }
+    uint64_t base = 0;
+    memcpy(&base, in + 1, tb);
+    for (uint32_t i = 0; i < m; i++) {
+        uint64_t v = l ? base + read_field(in + 1 + 2 * tb, (uint64_t)i * l, l) : base;
+        store_val(out, width, i, v & wmask_of(width));
+    }
+}
+
+/* FOR_int<T>::randomdecodeArray8 (:219-246) with read_FOR_int. */
+ORACLE_API uint64_t leco_oracle_for_lookup_block(const uint8_t *in, int width, uint32_t j)
+{
+    int l = in[0];
+    uint32_t tb = width / 8;
+    uint64_t v = 0;
+    if (l == width) {
+        memcpy(&v, in + 1 + (uint64_t)j * tb, tb);
+        return v;
+    }
+    memcpy(&v, in + 1, tb);
+    if (l) v += read_field(in + 1 + 2 * tb, (uint64_t)j * l, l);
+    return v & wmask_of(width);
+}
+
+/* FOR_int<T>::filter_range (:111-158) with filter_read_all_bit_FOR: positions of values
+   above filter (T compare); a block whose max <= filter reports nothing. */
+ORACLE_API uint64_t leco_oracle_for_filter_block(const uint8_t *in, int width, uint32_t m, uint64_t filter,
+                                                 uint32_t block_start, uint32_t *out, uint64_t cap)
+{
+    uint64_t wm = wmask_of(width), cnt = 0;
+    uint32_t tb = width / 8;
+    filter &= wm;
+    int l = in[0];
+    if (l >= width) {
+        for (uint32_t i = 0; i < m; i++) {
+            uint64_t v = 0;
+            memcpy(&v, in + 1 + (uint64_t)i * tb, tb);
+            if (v > filter) cnt = emit(out, cap, cnt, block_start + i);
+        }
+        return cnt;
+    }
+    uint64_t base = 0, mx = 0;
+    memcpy(&base, in + 1, tb);
+    memcpy(&mx, in + 1 + tb, tb);
+    if (mx <= filter) return 0;
+    for (uint32_t i = 0; i < m; i++) {
+        uint64_t v = (l ? base + read_field(in + 1 + 2 * tb, (uint64_t)i * l, l) : base) & wm;
+        if (v > filter) cnt = emit(out, cap, cnt, block_start + i);
+    }
+    return cnt;
+}
+
+/* FOR_int<T>::filter_range_close (:160-216): positions of values with
+   v % base in (filter1, filter2) (T arithmetic; base > 0). */
+ORACLE_API uint64_t leco_oracle_for_filter_close_block(const uint8_t *in, int width, uint32_t m, uint64_t f1,
+                                                       uint64_t f2, int32_t base, uint32_t block_start,
+                                                       uint32_t *out, uint64_t cap)
+{
+    uint64_t wm = wmask_of(width), cnt = 0, bt = (uint64_t)(int64_t)base & wm;
+    uint32_t tb = width / 8;
+    f1 &= wm;
+    f2 &= wm;
+    int l = in[0];
+    uint64_t b0 = 0;
+    if (l < width) memcpy(&b0, in + 1, tb);
+    for (uint32_t i = 0; i < m; i++) {
+        uint64_t v = 0;
+        if (l >= width) memcpy(&v, in + 1 + (uint64_t)i * tb, tb);
+        else v = (l ? b0 + read_field(in + 1 + 2 * tb, (uint64_t)i * l, l) : b0) & wm;
+        uint64_t r = v % bt;
+        if (r > f1 && r < f2) cnt = emit(out, cap, cnt, block_start + i);
+    }
+    return cnt;
+}
+
+/* Delta_int<T>::encodeArray8_int (:25-83) with write_delta_int_T. */
+ORACLE_API uint64_t leco_oracle_delta_encode_block(const void *y, int width, uint32_t n, uint8_t *out)
+{
+    uint32_t tb = width / 8;
+    uint64_t wm = wmask_of(width), mxd = 0;
+    uint64_t *f = (uint64_t *)calloc((uint64_t)n + 1, 8);
+    int *sg = (int *)calloc((uint64_t)n + 1, sizeof(int));
+    for (uint32_t i = 0; i + 1 < n; i++) {
+        uint64_t a = load_val(y, width, i), b = load_val(y, width, i + 1);
+        uint64_t d = b > a ? b - a : a - b;
+        sg[i] = b > a;
+        f[i] = d & wm;
+        if (f[i] > mxd) mxd = f[i];
+    }
+    int l = mxd ? bitlen64(mxd) + 1 : 0;
+    if (l > width) l = width;
+    uint64_t size = leco_oracle_delta_block_bytes(l, width, n);
+    memset(out, 0, size);
+    out[0] = (uint8_t)l;
+    if (l == width) {
+        memcpy(out + 1, y, (uint64_t)n * tb);
+    } else {
+        uint64_t y0 = load_val(y, width, 0);
+        memcpy(out + 1, &y0, tb);
+        if (l) {
+            /* fields 0..n-2 are the deltas; field n-1 (past the reference's vector) is zero */
+            for (uint32_t i = 0; i + 1 < n; i++)
+                f[i] = (f[i] & (((uint64_t)1 << (l - 1)) - 1)) | ((uint64_t)sg[i] << (l - 1));
+            f[n - 1] = 0;
+            pack_fields(out + 1 + tb, f, n, l);
+        }
+    }
+    free(f);
+    free(sg);
+    return size;
+}
+
+/* Delta_int<T>::decodeArray8 (:85-126) with read_all_bit_Delta; raw blocks are decoded as
+   raw values (see the header comment). */
+ORACLE_API void leco_oracle_delta_decode_block(const uint8_t *in, int width, uint32_t m, void *out)
+{
+    int l = in[0];
+    uint32_t tb = width / 8;
+    uint64_t wm = wmask_of(width);
+    if (l == 127 || l == 126) {
+        uint64_t v = 0;
+        memcpy(&v, in + 1, tb);
+        store_val(out, width, 0, v);
+        if (l == 126) { v = 0; memcpy(&v, in + 1 + tb, tb); store_val(out, width, 1, v); }
+        return;
+    }
+    if (l >= width) {
+        memcpy(out, in + 1, (uint64_t)m * tb);
+        return;
+    }
+    uint64_t base = 0;
+    memcpy(&base, in + 1, tb);
+    store_val(out, width, 0, base);
+    for (uint32_t i = 1; i < m; i++) {
+        if (l) {
+            uint64_t fld = read_field(in + 1 + tb, (uint64_t)(i - 1) * l, l);
+            uint64_t mag = fld & (((uint64_t)1 << (l - 1)) - 1);
+            base = ((fld >> (l - 1)) & 1) ? base + mag : base - mag;
+        }
+        store_val(out, width, i, base & wm);
+    }
+}
+
+/* Delta_int<T>::randomdecodeArray8 (:131-153) with read_Delta_int: y_0 plus the first
+   j signed deltas. */
+ORACLE_API uint64_t leco_oracle_delta_lookup_block(const uint8_t *in, int width, uint32_t j)
+{
+    int l = in[0];
+    uint32_t tb = width / 8;
+    uint64_t v = 0;
+    if (l == width) {
+        memcpy(&v, in + 1 + (uint64_t)j * tb, tb);
+        return v;
+    }
+    memcpy(&v, in + 1, tb);
+    if (l)
+        for (uint32_t i = 0; i < j; i++) {
+            uint64_t fld = read_field(in + 1 + tb, (uint64_t)i * l, l);
+            uint64_t mag = fld & (((uint64_t)1 << (l - 1)) - 1);
+            v = ((fld >> (l - 1)) & 1) ? v + mag : v - mag;
+        }
+    return v & wmask_of(width);
+}
+
+/* Columns of FOR / Delta blocks (codec 1 = FOR, 2 = Delta), block by block as the drivers
+   experiments/FOR_int_fix_template.cpp and Delta_int_fix_template.cpp do. */
+ORACLE_API uint64_t leco_oracle_base_encode_column(int codec, const void *y, int width, uint64_t n, uint32_t bs,
+                                                   uint8_t *stream, uint64_t *offsets)
+{
+    uint64_t nb = (n + bs - 1) / bs;
+    offsets[0] = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+        uint64_t s = b * bs, m = n - s < bs ? n - s : bs;
+        const uint8_t *src = (const uint8_t *)y + s * (width / 8);
+        offsets[b + 1] = offsets[b] + (codec == 1 ? leco_oracle_for_encode_block(src, width, (uint32_t)m, stream + offsets[b])
+                                                  : leco_oracle_delta_encode_block(src, width, (uint32_t)m, stream + offsets[b]));
+    }
+    return offsets[nb];
+}
+
+ORACLE_API void leco_oracle_base_decode_column(int codec, const uint8_t *stream, const uint64_t *offsets, int width,
+                                               uint64_t n, uint32_t bs, void *out)
+{
+    uint64_t nb = (n + bs - 1) / bs;
+    for (uint64_t b = 0; b < nb; b++) {
+        uint64_t s = b * bs, m = n - s < bs ? n - s : bs;
+        uint8_t *dst = (uint8_t *)out + s * (width / 8);
+        if (codec == 1) leco_oracle_for_decode_block(stream + offsets[b], width, (uint32_t)m, dst, 0);
+        else leco_oracle_delta_decode_block(stream + offsets[b], width, (uint32_t)m, dst);
+    }
+}
+
+ORACLE_API void leco_oracle_base_lookup_column(int codec, const uint8_t *stream, const uint64_t *offsets, int width,
+                                               uint64_t n, uint32_t bs, const uint64_t *idx, uint64_t nq, void *out)
+{
+    for (uint64_t q = 0; q < nq; q++) {
+        uint64_t i = idx[q], b = i / bs;
+        uint32_t j = (uint32_t)(i - b * bs);
+        uint64_t v = codec == 1 ? leco_oracle_for_lookup_block(stream + offsets[b], width, j)
+                                : leco_oracle_delta_lookup_block(stream + offsets[b], width, j);
+        store_val(out, width, q, v);
+    }
+}
+
+/* FOR filters over a column (positions b*bs + i in block order); close = 1 is
+   filter_range_close with (f1, f2, base) for every block. */
+ORACLE_API uint64_t leco_oracle_for_filter_column(const uint8_t *stream, const uint64_t *offsets, int width,
+                                                  uint64_t n, uint32_t bs, int close, uint64_t f1, uint64_t f2,
+                                                  int32_t base, uint32_t *out, uint64_t cap)
+{
+    uint64_t nb = (n + bs - 1) / bs, total = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+        uint64_t s = b * bs, m = n - s < bs ? n - s : bs;
+        uint64_t room = total < cap ? cap - total : 0;
+        uint32_t *o = out + (total < cap ? total : 0);
+        total += close ? leco_oracle_for_filter_close_block(stream + offsets[b], width, (uint32_t)m, f1, f2, base,
+                                                            (uint32_t)s, o, room)
+                       : leco_oracle_for_filter_block(stream + offsets[b], width, (uint32_t)m, f1, (uint32_t)s, o,
+                                                      room);
+    }
+    return total;
+}

@@ -76,6 +76,34 @@ def lib() -> ctypes.CDLL:
                                                       ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                                       ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
         L.leco_oracle_filter_close_column.restype = ctypes.c_int
+        for name, res, args in (
+                ("leco_oracle_for_block_bytes", ctypes.c_uint64, [ctypes.c_int, ctypes.c_int, ctypes.c_uint32]),
+                ("leco_oracle_delta_block_bytes", ctypes.c_uint64, [ctypes.c_int, ctypes.c_int, ctypes.c_uint32]),
+                ("leco_oracle_for_encode_block", ctypes.c_uint64,
+                 [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p]),
+                ("leco_oracle_delta_encode_block", ctypes.c_uint64,
+                 [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p]),
+                ("leco_oracle_for_decode_block", None,
+                 [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_int]),
+                ("leco_oracle_delta_decode_block", None,
+                 [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p]),
+                ("leco_oracle_for_lookup_block", ctypes.c_uint64, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32]),
+                ("leco_oracle_delta_lookup_block", ctypes.c_uint64, [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32]),
+                ("leco_oracle_base_encode_column", ctypes.c_uint64,
+                 [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
+                  ctypes.c_void_p]),
+                ("leco_oracle_base_decode_column", None,
+                 [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+                  ctypes.c_void_p]),
+                ("leco_oracle_base_lookup_column", None,
+                 [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32,
+                  ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
+                ("leco_oracle_for_filter_column", ctypes.c_uint64,
+                 [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
+                  ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint64])):
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
         _lib = L
     return _lib
 
@@ -232,3 +260,83 @@ def filter_close_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, d
         if cnt.value <= cap:
             return out[:cnt.value].copy()
         cap = cnt.value
+
+
+# ---- FOR_int / Delta_int baselines (codec "for" or "delta") ----------------------------
+
+_CODEC = {"for": 1, "delta": 2}
+
+
+def base_block_bytes(codec: str, l: int, width: int, n: int) -> int:
+    fn = lib().leco_oracle_for_block_bytes if codec == "for" else lib().leco_oracle_delta_block_bytes
+    return fn(l, width, n)
+
+
+def base_encode_block(codec: str, y: np.ndarray) -> bytes:
+    y = np.ascontiguousarray(y)
+    w = _width(y)
+    out = np.zeros(len(y) * 8 + 64, np.uint8)
+    fn = lib().leco_oracle_for_encode_block if codec == "for" else lib().leco_oracle_delta_encode_block
+    nb = fn(_ptr(y), w, len(y), _ptr(out))
+    return out[:nb].tobytes()
+
+
+def base_decode_block(codec: str, blk: bytes, n: int, dtype, ref_compat: bool = False) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    src = np.frombuffer(bytes(blk) + bytes(64), np.uint8)
+    out = np.zeros(max(n, 2), dtype)
+    if codec == "for":
+        lib().leco_oracle_for_decode_block(_ptr(src), 8 * dtype.itemsize, n, _ptr(out), int(ref_compat))
+    else:
+        lib().leco_oracle_delta_decode_block(_ptr(src), 8 * dtype.itemsize, n, _ptr(out))
+    return out[:n]
+
+
+def base_lookup_block(codec: str, blk: bytes, j: int, dtype) -> int:
+    dtype = np.dtype(dtype)
+    src = np.frombuffer(bytes(blk) + bytes(64), np.uint8)
+    fn = lib().leco_oracle_for_lookup_block if codec == "for" else lib().leco_oracle_delta_lookup_block
+    return int(fn(_ptr(src), 8 * dtype.itemsize, j))
+
+
+def base_encode_column(codec: str, y: np.ndarray, bs: int) -> tuple[np.ndarray, np.ndarray]:
+    y = np.ascontiguousarray(y)
+    w = _width(y)
+    n = len(y)
+    nb = (n + bs - 1) // bs
+    stream = np.zeros(n * (w // 8) + nb * 48 + 128, np.uint8)
+    offs = np.zeros(nb + 1, np.uint64)
+    total = lib().leco_oracle_base_encode_column(_CODEC[codec], _ptr(y), w, n, bs, _ptr(stream), _ptr(offs))
+    return stream[:total].copy(), offs
+
+
+def base_decode_column(codec: str, stream: np.ndarray, offs: np.ndarray, n: int, bs: int, dtype) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    out = np.empty(n, dtype)
+    padded = _padded(stream, offs)
+    lib().leco_oracle_base_decode_column(_CODEC[codec], _ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs,
+                                         _ptr(out))
+    return out
+
+
+def base_lookup_column(codec: str, stream: np.ndarray, offs: np.ndarray, n: int, bs: int, idx: np.ndarray,
+                       dtype) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    idx = np.ascontiguousarray(idx, np.uint64)
+    out = np.empty(len(idx), dtype)
+    padded = _padded(stream, offs)
+    lib().leco_oracle_base_lookup_column(_CODEC[codec], _ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs,
+                                         _ptr(idx), len(idx), _ptr(out))
+    return out
+
+
+def for_filter_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, dtype, filt: int = 0,
+                      close: bool = False, f1: int = 0, f2: int = 0, base: int = 1) -> np.ndarray:
+    """FOR_int<T>::filter_range (close=False: values > filt) or filter_range_close
+    (v % base in (f1, f2)) over every block."""
+    dtype = np.dtype(dtype)
+    out = np.zeros(n, np.uint32)
+    padded = _padded(stream, offs)
+    c = lib().leco_oracle_for_filter_column(_ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs, int(close),
+                                            f1 if close else filt, f2, base, _ptr(out), n)
+    return out[:c].copy()
