@@ -86,6 +86,42 @@ def cpu_baseline(vals_dev, bs: int, sample: int, repeat: int):
     return gbs1, gbsn, nthr, gbsf
 
 
+def baselines_section(torch, vals, bs: int, raw: int, leco_bytes: int, leco_ms: float, steps: int) -> dict:
+    """FOR_int and Delta_int (the reference's fixed-partition baselines) on the
+    same column and block size: compression ratio, encode / decode GB/s,
+    lookups (Delta's cost per lookup grows with the position in the block, as
+    read_Delta_int's does), verified round trips."""
+    from leco_fix.baselines import BaseColumn
+
+    n = vals.numel()
+    out = {"leco": {"compression_ratio": round(leco_bytes / raw, 5), "decode_gbs": round(raw / (leco_ms * 1e-3) / 1e9, 1)}}
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    for codec, nq in (("for", 64_000_000), ("delta", 4_000_000)):
+        col = BaseColumn.encode(codec, vals, bs)
+        dec = torch.empty_like(vals)
+        ms_enc, _ = timed(torch, lambda: BaseColumn.encode(codec, vals, bs), 3, 1)
+        ms_dec, _ = timed(torch, lambda: col.decode(out=dec), steps, 2)
+        ok = bool(torch.equal(dec, vals))
+        idx = torch.randint(0, n, (nq,), device="cuda", generator=g)
+        res = torch.empty(nq, dtype=vals.dtype, device="cuda")
+        ms_lk, _ = timed(torch, lambda: col.lookup(idx, out=res), 3, 1)
+        ok = ok and bool(torch.equal(res, vals[idx]))
+        out[codec] = {"compression_ratio": round(col.nbytes / raw, 5), "compressed_bytes": col.nbytes,
+                      "encode_gbs": round(raw / (ms_enc * 1e-3) / 1e9, 1), "encode_ms_incl_alloc": round(ms_enc, 3),
+                      "decode_gbs": round(raw / (ms_dec * 1e-3) / 1e9, 1), "decode_ms": round(ms_dec, 4),
+                      "lookups": nq, "lookups_per_s": round(nq / (ms_lk * 1e-3), 0), "verified": ok}
+        if codec == "for":
+            fout = torch.empty(n, dtype=torch.int32, device="cuda")
+            f = int(vals[n // 2].item())
+            ms_f, _ = timed(torch, lambda: col.filter_range(f, capacity=n), 3, 1)
+            out[codec]["filter_range_ms_incl_alloc"] = round(ms_f, 3)
+            del fout
+        del col, dec, idx, res
+        torch.cuda.empty_cache()
+    return out
+
+
 def io_section(torch, keys, bs: int) -> dict:
     """SOSD binary -> HBM -> encode -> column file -> HBM, in a temporary
     directory (the host page cache serves the reads)."""
@@ -195,6 +231,7 @@ def main() -> int:
     ap.add_argument("--copy-reps", type=int, default=3,
                     help="plain device copies of the decoded column (a known-size HBM reference kernel)")
     ap.add_argument("--no-filter", action="store_true", help="skip the filter_range / filter_range_close section")
+    ap.add_argument("--no-baselines", action="store_true", help="skip the FOR_int / Delta_int section")
     ap.add_argument("--io-values", type=int, default=128_000_000,
                     help="keys written / read through SOSD and column files (0 = skip)")
     ap.add_argument("--filter-reps", type=int, default=10)
@@ -261,6 +298,12 @@ def main() -> int:
 
     log("lookup timing")
     ms_lk, _ = timed(torch, lambda: col.lookup(idx, out=lk), 5, 1)
+
+    base = None
+    if not args.no_baselines:
+        log("baselines: FOR_int / Delta_int on the same column")
+        base = baselines_section(torch, vals, bs, raw_bytes, comp, ms_dec, args.steps)
+        log(f"baselines: {base}")
 
     files = None
     if args.io_values > 0:
@@ -371,6 +414,7 @@ def main() -> int:
             "bytes_per_launch": comp + raw_bytes,
         },
         "cpu_baseline": cpu,
+        "baselines": base,
         "filter": filt,
         "files": files,
         "batched": batched,
