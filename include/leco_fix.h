@@ -264,6 +264,64 @@ int leco_fix_for_filter_close_block_u64(const uint8_t *h_in, uint64_t length, ui
                                         uint64_t out_capacity, int32_t block_id, uint32_t block_size,
                                         uint64_t filter1, uint64_t filter2, int32_t base, uint64_t *count);
 
+/* ---- piecewise_fix_op_max: the uint32 IntegerCODEC ----------------------- */
+
+/* The reference's piecewise_fix_op_max (headers/piecewise_fix_op_minimize_maxerror.h:18-212),
+ * the paper's uint32 LeCo_fix: a float slope, block = [u8 tmp_bit][f64 theta0]
+ * [f32 theta1] + payload (nothing for tmp_bit 0, the raw values for tmp_bit >= 31,
+ * else tmp_bit-bit sign/magnitude residuals, write_delta_T).  Its decoder
+ * predicts with an additive chain that can truncate one away from the
+ * encoder's prediction; the encoder lists those positions in
+ * IntegerCODEC::mul_add_diff_set (+1) and mul_add_diff_set_minus (-1)
+ * (codecs.h:30-31) and the harness adds them to the decoded values
+ * (experiments/fix_int.cpp:110-117).  Here the lists are d_fix: one int8 per
+ * value (+1, -1 or 0), identical to the reference's lists position for
+ * position, raw blocks included.
+ * Column layout and params as the LeCo calls; d_offsets[nb] = stream size;
+ * params.theta1 holds the float slope as a double, params.bits = tmp_bit.
+ * stream_bound includes LECO_FIX_TAIL_PAD. */
+uint64_t leco_fix_opmax_stream_bound(uint64_t n, uint32_t block_size);
+/* encodeArray8 (:45-131) of every block; d_fix (n int8) may be NULL. */
+int leco_fix_opmax_encode_column(const uint32_t *d_in, uint64_t n, uint32_t block_size, uint8_t *d_stream,
+                                 uint64_t stream_capacity, uint64_t *d_offsets, leco_fix_block_param *d_params,
+                                 int8_t *d_fix, void *stream);
+/* decodeArray8 (:134-169) of every block.  d_fix NULL: exactly the reference
+ * decoder's output.  d_fix given: each value of a packed or model-only block
+ * plus its flag (the harness's correction; the input, bit for bit).  Raw
+ * blocks are never adjusted: they hold the values themselves (the reference
+ * harness adjusts them too, which corrupts them). */
+int leco_fix_opmax_decode_column(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                 uint32_t block_size, const int8_t *d_fix, uint32_t *d_out, void *stream);
+/* randomdecodeArray8 (:171-197, read_bit_fix_T bit_read.h:1107-1132): exact
+ * values without fix-ups.  Indices >= n give 0. */
+int leco_fix_opmax_lookup(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n,
+                          uint32_t block_size, const uint64_t *d_idx, uint64_t nq, uint32_t *d_out, void *stream);
+/* The param table of a stored column, from its block headers. */
+int leco_fix_opmax_build_params(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                uint32_t block_size, leco_fix_block_param *d_params, void *stream);
+/* The reference's two lists from d_fix, ascending, as int32 (std::vector<int>;
+ * n <= 2^31): d_counts[0] = |mul_add_diff_set|, d_counts[1] =
+ * |mul_add_diff_set_minus| (all counted, at most `capacity` of each written).
+ * d_work: leco_fix_opmax_lists_workspace_bytes(n) bytes. */
+uint64_t leco_fix_opmax_lists_workspace_bytes(uint64_t n);
+int leco_fix_opmax_fix_lists(const int8_t *d_fix, uint64_t n, int32_t *d_plus, int32_t *d_minus,
+                             uint64_t capacity, uint64_t *d_counts, void *d_work, uint64_t work_bytes, void *stream);
+/* The harness's correction loops (experiments/fix_int.cpp:110-117):
+ * d_values[p] += 1 for p in d_plus, -= 1 for p in d_minus (mod 2^32; entries
+ * outside [0, n) skipped). */
+int leco_fix_opmax_apply_lists(uint32_t *d_values, uint64_t n, const int32_t *d_plus, uint64_t n_plus,
+                               const int32_t *d_minus, uint64_t n_minus, void *stream);
+/* One block from / to host memory (staged through HBM): encodeArray8 with
+ * the block's flags in h_fix (length int8, may be NULL; position i of the
+ * block is list entry i + nvalue * block_size of the reference's lists),
+ * decodeArray8 (the reference decoder's output) and randomdecodeArray8
+ * (which reads the block only up to value to_find's bytes; length >
+ * to_find is all it checks). */
+int leco_fix_opmax_encode_block(const uint32_t *h_in, uint64_t length, uint8_t *h_res, uint64_t res_capacity,
+                                uint64_t *written, int8_t *h_fix);
+int leco_fix_opmax_decode_block(const uint8_t *h_in, uint64_t length, uint32_t *h_out);
+int leco_fix_opmax_random_decode(const uint8_t *h_in, uint64_t length, uint32_t to_find, uint32_t *h_value);
+
 /* ---- files ------------------------------------------------------------- */
 
 /* SOSD binary, the reference's input format (experiments/leco_int_fix_template.cpp:21-40,

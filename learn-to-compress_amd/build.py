@@ -24,7 +24,7 @@ LIB = os.path.join(LIBDIR, "libleco_fix.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 SOURCES = ["leco_kernels.hip", "leco_filter.hip", "leco_api.hip", "leco_batch.hip", "leco_serve.hip", "leco_io.hip",
-           "leco_base.hip"]
+           "leco_base.hip", "leco_opmax.hip"]
 CFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -387,9 +387,168 @@ int for_filter_column(const uint8_t *d_stream, const uint64_t *d_offs, uint64_t 
                                                  d_result, (uint64_t *)d_work, (hipStream_t)st));
 }
 
+// ---- piecewise_fix_op_max --------------------------------------------------
+
+int opmax_encode_block_host(const uint32_t *h, uint64_t len, uint8_t *res, uint64_t rcap, uint64_t *written,
+                            int8_t *h_fix) {
+    if (!h || !res || !written || len == 0 || len > kMaxBlockSize) return LECO_FIX_EINVAL;
+    const size_t in_b = up256(len * 4);
+    const size_t st_b = up256(leco::opmax_block_bound((uint32_t)len) + LECO_FIX_TAIL_PAD);
+    const size_t off_b = up256(2 * sizeof(uint64_t));
+    const size_t prm_b = up256(sizeof(leco_fix_block_param));
+    const size_t fix_b = up256(len);
+    int rc = g_stage.reserve(in_b + st_b + off_b + prm_b + fix_b);
+    if (rc) return rc;
+    uint8_t *base = (uint8_t *)g_stage.dev;
+    uint32_t *d_in = (uint32_t *)base;
+    uint8_t *d_st = base + in_b;
+    uint64_t *d_off = (uint64_t *)(base + in_b + st_b);
+    leco_fix_block_param *d_prm = (leco_fix_block_param *)(base + in_b + st_b + off_b);
+    int8_t *d_fix = (int8_t *)(base + in_b + st_b + off_b + prm_b);
+    hipStream_t s = g_stage.stream;
+    if (hipMemcpyAsync(d_in, h, len * 4, hipMemcpyHostToDevice, s) != hipSuccess) return LECO_FIX_EHIP;
+    if (leco::launch_opmax_encode(d_in, len, (uint32_t)len, d_st, d_off, d_prm, h_fix ? d_fix : nullptr, s) !=
+        hipSuccess)
+        return LECO_FIX_EHIP;
+    uint64_t offs[2];
+    if (hipMemcpyAsync(offs, d_off, sizeof(offs), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    if (offs[1] > rcap) return LECO_FIX_EINVAL;
+    if (hipMemcpyAsync(res, d_st, offs[1], hipMemcpyDeviceToHost, s) != hipSuccess ||
+        (h_fix && hipMemcpyAsync(h_fix, d_fix, len, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    *written = offs[1];
+    return LECO_FIX_OK;
+}
+
+// The first `bytes` bytes of a stored block staged (zero-padded) with a
+// 2-entry offsets table, its params and `extra` bytes.
+int opmax_stage_block(const uint8_t *h_in, uint64_t len, uint64_t bytes, size_t extra, uint8_t **d_st,
+                      uint64_t **d_off, leco_fix_block_param **d_prm, uint8_t **d_extra) {
+    const size_t st_b = up256(bytes + LECO_FIX_TAIL_PAD);
+    const size_t off_b = up256(2 * sizeof(uint64_t));
+    const size_t prm_b = up256(sizeof(leco_fix_block_param));
+    int rc = g_stage.reserve(st_b + off_b + prm_b + up256(extra));
+    if (rc) return rc;
+    uint8_t *base = (uint8_t *)g_stage.dev;
+    *d_st = base;
+    *d_off = (uint64_t *)(base + st_b);
+    *d_prm = (leco_fix_block_param *)(base + st_b + off_b);
+    *d_extra = base + st_b + off_b + prm_b;
+    hipStream_t s = g_stage.stream;
+    const uint64_t offs[2] = {0, bytes};
+    if (hipMemcpyAsync(*d_st, h_in, bytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(*d_st + bytes, 0, st_b - bytes, s) != hipSuccess ||
+        hipMemcpyAsync(*d_off, offs, sizeof(offs), hipMemcpyHostToDevice, s) != hipSuccess ||
+        leco::launch_opmax_params(*d_st, *d_off, len, (uint32_t)len, *d_prm, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    return LECO_FIX_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+uint64_t leco_fix_opmax_stream_bound(uint64_t n, uint32_t bs) {
+    if (n == 0 || bs == 0) return 0;
+    const uint64_t nb = (n + bs - 1) / bs, last = n - (nb - 1) * bs;
+    return (nb - 1) * leco::opmax_block_bound(bs) + leco::opmax_block_bound((uint32_t)last) + LECO_FIX_TAIL_PAD;
+}
+
+int leco_fix_opmax_encode_column(const uint32_t *d_in, uint64_t n, uint32_t bs, uint8_t *d_stream, uint64_t cap,
+                                 uint64_t *d_offsets, leco_fix_block_param *d_params, int8_t *d_fix, void *stream) {
+    if (n == 0 || !valid_bs(bs) || !d_in || !d_stream || !d_offsets || !d_params ||
+        cap < leco_fix_opmax_stream_bound(n, bs))
+        return LECO_FIX_EINVAL;
+    return hip_status(
+        leco::launch_opmax_encode(d_in, n, bs, d_stream, d_offsets, d_params, d_fix, (hipStream_t)stream));
+}
+
+int leco_fix_opmax_decode_column(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                                 const int8_t *d_fix, uint32_t *d_out, void *stream) {
+    if (n == 0 || !valid_bs(bs) || !d_stream || !d_offsets || !d_out) return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_opmax_decode(d_stream, d_offsets, n, bs, d_fix, d_out, (hipStream_t)stream));
+}
+
+int leco_fix_opmax_lookup(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n, uint32_t bs,
+                          const uint64_t *d_idx, uint64_t nq, uint32_t *d_out, void *stream) {
+    if (n == 0 || !valid_bs(bs) || !d_stream || !d_params) return LECO_FIX_EINVAL;
+    if (nq == 0) return LECO_FIX_OK;
+    if (!d_idx || !d_out) return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_opmax_lookup(d_stream, d_params, n, bs, d_idx, nq, d_out, (hipStream_t)stream));
+}
+
+int leco_fix_opmax_build_params(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                                leco_fix_block_param *d_params, void *stream) {
+    if (n == 0 || !valid_bs(bs) || !d_stream || !d_offsets || !d_params) return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_opmax_params(d_stream, d_offsets, n, bs, d_params, (hipStream_t)stream));
+}
+
+uint64_t leco_fix_opmax_lists_workspace_bytes(uint64_t n) {
+    return n ? 16 * ((n + leco::kFixTile - 1) / leco::kFixTile) : 0;
+}
+
+int leco_fix_opmax_fix_lists(const int8_t *d_fix, uint64_t n, int32_t *d_plus, int32_t *d_minus, uint64_t capacity,
+                             uint64_t *d_counts, void *d_work, uint64_t work_bytes, void *stream) {
+    if (n == 0 || n > 0x80000000ull || !d_fix || !d_counts || !d_work || (capacity && (!d_plus || !d_minus)) ||
+        work_bytes < leco_fix_opmax_lists_workspace_bytes(n))
+        return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_fix_lists(d_fix, n, d_plus, d_minus, capacity, d_counts, (uint64_t *)d_work,
+                                             (hipStream_t)stream));
+}
+
+int leco_fix_opmax_apply_lists(uint32_t *d_values, uint64_t n, const int32_t *d_plus, uint64_t n_plus,
+                               const int32_t *d_minus, uint64_t n_minus, void *stream) {
+    if (!d_values || (n_plus && !d_plus) || (n_minus && !d_minus)) return LECO_FIX_EINVAL;
+    return hip_status(
+        leco::launch_apply_lists(d_values, n, d_plus, n_plus, d_minus, n_minus, (hipStream_t)stream));
+}
+
+int leco_fix_opmax_encode_block(const uint32_t *h_in, uint64_t length, uint8_t *h_res, uint64_t res_capacity,
+                                uint64_t *written, int8_t *h_fix) {
+    return opmax_encode_block_host(h_in, length, h_res, res_capacity, written, h_fix);
+}
+
+int leco_fix_opmax_decode_block(const uint8_t *h_in, uint64_t length, uint32_t *h_out) {
+    if (!h_in || !h_out || length == 0 || length > kMaxBlockSize) return LECO_FIX_EINVAL;
+    uint8_t *d_st, *d_x;
+    uint64_t *d_off;
+    leco_fix_block_param *d_prm;
+    int rc = opmax_stage_block(h_in, length, leco::opmax_block_bytes(h_in[0], (uint32_t)length), length * 4, &d_st,
+                               &d_off, &d_prm, &d_x);
+    if (rc) return rc;
+    hipStream_t s = g_stage.stream;
+    if (leco::launch_opmax_decode(d_st, d_off, length, (uint32_t)length, nullptr, (uint32_t *)d_x, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(h_out, d_x, length * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    return hip_status(hipStreamSynchronize(s));
+}
+
+int leco_fix_opmax_random_decode(const uint8_t *h_in, uint64_t length, uint32_t to_find, uint32_t *h_value) {
+    if (!h_in || !h_value || length == 0 || length > kMaxBlockSize || to_find >= length) return LECO_FIX_EINVAL;
+    uint8_t *d_st, *d_x;
+    uint64_t *d_off;
+    leco_fix_block_param *d_prm;
+    // only the bytes up to the value's field (randomdecodeArray8 reads no further)
+    const uint32_t l = h_in[0];
+    const uint64_t need = l == 0 ? leco::kOpmaxHdr
+                          : l >= 31 ? leco::kOpmaxHdr + 4ull * (to_find + 1)
+                                    : leco::kOpmaxHdr + ((uint64_t)l * (to_find + 1) + 7) / 8;
+    int rc = opmax_stage_block(h_in, length, need, 16, &d_st, &d_off, &d_prm, &d_x);
+    if (rc) return rc;
+    hipStream_t s = g_stage.stream;
+    const uint64_t q = to_find;
+    if (hipMemcpyAsync(d_x, &q, sizeof(q), hipMemcpyHostToDevice, s) != hipSuccess ||
+        leco::launch_opmax_lookup(d_st, d_prm, length, (uint32_t)length, (const uint64_t *)d_x, 1,
+                                  (uint32_t *)(d_x + 8), s) != hipSuccess ||
+        hipMemcpyAsync(h_value, d_x + 8, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return LECO_FIX_EHIP;
+    return hip_status(hipStreamSynchronize(s));
+}
 
 uint64_t leco_fix_base_stream_bound(int codec, uint64_t n, uint32_t bs, uint32_t w) {
     if (!valid_codec(codec) || n == 0 || bs == 0 || (w != 32 && w != 64)) return 0;

@@ -62,6 +62,28 @@ __host__ __device__ inline uint64_t delta_block_bytes(uint32_t l, uint32_t w, ui
     return 1 + w / 8 + e + 1 + (8 - e % 8);
 }
 
+// piecewise_fix_op_max (uint32): block = [u8 tmp_bit][f64 theta0][f32 theta1]
+// then nothing (tmp_bit 0), n raw uint32 (tmp_bit >= 31) or n tmp_bit-bit
+// fields, E+1 bytes and (32 - l)/8 zero bytes (write_delta_T, bit_write.h:186-247).
+// Encoded one block per wave, opmax_unit_blocks() consecutive blocks per
+// workgroup and look-back unit; decoded kOpmaxRun consecutive values per thread.
+constexpr uint32_t kOpmaxHdr = 13;
+constexpr uint32_t kOpmaxRun = 8;
+constexpr uint32_t kFixTile = 8192;  // fix-up list compaction: flags per workgroup
+__host__ __device__ inline uint64_t opmax_block_bytes(uint32_t l, uint32_t n) {
+    if (l == 0) return kOpmaxHdr;
+    if (l >= 31) return kOpmaxHdr + 4ull * n;
+    return kOpmaxHdr + ((uint64_t)l * n + 7) / 8 + 1 + (32 - l) / 8;
+}
+__host__ __device__ inline uint64_t opmax_block_bound(uint32_t n) {
+    const uint64_t a = opmax_block_bytes(30, n), b = opmax_block_bytes(32, n);
+    return a > b ? a : b;
+}
+__host__ __device__ inline uint32_t opmax_unit_blocks(uint32_t bs) {
+    const uint32_t per = bs > kWaveBlock ? 4 : 4 * (kWaveBlock / bs);
+    return per < kUnitMaxBlocks ? per : kUnitMaxBlocks;
+}
+
 // One column of a batch, as the device sees it.
 struct BatchColumn {
     const void *in;   // values to encode (device)
@@ -202,5 +224,24 @@ hipError_t launch_for_filter(const uint8_t *stream, const uint64_t *offs, uint64
 // Served batch: `grid` workgroups stride over nitems items.
 hipError_t launch_serve(const ServeColumn *cols, const ServeItem *items, uint32_t nitems, uint32_t grid,
                         hipStream_t st);
+
+// K1 of the LeCo encoder alone: per-block fit (theta0, theta1, length) into
+// prm, look-back status words offs[1..nb] and the ticket offs[0] cleared.
+hipError_t launch_fit(const void *in, uint64_t n, uint32_t bs, uint32_t width, leco_fix_block_param *prm,
+                      uint64_t *offs, hipStream_t st);
+// piecewise_fix_op_max (leco_opmax.hip).  fix: n int8 (+1 / -1 / 0) or null.
+hipError_t launch_opmax_encode(const uint32_t *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t *offs,
+                               leco_fix_block_param *prm, int8_t *fix, hipStream_t st);
+hipError_t launch_opmax_decode(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
+                               const int8_t *fix, uint32_t *out, hipStream_t st);
+hipError_t launch_opmax_lookup(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, uint32_t bs,
+                               const uint64_t *idx, uint64_t nq, uint32_t *out, hipStream_t st);
+hipError_t launch_opmax_params(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
+                               leco_fix_block_param *prm, hipStream_t st);
+// work: 2 * ceil(n / kFixTile) uint64; counts[0..1] = list lengths.
+hipError_t launch_fix_lists(const int8_t *fix, uint64_t n, int32_t *plus, int32_t *minus, uint64_t cap,
+                            uint64_t *counts, uint64_t *work, hipStream_t st);
+hipError_t launch_apply_lists(uint32_t *values, uint64_t n, const int32_t *plus, uint64_t np, const int32_t *minus,
+                              uint64_t nm, hipStream_t st);
 
 }  // namespace leco

@@ -123,6 +123,7 @@ struct PackedBlock {
     int l;                 // bit width
     uint32_t hdr;          // header bytes (1 raw, 17 otherwise)
     uint32_t nchunks;
+    int lbyte = -1;        // header byte 0 when it differs from the field width l
 };
 
 // Chunk c (CH values) of a block, packed by NT threads (thread `tid`) through
@@ -145,7 +146,7 @@ __device__ __forceinline__ void pack_chunk(const PackedBlock &pb, uint32_t c, in
     sync();
     if (c == 0 && (uint32_t)tid < pb.hdr) {
         uint32_t byte;
-        if (tid == 0) byte = (uint32_t)l;
+        if (tid == 0) byte = (uint32_t)(pb.lbyte >= 0 ? pb.lbyte : l);
         else if (tid < 9) byte = (uint32_t)(pb.t0b >> (8 * (tid - 1))) & 0xFF;
         else byte = (uint32_t)(pb.t1b >> (8 * (tid - 9))) & 0xFF;
         const uint32_t pos = lead + tid;

@@ -855,6 +855,14 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
     return hipGetLastError();
 }
 
+hipError_t launch_fit(const void *in, uint64_t n, uint32_t bs, uint32_t width, leco_fix_block_param *prm,
+                      uint64_t *offs, hipStream_t st) {
+    const uint64_t nb = (n + bs - 1) / bs;
+    PlainFit fit{(const uint8_t *)in, n, bs, width};
+    k_fit<PlainFit><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_decode(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, uint64_t lo,
                          uint64_t hi, T *out, hipStream_t st) {

@@ -138,6 +138,20 @@ SIGNATURES.update({
                                        ctypes.POINTER(c_u32), ctypes.POINTER(c_u64)]),
 })
 
+SIGNATURES.update({
+    "leco_fix_opmax_stream_bound": (c_u64, [c_u64, c_u32]),
+    "leco_fix_opmax_encode_column": (c_int, [c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr, c_ptr, c_ptr]),
+    "leco_fix_opmax_decode_column": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_ptr, c_ptr]),
+    "leco_fix_opmax_lookup": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr]),
+    "leco_fix_opmax_build_params": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_ptr]),
+    "leco_fix_opmax_lists_workspace_bytes": (c_u64, [c_u64]),
+    "leco_fix_opmax_fix_lists": (c_int, [c_ptr, c_u64, c_ptr, c_ptr, c_u64, c_ptr, c_ptr, c_u64, c_ptr]),
+    "leco_fix_opmax_apply_lists": (c_int, [c_ptr, c_u64, c_ptr, c_u64, c_ptr, c_u64, c_ptr]),
+    "leco_fix_opmax_encode_block": (c_int, [c_ptr, c_u64, c_ptr, c_u64, ctypes.POINTER(c_u64), c_ptr]),
+    "leco_fix_opmax_decode_block": (c_int, [c_ptr, c_u64, c_ptr]),
+    "leco_fix_opmax_random_decode": (c_int, [c_ptr, c_u64, c_u32, c_ptr]),
+})
+
 _lib = None
 
 

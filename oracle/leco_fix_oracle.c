@@ -893,3 +893,149 @@ ORACLE_API uint64_t leco_oracle_for_filter_column(const uint8_t *stream, const u
     }
     return total;
 }
+
+/* ---- piecewise_fix_op_max: the uint32 IntegerCODEC ------------------------
+ * headers/piecewise_fix_op_minimize_maxerror.h:45-186 with lr_int::caltheta
+ * (lr.h:89-111, the same FMA recipe as lr_int_T), write_delta_T /
+ * write_delta_default (bit_write.h:186-247, 395-410), read_all_bit_fix_add
+ * (bit_read.h:559-604), read_all_default / read_bit_default (bit_read.h:26-75)
+ * and read_bit_fix_T (bit_read.h:1107-1132).
+ *   block: [u8 l][f64 theta0][f32 theta1]; l = 0: nothing more; l >= 31: n raw uint32;
+ *          else n fields of l bits, field = (|d| & (2^(l-1)-1)) | (d > 0) << (l-1),
+ *          d = (int32)(y - pm_i), E+1 bytes then (32 - l)/8 zero bytes.
+ *   pm_i = cvtt64(fma((double)theta1, i, theta0))      (encoder, random access)
+ *   pa_i = cvtt64(p_i), p_0 = theta0, p_{i+1} = p_i + (double)theta1   (decoder)
+ * The decoder rebuilds y_i + pa_i - pm_i; the encoder records sign(pm_i - pa_i) per
+ * value (the harness's mul_add_diff_set / _minus lists, experiments/fix_int.cpp:110-117),
+ * here as fix[i] in {-1, 0, +1}; applying fix[i] to the decoded value is the harness's
+ * fix-up. */
+
+static inline int64_t opmax_pm(double t0, float t1, uint32_t i)
+{
+    return leco_oracle_cvtt64(fma((double)t1, (double)i, t0));
+}
+
+ORACLE_API uint64_t leco_oracle_opmax_block_bytes(int l, uint32_t n)
+{
+    if (l == 0) return 13;
+    if (l >= 31) return 13 + 4 * (uint64_t)n;
+    uint64_t e = ((uint64_t)l * n + 7) / 8;
+    return 13 + e + 1 + (uint64_t)((32 - l) / 8);
+}
+
+/* piecewise_fix_op_max::encodeArray8 (:45-131); fix may be NULL. */
+ORACLE_API uint64_t leco_oracle_opmax_encode_block(const uint32_t *y, uint32_t n, uint8_t *out, int8_t *fix)
+{
+    double t0, t1d;
+    leco_oracle_fit(y, 32, n, &t0, &t1d);
+    const float t1 = (float)t1d;
+    int32_t mx = INT32_MIN, mn = INT32_MAX;
+    for (uint32_t i = 0; i < n; i++) {
+        int32_t d = (int32_t)(uint32_t)((int64_t)y[i] - opmax_pm(t0, t1, i));
+        if (d > mx) mx = d;
+        if (d < mn) mn = d;
+    }
+    t0 += (double)(int32_t)((uint32_t)mx + (uint32_t)mn) / 2.0;
+    int32_t *d = (int32_t *)malloc((uint64_t)n * sizeof(int32_t));
+    int32_t maxe = 0;
+    double p = t0;
+    for (uint32_t i = 0; i < n; i++) {
+        int64_t pm = opmax_pm(t0, t1, i), pa = leco_oracle_cvtt64(p);
+        if (fix) fix[i] = (int8_t)(pm > pa ? 1 : pm < pa ? -1 : 0);
+        p += (double)t1;
+        d[i] = (int32_t)(uint32_t)((int64_t)y[i] - pm);
+        if (d[i] != INT32_MIN && (d[i] < 0 ? -d[i] : d[i]) > maxe) maxe = d[i] < 0 ? -d[i] : d[i];
+    }
+    int l = maxe ? (32 - __builtin_clz((uint32_t)maxe)) + 1 : 0;
+    uint64_t size = leco_oracle_opmax_block_bytes(l, n);
+    memset(out, 0, size);
+    out[0] = (uint8_t)l;
+    memcpy(out + 1, &t0, 8);
+    memcpy(out + 9, &t1, 4);
+    if (l >= 31) {
+        memcpy(out + 13, y, 4 * (uint64_t)n);
+    } else if (l) {
+        uint64_t *f = (uint64_t *)malloc((uint64_t)n * 8);
+        uint64_t low = ((uint64_t)1 << (l - 1)) - 1;
+        for (uint32_t i = 0; i < n; i++) {
+            int32_t v = d[i];
+            int sign = v > 0;
+            int32_t mag = v > 0 ? v : (int32_t)(0u - (uint32_t)v);
+            f[i] = ((uint64_t)(int64_t)mag & low) | ((uint64_t)sign << (l - 1));
+        }
+        pack_fields(out + 13, f, n, l);
+        free(f);
+    }
+    free(d);
+    return size;
+}
+
+/* piecewise_fix_op_max::decodeArray8 (:134-169): the decoder's output, before the
+   harness's fix-up. */
+ORACLE_API void leco_oracle_opmax_decode_block(const uint8_t *in, uint32_t m, uint32_t *out)
+{
+    int l = in[0];
+    double t0;
+    float t1;
+    memcpy(&t0, in + 1, 8);
+    memcpy(&t1, in + 9, 4);
+    if (l >= 31) {
+        memcpy(out, in + 13, 4 * (uint64_t)m);
+        return;
+    }
+    double p = t0;
+    for (uint32_t i = 0; i < m; i++) {
+        int64_t v = leco_oracle_cvtt64(p);
+        if (l) {
+            uint64_t fld = read_field(in + 13, (uint64_t)i * l, l);
+            uint64_t mag = fld & (((uint64_t)1 << (l - 1)) - 1);
+            v = ((fld >> (l - 1)) & 1) ? v + (int64_t)mag : v - (int64_t)mag;
+        }
+        out[i] = (uint32_t)v;
+        p += (double)t1;
+    }
+}
+
+/* piecewise_fix_op_max::randomdecodeArray8 (:171-197) with read_bit_fix_T. */
+ORACLE_API uint32_t leco_oracle_opmax_lookup_block(const uint8_t *in, uint32_t j)
+{
+    int l = in[0];
+    double t0;
+    float t1;
+    memcpy(&t0, in + 1, 8);
+    memcpy(&t1, in + 9, 4);
+    if (l >= 31) {
+        uint32_t v;
+        memcpy(&v, in + 13 + 4 * (uint64_t)j, 4);
+        return v;
+    }
+    int64_t pm = opmax_pm(t0, t1, j);
+    if (!l) return (uint32_t)pm;
+    uint64_t fld = read_field(in + 13, (uint64_t)j * l, l);
+    int64_t mag = (int64_t)(fld & (((uint64_t)1 << (l - 1)) - 1));
+    return (uint32_t)(((fld >> (l - 1)) & 1 ? mag : -mag) + pm);
+}
+
+/* The harness's column loops (experiments/fix_int.cpp:62-79, 95-117). fix: n values. */
+ORACLE_API uint64_t leco_oracle_opmax_encode_column(const uint32_t *y, uint64_t n, uint32_t bs, uint8_t *stream,
+                                                    uint64_t *offsets, int8_t *fix)
+{
+    uint64_t nb = (n + bs - 1) / bs;
+    offsets[0] = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+        uint64_t s = b * bs, m = n - s < bs ? n - s : bs;
+        offsets[b + 1] = offsets[b] + leco_oracle_opmax_encode_block(y + s, (uint32_t)m, stream + offsets[b],
+                                                                     fix ? fix + s : NULL);
+    }
+    return offsets[nb];
+}
+
+ORACLE_API void leco_oracle_opmax_decode_column(const uint8_t *stream, const uint64_t *offsets, uint64_t n,
+                                                uint32_t bs, uint32_t *out)
+{
+    uint64_t nb = (n + bs - 1) / bs;
+    for (uint64_t b = 0; b < nb; b++) {
+        uint64_t s = b * bs, m = n - s < bs ? n - s : bs;
+        leco_oracle_opmax_decode_block(stream + offsets[b], (uint32_t)m, out + s);
+    }
+}

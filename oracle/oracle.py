@@ -100,7 +100,17 @@ def lib() -> ctypes.CDLL:
                   ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]),
                 ("leco_oracle_for_filter_column", ctypes.c_uint64,
                  [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int,
-                  ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint64])):
+                  ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int32, ctypes.c_void_p, ctypes.c_uint64]),
+                ("leco_oracle_opmax_block_bytes", ctypes.c_uint64, [ctypes.c_int, ctypes.c_uint32]),
+                ("leco_oracle_opmax_encode_block", ctypes.c_uint64,
+                 [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]),
+                ("leco_oracle_opmax_decode_block", None, [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p]),
+                ("leco_oracle_opmax_lookup_block", ctypes.c_uint32, [ctypes.c_void_p, ctypes.c_uint32]),
+                ("leco_oracle_opmax_encode_column", ctypes.c_uint64,
+                 [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p]),
+                ("leco_oracle_opmax_decode_column", None,
+                 [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p])):
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -340,3 +350,75 @@ def for_filter_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, dty
     c = lib().leco_oracle_for_filter_column(_ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs, int(close),
                                             f1 if close else filt, f2, base, _ptr(out), n)
     return out[:c].copy()
+
+
+# ---- piecewise_fix_op_max (uint32) ------------------------------------------------------
+# fix arrays: int8 per value, +1 where the reference lists the position in
+# mul_add_diff_set, -1 in mul_add_diff_set_minus, 0 elsewhere.
+
+
+def opmax_block_bytes(l: int, n: int) -> int:
+    return lib().leco_oracle_opmax_block_bytes(l, n)
+
+
+def opmax_encode_block(y: np.ndarray) -> tuple[bytes, np.ndarray]:
+    y = np.ascontiguousarray(y, np.uint32)
+    out = np.zeros(len(y) * 4 + 64, np.uint8)
+    fix = np.zeros(max(len(y), 1), np.int8)
+    nb = lib().leco_oracle_opmax_encode_block(_ptr(y), len(y), _ptr(out), _ptr(fix))
+    return out[:nb].tobytes(), fix[:len(y)].copy()
+
+
+def opmax_decode_block(blk: bytes, n: int) -> np.ndarray:
+    """decodeArray8's output (before the fix-up lists are applied)."""
+    src = np.frombuffer(bytes(blk) + bytes(64), np.uint8)
+    out = np.zeros(max(n, 1), np.uint32)
+    lib().leco_oracle_opmax_decode_block(_ptr(src), n, _ptr(out))
+    return out[:n]
+
+
+def opmax_lookup_block(blk: bytes, j: int) -> int:
+    src = np.frombuffer(bytes(blk) + bytes(64), np.uint8)
+    return int(lib().leco_oracle_opmax_lookup_block(_ptr(src), j))
+
+
+def opmax_encode_column(y: np.ndarray, bs: int) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """-> (stream, offsets, fix)."""
+    y = np.ascontiguousarray(y, np.uint32)
+    n = len(y)
+    nb = (n + bs - 1) // bs
+    stream = np.zeros(n * 4 + nb * 20 + 128, np.uint8)
+    offs = np.zeros(nb + 1, np.uint64)
+    fix = np.zeros(max(n, 1), np.int8)
+    total = lib().leco_oracle_opmax_encode_column(_ptr(y), n, bs, _ptr(stream), _ptr(offs), _ptr(fix))
+    return stream[:total].copy(), offs, fix[:n].copy()
+
+
+def opmax_decode_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int) -> np.ndarray:
+    out = np.empty(max(n, 1), np.uint32)
+    padded = _padded(stream, offs)
+    lib().leco_oracle_opmax_decode_column(_ptr(padded), _ptr(offs), n, bs, _ptr(out))
+    return out[:n]
+
+
+def opmax_lookup_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, idx: np.ndarray) -> np.ndarray:
+    padded = _padded(stream, offs)
+    idx = np.asarray(idx, np.uint64)
+    out = np.empty(len(idx), np.uint32)
+    for q, i in enumerate(idx):
+        b, j = divmod(int(i), bs)
+        out[q] = lib().leco_oracle_opmax_lookup_block(_ptr(padded) + int(offs[b]), j)
+    return out
+
+
+def opmax_fix_lists(fix: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """The reference's (mul_add_diff_set, mul_add_diff_set_minus) from a fix array."""
+    return np.flatnonzero(fix > 0).astype(np.int32), np.flatnonzero(fix < 0).astype(np.int32)
+
+
+def opmax_apply_lists(decoded: np.ndarray, plus: np.ndarray, minus: np.ndarray) -> np.ndarray:
+    """The harness's fix-up loops (experiments/fix_int.cpp:110-117)."""
+    out = decoded.copy()
+    np.add.at(out, plus, np.uint32(1))
+    np.subtract.at(out, minus, np.uint32(1))
+    return out
