@@ -122,6 +122,80 @@ def baselines_section(torch, vals, bs: int, raw: int, leco_bytes: int, leco_ms: 
     return out
 
 
+def opmax_section(torch, n: int, steps: int, cpu_sample: int) -> dict:
+    """piecewise_fix_op_max, the uint32 IntegerCODEC, on the shape of its
+    published runs (benchmark/fix_int.sh: 200M uint32 keys in 100,000 blocks,
+    bs 2000, and in 1,000,000 blocks, bs 200): synthetic linear keys with
+    noise.  Encode (fit, pack, fix-up flags), decode with the fix-ups applied
+    (the harness's output), the decoder alone, lookups, list compaction; the C
+    oracle's decode + list correction on a sample as the CPU figure."""
+    import numpy as np
+
+    from leco_fix.opmax import OpMaxColumn
+
+    g = torch.Generator(device="cuda")
+    g.manual_seed(21)
+    i = torch.arange(n, device="cuda", dtype=torch.float64)
+    y = torch.round(1e6 + 21.3 * i + torch.randint(-3, 4, (n,), device="cuda", generator=g)).to(torch.int64)
+    y = y.to(torch.int32)
+    del i
+    torch.cuda.empty_cache()
+    raw = 4 * n
+    out = {"workload": f"{n // 1_000_000}M uint32 linear keys with noise (y = round(1e6 + 21.3 i + U[-3,3]))",
+           "reference_published": {"source": "benchmark/fix_int_benchmark_intel.log:4 (linear_200M_uint32, "
+                                             "100000 blocks, 1 CPU thread)",
+                                   "da_ns": 1.75531, "ra_ns": 28.285, "cr_percent": 4.48606,
+                                   "compression_gbs": 0.931192}}
+    for bs in (2000, 200):
+        col = OpMaxColumn.encode(y, bs)
+        plus, minus = col.fix_lists()  # the reference's lists, kept with the column
+        dec = torch.empty_like(y)
+        ms_enc, _ = timed(torch, lambda: OpMaxColumn.encode(y, bs), 3, 1)
+        ms_dec, _ = timed(torch, lambda: col.decode_lists(plus, minus, out=dec), steps, 2)
+        ok = bool(torch.equal(dec, y))
+        ms_dense, _ = timed(torch, lambda: col.decode(fixed=True, out=dec), steps, 2)
+        ok = ok and bool(torch.equal(dec, y))
+        ms_raw, _ = timed(torch, lambda: col.decode(fixed=False, out=dec), steps, 2)
+        nq = 16_000_000
+        idx = torch.randint(0, n, (nq,), device="cuda", generator=g)
+        res = torch.empty(nq, dtype=torch.int32, device="cuda")
+        ms_lk, _ = timed(torch, lambda: col.lookup(idx, out=res), 3, 1)
+        ok = ok and bool(torch.equal(res, y[idx]))
+        ms_lists, _ = timed(torch, lambda: col.fix_lists(), 3, 1)
+        alg = col.nbytes + raw  # stream read, values written (the lists are sparse)
+        out[f"bs{bs}"] = {
+            "blocks": col.num_blocks, "compression_ratio": round(col.compression_ratio, 5),
+            "compressed_bytes": col.nbytes, "fixups": [len(plus), len(minus)],
+            "encode_gbs": round(raw / (ms_enc * 1e-3) / 1e9, 1), "encode_ms_incl_alloc": round(ms_enc, 3),
+            "decode_gbs": round(raw / (ms_dec * 1e-3) / 1e9, 1), "decode_ms": round(ms_dec, 4),
+            "decode_hbm_gbs": round(alg / (ms_dec * 1e-3) / 1e9, 1),
+            "decode_hbm_frac": round(alg / (ms_dec * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "decode_ns_per_value": round(ms_dec * 1e6 / n, 5),
+            "decode_dense_flags_gbs": round(raw / (ms_dense * 1e-3) / 1e9, 1),
+            "decoder_only_gbs": round(raw / (ms_raw * 1e-3) / 1e9, 1),
+            "lookups": nq, "lookups_per_s": round(nq / (ms_lk * 1e-3), 0),
+            "fix_lists_ms_incl_alloc": round(ms_lists, 3), "verified": ok}
+        del col, dec, idx, res, plus, minus
+        torch.cuda.empty_cache()
+    if cpu_sample > 0:
+        sys.path.insert(0, ROOT)
+        from oracle import oracle
+        m = min(cpu_sample, n)
+        yh = y[:m].cpu().numpy().view(np.uint32)
+        stream, offs, fix = oracle.opmax_encode_column(yh, 2000)
+        plus, minus = oracle.opmax_fix_lists(fix)
+        t0 = time.perf_counter()
+        d = oracle.opmax_decode_column(stream, offs, m, 2000)
+        d = oracle.opmax_apply_lists(d, plus, minus)
+        dt = time.perf_counter() - t0
+        out["cpu_oracle"] = {"sample": f"first {m} values, bs 2000: decode + list correction, C oracle, 1 thread",
+                             "gbs": round(4 * m / dt / 1e9, 3), "ns_per_value": round(dt * 1e9 / m, 3),
+                             "verified": bool(np.array_equal(d, yh))}
+    del y
+    torch.cuda.empty_cache()
+    return out
+
+
 def io_section(torch, keys, bs: int) -> dict:
     """SOSD binary -> HBM -> encode -> column file -> HBM, in a temporary
     directory (the host page cache serves the reads)."""
@@ -232,6 +306,8 @@ def main() -> int:
                     help="plain device copies of the decoded column (a known-size HBM reference kernel)")
     ap.add_argument("--no-filter", action="store_true", help="skip the filter_range / filter_range_close section")
     ap.add_argument("--no-baselines", action="store_true", help="skip the FOR_int / Delta_int section")
+    ap.add_argument("--opmax-n", type=int, default=200_000_000,
+                    help="values of the uint32 piecewise_fix_op_max section (0 = skip)")
     ap.add_argument("--io-values", type=int, default=128_000_000,
                     help="keys written / read through SOSD and column files (0 = skip)")
     ap.add_argument("--filter-reps", type=int, default=10)
@@ -304,6 +380,12 @@ def main() -> int:
         log("baselines: FOR_int / Delta_int on the same column")
         base = baselines_section(torch, vals, bs, raw_bytes, comp, ms_dec, args.steps)
         log(f"baselines: {base}")
+
+    opmax = None
+    if args.opmax_n > 0:
+        log("piecewise_fix_op_max (uint32)")
+        opmax = opmax_section(torch, args.opmax_n, args.steps, 0 if args.no_cpu else 20_000_000)
+        log(f"opmax: {opmax}")
 
     files = None
     if args.io_values > 0:
@@ -415,6 +497,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "baselines": base,
+        "opmax": opmax,
         "filter": filt,
         "files": files,
         "batched": batched,

@@ -292,6 +292,13 @@ int leco_fix_opmax_encode_column(const uint32_t *d_in, uint64_t n, uint32_t bloc
  * harness adjusts them too, which corrupts them). */
 int leco_fix_opmax_decode_column(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
                                  uint32_t block_size, const int8_t *d_fix, uint32_t *d_out, void *stream);
+/* decodeArray8 of every block, then the reference's lists (as
+ * leco_fix_opmax_fix_lists writes them) applied to packed and model-only
+ * blocks: the input, bit for bit, reading the sparse lists instead of one
+ * flag byte per value. */
+int leco_fix_opmax_decode_column_lists(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                       uint32_t block_size, const int32_t *d_plus, uint64_t n_plus,
+                                       const int32_t *d_minus, uint64_t n_minus, uint32_t *d_out, void *stream);
 /* randomdecodeArray8 (:171-197, read_bit_fix_T bit_read.h:1107-1132): exact
  * values without fix-ups.  Indices >= n give 0. */
 int leco_fix_opmax_lookup(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n,

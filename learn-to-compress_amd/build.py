@@ -67,14 +67,22 @@ DRIVER = os.path.join(ROOT, "tests", "cpp", "leco_int_driver")
 
 BASE_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "base_driver.cpp")
 BASE_DRIVER = os.path.join(ROOT, "tests", "cpp", "base_driver")
+OPMAX_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "opmax_driver.cpp")
+OPMAX_DRIVER = os.path.join(ROOT, "tests", "cpp", "opmax_driver")
+CHAIN_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "chain_check.cpp")
+CHAIN_CHECK = os.path.join(ROOT, "tests", "cpp", "chain_check")
 
 
 def build_driver() -> str:
-    """The C++ drivers over include/leco_int.hpp and include/leco_baselines.hpp
-    (host code: g++ links the library)."""
-    for src, exe in ((DRIVER_SRC, DRIVER), (BASE_DRIVER_SRC, BASE_DRIVER)):
+    """The C++ drivers over include/leco_int.hpp, leco_baselines.hpp and
+    leco_opmax.hpp (host code: g++ links the library), and the host-side check
+    of the op_max chain evaluation (leco_chain.h is HIP host+device code, so
+    hipcc compiles it; only its host code runs)."""
+    for src, exe in ((DRIVER_SRC, DRIVER), (BASE_DRIVER_SRC, BASE_DRIVER), (OPMAX_DRIVER_SRC, OPMAX_DRIVER)):
         _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src,
               "-o", exe, "-L", LIBDIR, "-lleco_fix", "-Wl,-rpath,$ORIGIN/../../learn-to-compress_amd/lib"])
+    _run([HIPCC, "-x", "hip", f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-ffp-contract=off", "-I", CSRC,
+          CHAIN_CHECK_SRC, "-o", CHAIN_CHECK])
     return DRIVER
 
 

@@ -473,6 +473,19 @@ int leco_fix_opmax_decode_column(const uint8_t *d_stream, const uint64_t *d_offs
     return hip_status(leco::launch_opmax_decode(d_stream, d_offsets, n, bs, d_fix, d_out, (hipStream_t)stream));
 }
 
+int leco_fix_opmax_decode_column_lists(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,
+                                       const int32_t *d_plus, uint64_t n_plus, const int32_t *d_minus,
+                                       uint64_t n_minus, uint32_t *d_out, void *stream) {
+    if (n == 0 || !valid_bs(bs) || !d_stream || !d_offsets || !d_out || (n_plus && !d_plus) ||
+        (n_minus && !d_minus))
+        return LECO_FIX_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    if (leco::launch_opmax_decode(d_stream, d_offsets, n, bs, nullptr, d_out, st) != hipSuccess)
+        return LECO_FIX_EHIP;
+    return hip_status(
+        leco::launch_opmax_apply_exact(d_out, n, bs, d_stream, d_offsets, d_plus, n_plus, d_minus, n_minus, st));
+}
+
 int leco_fix_opmax_lookup(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n, uint32_t bs,
                           const uint64_t *d_idx, uint64_t nq, uint32_t *d_out, void *stream) {
     if (n == 0 || !valid_bs(bs) || !d_stream || !d_params) return LECO_FIX_EINVAL;

@@ -68,7 +68,7 @@ __host__ __device__ inline uint64_t delta_block_bytes(uint32_t l, uint32_t w, ui
 // Encoded one block per wave, opmax_unit_blocks() consecutive blocks per
 // workgroup and look-back unit; decoded kOpmaxRun consecutive values per thread.
 constexpr uint32_t kOpmaxHdr = 13;
-constexpr uint32_t kOpmaxRun = 8;
+constexpr uint32_t kOpmaxRun = 16;
 constexpr uint32_t kFixTile = 8192;  // fix-up list compaction: flags per workgroup
 __host__ __device__ inline uint64_t opmax_block_bytes(uint32_t l, uint32_t n) {
     if (l == 0) return kOpmaxHdr;
@@ -234,6 +234,10 @@ hipError_t launch_opmax_encode(const uint32_t *in, uint64_t n, uint32_t bs, uint
                                leco_fix_block_param *prm, int8_t *fix, hipStream_t st);
 hipError_t launch_opmax_decode(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
                                const int8_t *fix, uint32_t *out, hipStream_t st);
+// The reference's lists applied to decoded values, raw blocks skipped.
+hipError_t launch_opmax_apply_exact(uint32_t *values, uint64_t n, uint32_t bs, const uint8_t *stream,
+                                    const uint64_t *offs, const int32_t *plus, uint64_t np, const int32_t *minus,
+                                    uint64_t nm, hipStream_t st);
 hipError_t launch_opmax_lookup(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, uint32_t bs,
                                const uint64_t *idx, uint64_t nq, uint32_t *out, hipStream_t st);
 hipError_t launch_opmax_params(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,

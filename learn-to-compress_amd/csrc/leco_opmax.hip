@@ -20,7 +20,7 @@
 // workgroup: residual range, centring, bit width, size; one decoupled
 // look-back per unit; pack through the shared LDS-image packer; fix flags ->
 // offsets from the param table.  Decode = kOpmaxRun consecutive values per
-// thread.  Lookup = one lane per query.
+// thread, fields staged through LDS.  Lookup = one lane per query.
 #include "leco_chain.h"
 #include "leco_kernels.h"
 
@@ -204,12 +204,13 @@ __global__ __launch_bounds__(kThreads) void k_opmax_encode(const uint32_t *__res
                                                 [&](int q, uint32_t) -> uint64_t { return f[q]; });
         }
         if (fix) {
-            // runs of kOpmaxRun values per lane: p_j of the decoder's chain against pm_j
-            for (uint32_t r0 = lane * kOpmaxRun; r0 < m; r0 += kWave * kOpmaxRun) {
+            // runs of 8 values per lane (one 8-byte store): p_j of the decoder's chain against pm_j
+            constexpr uint32_t kRun = 8;
+            for (uint32_t r0 = lane * kRun; r0 < m; r0 += kWave * kRun) {
                 double p = chain_at(t0, t1, r0, m);
                 uint64_t word = 0;
 #pragma unroll
-                for (uint32_t q = 0; q < kOpmaxRun; ++q) {
+                for (uint32_t q = 0; q < kRun; ++q) {
                     const uint32_t j = r0 + q;
                     const int64_t pa = cvtt64(p), pm = opmax_pm(t0, t1, j);
                     const int8_t fv = pm > pa ? 1 : (pm < pa ? -1 : 0);
@@ -217,10 +218,10 @@ __global__ __launch_bounds__(kThreads) void k_opmax_encode(const uint32_t *__res
                     p = p + t1;
                 }
                 int8_t *dst = fix + s + r0;
-                if (r0 + kOpmaxRun <= m && ((uintptr_t)dst & 7) == 0) {
+                if (r0 + kRun <= m && ((uintptr_t)dst & 7) == 0) {
                     *(uint64_t *)dst = word;
                 } else {
-                    for (uint32_t q = 0; q < kOpmaxRun && r0 + q < m; ++q) dst[q] = (int8_t)(word >> (8 * q));
+                    for (uint32_t q = 0; q < kRun && r0 + q < m; ++q) dst[q] = (int8_t)(word >> (8 * q));
                 }
             }
         }
@@ -238,15 +239,23 @@ __global__ __launch_bounds__(kThreads) void k_opmax_fixup(const leco_fix_block_p
 }
 
 // ---------------------------------------------------------------------------
-// Decode: decodeArray8 (:134-169).  Thread t of tile g owns positions
-// [g*kThreads*kOpmaxRun + t*kOpmaxRun, +kOpmaxRun): it places p at its first
-// position with chain_at, then steps the chain exactly as the reference
-// does, crossing into the next block where its run does.  FIX: add the
-// fix-up flag of every value of a packed or model-only block (raw blocks
-// hold the values themselves).
+// Decode: decodeArray8 (:134-169).  Thread t owns a run of kOpmaxRun
+// consecutive values of one block (runs start at multiples of kOpmaxRun
+// inside their block; a block's last run may be shorter), a workgroup
+// kThreads consecutive runs.  The stream bytes of the workgroup's fields (a
+// contiguous range across its blocks) are staged through LDS with 16-byte
+// loads; each thread places p at its first value with chain_at, then steps
+// the chain exactly as the reference does, reading each field as two LDS
+// words and one funnel shift.  A workgroup whose bytes exceed the staging
+// buffer (tiny blocks: mostly headers) reads fields from global memory.  FIX:
+// add the fix-up flag of every value of a packed or model-only block (raw
+// blocks hold the values themselves).
 // ---------------------------------------------------------------------------
+constexpr uint32_t kOpmaxTile = kThreads * kOpmaxRun;
+constexpr uint32_t kOpmaxStage = kOpmaxTile * 4 + 1024;  // raw fields of a tile + headers + slack
+
 struct OpmaxBlock {
-    const uint8_t *pay;
+    uint64_t off;  // stream offset of the block
     double t0, t1;
     uint32_t m;
     int l;
@@ -254,79 +263,206 @@ struct OpmaxBlock {
 
 __device__ __forceinline__ OpmaxBlock opmax_block(const uint8_t *__restrict__ st, const uint64_t *__restrict__ offs,
                                                   uint64_t n, uint32_t bs, uint64_t b) {
-    const uint8_t *blk = st + offs[b];
     OpmaxBlock k;
+    k.off = offs[b];
+    const uint8_t *blk = st + k.off;
     k.l = blk[0];
     k.t0 = bits_to_double(load_u64_any(blk + 1));
     k.t1 = (double)__uint_as_float((uint32_t)load_u64_any(blk + 9));
-    k.pay = blk + kOpmaxHdr;
     const uint64_t s = b * bs;
     k.m = (uint32_t)(n - s < bs ? n - s : bs);
     return k;
 }
 
+// Field width as stored: raw blocks (l >= 31) hold 32-bit values.
+__device__ __forceinline__ uint32_t opmax_lf(int l) { return l >= 31 ? 32u : (uint32_t)l; }
+
+// Low 32 bits of cvtt64(x), the only bits a uint32 output keeps (fast path:
+// the low word of t + 1.5*2^52 is t mod 2^32 for |t| < 2^51).
+__device__ __forceinline__ uint32_t cvtt_lo32(double x) {
+    const double t = __builtin_trunc(x);
+    if (__builtin_expect(__builtin_fabs(t) < 0x1p51, 1)) return (uint32_t)__double_as_longlong(t + 0x1.8p52);
+    return (uint32_t)cvtt64(x);
+}
+
+// Value from a field f (l bits, 2 <= l <= 30) and the prediction's low word.
+__device__ __forceinline__ uint32_t opmax_apply(uint32_t pa, uint32_t f, int l) {
+    const uint32_t mag = f & ((1u << (l - 1)) - 1);
+    return (f >> (l - 1)) & 1 ? pa + mag : pa - mag;
+}
+
+// Runs of a column: each block splits into ceil(m / kOpmaxRun) runs that
+// start at multiples of kOpmaxRun inside the block, so no run crosses a block.
+__host__ __device__ inline uint32_t opmax_runs_per_block(uint32_t bs) { return (bs + kOpmaxRun - 1) / kOpmaxRun; }
+__host__ __device__ inline uint64_t opmax_runs(uint64_t n, uint32_t bs) {
+    const uint64_t nb = (n + bs - 1) / bs;
+    const uint32_t last = (uint32_t)(n - (nb - 1) * bs);
+    return (nb - 1) * opmax_runs_per_block(bs) + (last + kOpmaxRun - 1) / kOpmaxRun;
+}
+
+// Byte just past the field of value j in a block at stream offset `off` (raw: 4-byte values).
+__device__ __forceinline__ uint64_t opmax_field_end(uint64_t off, int l, uint32_t j) {
+    const uint32_t lf = opmax_lf(l);
+    return off + kOpmaxHdr + (((uint64_t)(j + 1) * lf + 7) >> 3);
+}
+
 template <bool FIX>
 __global__ __launch_bounds__(kThreads) void k_opmax_decode(const uint8_t *__restrict__ st,
                                                            const uint64_t *__restrict__ offs, uint64_t n, uint32_t bs,
-                                                           const int8_t *__restrict__ fix,
+                                                           uint64_t runs, const int8_t *__restrict__ fix,
                                                            uint32_t *__restrict__ out) {
-    constexpr uint32_t kTile = kThreads * kOpmaxRun;
-    const uint64_t t0pos = (uint64_t)blockIdx.x * kTile;
-    const uint64_t b0 = t0pos / bs;
-    const uint32_t r0 = (uint32_t)(t0pos - b0 * bs);
-    const uint64_t i0 = t0pos + threadIdx.x * kOpmaxRun;
-    if (i0 >= n) return;
-    const uint32_t q = r0 + threadIdx.x * kOpmaxRun;
-    uint64_t b = b0 + q / bs;
-    uint32_t j = q % bs;
-    OpmaxBlock k = opmax_block(st, offs, n, bs, b);
-    double p = k.l >= 31 ? 0.0 : chain_at(k.t0, k.t1, j, k.m);
-    uint32_t o[kOpmaxRun];
-    uint64_t fw = 0;
-    const bool whole = i0 + kOpmaxRun <= n;
+    __shared__ uint32_t lds[kOpmaxStage / 4];
+    __shared__ uint64_t s_a0;
+    __shared__ uint32_t s_words, s_blocks;
+    const int tid = threadIdx.x;
+    const uint32_t rpb = opmax_runs_per_block(bs);
+    const uint64_t g0 = (uint64_t)blockIdx.x * kThreads;            // first run of the workgroup
+    const uint64_t b0 = g0 / rpb;
+    const uint32_t rr0 = (uint32_t)(g0 - b0 * rpb);
+    const uint64_t g = g0 + tid;
+    const bool live = g < runs;
+    const uint32_t rq = rr0 + tid;
+    const uint64_t b = b0 + rq / rpb;
+    const uint32_t j0 = (rq % rpb) * kOpmaxRun;
+    uint64_t boff = live ? offs[b] : 0;  // in flight while the range is found
+    if (tid == 0) {
+        // the workgroup's whole blocks (headers included) when they fit, else
+        // just the stream bytes of its fields
+        const uint64_t gl = (g0 + kThreads < runs ? g0 + kThreads : runs) - 1;
+        const uint64_t bl = gl / rpb;
+        const uint64_t o0 = offs[b0], oe = offs[bl + 1];
+        uint64_t a0 = o0 & ~15ull, a1 = ((oe + 15) & ~15ull) + 16;
+        s_blocks = a1 - a0 <= kOpmaxStage;
+        if (!s_blocks) {
+            const uint64_t sl = bl * bs;
+            const uint32_t ml = (uint32_t)(n - sl < bs ? n - sl : bs);
+            const uint32_t jl0 = (uint32_t)(gl - bl * rpb) * kOpmaxRun;
+            const uint32_t jl = (jl0 + kOpmaxRun < ml ? jl0 + kOpmaxRun : ml) - 1;
+            const uint64_t ol = offs[bl];
+            const uint64_t ga = o0 + kOpmaxHdr + (((uint64_t)rr0 * kOpmaxRun * opmax_lf(st[o0])) >> 3);
+            const uint64_t gb = opmax_field_end(ol, st[ol], jl);
+            a0 = ga & ~15ull;
+            a1 = ((gb + 15) & ~15ull) + 16;
+        }
+        s_a0 = a0;
+        s_words = a1 - a0 <= kOpmaxStage ? (uint32_t)((a1 - a0) / 4) : 0u;
+    }
+    __syncthreads();
+    const uint64_t a0 = s_a0;
+    const uint32_t words = s_words;
+    for (uint32_t w = tid; w < words / 4; w += kThreads) ((uint4 *)lds)[w] = *(const uint4 *)(st + a0 + 16ull * w);
+    __syncthreads();
+    if (!live) return;
+
+    OpmaxBlock k;
+    if (s_blocks) {
+        // header from the staged block: [l][theta0][theta1] at LDS byte boff - a0
+        const uint32_t x = (uint32_t)(boff - a0);
+        auto u32_at = [&](uint32_t byte) -> uint32_t {
+            return __builtin_amdgcn_alignbit(lds[(byte >> 2) + 1], lds[byte >> 2], 8 * (byte & 3));
+        };
+        k.off = boff;
+        k.l = (int)(u32_at(x) & 0xFF);
+        k.t0 = bits_to_double((uint64_t)u32_at(x + 1) | ((uint64_t)u32_at(x + 5) << 32));
+        k.t1 = (double)__uint_as_float(u32_at(x + 9));
+        const uint64_t sb = b * bs;
+        k.m = (uint32_t)(n - sb < bs ? n - sb : bs);
+    } else {
+        k = opmax_block(st, offs, n, bs, b);
+    }
+    const uint32_t len = k.m - j0 < kOpmaxRun ? k.m - j0 : kOpmaxRun;
+    const uint64_t i0 = b * bs + j0;
+    const bool whole = len == kOpmaxRun;
+    uint64_t fw[kOpmaxRun / 8] = {};
     if (FIX) {
         if (whole && ((uintptr_t)(fix + i0) & 7) == 0) {
-            fw = *(const uint64_t *)(fix + i0);
+#pragma unroll
+            for (uint32_t h = 0; h < kOpmaxRun / 8; ++h) fw[h] = *(const uint64_t *)(fix + i0 + 8 * h);
         } else {
-            for (uint32_t r = 0; r < kOpmaxRun && i0 + r < n; ++r) fw |= (uint64_t)(uint8_t)fix[i0 + r] << (8 * r);
+            for (uint32_t r = 0; r < len; ++r) fw[r / 8] |= (uint64_t)(uint8_t)fix[i0 + r] << (8 * (r % 8));
         }
     }
+    auto flag = [&](uint32_t r) -> uint32_t { return (uint32_t)(int32_t)(int8_t)(fw[r / 8] >> (8 * (r % 8))); };
+    const uint32_t lf = opmax_lf(k.l);
+    const uint32_t bit0 = words ? (uint32_t)((int64_t)(k.off + kOpmaxHdr - a0) * 8) + j0 * lf : 0u;
+    const uint8_t *pay = st + k.off + kOpmaxHdr;
+    // field r of the run: from LDS when staged, else from global memory
+    auto field = [&](uint32_t r) -> uint32_t {
+        if (words) {
+            const uint32_t qb = bit0 + r * lf;
+            const uint32_t v = __builtin_amdgcn_alignbit(lds[(qb >> 5) + 1], lds[qb >> 5], qb & 31);
+            return lf == 32 ? v : v & ((1u << lf) - 1);
+        }
+        return (uint32_t)load_bits(pay, (uint64_t)(j0 + r) * lf, (int)lf);
+    };
+    uint32_t o[kOpmaxRun];
+    if (k.l >= 31) {
 #pragma unroll
-    for (uint32_t r = 0; r < kOpmaxRun; ++r) {
-        o[r] = 0;
-        if (i0 + r < n) {
-            if (j == k.m) {  // the run crosses into the next block
-                ++b;
-                j = 0;
-                k = opmax_block(st, offs, n, bs, b);
-                p = k.t0;
+        for (uint32_t r = 0; r < kOpmaxRun; ++r) {
+            o[r] = 0;
+            if (r < len) o[r] = field(r);
+        }
+    } else {
+        double p = chain_at(k.t0, k.t1, j0, k.m);
+        const int l = k.l;
+        // the run's predictions lie between p and p + kOpmaxRun*t (monotone chain), so
+        // one test decides whether every truncation takes cvtt_lo32's fast path
+        const double pe = __builtin_fma((double)kOpmaxRun, k.t1, p);
+        const bool fast = whole && words && __builtin_fabs(p) < 0x1p50 && __builtin_fabs(pe) < 0x1p50;
+        if (fast && l) {
+            const uint32_t fm = (2u << (l - 1)) - 1;
+            uint32_t f[kOpmaxRun];
+#pragma unroll
+            for (uint32_t r = 0; r < kOpmaxRun; ++r) {  // all LDS reads in flight together
+                const uint32_t qb = bit0 + r * lf;
+                f[r] = __builtin_amdgcn_alignbit(lds[(qb >> 5) + 1], lds[qb >> 5], qb & 31) & fm;
             }
-            uint32_t v;
-            if (k.l >= 31) {
-                v = (uint32_t)load_bits(k.pay, (uint64_t)j * 32, 32);
-            } else {
-                const int64_t pa = cvtt64(p);
-                if (k.l) {
-                    const uint64_t f = load_bits(k.pay, (uint64_t)j * k.l, k.l);
-                    const uint64_t mag = f & ((1ull << (k.l - 1)) - 1);
-                    v = (uint32_t)((f >> (k.l - 1)) & 1 ? (uint64_t)pa + mag : (uint64_t)pa - mag);
-                } else {
-                    v = (uint32_t)pa;
-                }
-                if (FIX) v += (uint32_t)(int32_t)(int8_t)(fw >> (8 * r));
+#pragma unroll
+            for (uint32_t r = 0; r < kOpmaxRun; ++r) {
+                const uint32_t pa = (uint32_t)__double_as_longlong(__builtin_trunc(p) + 0x1.8p52);
+                o[r] = opmax_apply(pa, f[r], l) + (FIX ? flag(r) : 0u);
                 p = p + k.t1;
             }
-            o[r] = v;
-            ++j;
+        } else if (fast) {
+#pragma unroll
+            for (uint32_t r = 0; r < kOpmaxRun; ++r) {
+                o[r] = (uint32_t)__double_as_longlong(__builtin_trunc(p) + 0x1.8p52) + (FIX ? flag(r) : 0u);
+                p = p + k.t1;
+            }
+        } else {
+            // partial runs (a block's last), unstaged workgroups, predictions beyond 2^50
+            for (uint32_t r = 0; r < kOpmaxRun; ++r) {
+                o[r] = 0;
+                if (r < len) {
+                    const uint32_t pa = cvtt_lo32(p);
+                    o[r] = (l ? opmax_apply(pa, field(r), l) : pa) + (FIX ? flag(r) : 0u);
+                    p = p + k.t1;
+                }
+            }
         }
     }
     uint32_t *dst = out + i0;
     if (whole && ((uintptr_t)dst & 15) == 0) {
-        static_assert(kOpmaxRun == 8, "two 16-byte stores per run");
-        *(uint4 *)dst = make_uint4(o[0], o[1], o[2], o[3]);
-        *(uint4 *)(dst + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+#pragma unroll
+        for (uint32_t h = 0; h < kOpmaxRun; h += 4) *(uint4 *)(dst + h) = make_uint4(o[h], o[h + 1], o[h + 2], o[h + 3]);
     } else {
-        for (uint32_t r = 0; r < kOpmaxRun && i0 + r < n; ++r) dst[r] = o[r];
+        for (uint32_t r = 0; r < len; ++r) dst[r] = o[r];
+    }
+}
+
+// The reference's fix-up lists applied to decoded values, skipping raw blocks
+// (their values are stored, not predicted): the input, bit for bit.
+__global__ __launch_bounds__(kThreads) void k_opmax_apply_exact(uint32_t *__restrict__ values, uint64_t n,
+                                                                uint32_t bs, const uint8_t *__restrict__ st,
+                                                                const uint64_t *__restrict__ offs,
+                                                                const int32_t *__restrict__ plus, uint64_t np,
+                                                                const int32_t *__restrict__ minus, uint64_t nm) {
+    const uint64_t q = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    for (int h = 0; h < 2; ++h) {
+        if (q >= (h ? nm : np)) continue;
+        const int32_t p = (h ? minus : plus)[q];
+        if (p < 0 || (uint64_t)p >= n || st[offs[(uint64_t)p / bs]] >= 31) continue;
+        atomicAdd(&values[p], h ? 0xFFFFFFFFu : 1u);
     }
 }
 
@@ -521,10 +657,20 @@ hipError_t launch_opmax_encode(const uint32_t *in, uint64_t n, uint32_t bs, uint
 
 hipError_t launch_opmax_decode(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
                                const int8_t *fix, uint32_t *out, hipStream_t st) {
-    constexpr uint64_t kTile = kThreads * kOpmaxRun;
-    const dim3 grid((unsigned)((n + kTile - 1) / kTile));
-    if (fix) k_opmax_decode<true><<<grid, dim3(kThreads), 0, st>>>(stream, offs, n, bs, fix, out);
-    else k_opmax_decode<false><<<grid, dim3(kThreads), 0, st>>>(stream, offs, n, bs, nullptr, out);
+    const uint64_t runs = opmax_runs(n, bs);
+    const dim3 grid((unsigned)((runs + kThreads - 1) / kThreads));
+    if (fix) k_opmax_decode<true><<<grid, dim3(kThreads), 0, st>>>(stream, offs, n, bs, runs, fix, out);
+    else k_opmax_decode<false><<<grid, dim3(kThreads), 0, st>>>(stream, offs, n, bs, runs, nullptr, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_opmax_apply_exact(uint32_t *values, uint64_t n, uint32_t bs, const uint8_t *stream,
+                                    const uint64_t *offs, const int32_t *plus, uint64_t np, const int32_t *minus,
+                                    uint64_t nm, hipStream_t st) {
+    const uint64_t q = np > nm ? np : nm;
+    if (q == 0) return hipSuccess;
+    k_opmax_apply_exact<<<dim3((unsigned)((q + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(
+        values, n, bs, stream, offs, plus, np, minus, nm);
     return hipGetLastError();
 }
 

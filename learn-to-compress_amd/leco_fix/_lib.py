@@ -142,6 +142,8 @@ SIGNATURES.update({
     "leco_fix_opmax_stream_bound": (c_u64, [c_u64, c_u32]),
     "leco_fix_opmax_encode_column": (c_int, [c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr, c_ptr, c_ptr]),
     "leco_fix_opmax_decode_column": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_ptr, c_ptr]),
+    "leco_fix_opmax_decode_column_lists": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_u64, c_ptr,
+                                                   c_ptr]),
     "leco_fix_opmax_lookup": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr]),
     "leco_fix_opmax_build_params": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_ptr]),
     "leco_fix_opmax_lists_workspace_bytes": (c_u64, [c_u64]),

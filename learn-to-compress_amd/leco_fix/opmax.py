@@ -91,6 +91,20 @@ class OpMaxColumn:
                                                  out.data_ptr(), _stream_handle(stream)), "opmax decode_column")
         return out
 
+    def decode_lists(self, plus: torch.Tensor, minus: torch.Tensor, out: torch.Tensor | None = None,
+                     stream=None) -> torch.Tensor:
+        """The input values from the decoder's output and the reference's sparse
+        fix-up lists (fix_lists()), raw blocks left as stored."""
+        plus = plus.contiguous().to(torch.int32)
+        minus = minus.contiguous().to(torch.int32)
+        if out is None:
+            out = torch.empty(self.n, dtype=torch.int32, device=self.stream.device)
+        check(lib().leco_fix_opmax_decode_column_lists(self.stream.data_ptr(), self.offsets.data_ptr(), self.n,
+                                                       self.block_size, plus.data_ptr(), plus.numel(),
+                                                       minus.data_ptr(), minus.numel(), out.data_ptr(),
+                                                       _stream_handle(stream)), "opmax decode_column_lists")
+        return out
+
     def lookup(self, idx: torch.Tensor, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
         idx = idx.contiguous().to(torch.int64)
         if out is None:

@@ -31,6 +31,7 @@ static void check(double p0, double t, uint32_t m) {
     }
     uint32_t s = 0;
     double q = p0;
+    std::vector<uint32_t> probes;  // segment edges, for chain_at below
     while (s < m) {
         const ChainSeg g = leco::chain_segment(q, t, s, m - s);
         if (g.len < 1 || g.s != s || s + g.len > m) {
@@ -47,12 +48,16 @@ static void check(double p0, double t, uint32_t m) {
             }
         }
         q = leco::chain_after(g, t);
+        if (probes.size() < 64) {
+            probes.push_back(s + g.len - 1);
+            if (s + g.len < m) probes.push_back(s + g.len);
+        }
         s += g.len;
         ++g_segments;
     }
-    // random point queries through chain_at
-    for (int z = 0; z < 4; ++z) {
-        const uint32_t j = (uint32_t)((uint64_t)rand() * m / ((uint64_t)RAND_MAX + 1));
+    // point queries through chain_at: segment edges and random positions
+    for (int z = 0; z < 8; ++z) probes.push_back((uint32_t)((uint64_t)rand() * m / ((uint64_t)RAND_MAX + 1)));
+    for (const uint32_t j : probes) {
         const double v = leco::chain_at(p0, t, j, m);
         if (!same(v, ref[j])) {
             if (g_bad++ < 10) std::printf("chain_at mismatch p0=%a t=%a m=%u j=%u\n", p0, t, m, j);

@@ -56,6 +56,7 @@ def test_opmax_golden_on_gpu(case):
     raw_dec = col.decode(fixed=False)
     assert np.array_equal(host32(raw_dec), g["decoded"])
     assert np.array_equal(host32(col.decode(fixed=True)), y)
+    assert np.array_equal(host32(col.decode_lists(plus, minus)), y)
     # the harness's correction on the reference decoder's output
     fixed = host32(col.apply_lists(raw_dec.clone(), plus, minus))
     keep = ~raw_mask(g["stream"], g["offsets"], len(y), bs)
@@ -101,6 +102,7 @@ def test_opmax_matches_oracle(kind, bs):
     plus, minus = col.fix_lists()
     op, om = oracle.opmax_fix_lists(fix)
     assert np.array_equal(plus.cpu().numpy(), op) and np.array_equal(minus.cpu().numpy(), om)
+    assert np.array_equal(host32(col.decode_lists(plus, minus)), y)
     idx = rng.integers(0, n, 50_000).astype(np.int64)
     assert np.array_equal(host32(col.lookup(torch.from_numpy(idx).cuda())), y[idx])
 
@@ -174,6 +176,8 @@ def test_opmax_200m_round_trip():
     del i
     col = OpMaxColumn.encode(y, bs)
     assert torch.equal(col.decode(fixed=True), y)
+    plus, minus = col.fix_lists()
+    assert torch.equal(col.decode_lists(plus, minus), y)
     idx = torch.randint(0, n, (1_000_000,), device="cuda", generator=g)
     assert torch.equal(col.lookup(idx), y[idx])
     yh = y.cpu().numpy().view(np.uint32)
@@ -187,3 +191,13 @@ def test_opmax_200m_round_trip():
         assert np.array_equal(fx[b * bs:(b + 1) * bs], ofix)
     plus, minus = col.fix_lists()
     assert len(plus) == int((col.fix > 0).sum()) and len(minus) == int((col.fix < 0).sum())
+
+
+def test_opmax_cpp_driver_flow():
+    # experiments/fix_int.cpp's flow compiled against include/leco_opmax.hpp
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "opmax_driver")
+    assert os.path.exists(exe), "built by learn-to-compress_amd/build.py"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.startswith("ok plus=254 minus=289"), r.stdout
