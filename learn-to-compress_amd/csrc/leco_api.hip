@@ -204,13 +204,13 @@ int filter_block_host(const uint8_t *h_in, uint64_t len, bool close, uint64_t fi
     uint64_t dcap = cap < len + 1 ? cap : len + 1;
     for (;;) {
         const size_t st_b = up256(bytes + LECO_FIX_TAIL_PAD);
-        const size_t small_b = up256(8 * sizeof(uint64_t));
+        const size_t small_b = up256(6 * sizeof(uint64_t) + leco_fix_filter_workspace_bytes(len, (uint32_t)len));
         const size_t out_b = up256((dcap ? dcap : 1) * sizeof(uint32_t));
         int rc = g_stage.reserve(st_b + small_b + out_b);
         if (rc) return rc;
         uint8_t *base_p = (uint8_t *)g_stage.dev;
         uint8_t *d_st = base_p;
-        uint64_t *d_small = (uint64_t *)(base_p + st_b);  // offsets[2], f1, f2, result[2], work[2]
+        uint64_t *d_small = (uint64_t *)(base_p + st_b);  // offsets[2], f1, f2, result[2], work[...]
         uint32_t *d_out = (uint32_t *)(base_p + st_b + small_b);
         hipStream_t s = g_stage.stream;
         uint64_t small[8] = {0, bytes, 0, 0, 0, 0, 0, 0};
@@ -356,11 +356,12 @@ int for_filter_block_host(const uint8_t *h_in, uint64_t len, bool close, uint64_
     const uint64_t dcap = cap < len ? cap : len;
     uint8_t *d_st, *d_x;
     uint64_t *d_off;
-    int rc = stage_block<T>(leco::kCodecFor, h_in, len, 64 + (dcap ? dcap : 1) * 4, &d_st, &d_off, &d_x);
+    const size_t small_b = up256(16 + leco_fix_filter_workspace_bytes(len, (uint32_t)len));
+    int rc = stage_block<T>(leco::kCodecFor, h_in, len, small_b + (dcap ? dcap : 1) * 4, &d_st, &d_off, &d_x);
     if (rc) return rc;
     hipStream_t s = g_stage.stream;
     uint64_t *d_res = (uint64_t *)d_x, *d_work = d_res + 2;
-    uint32_t *d_out = (uint32_t *)(d_x + 64);
+    uint32_t *d_out = (uint32_t *)(d_x + small_b);
     uint64_t res[2];
     if (leco::launch_for_filter<T>(d_st, d_off, len, (uint32_t)len, close, f1, f2, base,
                                    (uint32_t)block_id * block_size, d_out, dcap, d_res, d_work, s) != hipSuccess ||
@@ -632,7 +633,7 @@ LECO_BASE_ABI(u64, uint64_t)
 
 uint64_t leco_fix_filter_workspace_bytes(uint64_t n, uint32_t block_size) {
     if (n == 0 || block_size == 0) return 0;
-    return 8 * (leco::filter_tiles(n, block_size) + 1);
+    return 8 * (leco::filter_work_tiles(n, block_size) + 1);
 }
 
 int leco_fix_filter_range_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,

@@ -95,9 +95,10 @@ __device__ __forceinline__ bool passes(const uint8_t *blk, uint32_t i, bool raw,
     return CLOSE ? (above<W>(v, f1) && below<W>(v, f2)) : above<W>(v, f1);
 }
 
-// Workgroup exclusive scan of one value per thread; returns this thread's
-// prefix and the total in *total (every thread).
-__device__ __forceinline__ uint32_t wg_exclusive_scan(FilterShared &sh, uint32_t v, uint32_t *total);
+// Workgroup exclusive scan of one value per thread (through sh.wcnt);
+// returns this thread's prefix and the total in *total (every thread).
+template <class Shared>
+__device__ __forceinline__ uint32_t wg_exclusive_scan(Shared &sh, uint32_t v, uint32_t *total);
 
 // Candidates q in [0, total) in order; cand(q, &position) says whether q
 // passes and where it is.  Pass 1 (WRITE = false) adds this thread's hits to
@@ -165,7 +166,8 @@ __device__ __forceinline__ void scan_window(const FilterArgs &a, FilterShared &s
                         acc);
 }
 
-__device__ __forceinline__ uint32_t wg_exclusive_scan(FilterShared &sh, uint32_t v, uint32_t *total) {
+template <class Shared>
+__device__ __forceinline__ uint32_t wg_exclusive_scan(Shared &sh, uint32_t v, uint32_t *total) {
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     uint32_t inc = v;
     for (int o = 1; o < kWave; o <<= 1) {
@@ -185,28 +187,16 @@ __device__ __forceinline__ uint32_t wg_exclusive_scan(FilterShared &sh, uint32_t
     return before + inc - v;
 }
 
-// filter_range (:206-249) on block b: one window from the model threshold.
-template <typename T, int W, bool WRITE>
-__device__ __forceinline__ void gt_block(const FilterArgs &a, FilterShared &sh, uint64_t b, uint32_t m,
-                                         uint32_t bstart, uint64_t &acc) {
-    constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
-    const uint8_t *blk = a.stream + a.offs[b];
-    const int l = blk[0];
-    const uint64_t f = a.filter & wmask;
-    if (l >= W) {
-        // raw blocks (and the one-/two-value blocks 127/126): every stored value
-        const uint32_t mm = l == 127 ? (m < 1 ? m : 1) : l == 126 ? (m < 2 ? m : 2) : m;
-        scan_window<W, false, WRITE>(a, sh, blk, 0, mm, true, l, 0.0, 0.0, f, 0, bstart, acc);
-        return;
-    }
-    const double t0 = bits_to_double(load_u64_any(blk + 1)), t1 = bits_to_double(load_u64_any(blk + 9));
+// filter_range (:206-249): the first position of block b's window, from the
+// model threshold (positions before it are not scanned).  m when the window is empty.
+template <int W>
+__device__ __forceinline__ uint32_t gt_window(uint64_t f, int l, double t0, double t1, uint32_t m) {
     const int64_t delta = l ? (int64_t)1 << (l - 1) : 0;
     double x;
     if (W == 64) x = (double)(f + 1 - (uint64_t)delta) - t0;
     else x = (double)((int64_t)(uint32_t)(f + 1) - delta) - t0;
     const int32_t thre = cvtt32(std_max(x / t1, 0.0));
-    if ((uint64_t)(int64_t)thre >= m) return;
-    scan_window<W, false, WRITE>(a, sh, blk, (uint32_t)thre, m, false, l, t0, t1, f, 0, bstart, acc);
+    return (uint64_t)(int64_t)thre >= m ? m : (uint32_t)thre;
 }
 
 // ---- filter_range_close (:251-303) ----------------------------------------
@@ -460,7 +450,7 @@ __device__ __forceinline__ void for_block(const FilterArgs &a, FilterShared &sh,
                         acc);
 }
 
-// MODE: 0 LeCo filter_range, 1 LeCo filter_range_close, 2 FOR filter_range, 3 FOR filter_range_close.
+// The close filters, MODE 1 (LeCo filter_range_close) and 3 (FOR filter_range_close).
 template <typename T, int W, int MODE>
 __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     __shared__ FilterShared sh;
@@ -482,8 +472,7 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
         if (MODE == 1) ok &= close_block<T, W, false>(a, sh, b, block_len(b), k, 0, mine);
-        else if (MODE == 0) gt_block<T, W, false>(a, sh, b, block_len(b), 0, mine);
-        else for_block<T, W, MODE == 3, false>(a, sh, b, block_len(b), 0, mine);
+        else for_block<T, W, true, false>(a, sh, b, block_len(b), 0, mine);
     }
     for (int o = kWave / 2; o > 0; o >>= 1) mine += (uint64_t)__shfl_xor((long long)mine, o, kWave);
     if (lane == 0) sh.wsum[wid] = mine;
@@ -507,8 +496,182 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
         const uint64_t b = b0 + k;
         const uint32_t bstart = a.pos_base + (uint32_t)(b * a.bs);
         if (MODE == 1) close_block<T, W, true>(a, sh, b, block_len(b), k, bstart, pos);
-        else if (MODE == 0) gt_block<T, W, true>(a, sh, b, block_len(b), bstart, pos);
-        else for_block<T, W, MODE == 3, true>(a, sh, b, block_len(b), bstart, pos);
+        else for_block<T, W, true, true>(a, sh, b, block_len(b), bstart, pos);
+    }
+}
+
+// ---- filter_range of LeCo (:206-249) and FOR (:111-158) columns ------------
+// A workgroup tile is gt_tile_blocks() whole blocks (blocks of up to
+// kFilterTile values) or a kFilterTile-value chunk of one larger block.  Its
+// field bytes are staged through LDS when they fit.  Block by block (the
+// header is workgroup-uniform), the scanned window is tested once, in rounds
+// of kThreads consecutive positions (neighbouring lanes read neighbouring
+// fields); each wave parks its round ballot in LDS.  A workgroup scan of the
+// (round, wave) counts gives the tile's total, a look-back over the tiles its
+// output slot, and every hit is written at slot + the hits before it in
+// (round, wave, lane) order -- position order -- with coalesced stores.
+constexpr uint32_t kGtStage = 16384;
+
+// Rounds a tile can need: its full rounds plus one partial round per block.
+__host__ __device__ inline uint32_t gt_max_rounds(uint32_t bs) {
+    return kFilterTile / kThreads + (bs > kFilterTile ? 1 : filter_tile_blocks(bs));
+}
+// Dynamic LDS of k_filter_gt: the staged bytes, then per (round, wave) the
+// ballot and its exclusive prefix, then per round its first position.
+__host__ __device__ inline size_t gt_lds_bytes(uint32_t bs) {
+    return kGtStage + (size_t)gt_max_rounds(bs) * (kWaves * 12 + 4);
+}
+
+struct GtShared {
+    uint64_t base, tile, a0;
+    uint32_t words;
+    uint32_t wcnt[kWaves];
+};
+
+template <typename T, int W, bool FOR>
+__global__ __launch_bounds__(kThreads) void k_filter_gt(FilterArgs a) {
+    extern __shared__ uint4 gt_dyn[];  // 16-byte aligned: the staging writes are 16-byte stores
+    __shared__ GtShared sh;
+    const uint32_t max_rounds = gt_max_rounds(a.bs);
+    uint32_t *lds = (uint32_t *)gt_dyn;
+    uint64_t *rbal = (uint64_t *)(gt_dyn + kGtStage / 16);
+    uint32_t *rpre = (uint32_t *)(rbal + max_rounds * kWaves);
+    uint32_t *rstart = rpre + max_rounds * kWaves;
+    constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
+    constexpr uint32_t tb = W / 8;
+    constexpr uint32_t hdr = FOR ? 1 + 2 * tb : 17;  // payload offset of a packed block
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    if (tid == 0) sh.tile = atomicAdd((unsigned long long *)a.work, 1ull);  // work[0] is the ticket
+    __syncthreads();
+    const uint64_t t = sh.tile;
+    const uint64_t nb = (a.n + a.bs - 1) / a.bs;
+    // the tile: blocks [b0, b0 + kbt) whole, or values [c0, c1) of block b0
+    uint64_t b0;
+    uint32_t kbt, c0 = 0, c1 = 0;
+    const bool chunked = a.bs > kFilterTile;
+    if (chunked) {
+        const uint32_t cpb = (a.bs + kFilterTile - 1) / kFilterTile;
+        b0 = t / cpb;
+        kbt = 1;
+        const uint64_t s = b0 * a.bs;
+        const uint32_t m = (uint32_t)(a.n - s < a.bs ? a.n - s : a.bs);
+        c0 = (uint32_t)(t - b0 * cpb) * kFilterTile;
+        c1 = c0 + kFilterTile < m ? c0 + kFilterTile : m;
+    } else {
+        b0 = t * a.kb;
+        kbt = (uint32_t)(nb - b0 < a.kb ? nb - b0 : a.kb);
+    }
+    if (tid == 0) {
+        uint64_t g0 = a.offs[b0], g1 = a.offs[b0 + kbt];
+        if (chunked) {  // the chunk's fields only
+            const uint8_t *blk = a.stream + g0;
+            const int l = blk[0];
+            const uint32_t lf = l >= W ? W : (uint32_t)l;
+            const uint64_t pay = g0 + (l >= W ? 1 : hdr);
+            g1 = pay + (((uint64_t)c1 * lf + 7) >> 3);
+            g0 = pay + (((uint64_t)c0 * lf) >> 3);
+        }
+        const uint64_t a0 = g0 & ~15ull, a1 = ((g1 + 15) & ~15ull) + 16;
+        sh.a0 = a0;
+        sh.words = a1 - a0 <= kGtStage ? (uint32_t)((a1 - a0) / 4) : 0u;
+    }
+    __syncthreads();
+    const uint64_t a0 = sh.a0;
+    const uint32_t words = sh.words;
+    for (uint32_t w = tid; w < words / 4; w += kThreads)
+        ((uint4 *)lds)[w] = *(const uint4 *)(a.stream + a0 + 16ull * w);
+    __syncthreads();
+
+    const uint64_t f = a.filter & wmask;
+    uint32_t rounds = 0;
+    for (uint32_t kk = 0; kk < kbt; ++kk) {
+        const uint64_t b = b0 + kk;
+        const uint64_t off = a.offs[b];
+        const uint8_t *blk = a.stream + off;
+        const uint64_t s = b * a.bs;
+        const uint32_t m = (uint32_t)(a.n - s < a.bs ? a.n - s : a.bs);
+        const int l = __builtin_amdgcn_readfirstlane(blk[0]);
+        const bool raw = l >= W;
+        const uint32_t lf = raw ? W : (uint32_t)l;
+        const uint64_t pay = raw ? 1 : hdr;  // payload byte offset in the block
+        // window [lo, hi) of positions the reference scans
+        uint32_t lo = chunked ? c0 : 0, hi = chunked ? c1 : m;
+        double t0 = 0.0, t1 = 0.0;
+        uint64_t base_v = 0;
+        if (raw) {
+            if (!FOR) {  // LeCo's one- and two-value raw blocks (127 / 126)
+                const uint32_t mm = l == 127 ? (m < 1 ? m : 1) : l == 126 ? (m < 2 ? m : 2) : m;
+                hi = hi < mm ? hi : mm;
+            }
+        } else if (FOR) {
+            base_v = load_u64_any(blk + 1) & wmask;
+            if ((load_u64_any(blk + 1 + tb) & wmask) <= f) hi = lo;  // max <= filter: nothing
+        } else {
+            t0 = bits_to_double(load_u64_any(blk + 1));
+            t1 = bits_to_double(load_u64_any(blk + 9));
+            const uint32_t w0 = gt_window<W>(f, l, t0, t1, m);
+            lo = lo > w0 ? lo : w0;
+        }
+        // bit offset of field 0 of this block in LDS (staged) or in the block
+        const int64_t fbit = words ? ((int64_t)(off + pay) - (int64_t)a0) * 8 : (int64_t)pay * 8;
+        auto field = [&](uint32_t j) -> uint64_t {
+            if (!words) return load_bits(blk, (uint64_t)fbit + (uint64_t)j * lf, (int)lf);
+            const uint32_t q = (uint32_t)(fbit + (int64_t)j * lf), w = q >> 5, sft = q & 31;
+            const uint32_t x0 = lds[w], x1 = lds[w + 1];
+            uint64_t v = __builtin_amdgcn_alignbit(x1, x0, sft);
+            if (W == 64) v |= (uint64_t)__builtin_amdgcn_alignbit(lds[w + 2], x1, sft) << 32;
+            return v & low_mask((int)lf);
+        };
+#pragma unroll 4
+        for (uint32_t p = lo; p < hi; p += kThreads) {
+            const uint32_t j = p + tid;
+            bool h = false;
+            if (j < hi) {
+                uint64_t v;
+                if (raw) v = field(j);
+                else if (FOR) v = l ? (base_v + field(j)) & wmask : base_v;
+                else {
+                    const int64_t pr = predict(t0, t1, j);
+                    v = l ? apply_field(pr, field(j), l) : (uint64_t)pr;
+                }
+                h = (raw || FOR) ? v > f : above<W>(v, f);
+            }
+            const uint64_t bal = __ballot(h);
+            if (lane == 0) rbal[rounds * kWaves + wid] = bal;
+            if (tid == 0) rstart[rounds] = a.pos_base + (uint32_t)(s + p);
+            ++rounds;
+        }
+    }
+    __syncthreads();
+
+    // exclusive prefix of the (round, wave) counts: each thread owns a run of them
+    const uint32_t ne = rounds * kWaves;
+    const uint32_t per = (ne + kThreads - 1) / kThreads;
+    const uint32_t e0 = tid * per < ne ? tid * per : ne, e1 = e0 + per < ne ? e0 + per : ne;
+    uint32_t mine = 0;
+    for (uint32_t e = e0; e < e1; ++e) mine += (uint32_t)__popcll(rbal[e]);
+    uint32_t total;
+    uint32_t run = wg_exclusive_scan(sh, mine, &total);
+    for (uint32_t e = e0; e < e1; ++e) {
+        rpre[e] = run;
+        run += (uint32_t)__popcll(rbal[e]);
+    }
+    if (wid == 0) {
+        const uint64_t excl = look_back(t, total, a.work);
+        if (lane == 0) {
+            sh.base = excl;
+            if (t + 1 == a.tiles) a.result[0] = excl + total;
+        }
+    }
+    __syncthreads();
+    const uint64_t base = sh.base;
+    const uint64_t below = (1ull << lane) - 1;
+    for (uint32_t r = 0; r < rounds; ++r) {
+        const uint64_t bal = rbal[r * kWaves + wid];
+        if ((bal >> lane) & 1) {
+            const uint64_t at = base + rpre[r * kWaves + wid] + (uint64_t)__popcll(bal & below);
+            if (at < a.cap) a.out[at] = rstart[r] + (uint32_t)tid;
+        }
     }
 }
 
@@ -526,13 +689,13 @@ hipError_t launch_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n
                          uint64_t filter, const T *f1, const T *f2, int32_t base, uint32_t pos_base, uint32_t *out,
                          uint64_t cap, uint64_t *result, uint64_t *work, hipStream_t st) {
     constexpr int W = 8 * sizeof(T);
-    FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), filter_tiles(n, bs), filter, 0, f1, f2, base,
-                 pos_base, out, cap, result, work};
+    FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), close ? filter_tiles(n, bs) : gt_tiles(n, bs), filter,
+                 0, f1, f2, base, pos_base, out, cap, result, work};
     const uint64_t words = a.tiles + 1;
     k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
                                                                                                     result);
     if (close) k_filter<T, W, 1><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
-    else k_filter<T, W, 0><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    else k_filter_gt<T, W, false><<<dim3((unsigned)a.tiles), dim3(kThreads), gt_lds_bytes(bs), st>>>(a);
     return hipGetLastError();
 }
 
@@ -541,13 +704,13 @@ hipError_t launch_for_filter(const uint8_t *stream, const uint64_t *offs, uint64
                              uint64_t f1, uint64_t f2, int32_t base, uint32_t pos_base, uint32_t *out, uint64_t cap,
                              uint64_t *result, uint64_t *work, hipStream_t st) {
     constexpr int W = 8 * sizeof(T);
-    FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), filter_tiles(n, bs), f1, f2, nullptr, nullptr, base,
-                 pos_base, out, cap, result, work};
+    FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), close ? filter_tiles(n, bs) : gt_tiles(n, bs), f1, f2,
+                 nullptr, nullptr, base, pos_base, out, cap, result, work};
     const uint64_t words = a.tiles + 1;
     k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
                                                                                                     result);
     if (close) k_filter<T, W, 3><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
-    else k_filter<T, W, 2><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    else k_filter_gt<T, W, true><<<dim3((unsigned)a.tiles), dim3(kThreads), gt_lds_bytes(bs), st>>>(a);
     return hipGetLastError();
 }
 

@@ -57,14 +57,17 @@ struct EncodeShared {
 // Decoupled look-back of unit b (one wave; every lane returns the same
 // exclusive prefix) over the status words offs[p+1] of the units before it
 // (offs[0] is the ticket that orders the units).  Publishes the unit's size,
-// then reads kLookWin predecessors' words per step (kLookPer per lane,
-// nearest first) until the nearest one that carries an inclusive prefix; the
-// prefix is its value plus the sizes of the units in between; finally
-// publishes the unit's inclusive prefix.  Every lower unit belongs to a wave
+// then loads kWave*PER predecessors' words per step (PER per lane, all in
+// flight together) and walks them 64 at a time, nearest first, until the
+// nearest one that carries an inclusive prefix; the prefix is its value plus
+// the sizes of the units in between; finally publishes the unit's inclusive
+// prefix.  Every lower unit belongs to a wave
 // that is already running and publishes its size before it waits, so the
 // waits end; the spin bound (~seconds) only keeps a broken invariant from
 // hanging the GPU.
+template <int PER = kLookPer>
 __device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_t *__restrict__ offs) {
+    constexpr int kWin = kWave * PER;
     const int lane = threadIdx.x % kWave;
     if (b == 0) {
         if (lane == 0) store_status(&offs[1], kStatusInclusive | size);
@@ -72,35 +75,35 @@ __device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_
     }
     if (lane == 0) store_status(&offs[b + 1], kStatusAggregate | size);
     uint64_t excl = 0;
-    for (int64_t top = (int64_t)b - 1;; top -= kLookWin) {
-        uint64_t stw[kLookPer];
+    for (int64_t top = (int64_t)b - 1;; top -= kWin) {
+        uint64_t stw[PER];
 #pragma unroll
-        for (int k = 0; k < kLookPer; ++k) {
+        for (int k = 0; k < PER; ++k) {
             const int64_t p = top - (k * kWave + lane);
             stw[k] = p >= 0 ? load_status(&offs[p + 1]) : kStatusInclusive;
         }
+        // chunks of 64 predecessors, nearest first: wait for a chunk, add the
+        // sizes up to its nearest inclusive prefix, stop there (predecessors
+        // behind that one are never waited for)
+        bool done = false;
 #pragma unroll
-        for (int k = 0; k < kLookPer; ++k) {
-            const int64_t p = top - (k * kWave + lane);
-            uint32_t spins = 0;
-            while ((stw[k] & kStatusMask) == 0 && ++spins < kSpinLimit) {
-                __builtin_amdgcn_s_sleep(1);
-                stw[k] = load_status(&offs[p + 1]);
+        for (int k = 0; k < PER; ++k) {
+            if (!done) {
+                const int64_t p = top - (k * kWave + lane);
+                uint32_t spins = 0;
+                while ((stw[k] & kStatusMask) == 0 && ++spins < kSpinLimit) {
+                    __builtin_amdgcn_s_sleep(1);
+                    stw[k] = load_status(&offs[p + 1]);
+                }
+                const uint64_t incl = __ballot((stw[k] & kStatusMask) == kStatusInclusive);
+                const int stop = incl ? __ffsll((long long)incl) - 1 : kWave;
+                uint64_t part = lane <= stop ? stw[k] & kStatusValue : 0;
+                for (int o = kWave / 2; o > 0; o >>= 1) part += (uint64_t)__shfl_xor((long long)part, o, kWave);
+                excl += part;
+                done = incl != 0;
             }
         }
-        int stop = kLookWin;  // nearest predecessor (q = k*64 + lane) with an inclusive prefix
-#pragma unroll
-        for (int k = kLookPer - 1; k >= 0; --k) {
-            const uint64_t incl = __ballot((stw[k] & kStatusMask) == kStatusInclusive);
-            if (incl) stop = k * kWave + __ffsll((long long)incl) - 1;
-        }
-        uint64_t part = 0;
-#pragma unroll
-        for (int k = 0; k < kLookPer; ++k)
-            if (k * kWave + lane <= stop) part += stw[k] & kStatusValue;
-        for (int o = kWave / 2; o > 0; o >>= 1) part += (uint64_t)__shfl_xor((long long)part, o, kWave);
-        excl += part;
-        if (stop < kLookWin) break;
+        if (done) break;
     }
     if (lane == 0) store_status(&offs[b + 1], kStatusInclusive | (excl + size));
     return excl;
