@@ -112,6 +112,23 @@ def test_base_host_surface(oracle_mod):
                     assert got.tolist() == [500 + i for i in range(500) if 10 < int(y[i]) % 1000 < 600]
 
 
+@pytest.mark.parametrize("bs", [8193, 30000])
+def test_for_host_filter_large_block(bs):
+    # one FOR block split into several filter tiles on the GPU; brute force is the reference
+    from leco_fix.baselines import FOR_int
+    rng = np.random.default_rng(bs)
+    for dt in (np.uint32, np.uint64):
+        width = 8 * np.dtype(dt).itemsize
+        c = FOR_int(dt)
+        c.init(1, bs)
+        y = rng.integers(0, 1 << (width - 6), bs, dtype=np.uint64).astype(dt)
+        blk = c.encodeArray8_int(y, bs)
+        say(f"host for filter large block width={width} bs={bs}")
+        for f in (0, int(y[bs // 2]), int(y.max())):
+            want = [3 * bs + i for i in range(bs) if int(y[i]) > f]
+            assert c.filter_range(blk, bs, f, nvalue=3).tolist() == want
+
+
 def test_base_cpp_driver_flow():
     # the reference driver's FOR / Delta flow compiled against include/leco_baselines.hpp
     import subprocess

@@ -164,6 +164,23 @@ def test_filter_host_block_surface(oracle_mod):
                 assert np.array_equal(got, want), (width, kind, f1, f2, base)
 
 
+@pytest.mark.parametrize("bs", [8193, 20000, 65536])
+def test_filter_host_large_block(oracle_mod, bs):
+    # one block split into several filter tiles (and look-back slots) on the GPU
+    from leco_fix import Leco_int
+    rng = np.random.default_rng(bs)
+    for width, dt in ((32, np.uint32), (64, np.uint64)):
+        codec = Leco_int(dt)
+        codec.init(1, bs)
+        y = np.sort(rng.integers(0, 1 << (width - 4), bs, dtype=np.uint64)).astype(dt)
+        blk = codec.encodeArray8_int(y, bs)
+        say(f"host filter large block width={width} bs={bs}")
+        for f in (0, int(y[bs // 3]), int(y[-2]), int(y.max())):
+            got = codec.filter_range(blk, bs, f, block_id=5)
+            want = oracle_mod.filter_block(blk, bs, dt, f, 5 * bs)
+            assert np.array_equal(got, want), (width, f)
+
+
 def test_filter_batch_arena_column(oracle_mod):
     import torch
     from leco_fix.batch import Batch
