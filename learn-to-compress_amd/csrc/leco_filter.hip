@@ -40,6 +40,8 @@ __device__ __forceinline__ bool below(uint64_t v, uint64_t f) {
 
 constexpr int kWaves = kThreads / kWave;
 constexpr int kEmitPer = 8;  // rounds of kThreads candidates per compaction step when writing positions
+constexpr uint32_t kCloseRounds = 64;  // scan rounds of a tile whose ballots pass 1 parks for pass 2
+constexpr uint32_t kCloseWin = 512;    // periods whose windows pass 1 keeps for the scan
 
 struct FilterShared {
     uint64_t wsum[kWaves];
@@ -53,9 +55,14 @@ struct FilterShared {
     uint64_t pf1[kThreads], pf2[kThreads];
     uint32_t plo[kThreads];
     uint32_t ppre[kThreads + 1];
-    // per block of the tile: the close loop's period count and whether pass 1 chose the scan
+    // per block of the tile: the close loop's period count, how pass 2 reports it (0 walk,
+    // 1 scan evaluated again, 2 scan from the ballots pass 1 parked) and its first parked round
     uint32_t bK[256];
     uint8_t bscan[256];
+    uint16_t bround[256];
+    uint64_t rb[kCloseRounds * kWaves];  // parked scan ballots of (round, wave)
+    // the current block's window of each of its first kCloseWin periods (filter_range_close)
+    int32_t pw1[kCloseWin], pw2[kCloseWin];
 };
 
 // Arguments of one filter launch.
@@ -252,6 +259,10 @@ __device__ __forceinline__ uint32_t close_periods(const CloseBlock &c, FilterSha
         int32_t thre1, thre2;
         period_window<W>((c.f1 + (uint64_t)k * c.step) & wmask, (c.f2 + (uint64_t)k * c.step) & wmask, c.delta,
                          c.t0, c.t1, c.m, &thre1, &thre2);
+        if (k < kCloseWin) {
+            sh.pw1[k] = thre1;
+            sh.pw2[k] = thre2;
+        }
         const bool cond = (uint64_t)(int64_t)thre1 < c.m && thre2 > 0;
         const uint64_t sb = __ballot(!cond || k >= kFilterMaxPeriods);
         if (lane == 0) sh.wfail[wid] = sb ? (uint32_t)(wid * kWave + __ffsll((long long)sb) - 1) : kThreads;
@@ -316,10 +327,11 @@ __device__ __forceinline__ uint64_t close_value(const CloseBlock &c, uint32_t i)
     return c.l ? apply_field(p, load_bits(c.blk + 17, (uint64_t)i * c.l, c.l), c.l) : (uint64_t)p;
 }
 
-// Scan: does position i (value v) pass in its own period?
+// Scan: does position i (value v) pass in its own period?  The windows of
+// periods below nwin come from the table close_periods filled.
 template <int W>
 __device__ __forceinline__ bool close_scan_hit(const CloseBlock &c, uint32_t K, uint32_t i, uint64_t v,
-                                               int32_t base) {
+                                               int32_t base, const FilterShared &sh, uint32_t nwin) {
     uint64_t d;
     if (W == 64) {
         if (v <= c.f1) return false;
@@ -333,14 +345,19 @@ __device__ __forceinline__ bool close_scan_hit(const CloseBlock &c, uint32_t K, 
     const uint64_t f1 = c.f1 + k * (uint64_t)base, f2 = c.f2 + k * (uint64_t)base;
     if (!(W == 64 ? v < f2 : (int64_t)v < (int64_t)f2)) return false;
     int32_t thre1, thre2;
-    period_window<W>(f1, f2, c.delta, c.t0, c.t1, c.m, &thre1, &thre2);
+    if (k < nwin) {
+        thre1 = sh.pw1[k];
+        thre2 = sh.pw2[k];
+    } else {
+        period_window<W>(f1, f2, c.delta, c.t0, c.t1, c.m, &thre1, &thre2);
+    }
     const uint32_t hi = c.l ? (uint32_t)thre2 : (uint32_t)thre2 + 1;
     return i >= (uint32_t)thre1 && i < hi;
 }
 
 template <typename T, int W, bool WRITE>
 __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &sh, uint64_t b, uint32_t m,
-                                            uint32_t slot, uint32_t bstart, uint64_t &acc) {
+                                            uint32_t slot, uint32_t bstart, uint64_t &acc, uint32_t &rounds_used) {
     constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     CloseBlock c;
@@ -361,6 +378,7 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
     bool cut = false;
     uint32_t K;
     bool scan;
+    uint32_t mode = 0;
     if (!WRITE) {
         K = close_periods<W>(c, sh, &cut);
         // scan needs disjoint, non-wrapping intervals up to period K
@@ -369,23 +387,37 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
                c.f1 <= wmask - reach && c.f2 <= wmask - reach;
     } else {
         K = sh.bK[slot];
-        scan = sh.bscan[slot] != 0;
+        mode = sh.bscan[slot];
+        scan = mode != 0;
     }
     if (K == 0) {
         if (!WRITE && tid == 0) { sh.bK[slot] = 0; sh.bscan[slot] = 0; }
         return !cut;
     }
     const uint32_t mp = c.l ? m : m + 1;  // positions the windows can reach
+    const uint32_t nr = (mp + kThreads - 1) / kThreads;  // scan rounds of kThreads positions
     if (!WRITE && scan) {
-        // one sweep: the scan's count and whether the values never decrease
+        // one sweep, each value decoded once: the scan's count, whether the values never
+        // decrease (the next value from the next lane; the last lane decodes it), and,
+        // while the tile has room, the round ballots for pass 2
+        const bool park = rounds_used + nr <= kCloseRounds;
         uint64_t cnt = 0;
         bool sorted = true;
-        for (uint32_t i = tid; i < mp; i += kThreads) {
-            const uint64_t v = close_value(c, i);
-            cnt += close_scan_hit<W>(c, K, i, v, a.base);
-            if (i + 1 < mp) {
-                const uint64_t u = close_value(c, i + 1);
-                sorted &= W == 64 ? v <= u : (int64_t)v <= (int64_t)u;
+        for (uint32_t r = 0; r < nr; ++r) {
+            const uint32_t i = r * kThreads + tid;
+            uint64_t v = 0;
+            bool h = false;
+            if (i < mp) {
+                v = close_value(c, i);
+                h = close_scan_hit<W>(c, K, i, v, a.base, sh, K < kCloseWin ? K : kCloseWin);
+            }
+            uint64_t u = (uint64_t)__shfl_down((long long)v, 1, kWave);
+            if (lane == kWave - 1 && i + 1 < mp) u = close_value(c, i + 1);
+            if (i + 1 < mp) sorted &= W == 64 ? v <= u : (int64_t)v <= (int64_t)u;
+            cnt += h;
+            if (park) {
+                const uint64_t bal = __ballot(h);
+                if (lane == 0) sh.rb[(rounds_used + r) * kWaves + wid] = bal;
             }
         }
         const uint64_t bad = __ballot(!sorted);
@@ -397,19 +429,42 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
         __syncthreads();
         scan = all_sorted;
         if (scan) acc += cnt;
+        mode = scan ? (park ? 2u : 1u) : 0u;
     }
     if (!WRITE && tid == 0) {
         sh.bK[slot] = K;
-        sh.bscan[slot] = scan ? 1 : 0;
+        sh.bscan[slot] = (uint8_t)mode;
+        sh.bround[slot] = (uint16_t)rounds_used;
     }
+    if (!WRITE && mode == 2) rounds_used += nr;
     if (scan) {
-        if (WRITE)
+        if (WRITE && mode == 2) {
+            // the parked ballots, in (round, wave, lane) order: position order
+            const uint32_t r0 = sh.bround[slot];
+            const uint64_t below = (1ull << lane) - 1;
+            for (uint32_t r = 0; r < nr; ++r) {
+                uint32_t before = 0, all = 0;
+#pragma unroll
+                for (int w = 0; w < kWaves; ++w) {
+                    const uint32_t cw = (uint32_t)__popcll(sh.rb[(r0 + r) * kWaves + w]);
+                    before += w < wid ? cw : 0;
+                    all += cw;
+                }
+                const uint64_t bal = sh.rb[(r0 + r) * kWaves + wid];
+                if ((bal >> lane) & 1) {
+                    const uint64_t at = acc + before + (uint64_t)__popcll(bal & below);
+                    if (at < a.cap) a.out[at] = bstart + r * kThreads + (uint32_t)tid;
+                }
+                acc += all;
+            }
+        } else if (WRITE) {
             emit_ordered<true>(a, sh, mp,
                                [&](uint32_t i, uint32_t *where) {
                                    *where = bstart + i;
-                                   return close_scan_hit<W>(c, K, i, close_value(c, i), a.base);
+                                   return close_scan_hit<W>(c, K, i, close_value(c, i), a.base, sh, 0);
                                },
                                acc);
+        }
     } else {
         close_walk<W, WRITE>(a, c, sh, K, bstart, acc);
     }
@@ -469,9 +524,10 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     // pass 1: count
     uint64_t mine = 0;
     bool ok = true;
+    uint32_t rounds_used = 0;
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
-        if (MODE == 1) ok &= close_block<T, W, false>(a, sh, b, block_len(b), k, 0, mine);
+        if (MODE == 1) ok &= close_block<T, W, false>(a, sh, b, block_len(b), k, 0, mine, rounds_used);
         else for_block<T, W, true, false>(a, sh, b, block_len(b), 0, mine);
     }
     for (int o = kWave / 2; o > 0; o >>= 1) mine += (uint64_t)__shfl_xor((long long)mine, o, kWave);
@@ -495,7 +551,7 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
         const uint32_t bstart = a.pos_base + (uint32_t)(b * a.bs);
-        if (MODE == 1) close_block<T, W, true>(a, sh, b, block_len(b), k, bstart, pos);
+        if (MODE == 1) close_block<T, W, true>(a, sh, b, block_len(b), k, bstart, pos, rounds_used);
         else for_block<T, W, true, true>(a, sh, b, block_len(b), bstart, pos);
     }
 }
@@ -622,7 +678,6 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt(FilterArgs a) {
             if (W == 64) v |= (uint64_t)__builtin_amdgcn_alignbit(lds[w + 2], x1, sft) << 32;
             return v & low_mask((int)lf);
         };
-#pragma unroll 4
         for (uint32_t p = lo; p < hi; p += kThreads) {
             const uint32_t j = p + tid;
             bool h = false;
