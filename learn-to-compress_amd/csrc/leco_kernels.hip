@@ -442,7 +442,8 @@ __device__ __forceinline__ void decode_run(const uint64_t *__restrict__ buf, uin
         const uint32_t w = (uint32_t)(q >> 6);
         const int sh = (int)(q & 63);
         const uint64_t field = ((buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh))) & mask;
-        o[j] = RAW ? (T)field : (T)apply_field(predict(t0, t1, j), field, l);
+        // decoded values are written once and not read back here: streaming stores
+        __builtin_nontemporal_store(RAW ? (T)field : (T)apply_field(predict(t0, t1, j), field, l), &o[j]);
     };
     if (ja == j0 && jb == j0 + kChunk) {
 #pragma unroll
@@ -596,7 +597,7 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
                 const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
                 const uint64_t val = raw ? field : (l ? v : (uint64_t)pr);
                 const bool wr = p < pend && (l < 126 || j < (uint32_t)(128 - l));
-                if (wr) o[p] = (T)val;
+                if (wr) __builtin_nontemporal_store((T)val, &o[p]);
             }
         }
         __syncthreads();
