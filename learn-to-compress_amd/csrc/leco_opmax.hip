@@ -443,11 +443,10 @@ __global__ __launch_bounds__(kThreads) void k_opmax_decode(const uint8_t *__rest
     }
     uint32_t *dst = out + i0;
     if (whole && ((uintptr_t)dst & 15) == 0) {
-        // decoded values are written once and not read back here: streaming stores
-        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        // (ordinary stores: a lane's 16-byte pieces lie 64 bytes apart, and L2 merges them into
+        // whole lines; streaming stores would reach HBM as partial lines)
 #pragma unroll
-        for (uint32_t h = 0; h < kOpmaxRun; h += 4)
-            __builtin_nontemporal_store(u32x4{o[h], o[h + 1], o[h + 2], o[h + 3]}, (u32x4 *)(dst + h));
+        for (uint32_t h = 0; h < kOpmaxRun; h += 4) *(uint4 *)(dst + h) = make_uint4(o[h], o[h + 1], o[h + 2], o[h + 3]);
     } else {
         for (uint32_t r = 0; r < len; ++r) dst[r] = o[r];
     }
