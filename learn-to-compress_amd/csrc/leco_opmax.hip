@@ -352,8 +352,11 @@ __global__ __launch_bounds__(kThreads) void k_opmax_decode(const uint8_t *__rest
     const uint32_t words = s_words;
     for (uint32_t w = tid; w < words / 4; w += kThreads) ((uint4 *)lds)[w] = *(const uint4 *)(st + a0 + 16ull * w);
     __syncthreads();
-    if (!live) return;
 
+    uint32_t o[kOpmaxRun];
+    uint32_t len = 0;
+    uint64_t i0 = 0;
+    if (live) {
     OpmaxBlock k;
     if (s_blocks) {
         // header from the staged block: [l][theta0][theta1] at LDS byte boff - a0
@@ -370,8 +373,8 @@ __global__ __launch_bounds__(kThreads) void k_opmax_decode(const uint8_t *__rest
     } else {
         k = opmax_block(st, offs, n, bs, b);
     }
-    const uint32_t len = k.m - j0 < kOpmaxRun ? k.m - j0 : kOpmaxRun;
-    const uint64_t i0 = b * bs + j0;
+    len = k.m - j0 < kOpmaxRun ? k.m - j0 : kOpmaxRun;
+    i0 = b * bs + j0;
     const bool whole = len == kOpmaxRun;
     uint64_t fw[kOpmaxRun / 8] = {};
     if (FIX) {
@@ -395,7 +398,6 @@ __global__ __launch_bounds__(kThreads) void k_opmax_decode(const uint8_t *__rest
         }
         return (uint32_t)load_bits(pay, (uint64_t)(j0 + r) * lf, (int)lf);
     };
-    uint32_t o[kOpmaxRun];
     if (k.l >= 31) {
 #pragma unroll
         for (uint32_t r = 0; r < kOpmaxRun; ++r) {
@@ -441,15 +443,37 @@ __global__ __launch_bounds__(kThreads) void k_opmax_decode(const uint8_t *__rest
             }
         }
     }
-    uint32_t *dst = out + i0;
-    if (whole && ((uintptr_t)dst & 15) == 0) {
-        // (ordinary stores: a lane's 16-byte pieces lie 64 bytes apart, and L2 merges them into
-        // whole lines; streaming stores would reach HBM as partial lines)
-#pragma unroll
-        for (uint32_t h = 0; h < kOpmaxRun; h += 4) *(uint4 *)(dst + h) = make_uint4(o[h], o[h + 1], o[h + 2], o[h + 3]);
-    } else {
-        for (uint32_t r = 0; r < len; ++r) dst[r] = o[r];
     }
+
+    // The workgroup's runs cover consecutive positions [p0, p1): transpose the
+    // values through LDS (free again once every thread has read its fields)
+    // and write the range with lane-contiguous 16-byte streaming stores.
+    const uint64_t p0 = b0 * bs + (uint64_t)rr0 * kOpmaxRun;
+    const uint64_t gl = (g0 + kThreads < runs ? g0 + kThreads : runs) - 1;
+    const uint32_t rl = rr0 + (uint32_t)(gl - g0);
+    const uint64_t bl = b0 + rl / rpb;
+    const uint32_t jl = (rl % rpb) * kOpmaxRun;
+    const uint64_t sl = bl * bs;
+    const uint32_t ml = (uint32_t)(n - sl < bs ? n - sl : bs);
+    const uint32_t cnt = (uint32_t)(sl + (jl + kOpmaxRun < ml ? jl + kOpmaxRun : ml) - p0);
+    // value q sits at LDS word q + q/16 (one pad word per run keeps the threads' writes on distinct banks)
+    auto at = [](uint32_t q) { return q + (q >> 4); };
+    static_assert(kOpmaxTile + kOpmaxTile / 16 <= kOpmaxStage / 4, "a padded tile fits the staging buffer");
+    __syncthreads();
+    for (uint32_t r = 0; r < len; ++r) lds[at((uint32_t)(i0 - p0) + r)] = o[r];
+    __syncthreads();
+    uint32_t *dst = out + p0;
+    uint32_t head = (uint32_t)((4 - (((uintptr_t)dst >> 2) & 3)) & 3);  // values before a 16-byte boundary
+    head = head < cnt ? head : cnt;
+    if ((uint32_t)tid < head) dst[tid] = lds[at(tid)];
+    const uint32_t nvec = (cnt - head) / 4;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    for (uint32_t v = tid; v < nvec; v += kThreads) {
+        const uint32_t q = head + 4 * v;
+        __builtin_nontemporal_store(u32x4{lds[at(q)], lds[at(q + 1)], lds[at(q + 2)], lds[at(q + 3)]},
+                                    (u32x4 *)(dst + q));
+    }
+    for (uint32_t q = head + 4 * nvec + tid; q < cnt; q += kThreads) dst[q] = lds[at(q)];
 }
 
 // The reference's fix-up lists applied to decoded values, skipping raw blocks
