@@ -342,6 +342,20 @@ int leco_fix_sosd_load(const char *path, uint32_t width_bits, void *d_out, uint6
                        void *stream);
 int leco_fix_sosd_save(const char *path, uint32_t width_bits, const void *d_in, uint64_t count, void *stream);
 
+/* Text, the reference's other input format (experiments/leco_int_fix_template.cpp:43-58
+ * load_data, experiments/fix_int.cpp:30-44): the values that
+ *     while (in.good()) { T v; in >> v; if (!in.good()) break; push(v); }
+ * reads from the file in the C locale, T of width_bits: whitespace-separated
+ * decimal integers, an optional sign ('-' negates modulo 2^width_bits), up to
+ * the first failed extraction (a character that starts no value, a value above
+ * T's maximum); a value directly at the end of the file (no character after
+ * it) is not kept, as in that loop.  _info parses the file and returns the
+ * count (host only, no device call); _load parses it again into device memory
+ * (d_out holds capacity values) through pinned staging on `stream`. */
+int leco_fix_text_info(const char *path, uint32_t width_bits, uint64_t *count);
+int leco_fix_text_load(const char *path, uint32_t width_bits, void *d_out, uint64_t capacity, uint64_t *count,
+                       void *stream);
+
 /* Column file: an encoded column as it lives in HBM (the reference keeps it as
  * block_start_vec, experiments/leco_int_fix_template.cpp:108,125): 64-byte header
  * ("LECOCOL1", version, width, n, block size, block count, stream bytes,

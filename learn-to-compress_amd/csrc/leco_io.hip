@@ -152,6 +152,126 @@ int device_to_file(const uint8_t *src, uint64_t len, int fd, uint64_t off, hipSt
     return LECO_FIX_OK;
 }
 
+// The reference's text input (experiments/leco_int_fix_template.cpp:43-58
+// load_data, experiments/fix_int.cpp:30-44): values read from a file stream
+// in the C locale by
+//     while (in.good()) { T v; in >> v; if (!in.good()) break; push(v); }
+// restated as a state machine over the file's bytes.  One extraction skips
+// whitespace, takes an optional sign and then decimal digits, and ends at the
+// first other character (which stays in the stream); it fails when no digit
+// follows or the magnitude exceeds T's maximum, and a '-' negates modulo
+// 2^width, as unsigned extraction does.  The loop keeps a value only when the
+// stream is still good after it, so a value that the end of the file ends
+// (no character after it) is not kept.
+class TextParser {
+  public:
+    explicit TextParser(uint32_t width_bits) : max_(width_bits == 32 ? 0xFFFFFFFFull : ~0ull) {}
+
+    // Feeds the next bytes of the file; emit(v) for every value kept.
+    // Returns false once the input has ended (a failed extraction).
+    template <class Emit>
+    bool feed(const uint8_t *p, size_t len, Emit &&emit) {
+        for (size_t i = 0; i < len; ++i) {
+            const uint8_t c = p[i];
+            if (c >= '0' && c <= '9') {
+                const uint64_t d = c - '0';
+                if (state_ != kDigits) {
+                    cur_ = 0;
+                    if (state_ == kIdle) neg_ = false;
+                    state_ = kDigits;
+                }
+                if (cur_ > (max_ - d) / 10) return false;  // does not fit T
+                cur_ = cur_ * 10 + d;
+            } else if (c == ' ' || (c >= '\t' && c <= '\r')) {
+                if (state_ == kSign) return false;
+                if (state_ == kDigits) keep(emit);
+                state_ = kIdle;
+            } else if (c == '-' || c == '+') {
+                if (state_ == kSign) return false;
+                if (state_ == kDigits) keep(emit);  // the sign starts the next extraction
+                neg_ = c == '-';
+                state_ = kSign;
+            } else {
+                if (state_ == kDigits) keep(emit);  // the next extraction fails on c
+                return false;
+            }
+        }
+        return true;
+    }
+
+  private:
+    enum { kIdle, kSign, kDigits };
+    template <class Emit>
+    void keep(Emit &&emit) {
+        const uint64_t v = neg_ ? (0 - cur_) & max_ : cur_;
+        emit(v);
+    }
+    uint64_t max_, cur_ = 0;
+    int state_ = kIdle;
+    bool neg_ = false;
+};
+
+constexpr size_t kTextRead = 16ull << 20;  // file bytes parsed per read
+
+// One pass over a text file: counts its values, and with d_out stores them
+// (width_bits each) to device memory through the pinned pair on st.
+int text_pass(const char *path, uint32_t width_bits, uint8_t *d_out, uint64_t capacity, uint64_t *count,
+              hipStream_t st) {
+    if (!path || !count || (width_bits != 32 && width_bits != 64)) return LECO_FIX_EINVAL;
+    Fd f;
+    f.fd = open(path, O_RDONLY | O_CLOEXEC);
+    if (f.fd < 0) return LECO_FIX_EINVAL;
+    if (d_out) {
+        const int rc = g_pinned.init();
+        if (rc) return rc;
+    }
+    const size_t wb = width_bits / 8, per = kChunk / wb;
+    std::vector<uint8_t> buf(kTextRead);
+    TextParser parser(width_bits);
+    uint64_t n = 0, copied = 0, pieces = 0;
+    size_t fill = 0;
+    int k = 0, rc = LECO_FIX_OK;
+    auto flush = [&]() {
+        if (!fill || rc) return;
+        if (hipMemcpyAsync(d_out + copied, g_pinned.buf[k], fill * wb, hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipEventRecord(g_pinned.ev[k], st) != hipSuccess) {
+            rc = LECO_FIX_EHIP;
+            return;
+        }
+        copied += fill * wb;
+        fill = 0;
+        k ^= 1;
+        if (++pieces >= 2 && hipEventSynchronize(g_pinned.ev[k]) != hipSuccess) rc = LECO_FIX_EHIP;
+    };
+    auto emit = [&](uint64_t v) {
+        if (d_out && !rc) {
+            if (n >= capacity) {
+                rc = LECO_FIX_EINVAL;
+            } else {
+                if (wb == 4)
+                    ((uint32_t *)g_pinned.buf[k])[fill] = (uint32_t)v;
+                else
+                    ((uint64_t *)g_pinned.buf[k])[fill] = v;
+                if (++fill == per) flush();
+            }
+        }
+        ++n;
+    };
+    for (uint64_t off = 0;;) {
+        const ssize_t r = pread(f.fd, buf.data(), buf.size(), (off_t)off);
+        if (r < 0 && errno == EINTR) continue;
+        if (r < 0) return LECO_FIX_EINVAL;
+        if (r == 0 || !parser.feed(buf.data(), (size_t)r, emit)) break;
+        off += (uint64_t)r;
+    }
+    if (d_out) {
+        flush();
+        if (rc == LECO_FIX_OK && hipStreamSynchronize(st) != hipSuccess) rc = LECO_FIX_EHIP;
+    }
+    if (rc == LECO_FIX_OK) *count = n;
+    return rc;
+}
+
 int read_header(int fd, FileHeader *h) {
     if (!read_full(fd, h, sizeof(*h), 0)) return LECO_FIX_EINVAL;
     if (memcmp(h->magic, kMagic, 8) != 0 || h->version != kFileVersion) return LECO_FIX_EINVAL;
@@ -203,6 +323,16 @@ int leco_fix_sosd_save(const char *path, uint32_t width_bits, const void *d_in, 
     if (f.fd < 0) return LECO_FIX_EINVAL;
     if (!write_full(f.fd, &count, 8, 0)) return LECO_FIX_EINVAL;
     return device_to_file((const uint8_t *)d_in, count * (width_bits / 8), f.fd, 8, (hipStream_t)stream);
+}
+
+int leco_fix_text_info(const char *path, uint32_t width_bits, uint64_t *count) {
+    return text_pass(path, width_bits, nullptr, 0, count, nullptr);
+}
+
+int leco_fix_text_load(const char *path, uint32_t width_bits, void *d_out, uint64_t capacity, uint64_t *count,
+                       void *stream) {
+    if (!d_out || !count) return LECO_FIX_EINVAL;
+    return text_pass(path, width_bits, (uint8_t *)d_out, capacity, count, (hipStream_t)stream);
 }
 
 int leco_fix_column_file_info(const char *path, leco_fix_file_info *info) {

@@ -99,6 +99,8 @@ SIGNATURES.update({
     "leco_fix_sosd_info": (c_int, [ctypes.c_char_p, c_u32, ctypes.POINTER(c_u64)]),
     "leco_fix_sosd_load": (c_int, [ctypes.c_char_p, c_u32, c_ptr, c_u64, ctypes.POINTER(c_u64), c_ptr]),
     "leco_fix_sosd_save": (c_int, [ctypes.c_char_p, c_u32, c_ptr, c_u64, c_ptr]),
+    "leco_fix_text_info": (c_int, [ctypes.c_char_p, c_u32, ctypes.POINTER(c_u64)]),
+    "leco_fix_text_load": (c_int, [ctypes.c_char_p, c_u32, c_ptr, c_u64, ctypes.POINTER(c_u64), c_ptr]),
     "leco_fix_column_file_info": (c_int, [ctypes.c_char_p, ctypes.POINTER(ColumnFileInfo)]),
     "leco_fix_column_save": (c_int, [ctypes.c_char_p, c_ptr, c_ptr, c_u64, c_u32, c_u32, c_ptr]),
     "leco_fix_column_load": (c_int, [ctypes.c_char_p, c_ptr, c_u64, c_ptr, c_ptr, c_ptr]),

@@ -39,6 +39,25 @@ def save_sosd(path, values: torch.Tensor, stream=None) -> None:
                                    _stream_handle(stream)), "sosd_save")
 
 
+def text_info(path, width: int) -> int:
+    """Value count of a text input file, parsed as the reference's drivers
+    read it (leco_fix_text_info; host only)."""
+    c = ctypes.c_uint64()
+    check(lib().leco_fix_text_info(_path(path), width, ctypes.byref(c)), "text_info")
+    return c.value
+
+
+def load_text(path, width: int, device="cuda", stream=None) -> torch.Tensor:
+    """experiments/leco_int_fix_template.cpp:43-58 (load_data) and
+    experiments/fix_int.cpp:30-44: whitespace-separated decimal values, into HBM."""
+    n = text_info(path, width)
+    out = torch.empty(max(n, 1), dtype=_STORAGE[width], device=device)
+    c = ctypes.c_uint64()
+    check(lib().leco_fix_text_load(_path(path), width, out.data_ptr(), n, ctypes.byref(c), _stream_handle(stream)),
+          "text_load")
+    return out[:c.value]
+
+
 def column_file_info(path) -> dict:
     info = ColumnFileInfo()
     check(lib().leco_fix_column_file_info(_path(path), ctypes.byref(info)), "column_file_info")

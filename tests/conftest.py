@@ -32,3 +32,20 @@ def load_golden(name):
 def oracle_mod():
     from oracle import oracle
     return oracle
+
+
+@pytest.fixture(scope="session")
+def istream_values(tmp_path_factory):
+    """tests/cpp/istream_values.cpp built with g++: (path, width) -> the values
+    libstdc++'s extraction loop reads from a text file."""
+    import subprocess
+    exe = tmp_path_factory.mktemp("istream") / "istream_values"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-o", str(exe), os.path.join(ROOT, "tests", "cpp", "istream_values.cpp")],
+                   check=True, capture_output=True)
+
+    def read(path, width):
+        out = subprocess.run([str(exe), str(path), str(width)], check=True, capture_output=True, text=True).stdout
+        lines = out.split()
+        n = int(lines[0])
+        return np.array([int(x) for x in lines[1:1 + n]], dtype=np.uint64)
+    return read

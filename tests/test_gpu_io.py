@@ -98,3 +98,31 @@ def test_column_load_rejects_small_buffers(tmp_path):
     assert rc == 1
     with pytest.raises(LecoError):
         io.load_sosd(tmp_path / "missing.bin", 64)
+
+
+@pytest.mark.parametrize("width", [32, 64])
+def test_text_load_values(tmp_path, istream_values, width):
+    """leco_fix_text_load's values against libstdc++'s extraction loop
+    (tests/cpp/istream_values.cpp) on the edge cases, on fuzz and on a file
+    of more than one 64 MB staging piece of values."""
+    from leco_fix import io
+    from text_corpus import edge_cases, fuzz_bytes
+    host = np.uint32 if width == 32 else np.uint64
+    rng = np.random.default_rng(width + 1)
+    say(f"text load edge cases width={width}")
+    files = dict(edge_cases(width))
+    for k in range(40):
+        files[f"fuzz{k}"] = fuzz_bytes(rng, int(rng.integers(0, 400)))
+    for name, body in files.items():
+        p = tmp_path / f"{name}.txt"
+        p.write_bytes(body)
+        got = io.load_text(p, width).cpu().numpy().view(host)
+        assert got.astype(np.uint64).tolist() == istream_values(p, width).tolist(), name
+    n = 17_000_000 if width == 32 else 9_000_000     # 68 / 72 MB of values
+    say(f"text load large width={width} n={n}")
+    v = rng.integers(0, (1 << width) - 1, n, dtype=np.uint64, endpoint=True)
+    p = tmp_path / "large.txt"
+    p.write_text("\n".join(map(str, v.tolist())) + "\n")
+    got = io.load_text(p, width)
+    assert got.numel() == n
+    assert np.array_equal(got.cpu().numpy().view(host), v.astype(host))
