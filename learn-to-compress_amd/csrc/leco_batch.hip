@@ -26,7 +26,8 @@ struct leco_fix_batch {
     BatchColumn *d_cols = nullptr;
     uint64_t *d_first = nullptr;
     uint64_t *d_fit_first = nullptr;
-    uint64_t tiles = 0;            // decode tiles: one per block (staged) or per kValueTile positions
+    uint64_t tiles = 0;            // decode tiles: one per large block, or a run of small blocks
+    uint64_t staged_tiles = 0;     // tiles [0, staged_tiles) are the large blocks' (all columns)
     uint64_t *d_tile_first = nullptr;
     uint32_t *d_tile_col = nullptr;
     uint8_t *d_arena = nullptr;
@@ -77,13 +78,10 @@ int enqueue(leco_fix_batch *b, int op, hipStream_t st) {
         return hip_status(leco::launch_batch_encode(b->d_cols, b->d_first, b->d_fit_first, b->d_unit_first,
                                                     b->d_unit_col, b->d_widths, b->ncols, b->total, b->fit_groups,
                                                     b->units, b->d_arena, b->d_offs, b->d_prm, st));
-    return hip_status(leco::launch_batch_decode(b->d_cols, b->d_first, b->d_tile_first, b->d_tile_col, b->tiles,
-                                                b->d_arena, b->d_offs, b->d_prm, b->decode_grid, st));
+    return hip_status(leco::launch_batch_decode(b->d_cols, b->d_first, b->d_tile_first, b->d_tile_col,
+                                                b->staged_tiles, b->tiles, b->d_arena, b->d_offs, b->decode_grid,
+                                                st));
 }
-
-// One workgroup per tile (capped at the launch limit; the kernel strides
-// over any tiles beyond the grid).
-uint32_t default_decode_grid(uint64_t tiles) { return (uint32_t)std::min<uint64_t>(tiles, 0x7FFFFFFFull); }
 
 }  // namespace
 
@@ -110,9 +108,6 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
         b->fit_groups += (nb + 63) / 64;
         unit_first[c] = b->units;
         b->units += leco::encode_units(nb, d.block_size);
-        tile_first[c] = b->tiles;
-        const uint32_t kb = leco::multi_blocks(d.block_size, d.width_bits);
-        b->tiles += leco::staged_decode(d.block_size) ? nb : (nb + kb - 1) / kb;
         const uint64_t last = d.n - (nb - 1) * d.block_size;
         arena += (nb - 1) * leco::block_bound(d.width_bits, d.block_size) +
                  leco::block_bound(d.width_bits, (uint32_t)last);
@@ -120,10 +115,30 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
     }
     b->h_first[ncols] = b->total;
     b->h_fit_first[ncols] = b->fit_groups;
-    tile_first[ncols] = b->tiles;
     unit_first[ncols] = b->units;
+    // decode tiles: every staged column's first, then every multi column's (column order within each)
+    std::vector<uint32_t> ord;
+    std::vector<uint64_t> ord_first;
+    ord.reserve(ncols);
+    ord_first.reserve(ncols + 1);
+    for (int staged = 1; staged >= 0; --staged) {
+        for (uint32_t c = 0; c < ncols; ++c) {
+            const leco_fix_column_desc &d = cols[c];
+            if (leco::staged_decode(d.block_size) != (staged == 1)) continue;
+            const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
+            const uint32_t kb = leco::multi_blocks(d.block_size, d.width_bits);
+            ord.push_back(c);
+            ord_first.push_back(b->tiles);
+            tile_first[c] = b->tiles;
+            b->tiles += staged ? nb : (nb + kb - 1) / kb;
+        }
+        if (staged) b->staged_tiles = b->tiles;
+    }
+    ord_first.push_back(b->tiles);
+    tile_first[ncols] = b->tiles;
     b->arena_cap = arena + LECO_FIX_TAIL_PAD;
-    b->decode_grid = default_decode_grid(b->tiles);
+    uint32_t *d_ord = nullptr;
+    uint64_t *d_ord_first = nullptr;
     bool ok = hipMalloc(&b->d_cols, ncols * sizeof(BatchColumn)) == hipSuccess &&
               hipMalloc(&b->d_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
               hipMalloc(&b->d_fit_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
@@ -134,8 +149,15 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
               hipMalloc(&b->d_prm, b->total * sizeof(leco_fix_block_param)) == hipSuccess &&
               hipMalloc(&b->d_widths, b->total * sizeof(uint32_t)) == hipSuccess &&
               hipMalloc(&b->d_unit_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
-              hipMalloc(&b->d_unit_col, b->units * sizeof(uint32_t)) == hipSuccess;
+              hipMalloc(&b->d_unit_col, b->units * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&d_ord, ncols * sizeof(uint32_t)) == hipSuccess &&
+              hipMalloc(&d_ord_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess;
+    auto free_ord = [&]() {
+        (void)hipFree(d_ord);
+        (void)hipFree(d_ord_first);
+    };
     if (!ok) {
+        free_ord();
         free_batch(b);
         return LECO_FIX_ENOMEM;
     }
@@ -148,14 +170,20 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
             hipSuccess ||
         hipMemcpy(b->d_unit_first, unit_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
             hipSuccess ||
+        hipMemcpy(d_ord, ord.data(), ncols * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_ord_first, ord_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+            hipSuccess ||
         hipMemset(b->d_arena + arena, 0, LECO_FIX_TAIL_PAD) != hipSuccess ||
-        leco::launch_tile_columns(b->d_tile_first, ncols, b->tiles, b->d_tile_col, nullptr) != hipSuccess ||
-        leco::launch_tile_columns(b->d_unit_first, ncols, b->units, b->d_unit_col, nullptr) != hipSuccess ||
+        leco::launch_tile_columns(d_ord_first, d_ord, ncols, b->tiles, b->d_tile_col, nullptr) != hipSuccess ||
+        leco::launch_tile_columns(b->d_unit_first, nullptr, ncols, b->units, b->d_unit_col, nullptr) !=
+            hipSuccess ||
         leco::launch_batch_blocks(b->d_cols, b->d_first, ncols, b->total, b->d_widths, nullptr) != hipSuccess ||
         hipDeviceSynchronize() != hipSuccess) {
+        free_ord();
         free_batch(b);
         return LECO_FIX_EHIP;
     }
+    free_ord();
     *out = b;
     return LECO_FIX_OK;
 }
@@ -261,7 +289,7 @@ int leco_fix_batch_set_output(leco_fix_batch *b, uint32_t c, void *d_out) {
 
 int leco_fix_batch_set_decode_grid(leco_fix_batch *b, uint32_t wgs) {
     if (!b) return LECO_FIX_EINVAL;
-    b->decode_grid = wgs ? wgs : default_decode_grid(b->tiles);
+    b->decode_grid = wgs;  // 0: one workgroup per tile
     if (b->graph[1]) {  // the captured launch has the old grid baked in
         (void)hipGraphExecDestroy(b->graph[1]);
         b->graph[1] = nullptr;

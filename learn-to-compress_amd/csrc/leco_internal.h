@@ -200,13 +200,17 @@ hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, c
 // Fills widths for every block of a batch (once, at creation).
 hipError_t launch_batch_blocks(const BatchColumn *cols, const uint64_t *first, uint32_t ncols, uint64_t total,
                                uint32_t *widths, hipStream_t st);
-// Batched decode: `grid` workgroups (one per tile, or fewer striding over them) cover the tile
-// table (tile_first[c]: column c's first tile; tile_col[t]: column of tile t).
+// Batched decode of the tile table (tile_first[c]: column c's first tile; tile_col[t]: column of
+// tile t).  Tiles [0, staged_tiles) hold one large block each, tiles [staged_tiles, tiles) runs of
+// small blocks; each range is one launch of at most `grid` workgroups (0: one per tile), which
+// stride over the range when there are fewer.
 hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, const uint64_t *tile_first,
-                               const uint32_t *tile_col, uint64_t tiles, const uint8_t *arena, const uint64_t *offs,
-                               const leco_fix_block_param *prm, uint32_t grid, hipStream_t st);
-hipError_t launch_tile_columns(const uint64_t *tile_first, uint32_t ncols, uint64_t tiles, uint32_t *tile_col,
-                               hipStream_t st);
+                               const uint32_t *tile_col, uint64_t staged_tiles, uint64_t tiles, const uint8_t *arena,
+                               const uint64_t *offs, uint32_t grid, hipStream_t st);
+// tile_col[t] for tiles numbered in the column order ord (null: 0, 1, ...), ord_first[k] being
+// the first tile of column ord[k] (monotone, ord_first[ncols] = tiles).
+hipError_t launch_tile_columns(const uint64_t *ord_first, const uint32_t *ord, uint32_t ncols, uint64_t tiles,
+                               uint32_t *tile_col, hipStream_t st);
 // Filter over a column: filter_range (close = false: value > filter) or
 // filter_range_close (per-block starting filters f1[b], f2[b], period base).
 // Positions pos_base + global index (uint32) go to out[0..cap) in the

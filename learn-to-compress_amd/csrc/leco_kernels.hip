@@ -746,47 +746,69 @@ __global__ __launch_bounds__(kThreads) void k_build_params(const uint8_t *__rest
 // one global list, items in column order.  Encode writes one arena; decode is
 // a persistent kernel striding over the items.
 // ---------------------------------------------------------------------------
-// Persistent batched decode over the tile table: a tile is one block of a
-// column decoded staged (large blocks) or multi_blocks() consecutive small
-// blocks decoded staged together; tile_col[t] names its column, so no search.
-__global__ __launch_bounds__(kThreads) void k_batch_decode(const BatchColumn *__restrict__ cols,
-                                                           const uint64_t *__restrict__ first,
-                                                           const uint64_t *__restrict__ tile_first,
-                                                           const uint32_t *__restrict__ tile_col, uint64_t tiles,
-                                                           const uint8_t *__restrict__ arena,
-                                                           const uint64_t *__restrict__ offs,
-                                                           const leco_fix_block_param *__restrict__ prm) {
-    __shared__ MultiShared sm;
-    for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+// Batched decode over the tile table.  A tile is one large block of a column
+// (staged) or multi_blocks() consecutive small blocks decoded together;
+// tile_col[t] names its column and tile_first[c] is column c's first tile.
+// The table lists every staged tile before every multi tile, and each kind
+// has a kernel of its own, so that each runs with only its own body's
+// registers and LDS (one kernel for both ran the staged tiles at the multi
+// body's occupancy).  Both stride over [t0, t1) when the grid is smaller.
+// jlo is the first value of each block to decode, 0 from the launcher: as a
+// kernel argument rather than a literal, because with a literal 0 the
+// compiler schedules the chunk loop with about twice the registers (88
+// instead of 42 VGPRs, 5 instead of 8 waves per SIMD).
+__global__ __launch_bounds__(kThreads) void k_batch_decode_staged(const BatchColumn *__restrict__ cols,
+                                                                  const uint64_t *__restrict__ first,
+                                                                  const uint64_t *__restrict__ tile_first,
+                                                                  const uint32_t *__restrict__ tile_col, uint64_t t0,
+                                                                  uint64_t t1, const uint8_t *__restrict__ arena,
+                                                                  const uint64_t *__restrict__ offs, uint32_t jlo) {
+    __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
+    for (uint64_t t = t0 + blockIdx.x; t < t1; t += gridDim.x) {
         const uint32_t c = tile_col[t];
         const BatchColumn col = cols[c];
         const uint64_t local = t - tile_first[c];
-        if (staged_decode(col.bs)) {
-            const uint64_t s = local * col.bs;
-            const uint32_t m = (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs);
-            const uint64_t off = offs[first[c] + local];
-            if (col.width == 64)
-                decode_block_body<uint64_t, 64>(arena, off, m, 0, m, (uint64_t *)col.out + s, sm.buf);
-            else
-                decode_block_body<uint32_t, 32>(arena, off, m, 0, m, (uint32_t *)col.out + s, sm.buf);
-        } else {
-            const uint32_t kb = multi_blocks(col.bs, col.width);
-            const uint64_t b0 = local * kb;
-            if (col.width == 64)
-                decode_multi_body<uint64_t, 64>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
-                                                (uint64_t *)col.out + b0 * col.bs, sm);
-            else
-                decode_multi_body<uint32_t, 32>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
-                                                (uint32_t *)col.out + b0 * col.bs, sm);
-        }
+        const uint64_t s = local * col.bs;
+        const uint32_t m = (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs);
+        const uint64_t off = offs[first[c] + local];
+        if (col.width == 64)
+            decode_block_body<uint64_t, 64>(arena, off, m, jlo, m, (uint64_t *)col.out + s, buf);
+        else
+            decode_block_body<uint32_t, 32>(arena, off, m, jlo, m, (uint32_t *)col.out + s, buf);
     }
 }
 
-// tile_col[t] = column of tile t (tile_first is the per-column prefix).
-__global__ __launch_bounds__(kThreads) void k_tile_columns(const uint64_t *__restrict__ tile_first, uint32_t ncols,
+__global__ __launch_bounds__(kThreads) void k_batch_decode_multi(const BatchColumn *__restrict__ cols,
+                                                                 const uint64_t *__restrict__ first,
+                                                                 const uint64_t *__restrict__ tile_first,
+                                                                 const uint32_t *__restrict__ tile_col, uint64_t t0,
+                                                                 uint64_t t1, const uint8_t *__restrict__ arena,
+                                                                 const uint64_t *__restrict__ offs) {
+    __shared__ MultiShared sm;
+    for (uint64_t t = t0 + blockIdx.x; t < t1; t += gridDim.x) {
+        const uint32_t c = tile_col[t];
+        const BatchColumn col = cols[c];
+        const uint32_t kb = multi_blocks(col.bs, col.width);
+        const uint64_t b0 = (t - tile_first[c]) * kb;
+        if (col.width == 64)
+            decode_multi_body<uint64_t, 64>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
+                                            (uint64_t *)col.out + b0 * col.bs, sm);
+        else
+            decode_multi_body<uint32_t, 32>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
+                                            (uint32_t *)col.out + b0 * col.bs, sm);
+    }
+}
+
+// tile_col[t] = ord[k] (k itself when ord is null) for the largest k with
+// ord_first[k] <= t: the tiles are numbered in the column order ord lists.
+__global__ __launch_bounds__(kThreads) void k_tile_columns(const uint64_t *__restrict__ ord_first,
+                                                           const uint32_t *__restrict__ ord, uint32_t ncols,
                                                            uint64_t tiles, uint32_t *__restrict__ tile_col) {
     const uint64_t t = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (t < tiles) tile_col[t] = find_column(tile_first, ncols, t);
+    if (t < tiles) {
+        const uint32_t k = find_column(ord_first, ncols, t);
+        tile_col[t] = ord ? ord[k] : k;
+    }
 }
 
 // Per-block width of a batch (one binary search per block, once).
@@ -927,16 +949,26 @@ hipError_t launch_batch_encode(const BatchColumn *cols, const uint64_t *first, c
 }
 
 hipError_t launch_batch_decode(const BatchColumn *cols, const uint64_t *first, const uint64_t *tile_first,
-                               const uint32_t *tile_col, uint64_t tiles, const uint8_t *arena, const uint64_t *offs,
-                               const leco_fix_block_param *prm, uint32_t grid, hipStream_t st) {
-    k_batch_decode<<<dim3(grid), dim3(kThreads), 0, st>>>(cols, first, tile_first, tile_col, tiles, arena, offs, prm);
+                               const uint32_t *tile_col, uint64_t staged_tiles, uint64_t tiles, const uint8_t *arena,
+                               const uint64_t *offs, uint32_t grid, hipStream_t st) {
+    auto wgs = [&](uint64_t count) {
+        const uint64_t cap = grid ? grid : 0x7FFFFFFFull;
+        return (unsigned)(count < cap ? count : cap);
+    };
+    if (staged_tiles)
+        k_batch_decode_staged<<<dim3(wgs(staged_tiles)), dim3(kThreads), 0, st>>>(cols, first, tile_first, tile_col, 0,
+                                                                                  staged_tiles, arena, offs, 0u);
+    if (tiles > staged_tiles)
+        k_batch_decode_multi<<<dim3(wgs(tiles - staged_tiles)), dim3(kThreads), 0, st>>>(
+            cols, first, tile_first, tile_col, staged_tiles, tiles, arena, offs);
     return hipGetLastError();
 }
 
-hipError_t launch_tile_columns(const uint64_t *tile_first, uint32_t ncols, uint64_t tiles, uint32_t *tile_col,
-                               hipStream_t st) {
-    k_tile_columns<<<dim3((unsigned)((tiles + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(tile_first, ncols,
-                                                                                                    tiles, tile_col);
+hipError_t launch_tile_columns(const uint64_t *ord_first, const uint32_t *ord, uint32_t ncols, uint64_t tiles,
+                               uint32_t *tile_col, hipStream_t st) {
+    k_tile_columns<<<dim3((unsigned)((tiles + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(ord_first, ord,
+                                                                                                    ncols, tiles,
+                                                                                                    tile_col);
     return hipGetLastError();
 }
 
