@@ -17,6 +17,8 @@
  *    (experiments/leco_int_fix_template.cpp:98-127).
  *  - A stream buffer must be readable for LECO_FIX_TAIL_PAD bytes past its
  *    last block: decode and lookup read whole 16-byte words.
+ *  - Block offsets (d_offsets) are ceil(n/bs) + 1 entries, the last one the
+ *    stream's size, as the encoders write them; offsets[b+1] bounds block b.
  *  - Every call returns LECO_FIX_OK (0) or a positive error code; nothing
  *    aborts.  leco_fix_strerror() names a code.
  */

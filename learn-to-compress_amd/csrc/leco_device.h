@@ -35,6 +35,15 @@ __device__ __forceinline__ int64_t predict(double t0, double t1, uint32_t i) {
     return cvtt64(__builtin_fma(t1, (double)i, t0));
 }
 
+// Low 32 bits of predict(): the fast path's bits(t + 1.5*2^52) already hold
+// them (the subtracted 0x4338000000000000 has zero low word).
+__device__ __forceinline__ uint32_t predict_lo32(double t0, double t1, uint32_t i) {
+    const double x = __builtin_fma(t1, (double)i, t0);
+    const double t = __builtin_trunc(x);
+    if (__builtin_expect(__builtin_fabs(t) < 0x1p51, 1)) return (uint32_t)__double_as_longlong(t + 0x1.8p52);
+    return (uint32_t)cvtt64(x);
+}
+
 // Bit length (lr.h:116-130: 64 - clzll(v)).
 __device__ __forceinline__ int bitlen64(uint64_t v) { return v ? 64 - __clzll((long long)v) : 0; }
 

@@ -130,9 +130,12 @@ constexpr uint32_t kValueTile = 4096;
 // Small blocks: as many consecutive blocks per workgroup as keep the tile
 // within kMultiPositions positions, at most kMultiMax.  Their bytes are staged
 // through kMultiLds bytes of LDS, whole blocks at a time (one block of fewer
-// than kStagedMinBlock values always fits).
+// than kStagedMinBlock values always fits).  kMultiMax = 128 keeps the
+// workgroup's LDS (staging plus block tables) near 20 KB, so that 8
+// workgroups fit a CU's 160 KB; it only shortens the tiles of blocks of
+// fewer than 64 values.
 constexpr uint32_t kMultiPositions = 8192;
-constexpr uint32_t kMultiMax = 256;
+constexpr uint32_t kMultiMax = 128;
 constexpr uint32_t kMultiLds = 2048 * 8 + 32;
 static_assert(kMultiLds >= 1 + (kStagedMinBlock - 1) * 8 + 15, "a small block fits the staging buffer");
 __host__ __device__ inline uint32_t multi_blocks(uint32_t bs, uint32_t /*w*/) {

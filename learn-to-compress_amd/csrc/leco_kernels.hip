@@ -423,27 +423,52 @@ __global__ __launch_bounds__(kThreads) void k_encode_fixup(const leco_fix_block_
 
 // ---------------------------------------------------------------------------
 // K5 decode body: Leco_int<T>::decodeArray8 (:142-204) with read_all_bit_fix
-// (bit_read.h:222-275).  The payload is staged through LDS kChunk values at a
-// time with aligned 16-byte loads; each lane extracts its field, adds it to
-// cvtt64(fma(i, theta1, theta0)) and stores.  Writes block values [jlo, jhi)
-// to o[j].  A block with l == W-1 is decoded, not copied as raw (the
-// reference's :171-177 treats it as raw and returns wrong values).
+// (bit_read.h:222-275).  A block whose bytes fit the LDS buffer is staged
+// whole, header included, with aligned 16-byte loads as soon as its offsets
+// are known, and its header is read from LDS; a larger one reads its header
+// from global memory and stages its payload kChunk values at a time.  Each
+// lane extracts its field, adds it to cvtt64(fma(i, theta1, theta0)) and
+// stores.  Writes block values [jlo, jhi) to o[j].  A block with l == W-1 is
+// decoded, not copied as raw (the reference's :171-177 treats it as raw and
+// returns wrong values).
 // ---------------------------------------------------------------------------
+// Little-endian 8 bytes at LDS byte offset `byte` (two aligned words).
+__device__ __forceinline__ uint64_t lds_u64_at(const uint64_t *__restrict__ buf, uint32_t byte) {
+    const uint32_t w = byte >> 3;
+    const int sh = (int)(byte & 7) * 8;
+    return (buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh));
+}
+
 // Values [ja, jb) of a staged chunk that starts at value j0: field at LDS
 // bit lbit + (j - j0) * lw, read as two aligned words and funnel-shifted
 // ((w1 << 1) << (63 - sh) is w1 << (64 - sh) and also right for sh = 0).
+// For 32-bit values a field (at most 32 bits) lies in two 32-bit words and
+// only the low 32 bits of the prediction matter.  LDS bit offsets fit 32 bits.
 template <typename T, bool RAW>
-__device__ __forceinline__ void decode_run(const uint64_t *__restrict__ buf, uint64_t lbit, int lw, int l, double t0,
+__device__ __forceinline__ void decode_run(const uint64_t *__restrict__ buf, uint32_t lbit, int lw, int l, double t0,
                                            double t1, uint32_t j0, uint32_t ja, uint32_t jb, T *__restrict__ o) {
     const int tid = threadIdx.x;
-    const uint64_t mask = low_mask(lw);
     auto one = [&](uint32_t j) {
-        const uint64_t q = lbit + (uint64_t)(j - j0) * lw;
-        const uint32_t w = (uint32_t)(q >> 6);
-        const int sh = (int)(q & 63);
-        const uint64_t field = ((buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh))) & mask;
+        const uint32_t q = lbit + (j - j0) * (uint32_t)lw;
+        T v;
+        if constexpr (sizeof(T) == 4) {
+            const uint32_t *b32 = (const uint32_t *)buf;
+            const uint32_t w = __builtin_amdgcn_alignbit(b32[(q >> 5) + 1], b32[q >> 5], q & 31);
+            if (RAW) {
+                v = w;
+            } else {
+                const uint32_t pr = predict_lo32(t0, t1, j);
+                const uint32_t mag = w & ((1u << (l - 1)) - 1);
+                v = ((w >> (l - 1)) & 1) ? pr + mag : pr - mag;
+            }
+        } else {
+            const uint32_t w = q >> 6;
+            const int sh = (int)(q & 63);
+            const uint64_t field = ((buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh))) & low_mask(lw);
+            v = RAW ? (T)field : (T)apply_field(predict(t0, t1, j), field, l);
+        }
         // decoded values are written once and not read back here: streaming stores
-        __builtin_nontemporal_store(RAW ? (T)field : (T)apply_field(predict(t0, t1, j), field, l), &o[j]);
+        __builtin_nontemporal_store(v, &o[j]);
     };
     if (ja == j0 && jb == j0 + kChunk) {
 #pragma unroll
@@ -453,11 +478,52 @@ __device__ __forceinline__ void decode_run(const uint64_t *__restrict__ buf, uin
     }
 }
 
+// Block bytes [off, end) (end: the next block's offset, an upper bound on
+// its last byte).
 template <typename T, int W>
-__device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st, uint64_t off, uint32_t m,
-                                                  uint32_t jlo, uint32_t jhi, T *__restrict__ o,
+__device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st, uint64_t off, uint64_t end,
+                                                  uint32_t m, uint32_t jlo, uint32_t jhi, T *__restrict__ o,
                                                   uint64_t *__restrict__ buf) {
     const int tid = threadIdx.x;
+    const uint64_t b0 = off & ~15ull;
+    const uint32_t nb16 = (uint32_t)((end - b0 + 15) / 16);
+    // whole block: the staged bytes plus one word of field over-read fit the buffer
+    if (end >= off && (uint64_t)nb16 * 16 + 16 <= (uint64_t)kLdsBytes) {
+        for (uint32_t u = tid; u < nb16; u += kThreads)
+            *(uint4 *)(buf + 2 * u) = *(const uint4 *)(st + b0 + 16ull * u);
+        __syncthreads();
+        const uint32_t hb = (uint32_t)(off - b0);
+        const int l = ((const uint8_t *)buf)[hb];
+        if (l == 127 || l == 126) {  // :152-170, single/double raw value blocks
+            const uint32_t cnt = l == 127 ? 1 : 2;
+            if ((uint32_t)tid < cnt && (uint32_t)tid >= jlo && (uint32_t)tid < jhi)
+                o[tid] = (T)lds_u64_at(buf, hb + 1 + tid * (uint32_t)sizeof(T));
+        } else {
+            const bool raw = l >= W;
+            const int lw = raw ? W : l;
+            double t0 = 0, t1 = 0;
+            if (!raw) {
+                t0 = bits_to_double(lds_u64_at(buf, hb + 1));
+                t1 = bits_to_double(lds_u64_at(buf, hb + 9));
+            }
+            if (lw == 0) {  // :190-195
+                for (uint32_t j = jlo + tid; j < jhi; j += kThreads) o[j] = (T)predict(t0, t1, j);
+            } else {
+                const uint32_t lbit = 8 * (hb + (raw ? 1 : 17));
+                for (uint32_t c = jlo / kChunk; c * kChunk < jhi; ++c) {
+                    const uint32_t j0 = c * kChunk;
+                    const uint32_t jn = m - j0 < kChunk ? m - j0 : kChunk;
+                    const uint32_t ja = jlo > j0 ? jlo : j0;
+                    const uint32_t jb = jhi < j0 + jn ? jhi : j0 + jn;
+                    const uint32_t cbit = lbit + j0 * (uint32_t)lw;
+                    if (raw) decode_run<T, true>(buf, cbit, lw, l, t0, t1, j0, ja, jb, o);
+                    else decode_run<T, false>(buf, cbit, lw, l, t0, t1, j0, ja, jb, o);
+                }
+            }
+        }
+        __syncthreads();  // the buffer is reused by the caller's next block
+        return;
+    }
     const int l = st[off];
     if (l == 127 || l == 126) {  // :152-170, single/double raw value blocks
         const uint32_t cnt = l == 127 ? 1 : 2;
@@ -488,20 +554,13 @@ __device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st
         for (uint32_t u = tid; u < n16; u += kThreads)
             *(uint4 *)(buf + 2 * u) = *(const uint4 *)(st + a0 + 16ull * u);
         __syncthreads();
-        const uint64_t lbit = 8ull * (g0 - a0);
+        const uint32_t lbit = (uint32_t)(8 * (g0 - a0));
         const uint32_t ja = jlo > j0 ? jlo : j0;
         const uint32_t jb = jhi < j0 + jn ? jhi : j0 + jn;
         if (raw) decode_run<T, true>(buf, lbit, lw, l, t0, t1, j0, ja, jb, o);
         else decode_run<T, false>(buf, lbit, lw, l, t0, t1, j0, ja, jb, o);
         __syncthreads();
     }
-}
-
-// Little-endian 8 bytes at LDS byte offset `byte` (two aligned words).
-__device__ __forceinline__ uint64_t lds_u64_at(const uint64_t *__restrict__ buf, uint32_t byte) {
-    const uint32_t w = byte >> 3;
-    const int sh = (int)(byte & 7) * 8;
-    return (buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh));
 }
 
 // Multi-block staged decode for small blocks: blocks [b0, b0 + kb) of one
@@ -560,11 +619,11 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
         const uint32_t p0 = ks * bs > plo ? ks * bs : plo;
         const uint32_t pend = ke * bs < cnt ? ke * bs : cnt;
         // kMultiUnroll positions per thread per step: table reads, then field
-        // reads, then the arithmetic, so that the LDS latencies overlap
+        // reads, then the arithmetic, so that the LDS latencies overlap.  The
+        // field's LDS bit offset q fits 32 bits (LDS is 160 KB).
         for (uint32_t pb = p0; pb < pend; pb += kMultiUnroll * kThreads) {
             uint32_t lb[kMultiUnroll], jj[kMultiUnroll];
             double2 th[kMultiUnroll];
-            uint64_t w0[kMultiUnroll], w1[kMultiUnroll];
 #pragma unroll
             for (int u = 0; u < kMultiUnroll; ++u) {
                 uint32_t p = pb + tid + u * kThreads;
@@ -574,30 +633,61 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
                 lb[u] = sm.hlb[k - ks];
                 th[u] = sm.ht[k - ks];
             }
+            if constexpr (W == 32) {
+                // A field of at most 32 bits lies in two consecutive 32-bit
+                // words: one funnel shift extracts it, and the value needs
+                // only the low 32 bits of the prediction.
+                const uint32_t *b32 = (const uint32_t *)sm.buf;
+                uint32_t x0[kMultiUnroll], x1[kMultiUnroll];
 #pragma unroll
-            for (int u = 0; u < kMultiUnroll; ++u) {
-                const int l = (int)(lb[u] & 127);
-                const int lw = l >= W ? W : l;
-                const uint64_t q = (uint64_t)(lb[u] >> 7) + (uint64_t)jj[u] * lw;
-                w0[u] = sm.buf[q >> 6];
-                w1[u] = sm.buf[(q >> 6) + 1];
-            }
+                for (int u = 0; u < kMultiUnroll; ++u) {
+                    const uint32_t l = lb[u] & 127;
+                    const uint32_t q = (lb[u] >> 7) + jj[u] * (l >= 32 ? 32 : l);
+                    x0[u] = b32[q >> 5];
+                    x1[u] = b32[(q >> 5) + 1];
+                }
 #pragma unroll
-            for (int u = 0; u < kMultiUnroll; ++u) {
-                const uint32_t p = pb + tid + u * kThreads;
-                const uint32_t j = jj[u];
-                const int l = (int)(lb[u] & 127);
-                const bool raw = l >= W;
-                const int lw = raw ? W : l;
-                const int sh = (int)(((lb[u] >> 7) + (uint64_t)j * lw) & 63);
-                const uint64_t field = ((w0[u] >> sh) | ((w1[u] << 1) << (63 - sh))) & low_mask(lw);
-                const int64_t pr = predict(th[u].x, th[u].y, j);
-                const int lc = l > 0 ? l : 1;
-                const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
-                const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
-                const uint64_t val = raw ? field : (l ? v : (uint64_t)pr);
-                const bool wr = p < pend && (l < 126 || j < (uint32_t)(128 - l));
-                if (wr) __builtin_nontemporal_store((T)val, &o[p]);
+                for (int u = 0; u < kMultiUnroll; ++u) {
+                    const uint32_t p = pb + tid + u * kThreads;
+                    const uint32_t j = jj[u];
+                    const uint32_t l = lb[u] & 127;
+                    const bool raw = l >= 32;
+                    const uint32_t q = (lb[u] >> 7) + j * (raw ? 32 : l);
+                    const uint32_t w = __builtin_amdgcn_alignbit(x1[u], x0[u], q & 31);
+                    const uint32_t pr = predict_lo32(th[u].x, th[u].y, j);
+                    const uint32_t lc = l > 0 ? l - 1 : 0;
+                    const uint32_t mag = w & ((1u << lc) - 1);
+                    const uint32_t v = ((w >> lc) & 1) ? pr + mag : pr - mag;
+                    const uint32_t val = raw ? w : (l ? v : pr);
+                    const bool wr = p < pend && (l < 126 || j < 128 - l);
+                    if (wr) __builtin_nontemporal_store((T)val, &o[p]);
+                }
+            } else {
+                uint64_t w0[kMultiUnroll], w1[kMultiUnroll];
+#pragma unroll
+                for (int u = 0; u < kMultiUnroll; ++u) {
+                    const int l = (int)(lb[u] & 127);
+                    const uint32_t q = (lb[u] >> 7) + jj[u] * (uint32_t)(l >= W ? W : l);
+                    w0[u] = sm.buf[q >> 6];
+                    w1[u] = sm.buf[(q >> 6) + 1];
+                }
+#pragma unroll
+                for (int u = 0; u < kMultiUnroll; ++u) {
+                    const uint32_t p = pb + tid + u * kThreads;
+                    const uint32_t j = jj[u];
+                    const int l = (int)(lb[u] & 127);
+                    const bool raw = l >= W;
+                    const int lw = raw ? W : l;
+                    const int sh = (int)(((lb[u] >> 7) + j * (uint32_t)lw) & 63);
+                    const uint64_t field = ((w0[u] >> sh) | ((w1[u] << 1) << (63 - sh))) & low_mask(lw);
+                    const int64_t pr = predict(th[u].x, th[u].y, j);
+                    const int lc = l > 0 ? l : 1;
+                    const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
+                    const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
+                    const uint64_t val = raw ? field : (l ? v : (uint64_t)pr);
+                    const bool wr = p < pend && (l < 126 || j < (uint32_t)(128 - l));
+                    if (wr) __builtin_nontemporal_store((T)val, &o[p]);
+                }
             }
         }
         __syncthreads();
@@ -632,7 +722,7 @@ __global__ __launch_bounds__(kThreads) void k_decode(const uint8_t *__restrict__
     const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
     const uint32_t jlo = lo > s ? (uint32_t)(lo - s) : 0;
     const uint32_t jhi = hi - s < m ? (uint32_t)(hi - s) : m;
-    decode_block_body<T, W>(st, offs[b], m, jlo, jhi, out - lo + s, buf);
+    decode_block_body<T, W>(st, offs[b], offs[b + 1], m, jlo, jhi, out - lo + s, buf);
 }
 
 // ---------------------------------------------------------------------------
@@ -770,11 +860,11 @@ __global__ __launch_bounds__(kThreads) void k_batch_decode_staged(const BatchCol
         const uint64_t local = t - tile_first[c];
         const uint64_t s = local * col.bs;
         const uint32_t m = (uint32_t)(col.n - s < col.bs ? col.n - s : col.bs);
-        const uint64_t off = offs[first[c] + local];
+        const uint64_t off = offs[first[c] + local], end = offs[first[c] + local + 1];
         if (col.width == 64)
-            decode_block_body<uint64_t, 64>(arena, off, m, jlo, m, (uint64_t *)col.out + s, buf);
+            decode_block_body<uint64_t, 64>(arena, off, end, m, jlo, m, (uint64_t *)col.out + s, buf);
         else
-            decode_block_body<uint32_t, 32>(arena, off, m, jlo, m, (uint32_t *)col.out + s, buf);
+            decode_block_body<uint32_t, 32>(arena, off, end, m, jlo, m, (uint32_t *)col.out + s, buf);
     }
 }
 
@@ -851,10 +941,10 @@ __global__ __launch_bounds__(kThreads) void k_serve(const ServeColumn *__restric
             const uint32_t jlo = w.lo > s ? (uint32_t)(w.lo - s) : 0;
             const uint32_t jhi = w.hi - s < m ? (uint32_t)(w.hi - s) : m;
             if (col.width == 64)
-                decode_block_body<uint64_t, 64>(col.stream, col.offsets[w.block], m, jlo, jhi,
+                decode_block_body<uint64_t, 64>(col.stream, col.offsets[w.block], col.offsets[w.block + 1], m, jlo, jhi,
                                                 (uint64_t *)w.out - w.lo + s, buf);
             else
-                decode_block_body<uint32_t, 32>(col.stream, col.offsets[w.block], m, jlo, jhi,
+                decode_block_body<uint32_t, 32>(col.stream, col.offsets[w.block], col.offsets[w.block + 1], m, jlo, jhi,
                                                 (uint32_t *)w.out - w.lo + s, buf);
         }
         __syncthreads();
