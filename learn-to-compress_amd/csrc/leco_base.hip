@@ -128,52 +128,133 @@ __global__ __launch_bounds__(kThreads) void k_base_fixup(uint64_t *__restrict__ 
     if (b == 0) offs[0] = 0;
 }
 
-// FOR_int<T>::decodeArray8 (:79-108) / Delta_int<T>::decodeArray8 (:85-126).
+// FOR values [j0, j0 + jn) of a run staged in LDS, field j at bit
+// lbit + (j - j0) * lw (lw = 0: every value is base; raw blocks: base 0,
+// lw = W); streaming stores.
+template <typename T, int W>
+__device__ __forceinline__ void for_run(const uint64_t *__restrict__ buf, uint32_t lbit, int lw, uint64_t base,
+                                        uint32_t j0, uint32_t jn, T *__restrict__ o) {
+    const int tid = threadIdx.x;
+    auto one = [&](uint32_t j) {
+        const uint64_t f = lds_field<W>(buf, lbit + (j - j0) * (uint32_t)lw, lw);
+        __builtin_nontemporal_store((T)(base + f), &o[j]);
+    };
+    if (jn == kChunk) {
+#pragma unroll
+        for (int kk = 0; kk < (int)(kChunk / kThreads); ++kk) one(j0 + tid + kk * kThreads);
+    } else {
+        for (uint32_t j = j0 + tid; j < j0 + jn; j += kThreads) one(j);
+    }
+}
+
+// Bytes [g0, g1) of the stream into buf from 16-byte aligned address a0 =
+// g0 & ~15; returns the LDS bit of byte g0.  Callers synchronize.
+__device__ __forceinline__ uint32_t stage_bytes(const uint8_t *__restrict__ st, uint64_t g0, uint64_t g1,
+                                                uint64_t *__restrict__ buf) {
+    const uint64_t a0 = g0 & ~15ull;
+    const uint32_t n16 = (uint32_t)((g1 - a0 + 15) / 16);
+    for (uint32_t u = threadIdx.x; u < n16; u += kThreads)
+        *(uint4 *)(buf + 2 * u) = *(const uint4 *)(st + a0 + 16ull * u);
+    return (uint32_t)(8 * (g0 - a0));
+}
+
+// FOR_int<T>::decodeArray8 (:79-108) / Delta_int<T>::decodeArray8 (:85-126),
+// one workgroup per block, fields staged through LDS.
+// FOR: a block whose bytes fit the buffer is staged whole, header included
+// (no dependent header read); a larger one stages kChunk values at a time.
+// Delta: kChunk deltas at a time; each thread sums 8 consecutive deltas, a
+// workgroup scan places the sums, and the values go back through the buffer
+// (index i at i + i/8, which spreads a wave's 8-apart writes over the banks)
+// so that the stores are lane-contiguous.
 template <int CODEC, typename T, int W>
 __global__ __launch_bounds__(kThreads) void k_base_decode(const uint8_t *__restrict__ st,
                                                           const uint64_t *__restrict__ offs, uint64_t n, uint32_t bs,
                                                           T *__restrict__ out) {
+    constexpr uint32_t K = 8;                             // Delta: consecutive deltas per thread per step
+    constexpr uint32_t kSlots = kChunk + kChunk / K;      // Delta: padded value slots of one step
+    constexpr uint32_t kWords = (kLdsBytes / 8 > kSlots ? kLdsBytes / 8 : kSlots);
+    __shared__ __align__(16) uint64_t buf[kWords];
     __shared__ uint64_t wsum[kThreads / kWave];
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     const uint64_t b = blockIdx.x;
     const uint64_t s = b * bs;
     const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
-    const uint8_t *blk = st + offs[b];
-    const int l = blk[0];
+    const uint64_t off = offs[b], end = offs[b + 1];
     T *o = out + s;
     constexpr uint32_t tb = W / 8;
-    if (CODEC == kCodecDelta && (l == 127 || l == 126)) {  // :98-113, one- and two-value raw blocks
+    if (CODEC == kCodecFor) {
+        const uint64_t b0 = off & ~15ull;
+        const uint32_t nb16 = (uint32_t)((end - b0 + 15) / 16);
+        if (end >= off && (uint64_t)nb16 * 16 + 16 <= sizeof(buf)) {
+            const uint32_t hbit = stage_bytes(st, off, end, buf);
+            __syncthreads();
+            const uint32_t hb = hbit / 8;
+            const int l = ((const uint8_t *)buf)[hb];
+            const bool raw = l >= W;
+            const int lw = raw ? W : l;
+            const uint64_t base = raw ? 0 : lds_u64_at(buf, hb + 1) & wmask<W>();
+            const uint32_t lbit = 8 * (hb + (raw ? 1 : 1 + 2 * tb));
+            for (uint32_t j0 = 0; j0 < m; j0 += kChunk)
+                for_run<T, W>(buf, lbit + j0 * (uint32_t)lw, lw, base, j0, m - j0 < kChunk ? m - j0 : kChunk, o);
+            return;
+        }
+        const int l = st[off];
+        const bool raw = l >= W;
+        const int lw = raw ? W : l;
+        const uint64_t base = raw ? 0 : load_u64_any(st + off + 1) & wmask<W>();
+        if (lw == 0) {
+            for (uint32_t j = tid; j < m; j += kThreads) __builtin_nontemporal_store((T)base, &o[j]);
+            return;
+        }
+        const uint64_t pay = off + (raw ? 1 : 1 + 2 * tb);
+        for (uint32_t j0 = 0; j0 < m; j0 += kChunk) {
+            const uint32_t jn = m - j0 < kChunk ? m - j0 : kChunk;
+            const uint64_t g0 = pay + (uint64_t)j0 / 8 * lw;
+            const uint32_t lbit = stage_bytes(st, g0, g0 + ((uint64_t)jn * lw + 7) / 8, buf);
+            __syncthreads();
+            for_run<T, W>(buf, lbit, lw, base, j0, jn, o);
+            __syncthreads();
+        }
+        return;
+    }
+    const int l = st[off];
+    if (l == 127 || l == 126) {  // :98-113, one- and two-value raw blocks
         const uint32_t cnt = l == 127 ? 1 : 2;
-        if ((uint32_t)tid < cnt && (uint32_t)tid < m) o[tid] = (T)load_bits(blk + 1, (uint64_t)tid * W, W);
+        if ((uint32_t)tid < cnt && (uint32_t)tid < m) o[tid] = (T)load_bits(st + off + 1, (uint64_t)tid * W, W);
         return;
     }
     if (l >= W) {
-        for (uint32_t j = tid; j < m; j += kThreads) o[j] = (T)load_bits(blk + 1, (uint64_t)j * W, W);
-        return;
-    }
-    const uint64_t base = load_u64_any(blk + 1) & wmask<W>();
-    if (CODEC == kCodecFor) {
-        const uint8_t *pay = blk + 1 + 2 * tb;
-        for (uint32_t j = tid; j < m; j += kThreads) o[j] = (T)(l ? base + load_bits(pay, (uint64_t)j * l, l) : base);
+        for (uint32_t j0 = 0; j0 < m; j0 += kChunk) {
+            const uint32_t jn = m - j0 < kChunk ? m - j0 : kChunk;
+            const uint64_t g0 = off + 1 + (uint64_t)j0 * tb;
+            const uint32_t lbit = stage_bytes(st, g0, g0 + (uint64_t)jn * tb, buf);
+            __syncthreads();
+            for_run<T, W>(buf, lbit, W, 0, j0, jn, o);
+            __syncthreads();
+        }
         return;
     }
     // Delta: o[0] = y_0, o[i] = y_0 + (signed deltas 0 .. i-1)
-    if (tid == 0) o[0] = (T)base;
+    const uint64_t y0 = load_u64_any(st + off + 1) & wmask<W>();
+    if (tid == 0) o[0] = (T)y0;
     if (!l) {
-        for (uint32_t j = 1 + tid; j < m; j += kThreads) o[j] = (T)base;
+        for (uint32_t j = 1 + tid; j < m; j += kThreads) __builtin_nontemporal_store((T)y0, &o[j]);
         return;
     }
-    const uint8_t *pay = blk + 1 + tb;
-    constexpr uint32_t K = 8;  // consecutive deltas per thread per step
-    uint64_t carry = base;
-    for (uint32_t c0 = 0; c0 + 1 < m; c0 += kThreads * K) {
-        const uint32_t d0 = c0 + tid * K;
+    const uint64_t pay = off + 1 + tb;
+    uint64_t carry = y0;
+    for (uint32_t c0 = 0; c0 + 1 < m; c0 += kChunk) {
+        const uint32_t cn = m - 1 - c0 < kChunk ? m - 1 - c0 : kChunk;  // deltas c0 .. c0+cn-1 (values c0+1 ..)
+        const uint64_t g0 = pay + (uint64_t)c0 / 8 * l;
+        const uint32_t lbit = stage_bytes(st, g0, g0 + ((uint64_t)cn * l + 7) / 8, buf);
+        __syncthreads();
+        const uint32_t d0 = tid * K;
         uint64_t d[K], run = 0;
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             d[k] = 0;
-            if (d0 + k + 1 < m) {
-                const uint64_t f = load_bits(pay, (uint64_t)(d0 + k) * l, l);
+            if (d0 + k < cn) {
+                const uint64_t f = lds_field<W>(buf, lbit + (d0 + k) * (uint32_t)l, l);
                 const uint64_t mag = f & ((1ull << (l - 1)) - 1);
                 d[k] = ((f >> (l - 1)) & 1) ? mag : 0 - mag;
             }
@@ -186,21 +267,24 @@ __global__ __launch_bounds__(kThreads) void k_base_decode(const uint8_t *__restr
             if (lane >= sft) inc += t;
         }
         if (lane == kWave - 1) wsum[wid] = inc;
-        __syncthreads();
+        __syncthreads();  // also: every field of this step is in registers
         uint64_t before = 0, all = 0;
 #pragma unroll
         for (int w = 0; w < kThreads / kWave; ++w) {
             before += w < wid ? wsum[w] : 0;
             all += wsum[w];
         }
-        __syncthreads();
         uint64_t v = carry + before + inc - run;
 #pragma unroll
         for (uint32_t k = 0; k < K; ++k) {
             v += d[k];
-            if (d0 + k + 1 < m) o[d0 + k + 1] = (T)v;
+            const uint32_t i = d0 + k;
+            buf[i + i / K] = v;
         }
+        __syncthreads();
+        for (uint32_t i = tid; i < cn; i += kThreads) __builtin_nontemporal_store((T)buf[i + i / K], &o[c0 + 1 + i]);
         carry += all;
+        __syncthreads();  // the buffer and wsum are reused by the next step
     }
 }
 

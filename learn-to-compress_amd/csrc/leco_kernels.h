@@ -200,6 +200,29 @@ __device__ __forceinline__ void pack_chunk(const PackedBlock &pb, uint32_t c, in
 
 // Small-block decode staging: the bytes of whole blocks, their offsets and
 // per-block header tables (width | first-field bit << 7, thetas).
+// Little-endian 8 bytes at LDS byte offset `byte` (two aligned words).
+__device__ __forceinline__ uint64_t lds_u64_at(const uint64_t *__restrict__ buf, uint32_t byte) {
+    const uint32_t w = byte >> 3;
+    const int sh = (int)(byte & 7) * 8;
+    return (buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh));
+}
+
+// lw-bit field (lw <= W) at LDS bit q, from two aligned words: 32-bit words
+// and one funnel shift for W = 32, 64-bit words otherwise.  Reads the word
+// after the field's first one.
+template <int W>
+__device__ __forceinline__ uint64_t lds_field(const uint64_t *__restrict__ buf, uint32_t q, int lw) {
+    if constexpr (W == 32) {
+        const uint32_t *b32 = (const uint32_t *)buf;
+        const uint32_t w = __builtin_amdgcn_alignbit(b32[(q >> 5) + 1], b32[q >> 5], q & 31);
+        return lw >= 32 ? w : (w & ((1u << lw) - 1));
+    } else {
+        const uint32_t w = q >> 6;
+        const int sh = (int)(q & 63);
+        return ((buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh))) & low_mask(lw);
+    }
+}
+
 struct MultiShared {
     alignas(16) uint64_t buf[kLdsBytes / 8];
     uint64_t offs[kMultiMax + 1];

@@ -432,13 +432,6 @@ __global__ __launch_bounds__(kThreads) void k_encode_fixup(const leco_fix_block_
 // decoded, not copied as raw (the reference's :171-177 treats it as raw and
 // returns wrong values).
 // ---------------------------------------------------------------------------
-// Little-endian 8 bytes at LDS byte offset `byte` (two aligned words).
-__device__ __forceinline__ uint64_t lds_u64_at(const uint64_t *__restrict__ buf, uint32_t byte) {
-    const uint32_t w = byte >> 3;
-    const int sh = (int)(byte & 7) * 8;
-    return (buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh));
-}
-
 // Values [ja, jb) of a staged chunk that starts at value j0: field at LDS
 // bit lbit + (j - j0) * lw, read as two aligned words and funnel-shifted
 // ((w1 << 1) << (63 - sh) is w1 << (64 - sh) and also right for sh = 0).
