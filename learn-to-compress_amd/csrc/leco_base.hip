@@ -288,6 +288,90 @@ __global__ __launch_bounds__(kThreads) void k_base_decode(const uint8_t *__restr
     }
 }
 
+// Delta_int<T>::decodeArray8 for blocks below kStagedMinBlock values: tiles
+// of kb consecutive blocks, their bytes staged into LDS whole (as many blocks
+// at a time as fit kMultiLds), one wave per block.  A wave takes 64
+// consecutive deltas per step, lane i the i-th: an inclusive wave scan plus
+// the running value gives 64 consecutive values, stored lane-contiguous.
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_delta_decode_multi(const uint8_t *__restrict__ st,
+                                                                 const uint64_t *__restrict__ offs, uint64_t n,
+                                                                 uint32_t bs, uint32_t kb, T *__restrict__ out) {
+    __shared__ MultiShared sm;
+    constexpr uint32_t tb = W / 8;
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    const uint64_t b0 = (uint64_t)blockIdx.x * kb;
+    const uint64_t nb = (n + bs - 1) / bs;
+    if (b0 + kb > nb) kb = (uint32_t)(nb - b0);
+    for (uint32_t k = tid; k <= kb; k += kThreads) sm.offs[k] = offs[b0 + k];
+    __syncthreads();
+    const uint8_t *bytes = (const uint8_t *)sm.buf;
+    for (uint32_t ks = 0; ks < kb;) {
+        const uint64_t a0 = sm.offs[ks] & ~15ull;
+        // largest ke in (ks, kb] whose bytes [a0, offs[ke]) fit (one block always does)
+        uint32_t ke = kb;
+        if (sm.offs[kb] - a0 > kMultiLds) {
+            uint32_t lo = ks + 1, hi = kb;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) / 2;
+                if (sm.offs[mid] - a0 <= kMultiLds) lo = mid;
+                else hi = mid - 1;
+            }
+            ke = lo;
+        }
+        const uint32_t n16 = (uint32_t)((sm.offs[ke] - a0 + 15) / 16);
+        for (uint32_t u = tid; u < n16; u += kThreads)
+            *(uint4 *)(sm.buf + 2 * u) = *(const uint4 *)(st + a0 + 16ull * u);
+        __syncthreads();
+        for (uint32_t k = ks + wid; k < ke; k += kThreads / kWave) {
+            const uint64_t s = (b0 + k) * bs;
+            const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
+            const uint32_t hb = (uint32_t)(sm.offs[k] - a0);
+            const uint32_t l = bytes[hb];
+            T *o = out + s;
+            if (l == 127 || l == 126) {  // :98-113, one- and two-value raw blocks
+                if ((uint32_t)lane < (l == 127 ? 1u : 2u) && (uint32_t)lane < m)
+                    o[lane] = (T)(lds_u64_at(sm.buf, hb + 1 + lane * tb) & wmask<W>());
+                continue;
+            }
+            if (l >= (uint32_t)W) {
+                for (uint32_t j = lane; j < m; j += kWave)
+                    __builtin_nontemporal_store((T)lds_field<W>(sm.buf, 8 * (hb + 1) + j * W, W), &o[j]);
+                continue;
+            }
+            const uint64_t y0 = lds_u64_at(sm.buf, hb + 1) & wmask<W>();
+            if (l == 0) {
+                for (uint32_t j = lane; j < m; j += kWave) __builtin_nontemporal_store((T)y0, &o[j]);
+                continue;
+            }
+            if (lane == 0) o[0] = (T)y0;
+            const uint32_t pbit = 8 * (hb + 1 + tb);
+            const T low = (T)((1ull << (l - 1)) - 1);
+            // sums modulo 2^W: T-wide arithmetic (one 32-bit shuffle per step for uint32)
+            T carry = (T)y0;
+            for (uint32_t i0 = 0; i0 + 1 < m; i0 += kWave) {  // deltas i0 .. i0+63, values i0+1 ..
+                const uint32_t i = i0 + lane;
+                T inc = 0;
+                if (i + 1 < m) {
+                    const T f = (T)lds_field<W>(sm.buf, pbit + i * l, (int)l);
+                    inc = ((f >> (l - 1)) & 1) ? (f & low) : (T)(0 - (f & low));
+                }
+                for (int sft = 1; sft < kWave; sft <<= 1) {
+                    T t;
+                    if constexpr (W == 32) t = (T)__shfl_up((int)inc, sft, kWave);
+                    else t = (T)__shfl_up((long long)inc, sft, kWave);
+                    if (lane >= sft) inc += t;
+                }
+                if (i + 1 < m) __builtin_nontemporal_store((T)(carry + inc), &o[i + 1]);
+                if constexpr (W == 32) carry += (T)__shfl((int)inc, kWave - 1, kWave);
+                else carry += (T)__shfl((long long)inc, kWave - 1, kWave);
+            }
+        }
+        __syncthreads();
+        ks = ke;
+    }
+}
+
 // FOR_int<T>::randomdecodeArray8 (:219-246): one lane per query.
 template <typename T, int W>
 __global__ __launch_bounds__(kThreads) void k_for_lookup(const uint8_t *__restrict__ st,
@@ -369,6 +453,13 @@ hipError_t launch_base_decode(int codec, const uint8_t *stream, const uint64_t *
                               T *out, hipStream_t st) {
     constexpr int W = 8 * sizeof(T);
     const uint64_t nb = (n + bs - 1) / bs;
+    if (codec == kCodecFor && !staged_decode(bs)) return launch_for_decode_multi<T>(stream, offs, n, bs, out, st);
+    if (!staged_decode(bs)) {
+        const uint32_t kb = multi_blocks(bs, W);
+        k_delta_decode_multi<T, W><<<dim3((unsigned)((nb + kb - 1) / kb)), dim3(kThreads), 0, st>>>(stream, offs, n,
+                                                                                                     bs, kb, out);
+        return hipGetLastError();
+    }
     if (codec == kCodecFor)
         k_base_decode<kCodecFor, T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(stream, offs, n, bs, out);
     else

@@ -46,6 +46,7 @@ __host__ __device__ inline uint64_t encode_units(uint64_t nb, uint32_t bs) {
 // Fixed-partition baselines (FOR_integer_template.h, delta_integer_template.h):
 // block sizes of write_FOR_int_T (E+1 bytes, then (32 - l)/8 zero bytes in C
 // integer division) and write_delta_int_T (n fields: E+1 bytes, then 8 - E%8).
+constexpr int kCodecLeco = 0;  // the multi-block decode body's codec parameter: LeCo blocks
 constexpr int kCodecFor = 1;
 constexpr int kCodecDelta = 2;
 __host__ __device__ inline uint64_t for_block_bytes(uint32_t l, uint32_t w, uint32_t n) {
@@ -228,6 +229,11 @@ hipError_t launch_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n
 template <typename T>
 hipError_t launch_base_encode(int codec, const T *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t *offs,
                               hipStream_t st);
+// FOR_int<T> columns of blocks below kStagedMinBlock values: the LeCo
+// multi-block decode body with FOR headers (tiles of multi_blocks() blocks).
+template <typename T>
+hipError_t launch_for_decode_multi(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, T *out,
+                                   hipStream_t st);
 template <typename T>
 hipError_t launch_base_decode(int codec, const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
                               T *out, hipStream_t st);

@@ -565,8 +565,10 @@ __device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st
 // from its block's table entry and field.  Tile positions p in [plo, phi)
 // are decoded to o[p]; the others are skipped.  Widths 127 and 126 are the
 // reference decoder's one- and two-value raw blocks (:152-170): only their
-// first one or two positions are written.
-template <typename T, int W>
+// first one or two positions are written.  CODEC kCodecFor decodes FOR_int
+// blocks instead (FOR_integer_template.h:79-108): header [l][base][max],
+// value base + field (raw when l >= W, every position written).
+template <int CODEC, typename T, int W>
 __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st, const uint64_t *__restrict__ offs,
                                                   uint64_t n, uint32_t bs, uint64_t b0, uint32_t kb, uint32_t plo,
                                                   uint32_t phi, T *__restrict__ o, MultiShared &sm) {
@@ -602,10 +604,16 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
             const uint32_t hb = (uint32_t)(sm.offs[k] - a0);
             const uint32_t l = bytes[hb];
             const bool raw = l >= (uint32_t)W;
-            sm.hlb[k - ks] = ((8 * (hb + (raw ? 1 : 17))) << 7) | l;
-            if (!raw) {
-                sm.ht[k - ks] = make_double2(bits_to_double(lds_u64_at(sm.buf, hb + 1)),
-                                             bits_to_double(lds_u64_at(sm.buf, hb + 9)));
+            if constexpr (CODEC == kCodecFor) {
+                sm.hlb[k - ks] = ((8 * (hb + (raw ? 1 : 1 + 2 * (W / 8)))) << 7) | (l & 127);
+                const uint64_t base = raw ? 0 : lds_u64_at(sm.buf, hb + 1) & (W == 32 ? 0xFFFFFFFFull : ~0ull);
+                sm.ht[k - ks] = make_double2(bits_to_double(base), 0.0);
+            } else {
+                sm.hlb[k - ks] = ((8 * (hb + (raw ? 1 : 17))) << 7) | l;
+                if (!raw) {
+                    sm.ht[k - ks] = make_double2(bits_to_double(lds_u64_at(sm.buf, hb + 1)),
+                                                 bits_to_double(lds_u64_at(sm.buf, hb + 9)));
+                }
             }
         }
         __syncthreads();
@@ -647,13 +655,19 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
                     const bool raw = l >= 32;
                     const uint32_t q = (lb[u] >> 7) + j * (raw ? 32 : l);
                     const uint32_t w = __builtin_amdgcn_alignbit(x1[u], x0[u], q & 31);
-                    const uint32_t pr = predict_lo32(th[u].x, th[u].y, j);
-                    const uint32_t lc = l > 0 ? l - 1 : 0;
-                    const uint32_t mag = w & ((1u << lc) - 1);
-                    const uint32_t v = ((w >> lc) & 1) ? pr + mag : pr - mag;
-                    const uint32_t val = raw ? w : (l ? v : pr);
-                    const bool wr = p < pend && (l < 126 || j < 128 - l);
-                    if (wr) __builtin_nontemporal_store((T)val, &o[p]);
+                    if constexpr (CODEC == kCodecFor) {
+                        const uint32_t base = (uint32_t)double_to_bits(th[u].x);
+                        const uint32_t val = raw ? w : base + (w & ((1u << l) - 1));
+                        if (p < pend) __builtin_nontemporal_store((T)val, &o[p]);
+                    } else {
+                        const uint32_t pr = predict_lo32(th[u].x, th[u].y, j);
+                        const uint32_t lc = l > 0 ? l - 1 : 0;
+                        const uint32_t mag = w & ((1u << lc) - 1);
+                        const uint32_t v = ((w >> lc) & 1) ? pr + mag : pr - mag;
+                        const uint32_t val = raw ? w : (l ? v : pr);
+                        const bool wr = p < pend && (l < 126 || j < 128 - l);
+                        if (wr) __builtin_nontemporal_store((T)val, &o[p]);
+                    }
                 }
             } else {
                 uint64_t w0[kMultiUnroll], w1[kMultiUnroll];
@@ -673,13 +687,18 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
                     const int lw = raw ? W : l;
                     const int sh = (int)(((lb[u] >> 7) + j * (uint32_t)lw) & 63);
                     const uint64_t field = ((w0[u] >> sh) | ((w1[u] << 1) << (63 - sh))) & low_mask(lw);
-                    const int64_t pr = predict(th[u].x, th[u].y, j);
-                    const int lc = l > 0 ? l : 1;
-                    const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
-                    const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
-                    const uint64_t val = raw ? field : (l ? v : (uint64_t)pr);
-                    const bool wr = p < pend && (l < 126 || j < (uint32_t)(128 - l));
-                    if (wr) __builtin_nontemporal_store((T)val, &o[p]);
+                    if constexpr (CODEC == kCodecFor) {
+                        const uint64_t val = raw ? field : double_to_bits(th[u].x) + field;
+                        if (p < pend) __builtin_nontemporal_store((T)val, &o[p]);
+                    } else {
+                        const int64_t pr = predict(th[u].x, th[u].y, j);
+                        const int lc = l > 0 ? l : 1;
+                        const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
+                        const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
+                        const uint64_t val = raw ? field : (l ? v : (uint64_t)pr);
+                        const bool wr = p < pend && (l < 126 || j < (uint32_t)(128 - l));
+                        if (wr) __builtin_nontemporal_store((T)val, &o[p]);
+                    }
                 }
             }
         }
@@ -690,7 +709,7 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
 
 // One workgroup per tile of kb blocks of one column, tiles starting at block
 // b_first; values [lo, hi) go to out[global - lo].
-template <typename T, int W>
+template <int CODEC, typename T, int W>
 __global__ __launch_bounds__(kThreads) void k_decode_multi(const uint8_t *__restrict__ st,
                                                            const uint64_t *__restrict__ offs, uint64_t n,
                                                            uint32_t bs, uint32_t kb, uint64_t b_first, uint64_t lo,
@@ -700,7 +719,7 @@ __global__ __launch_bounds__(kThreads) void k_decode_multi(const uint8_t *__rest
     const uint64_t s0 = b0 * bs;
     const uint32_t plo = lo > s0 ? (uint32_t)(lo - s0) : 0;
     const uint32_t phi = hi - s0 < (uint64_t)kb * bs ? (uint32_t)(hi - s0) : kb * bs;
-    decode_multi_body<T, W>(st, offs, n, bs, b0, kb, plo, phi, out - lo + s0, sm);
+    decode_multi_body<CODEC, T, W>(st, offs, n, bs, b0, kb, plo, phi, out - lo + s0, sm);
 }
 
 // One workgroup per block of one column; values [lo, hi) go to out[global - lo].
@@ -874,10 +893,10 @@ __global__ __launch_bounds__(kThreads) void k_batch_decode_multi(const BatchColu
         const uint32_t kb = multi_blocks(col.bs, col.width);
         const uint64_t b0 = (t - tile_first[c]) * kb;
         if (col.width == 64)
-            decode_multi_body<uint64_t, 64>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
+            decode_multi_body<kCodecLeco, uint64_t, 64>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
                                             (uint64_t *)col.out + b0 * col.bs, sm);
         else
-            decode_multi_body<uint32_t, 32>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
+            decode_multi_body<kCodecLeco, uint32_t, 32>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,
                                             (uint32_t *)col.out + b0 * col.bs, sm);
     }
 }
@@ -977,11 +996,22 @@ hipError_t launch_decode(const uint8_t *stream, const uint64_t *offs, uint64_t n
     if (!staged_decode(bs)) {
         const uint32_t kb = multi_blocks(bs, W);
         const uint64_t tiles = (bl - bf + 1 + kb - 1) / kb;
-        k_decode_multi<T, W><<<dim3((unsigned)tiles), dim3(kThreads), 0, st>>>(stream, offs, n, bs, kb, bf, lo, hi,
-                                                                                 out);
+        k_decode_multi<kCodecLeco, T, W><<<dim3((unsigned)tiles), dim3(kThreads), 0, st>>>(stream, offs, n, bs, kb,
+                                                                                             bf, lo, hi, out);
         return hipGetLastError();
     }
     k_decode<T, W><<<dim3((unsigned)(bl - bf + 1)), dim3(kThreads), 0, st>>>(stream, offs, n, bs, bf, lo, hi, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_for_decode_multi(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, T *out,
+                                   hipStream_t st) {
+    constexpr int W = 8 * sizeof(T);
+    const uint32_t kb = multi_blocks(bs, W);
+    const uint64_t nb = (n + bs - 1) / bs;
+    k_decode_multi<kCodecFor, T, W><<<dim3((unsigned)((nb + kb - 1) / kb)), dim3(kThreads), 0, st>>>(
+        stream, offs, n, bs, kb, 0, 0, n, out);
     return hipGetLastError();
 }
 
@@ -1069,6 +1099,10 @@ template hipError_t launch_decode<uint32_t>(const uint8_t *, const uint64_t *, u
                                             uint64_t, uint32_t *, hipStream_t);
 template hipError_t launch_decode<uint64_t>(const uint8_t *, const uint64_t *, uint64_t, uint32_t, uint64_t,
                                             uint64_t, uint64_t *, hipStream_t);
+template hipError_t launch_for_decode_multi<uint32_t>(const uint8_t *, const uint64_t *, uint64_t, uint32_t,
+                                                      uint32_t *, hipStream_t);
+template hipError_t launch_for_decode_multi<uint64_t>(const uint8_t *, const uint64_t *, uint64_t, uint32_t,
+                                                      uint64_t *, hipStream_t);
 template hipError_t launch_decode_values<uint32_t>(const uint8_t *, const leco_fix_block_param *, uint32_t, uint64_t,
                                                    uint64_t, uint32_t *, hipStream_t);
 template hipError_t launch_decode_values<uint64_t>(const uint8_t *, const leco_fix_block_param *, uint32_t, uint64_t,
