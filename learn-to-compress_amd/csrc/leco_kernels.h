@@ -10,7 +10,6 @@ namespace leco {
 constexpr int kThreads = 256;                       // workgroup of the per-block kernels (4 waves)
 constexpr uint32_t kChunk = 2048;                   // values staged through LDS at a time
 constexpr int kLdsBytes = kChunk * 8 + 64;          // one chunk of raw uint64 + alignment slack
-constexpr int kVPT = 16;                            // values per lane kept in registers (4096 per block)
 constexpr uint32_t kValTile = kValueTile;           // positions per workgroup in value-parallel decode
 constexpr int kFitRows = 64;                        // blocks per fit workgroup (one lane of wave 0 each)
 constexpr int kFitTK = 128;                         // values per block row in one fit tile
@@ -41,10 +40,13 @@ __device__ __forceinline__ uint64_t load_status(const uint64_t *p) {
     return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+constexpr int kEncodeMaxWaves = kThreads / kWave;  // reduction slots of the encode workgroups
+constexpr uint32_t kLargeEncodeBlock = 4096;          // larger blocks: k_encode_large (one chunk cached)
+
 struct EncodeShared {
     alignas(16) uint32_t buf[kLdsBytes / 4];
-    int64_t rmax[kThreads / kWave], rmin[kThreads / kWave];
-    uint64_t rmag[kThreads / kWave];
+    int64_t rmax[kEncodeMaxWaves], rmin[kEncodeMaxWaves];
+    uint64_t rmag[kEncodeMaxWaves];
     uint64_t off, end;
     double t0;
     int l;

@@ -133,33 +133,36 @@ __global__ __launch_bounds__(kFitThreads) void k_fit(Src src, leco_fix_block_par
 // Either way the block's values stay in registers across all passes.
 // ---------------------------------------------------------------------------
 
-// Workgroup body: a block of any size up to kMaxBlockSize; kVPT values per
-// thread are register-cached, the rest re-read in the passes.
-template <typename T, int W>
+// Workgroup body (NT threads): a block of any size up to kMaxBlockSize; the
+// values of the first CC pack chunks (VPT per thread) are register-cached,
+// the rest re-read in the passes (mostly from L2: the block was just loaded).
+template <int NT, int CC, typename T, int W>
 __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint32_t m, leco_fix_block_param *prm,
                                                   uint64_t b, uint64_t u, uint64_t *__restrict__ offs,
                                                   uint8_t *__restrict__ out, EncodeShared &sh) {
+    constexpr int VPT = CC * (int)(kChunk / NT);
+    static_assert(NT / kWave <= kEncodeMaxWaves, "the reduction slots hold one value per wave");
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     double t0 = prm[b].theta0;
     const double t1 = prm[b].theta1;
 
-    uint64_t v[kVPT];
+    uint64_t v[VPT];
 #pragma unroll
-    for (int k = 0; k < kVPT; ++k) {
-        const uint32_t j = tid + k * kThreads;
+    for (int k = 0; k < VPT; ++k) {
+        const uint32_t j = tid + k * NT;
         v[k] = j < m ? (uint64_t)y[j] : 0;
     }
 
     // pass 1 (lanes past the block's end are masked by selects, not branches)
     int64_t mx = INT64_MIN, mn = INT64_MAX;
 #pragma unroll
-    for (int k = 0; k < kVPT; ++k) {
-        const uint32_t j = tid + k * kThreads;
+    for (int k = 0; k < VPT; ++k) {
+        const uint32_t j = tid + k * NT;
         const int64_t d = (int64_t)(v[k] - (uint64_t)predict(t0, t1, j));
         mx = (j < m && d > mx) ? d : mx;
         mn = (j < m && d < mn) ? d : mn;
     }
-    for (uint32_t j = tid + kVPT * kThreads; j < m; j += kThreads) {
+    for (uint32_t j = tid + VPT * NT; j < m; j += NT) {
         const int64_t d = (int64_t)((uint64_t)y[j] - (uint64_t)predict(t0, t1, j));
         mx = d > mx ? d : mx;
         mn = d < mn ? d : mn;
@@ -171,7 +174,7 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
     mx = sh.rmax[0];
     mn = sh.rmin[0];
 #pragma unroll
-    for (int w = 1; w < kThreads / kWave; ++w) {
+    for (int w = 1; w < NT / kWave; ++w) {
         mx = sh.rmax[w] > mx ? sh.rmax[w] : mx;
         mn = sh.rmin[w] < mn ? sh.rmin[w] : mn;
     }
@@ -184,14 +187,14 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
     uint64_t mg = 0, mag;
     uint32_t signs = 0;
 #pragma unroll
-    for (int k = 0; k < kVPT; ++k) {
-        const uint32_t j = tid + k * kThreads;
+    for (int k = 0; k < VPT; ++k) {
+        const uint32_t j = tid + k * NT;
         const bool pos = residual_sign<W>(v[k], predict(t0, t1, j), &mag);
         v[k] = mag;
         signs |= (uint32_t)pos << k;
         mg = (j < m && mag > mg) ? mag : mg;
     }
-    for (uint32_t j = tid + kVPT * kThreads; j < m; j += kThreads) {
+    for (uint32_t j = tid + VPT * NT; j < m; j += NT) {
         residual_field<W>((uint64_t)y[j], predict(t0, t1, j), 0, &mag);
         mg = mag > mg ? mag : mg;
     }
@@ -201,7 +204,7 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
 
     // bit width and size (every thread, from the four wave maxima)
 #pragma unroll
-    for (int w = 0; w < kThreads / kWave; ++w) mg = sh.rmag[w] > mg ? sh.rmag[w] : mg;
+    for (int w = 0; w < NT / kWave; ++w) mg = sh.rmag[w] > mg ? sh.rmag[w] : mg;
     int lb = mg ? bitlen64(mg) + 1 : 0;
     if (lb > W) lb = W;
     const uint64_t size = block_bytes((uint32_t)lb, W, m);
@@ -226,7 +229,6 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
     const PackedBlock pb{sh.off, sh.end, double_to_bits(t0), double_to_bits(t1), m, l, raw ? 1u : 17u,
                          l == 0 ? 1 : (m + kChunk - 1) / kChunk};
 
-    static_assert(kVPT == 2 * (kChunk / kThreads), "two chunks of values are register-cached");
     const uint64_t low = l > 0 ? (1ull << (l - 1)) - 1 : 0;
     auto cached = [&](int k, uint32_t i) -> uint64_t {
         if (raw) return (uint64_t)y[i];
@@ -238,13 +240,13 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
         return residual_field<W>((uint64_t)y[i], predict(t0, t1, i), l, &mg2);
     };
     auto sync = [] { __syncthreads(); };
-    pack_chunk<kThreads, kChunk, true>(pb, 0, tid, sh.buf, out, sync,
+    pack_chunk<NT, kChunk, true>(pb, 0, tid, sh.buf, out, sync,
                                        [&](int kk, uint32_t i) { return cached(kk, i); });
-    if (pb.nchunks > 1)
-        pack_chunk<kThreads, kChunk, true>(pb, 1, tid, sh.buf, out, sync,
-                                           [&](int kk, uint32_t i) { return cached(kChunk / kThreads + kk, i); });
-    for (uint32_t c = 2; c < pb.nchunks; ++c)
-        pack_chunk<kThreads, kChunk, false>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) { return fresh(i); });
+    if (CC > 1 && pb.nchunks > 1)
+        pack_chunk<NT, kChunk, true>(pb, 1, tid, sh.buf, out, sync,
+                                     [&](int kk, uint32_t i) { return cached((CC > 1 ? kChunk / NT : 0) + kk, i); });
+    for (uint32_t c = CC; c < pb.nchunks; ++c)
+        pack_chunk<NT, kChunk, false>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) { return fresh(i); });
 }
 
 // Wave-encoded unit: `count` consecutive blocks of at most kWaveBlock values,
@@ -402,9 +404,28 @@ __global__ __launch_bounds__(kThreads, 4) void k_encode(Rows rows, leco_fix_bloc
     const uint64_t s = un.cb0 * un.bs;
     const uint32_t m = (uint32_t)(un.n - s < un.bs ? un.n - s : un.bs);
     if (un.width == 64)
-        encode_block_body<uint64_t, 64>((const uint64_t *)un.base + s, m, prm, un.gb0, u, offs, out, sh);
+        encode_block_body<kThreads, 2, uint64_t, 64>((const uint64_t *)un.base + s, m, prm, un.gb0, u, offs, out, sh);
     else
-        encode_block_body<uint32_t, 32>((const uint32_t *)un.base + s, m, prm, un.gb0, u, offs, out, sh);
+        encode_block_body<kThreads, 2, uint32_t, 32>((const uint32_t *)un.base + s, m, prm, un.gb0, u, offs, out, sh);
+}
+
+// A plain column of blocks of more than kLargeEncodeBlock values: the
+// workgroup body caching one chunk (8 values per thread) instead of two.  At
+// 68 instead of 124 VGPRs, 7 workgroups fit a CU instead of 4; blocks this
+// large are re-read in the passes either way.
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_encode_large(PlainRows rows, leco_fix_block_param *__restrict__ prm,
+                                                           uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
+    __shared__ EncodeShared sh;
+    __shared__ uint64_t s_u;
+    if (threadIdx.x == 0) s_u = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+    __syncthreads();
+    const uint64_t u = s_u;
+    const EncodeUnit un = rows.unit(u);
+    if (!un.count) return;
+    const uint64_t s = un.cb0 * un.bs;
+    const uint32_t m = (uint32_t)(un.n - s < un.bs ? un.n - s : un.bs);
+    encode_block_body<kThreads, 1, T, W>((const T *)un.base + s, m, prm, un.gb0, u, offs, out, sh);
 }
 
 // K3: block end offsets offs[b+1] from the param table (the unit status
@@ -974,7 +995,10 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
     PlainRows rows{(const uint8_t *)in, n, bs, W, nb};
     PlainFit fit{(const uint8_t *)in, n, bs, W};
     k_fit<PlainFit><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
-    k_encode<PlainRows><<<dim3((unsigned)encode_units(nb, bs)), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
+    if (bs > kLargeEncodeBlock)  // one block per unit
+        k_encode_large<T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
+    else
+        k_encode<PlainRows><<<dim3((unsigned)encode_units(nb, bs)), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     k_encode_fixup<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, nb,
                                                                                                 nullptr, W);
     return hipGetLastError();
