@@ -409,6 +409,22 @@ __global__ __launch_bounds__(kThreads, 4) void k_encode(Rows rows, leco_fix_bloc
         encode_block_body<kThreads, 2, uint32_t, 32>((const uint32_t *)un.base + s, m, prm, un.gb0, u, offs, out, sh);
 }
 
+// A plain column of blocks of at most kWaveBlock values: wave-encoded units
+// only (94 instead of 124 VGPRs without the workgroup body: 5 workgroups per
+// CU instead of 4).
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_encode_waves(PlainRows rows, leco_fix_block_param *__restrict__ prm,
+                                                           uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
+    __shared__ EncodeShared sh;
+    __shared__ uint64_t s_u;
+    if (threadIdx.x == 0) s_u = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+    __syncthreads();
+    const uint64_t u = s_u;
+    const EncodeUnit un = rows.unit(u);
+    if (!un.count) return;
+    encode_unit_waves<T, W>(un, u, prm, offs, out, sh);
+}
+
 // A plain column of blocks of more than kLargeEncodeBlock values: the
 // workgroup body caching one chunk (8 values per thread) instead of two.  At
 // 68 instead of 124 VGPRs, 7 workgroups fit a CU instead of 4; blocks this
@@ -997,6 +1013,8 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
     k_fit<PlainFit><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
     if (bs > kLargeEncodeBlock)  // one block per unit
         k_encode_large<T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
+    else if (bs <= kWaveBlock)
+        k_encode_waves<T, W><<<dim3((unsigned)encode_units(nb, bs)), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     else
         k_encode<PlainRows><<<dim3((unsigned)encode_units(nb, bs)), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     k_encode_fixup<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, nb,
