@@ -50,6 +50,18 @@ def timed(torch, fn, reps: int, warmup: int):
     return e0.elapsed_time(e1) / reps, wall
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), as lscpu reports it."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(vals_dev, bs: int, sample: int, repeat: int):
     """Oracle decode (1 thread) on the first `sample` values of the column."""
     import numpy as np
@@ -444,6 +456,7 @@ def main() -> int:
                "sample": f"first {sample} values of the column ({(sample + bs - 1) // bs} blocks), "
                          f"decoded {args.cpu_repeat}x by the C oracle, 1 thread",
                "all_cores_value": round(gbsn, 3), "all_cores": nthr,
+               "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
                "filter_range_value": round(gbsf, 3)}
 
     traffic, traffic_src = None, None
