@@ -633,7 +633,11 @@ LECO_BASE_ABI(u64, uint64_t)
 
 uint64_t leco_fix_filter_workspace_bytes(uint64_t n, uint32_t block_size) {
     if (n == 0 || block_size == 0) return 0;
-    return 8 * (leco::filter_work_tiles(n, block_size) + 1);
+    // filter_range_close: a ticket and a look-back word per tile; filter_range: a count
+    // and a kFilterTile-bit bitmap per tile
+    const uint64_t close_b = 8 * (leco::filter_tiles(n, block_size) + 1);
+    const uint64_t gt_b = (8 + leco::kFilterTile / 8) * leco::gt_tiles(n, block_size);
+    return close_b > gt_b ? close_b : gt_b;
 }
 
 int leco_fix_filter_range_u32(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,

@@ -1,13 +1,15 @@
 // Predicate pushdown on compressed columns: Leco_int<T>::filter_range and
 // filter_range_close (headers/piecewise_fix_integer_template.h:206-303 of the
 // reference, with read_all_bit_fix_range / read_all_bit_fix_range_close,
-// headers/bit_read.h:387-497) over every block of a column in one launch.
+// headers/bit_read.h:387-497) over every block of a column.
 //
-// One workgroup per tile of filter_tile_blocks() consecutive blocks, in
-// ticket order.  Pass 1 counts the tile's matches; a decoupled look-back over
-// the tiles' status words (caller workspace) gives the tile's first output
-// slot; pass 2 evaluates the same windows again and writes the positions,
-// compacted in order with wave ballots.  Output order is the reference's:
+// filter_range_close: one launch, one workgroup per tile of
+// filter_tile_blocks() consecutive blocks, in ticket order.  Pass 1 counts
+// the tile's matches; a decoupled look-back over the tiles' status words
+// (caller workspace) gives the tile's first output slot; pass 2 evaluates the
+// same windows again and writes the positions, compacted in order with wave
+// ballots.  filter_range: mark / scan / emit launches with a match bitmap per
+// tile instead of the look-back (below).  Output order is the reference's:
 // block by block, period by period (filter_range_close), position by
 // position.  Only the windows the reference scans are read: positions from
 // the model-derived threshold on.
@@ -81,7 +83,8 @@ struct FilterArgs {
     uint32_t *out;
     uint64_t cap;
     uint64_t *result;       // [0] = count, [1] = flags
-    uint64_t *work;         // [0] = ticket, [1 + t] = tile t's look-back status
+    uint64_t *work;         // close: [0] = ticket, [1 + t] = tile t's look-back status;
+                            // range: [t] = tile t's count (then offset), then the tiles' bitmaps
 };
 
 // Does position i of the block at blk pass?
@@ -557,75 +560,79 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
 }
 
 // ---- filter_range of LeCo (:206-249) and FOR (:111-158) columns ------------
-// A workgroup tile is gt_tile_blocks() whole blocks (blocks of up to
-// kFilterTile values) or a kFilterTile-value chunk of one larger block.  Its
-// field bytes are staged through LDS when they fit.  Block by block (the
-// header is workgroup-uniform), the scanned window is tested once, in rounds
-// of kThreads consecutive positions (neighbouring lanes read neighbouring
-// fields); each wave parks its round ballot in LDS.  A workgroup scan of the
-// (round, wave) counts gives the tile's total, a look-back over the tiles its
-// output slot, and every hit is written at slot + the hits before it in
-// (round, wave, lane) order -- position order -- with coalesced stores.
+// filter_range in three launches, with no serial dependence between tiles.
+// A tile is gt_tile_blocks() whole blocks (blocks of up to kFilterTile
+// values) or a kFilterTile-value chunk of one larger block: at most
+// kFilterTile consecutive positions.
+//   mark:  the tile's field bytes are staged through LDS when they fit; block
+//          by block (the header is workgroup-uniform) the scanned window is
+//          tested once, in rounds of kThreads consecutive positions
+//          (neighbouring lanes read neighbouring fields); each wave ORs its
+//          round ballot into the tile's LDS bitmap at the positions' offset
+//          from the tile start.  The bitmap (kFilterTile bits) and the tile's
+//          count go to the workspace.
+//   scan:  one workgroup turns the counts into exclusive offsets, and the
+//          total into result[0].
+//   emit:  every hit of a tile is written at its offset + the hits before it
+//          in the bitmap -- position order -- with coalesced stores.
+// The workspace holds tiles counts (8 bytes each), then tiles bitmaps.
 constexpr uint32_t kGtStage = 16384;
+constexpr uint32_t kGtWords = kFilterTile / 32;  // 32-bit words of a tile's bitmap
+static_assert(kGtWords == kThreads, "one bitmap word per thread");
 
-// Rounds a tile can need: its full rounds plus one partial round per block.
-__host__ __device__ inline uint32_t gt_max_rounds(uint32_t bs) {
-    return kFilterTile / kThreads + (bs > kFilterTile ? 1 : filter_tile_blocks(bs));
-}
-// Dynamic LDS of k_filter_gt: the staged bytes, then per (round, wave) the
-// ballot and its exclusive prefix, then per round its first position.
-__host__ __device__ inline size_t gt_lds_bytes(uint32_t bs) {
-    return kGtStage + (size_t)gt_max_rounds(bs) * (kWaves * 12 + 4);
+// The tile's first position and length: blocks [b0, b0 + kbt) whole, or
+// values [c0, c1) of block b0.
+struct GtTile {
+    uint64_t b0;
+    uint32_t kbt, c0, c1;
+    bool chunked;
+};
+__device__ __forceinline__ GtTile gt_tile(const FilterArgs &a, uint64_t t) {
+    GtTile g{0, 0, 0, 0, a.bs > kFilterTile};
+    const uint64_t nb = (a.n + a.bs - 1) / a.bs;
+    if (g.chunked) {
+        const uint32_t cpb = (a.bs + kFilterTile - 1) / kFilterTile;
+        g.b0 = t / cpb;
+        g.kbt = 1;
+        const uint64_t s = g.b0 * a.bs;
+        const uint32_t m = (uint32_t)(a.n - s < a.bs ? a.n - s : a.bs);
+        g.c0 = (uint32_t)(t - g.b0 * cpb) * kFilterTile;
+        g.c1 = g.c0 + kFilterTile < m ? g.c0 + kFilterTile : m;
+    } else {
+        g.b0 = t * a.kb;
+        g.kbt = (uint32_t)(nb - g.b0 < a.kb ? nb - g.b0 : a.kb);
+    }
+    return g;
 }
 
 struct GtShared {
-    uint64_t base, tile, a0;
+    uint64_t a0;
     uint32_t words;
     uint32_t wcnt[kWaves];
+    uint32_t bits[kGtWords];
 };
 
 template <typename T, int W, bool FOR>
-__global__ __launch_bounds__(kThreads) void k_filter_gt(FilterArgs a) {
-    extern __shared__ uint4 gt_dyn[];  // 16-byte aligned: the staging writes are 16-byte stores
+__global__ __launch_bounds__(kThreads) void k_filter_gt_mark(FilterArgs a) {
+    __shared__ __align__(16) uint32_t lds[kGtStage / 4];
     __shared__ GtShared sh;
-    const uint32_t max_rounds = gt_max_rounds(a.bs);
-    uint32_t *lds = (uint32_t *)gt_dyn;
-    uint64_t *rbal = (uint64_t *)(gt_dyn + kGtStage / 16);
-    uint32_t *rpre = (uint32_t *)(rbal + max_rounds * kWaves);
-    uint32_t *rstart = rpre + max_rounds * kWaves;
     constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
     constexpr uint32_t tb = W / 8;
     constexpr uint32_t hdr = FOR ? 1 + 2 * tb : 17;  // payload offset of a packed block
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
-    if (tid == 0) sh.tile = atomicAdd((unsigned long long *)a.work, 1ull);  // work[0] is the ticket
-    __syncthreads();
-    const uint64_t t = sh.tile;
-    const uint64_t nb = (a.n + a.bs - 1) / a.bs;
-    // the tile: blocks [b0, b0 + kbt) whole, or values [c0, c1) of block b0
-    uint64_t b0;
-    uint32_t kbt, c0 = 0, c1 = 0;
-    const bool chunked = a.bs > kFilterTile;
-    if (chunked) {
-        const uint32_t cpb = (a.bs + kFilterTile - 1) / kFilterTile;
-        b0 = t / cpb;
-        kbt = 1;
-        const uint64_t s = b0 * a.bs;
-        const uint32_t m = (uint32_t)(a.n - s < a.bs ? a.n - s : a.bs);
-        c0 = (uint32_t)(t - b0 * cpb) * kFilterTile;
-        c1 = c0 + kFilterTile < m ? c0 + kFilterTile : m;
-    } else {
-        b0 = t * a.kb;
-        kbt = (uint32_t)(nb - b0 < a.kb ? nb - b0 : a.kb);
-    }
+    const uint64_t t = blockIdx.x;
+    const GtTile g = gt_tile(a, t);
+    const uint64_t tile_start = g.b0 * a.bs + g.c0;
+    sh.bits[tid] = 0;
     if (tid == 0) {
-        uint64_t g0 = a.offs[b0], g1 = a.offs[b0 + kbt];
-        if (chunked) {  // the chunk's fields only
+        uint64_t g0 = a.offs[g.b0], g1 = a.offs[g.b0 + g.kbt];
+        if (g.chunked) {  // the chunk's fields only
             const uint8_t *blk = a.stream + g0;
             const int l = blk[0];
             const uint32_t lf = l >= W ? W : (uint32_t)l;
             const uint64_t pay = g0 + (l >= W ? 1 : hdr);
-            g1 = pay + (((uint64_t)c1 * lf + 7) >> 3);
-            g0 = pay + (((uint64_t)c0 * lf) >> 3);
+            g1 = pay + (((uint64_t)g.c1 * lf + 7) >> 3);
+            g0 = pay + (((uint64_t)g.c0 * lf) >> 3);
         }
         const uint64_t a0 = g0 & ~15ull, a1 = ((g1 + 15) & ~15ull) + 16;
         sh.a0 = a0;
@@ -639,9 +646,8 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt(FilterArgs a) {
     __syncthreads();
 
     const uint64_t f = a.filter & wmask;
-    uint32_t rounds = 0;
-    for (uint32_t kk = 0; kk < kbt; ++kk) {
-        const uint64_t b = b0 + kk;
+    for (uint32_t kk = 0; kk < g.kbt; ++kk) {
+        const uint64_t b = g.b0 + kk;
         const uint64_t off = a.offs[b];
         const uint8_t *blk = a.stream + off;
         const uint64_t s = b * a.bs;
@@ -651,7 +657,7 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt(FilterArgs a) {
         const uint32_t lf = raw ? W : (uint32_t)l;
         const uint64_t pay = raw ? 1 : hdr;  // payload byte offset in the block
         // window [lo, hi) of positions the reference scans
-        uint32_t lo = chunked ? c0 : 0, hi = chunked ? c1 : m;
+        uint32_t lo = g.chunked ? g.c0 : 0, hi = g.chunked ? g.c1 : m;
         double t0 = 0.0, t1 = 0.0;
         uint64_t base_v = 0;
         if (raw) {
@@ -678,6 +684,7 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt(FilterArgs a) {
             if (W == 64) v |= (uint64_t)__builtin_amdgcn_alignbit(lds[w + 2], x1, sft) << 32;
             return v & low_mask((int)lf);
         };
+        const uint32_t rel = (uint32_t)(s - tile_start);  // tile bit of block position 0
         for (uint32_t p = lo; p < hi; p += kThreads) {
             const uint32_t j = p + tid;
             bool h = false;
@@ -691,41 +698,80 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt(FilterArgs a) {
                 }
                 h = (raw || FOR) ? v > f : above<W>(v, f);
             }
+            // the wave's 64 positions start at tile bit q: lanes 0-2 OR the ballot, shifted
+            // by q % 32, into words q/32 .. q/32+2.  Bits of positions past the window are 0,
+            // so a nonzero part never falls past the tile's kGtWords words.
             const uint64_t bal = __ballot(h);
-            if (lane == 0) rbal[rounds * kWaves + wid] = bal;
-            if (tid == 0) rstart[rounds] = a.pos_base + (uint32_t)(s + p);
-            ++rounds;
+            const uint32_t q = rel + p + (uint32_t)wid * kWave, sft = q & 31;
+            if (bal && lane < 3) {
+                const uint32_t part = lane == 0   ? (uint32_t)(bal << sft)
+                                      : lane == 1 ? (uint32_t)((bal << sft) >> 32)
+                                                  : (sft ? (uint32_t)(bal >> (64 - sft)) : 0u);
+                if (part) atomicOr(&sh.bits[(q >> 5) + lane], part);
+            }
         }
     }
     __syncthreads();
-
-    // exclusive prefix of the (round, wave) counts: each thread owns a run of them
-    const uint32_t ne = rounds * kWaves;
-    const uint32_t per = (ne + kThreads - 1) / kThreads;
-    const uint32_t e0 = tid * per < ne ? tid * per : ne, e1 = e0 + per < ne ? e0 + per : ne;
-    uint32_t mine = 0;
-    for (uint32_t e = e0; e < e1; ++e) mine += (uint32_t)__popcll(rbal[e]);
+    const uint32_t word = sh.bits[tid];
     uint32_t total;
-    uint32_t run = wg_exclusive_scan(sh, mine, &total);
-    for (uint32_t e = e0; e < e1; ++e) {
-        rpre[e] = run;
-        run += (uint32_t)__popcll(rbal[e]);
-    }
-    if (wid == 0) {
-        const uint64_t excl = look_back(t, total, a.work);
-        if (lane == 0) {
-            sh.base = excl;
-            if (t + 1 == a.tiles) a.result[0] = excl + total;
+    (void)wg_exclusive_scan(sh, (uint32_t)__popc(word), &total);
+    uint32_t *bm = (uint32_t *)(a.work + a.tiles) + t * kGtWords;
+    bm[tid] = word;
+    if (tid == 0) a.work[t] = total;
+}
+
+// Exclusive prefix of the tiles' counts (in place) and the total.
+__global__ __launch_bounds__(kThreads) void k_filter_gt_scan(uint64_t *__restrict__ counts, uint64_t tiles,
+                                                             uint64_t *__restrict__ result) {
+    __shared__ uint64_t wsum[kWaves];
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    uint64_t carry = 0;
+    for (uint64_t c0 = 0; c0 < tiles; c0 += kThreads) {
+        const uint64_t t = c0 + tid;
+        const uint64_t v = t < tiles ? counts[t] : 0;
+        uint64_t inc = v;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint64_t x = (uint64_t)__shfl_up((long long)inc, o, kWave);
+            if (lane >= o) inc += x;
         }
+        if (lane == kWave - 1) wsum[wid] = inc;
+        __syncthreads();
+        uint64_t before = 0, all = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            before += w < wid ? wsum[w] : 0;
+            all += wsum[w];
+        }
+        if (t < tiles) counts[t] = carry + before + inc - v;
+        carry += all;
+        __syncthreads();
     }
+    if (tid == 0) result[0] = carry;
+}
+
+// The positions of tile t's bitmap, at its offset (position order).
+__global__ __launch_bounds__(kThreads) void k_filter_gt_emit(FilterArgs a) {
+    __shared__ GtShared sh;
+    __shared__ uint32_t pre[kGtWords];
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    const uint64_t t = blockIdx.x;
+    const GtTile g = gt_tile(a, t);
+    const uint32_t start = a.pos_base + (uint32_t)(g.b0 * a.bs + g.c0);  // uint32 position arithmetic
+    const uint64_t base = a.work[t];
+    const uint32_t word = ((const uint32_t *)(a.work + a.tiles))[t * kGtWords + tid];
+    uint32_t total;
+    const uint32_t mine = wg_exclusive_scan(sh, (uint32_t)__popc(word), &total);
+    if (!total) return;
+    sh.bits[tid] = word;
+    pre[tid] = mine;
     __syncthreads();
-    const uint64_t base = sh.base;
     const uint64_t below = (1ull << lane) - 1;
-    for (uint32_t r = 0; r < rounds; ++r) {
-        const uint64_t bal = rbal[r * kWaves + wid];
-        if ((bal >> lane) & 1) {
-            const uint64_t at = base + rpre[r * kWaves + wid] + (uint64_t)__popcll(bal & below);
-            if (at < a.cap) a.out[at] = rstart[r] + (uint32_t)tid;
+    for (uint32_t r = 0; r < kGtWords / 2 / kWaves; ++r) {  // rounds of kThreads positions
+        const uint32_t w64 = r * kWaves + wid;               // the wave's 64 positions
+        const uint64_t bits = (uint64_t)sh.bits[2 * w64] | (uint64_t)sh.bits[2 * w64 + 1] << 32;
+        if ((bits >> lane) & 1) {
+            const uint64_t at = base + pre[2 * w64] + (uint64_t)__popcll(bits & below);
+            if (at < a.cap) a.out[at] = start + w64 * kWave + (uint32_t)lane;
         }
     }
 }
@@ -735,6 +781,13 @@ __global__ __launch_bounds__(kThreads) void k_filter_clear(uint64_t *__restrict_
     const uint64_t i = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (i < words) work[i] = 0;
     if (i == 0) result[0] = result[1] = 0;
+}
+
+template <typename T, int W, bool FOR>
+void launch_gt(const FilterArgs &a, hipStream_t st) {
+    k_filter_gt_mark<T, W, FOR><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    k_filter_gt_scan<<<dim3(1), dim3(kThreads), 0, st>>>(a.work, a.tiles, a.result);
+    k_filter_gt_emit<<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
 }
 
 }  // namespace
@@ -750,7 +803,7 @@ hipError_t launch_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n
     k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
                                                                                                     result);
     if (close) k_filter<T, W, 1><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
-    else k_filter_gt<T, W, false><<<dim3((unsigned)a.tiles), dim3(kThreads), gt_lds_bytes(bs), st>>>(a);
+    else launch_gt<T, W, false>(a, st);
     return hipGetLastError();
 }
 
@@ -765,7 +818,7 @@ hipError_t launch_for_filter(const uint8_t *stream, const uint64_t *offs, uint64
     k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
                                                                                                     result);
     if (close) k_filter<T, W, 3><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
-    else k_filter_gt<T, W, true><<<dim3((unsigned)a.tiles), dim3(kThreads), gt_lds_bytes(bs), st>>>(a);
+    else launch_gt<T, W, true>(a, st);
     return hipGetLastError();
 }
 

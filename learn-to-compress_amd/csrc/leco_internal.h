@@ -166,11 +166,6 @@ __host__ __device__ inline uint64_t gt_tiles(uint64_t n, uint32_t bs) {
     if (bs <= kFilterTile) return filter_tiles(n, bs);
     return (n + bs - 1) / bs * ((bs + kFilterTile - 1) / kFilterTile);
 }
-// Look-back status words any filter launch of this column may use.
-__host__ __device__ inline uint64_t filter_work_tiles(uint64_t n, uint32_t bs) {
-    const uint64_t a = filter_tiles(n, bs), b = gt_tiles(n, bs);
-    return a > b ? a : b;
-}
 
 // Each launcher enqueues on `st` and returns the hipGetLastError() of the launch.
 template <typename T>
