@@ -636,7 +636,7 @@ uint64_t leco_fix_filter_workspace_bytes(uint64_t n, uint32_t block_size) {
     // filter_range_close: a ticket and a look-back word per tile; filter_range: a count
     // and a kFilterTile-bit bitmap per tile
     const uint64_t close_b = 8 * (leco::filter_tiles(n, block_size) + 1);
-    const uint64_t gt_b = (8 + leco::kFilterTile / 8) * leco::gt_tiles(n, block_size);
+    const uint64_t gt_b = leco::gt_workspace_bytes(n, block_size);
     return close_b > gt_b ? close_b : gt_b;
 }
 

@@ -570,13 +570,17 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
 //          (neighbouring lanes read neighbouring fields); each wave ORs its
 //          round ballot into the tile's LDS bitmap at the positions' offset
 //          from the tile start.  The bitmap (kFilterTile bits) and the tile's
-//          count go to the workspace.
-//   scan:  one workgroup turns the counts into exclusive offsets, and the
-//          total into result[0].
-//   emit:  every hit of a tile is written at its offset + the hits before it
-//          in the bitmap -- position order -- with coalesced stores.
-// The workspace holds tiles counts (8 bytes each), then tiles bitmaps.
+//          count go to the workspace, and the count is added to the sum of
+//          its group of kGtGroup tiles.
+//   emit:  a tile's output offset is the sum of the groups before its group
+//          plus the counts of the tiles before it in its group, read in
+//          parallel; every hit is written at that offset + the hits before it
+//          in the bitmap -- position order -- with coalesced stores.  The last
+//          tile writes the total to result[0].
+// The workspace holds the group sums (zeroed first), the tiles' counts, then
+// the tiles' bitmaps.
 constexpr uint32_t kGtStage = 16384;
+constexpr uint32_t kGtGroup = kGtGroupTiles;  // tiles per group sum
 constexpr uint32_t kGtWords = kFilterTile / 32;  // 32-bit words of a tile's bitmap
 static_assert(kGtWords == kThreads, "one bitmap word per thread");
 
@@ -609,8 +613,11 @@ struct GtShared {
     uint64_t a0;
     uint32_t words;
     uint32_t wcnt[kWaves];
+    uint64_t wsum[kWaves];
     uint32_t bits[kGtWords];
 };
+
+__host__ __device__ inline uint64_t gt_groups(uint64_t tiles) { return (tiles + kGtGroup - 1) / kGtGroup; }
 
 template <typename T, int W, bool FOR>
 __global__ __launch_bounds__(kThreads) void k_filter_gt_mark(FilterArgs a) {
@@ -715,38 +722,13 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt_mark(FilterArgs a) {
     const uint32_t word = sh.bits[tid];
     uint32_t total;
     (void)wg_exclusive_scan(sh, (uint32_t)__popc(word), &total);
-    uint32_t *bm = (uint32_t *)(a.work + a.tiles) + t * kGtWords;
+    const uint64_t groups = gt_groups(a.tiles);
+    uint32_t *bm = (uint32_t *)(a.work + groups + a.tiles) + t * kGtWords;
     bm[tid] = word;
-    if (tid == 0) a.work[t] = total;
-}
-
-// Exclusive prefix of the tiles' counts (in place) and the total.
-__global__ __launch_bounds__(kThreads) void k_filter_gt_scan(uint64_t *__restrict__ counts, uint64_t tiles,
-                                                             uint64_t *__restrict__ result) {
-    __shared__ uint64_t wsum[kWaves];
-    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
-    uint64_t carry = 0;
-    for (uint64_t c0 = 0; c0 < tiles; c0 += kThreads) {
-        const uint64_t t = c0 + tid;
-        const uint64_t v = t < tiles ? counts[t] : 0;
-        uint64_t inc = v;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint64_t x = (uint64_t)__shfl_up((long long)inc, o, kWave);
-            if (lane >= o) inc += x;
-        }
-        if (lane == kWave - 1) wsum[wid] = inc;
-        __syncthreads();
-        uint64_t before = 0, all = 0;
-#pragma unroll
-        for (int w = 0; w < kWaves; ++w) {
-            before += w < wid ? wsum[w] : 0;
-            all += wsum[w];
-        }
-        if (t < tiles) counts[t] = carry + before + inc - v;
-        carry += all;
-        __syncthreads();
+    if (tid == 0) {
+        a.work[groups + t] = total;
+        if (total) atomicAdd((unsigned long long *)&a.work[t / kGtGroup], (unsigned long long)total);
     }
-    if (tid == 0) result[0] = carry;
 }
 
 // The positions of tile t's bitmap, at its offset (position order).
@@ -757,10 +739,21 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt_emit(FilterArgs a) {
     const uint64_t t = blockIdx.x;
     const GtTile g = gt_tile(a, t);
     const uint32_t start = a.pos_base + (uint32_t)(g.b0 * a.bs + g.c0);  // uint32 position arithmetic
-    const uint64_t base = a.work[t];
-    const uint32_t word = ((const uint32_t *)(a.work + a.tiles))[t * kGtWords + tid];
+    const uint64_t groups = gt_groups(a.tiles);
+    // offset: the groups before t's group, then the tiles before t in its group
+    const uint64_t gi = t / kGtGroup;
+    uint64_t part = 0;
+    for (uint64_t i = tid; i < gi; i += kThreads) part += a.work[i];
+    if (gi * kGtGroup + tid < t) part += a.work[groups + gi * kGtGroup + tid];
+    for (int o = kWave / 2; o > 0; o >>= 1) part += (uint64_t)__shfl_xor((long long)part, o, kWave);
+    if (lane == 0) sh.wsum[wid] = part;
+    const uint32_t word = ((const uint32_t *)(a.work + groups + a.tiles))[t * kGtWords + tid];
     uint32_t total;
-    const uint32_t mine = wg_exclusive_scan(sh, (uint32_t)__popc(word), &total);
+    const uint32_t mine = wg_exclusive_scan(sh, (uint32_t)__popc(word), &total);  // (its barrier orders wsum)
+    uint64_t base = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) base += sh.wsum[w];
+    if (t + 1 == a.tiles && tid == 0) a.result[0] = base + total;
     if (!total) return;
     sh.bits[tid] = word;
     pre[tid] = mine;
@@ -785,8 +778,10 @@ __global__ __launch_bounds__(kThreads) void k_filter_clear(uint64_t *__restrict_
 
 template <typename T, int W, bool FOR>
 void launch_gt(const FilterArgs &a, hipStream_t st) {
+    const uint64_t words = gt_groups(a.tiles);  // the group sums start at 0
+    k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(a.work, words,
+                                                                                                    a.result);
     k_filter_gt_mark<T, W, FOR><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
-    k_filter_gt_scan<<<dim3(1), dim3(kThreads), 0, st>>>(a.work, a.tiles, a.result);
     k_filter_gt_emit<<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
 }
 
@@ -799,11 +794,14 @@ hipError_t launch_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n
     constexpr int W = 8 * sizeof(T);
     FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), close ? filter_tiles(n, bs) : gt_tiles(n, bs), filter,
                  0, f1, f2, base, pos_base, out, cap, result, work};
-    const uint64_t words = a.tiles + 1;
-    k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
-                                                                                                    result);
-    if (close) k_filter<T, W, 1><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
-    else launch_gt<T, W, false>(a, st);
+    if (close) {
+        const uint64_t words = a.tiles + 1;  // the ticket and the look-back words start at 0
+        k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
+                                                                                                        result);
+        k_filter<T, W, 1><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    } else {
+        launch_gt<T, W, false>(a, st);
+    }
     return hipGetLastError();
 }
 
@@ -814,11 +812,14 @@ hipError_t launch_for_filter(const uint8_t *stream, const uint64_t *offs, uint64
     constexpr int W = 8 * sizeof(T);
     FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), close ? filter_tiles(n, bs) : gt_tiles(n, bs), f1, f2,
                  nullptr, nullptr, base, pos_base, out, cap, result, work};
-    const uint64_t words = a.tiles + 1;
-    k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
-                                                                                                    result);
-    if (close) k_filter<T, W, 3><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
-    else launch_gt<T, W, true>(a, st);
+    if (close) {
+        const uint64_t words = a.tiles + 1;  // the ticket and the look-back words start at 0
+        k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
+                                                                                                        result);
+        k_filter<T, W, 3><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    } else {
+        launch_gt<T, W, true>(a, st);
+    }
     return hipGetLastError();
 }
 
