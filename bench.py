@@ -98,6 +98,49 @@ def cpu_baseline(vals_dev, bs: int, sample: int, repeat: int):
     return gbs1, gbsn, nthr, gbsf
 
 
+def cpu_rows(vals_dev, bs: int, sample: int) -> dict:
+    """The other rows' CPU figures, 1 thread, on the same sample: the C
+    oracle's encode, 1M random lookups, filter_range_close with the driver's
+    zonemap filters (period 2^20), and FOR / Delta decode."""
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, ROOT)
+    from leco_fix.column import close_filters_from_zonemap
+    from oracle import oracle
+
+    y = vals_dev[:sample].cpu().numpy().view(np.uint64)
+    n = len(y)
+    out = {}
+    t0 = time.perf_counter()
+    stream, offs = oracle.encode_column(y, bs, threads=1)
+    out["encode_gbs"] = round(n * 8 / (time.perf_counter() - t0) / 1e9, 3)
+    idx = np.random.default_rng(7).integers(0, n, 1_000_000).astype(np.uint64)
+    t0 = time.perf_counter()
+    got = oracle.lookup_column(stream, offs, n, bs, idx, np.uint64, threads=1)
+    out["lookups_per_s"] = round(len(idx) / (time.perf_counter() - t0), 0)
+    assert np.array_equal(got, y[idx])
+    base = 1 << 20
+    full = n // bs * bs
+    mins = torch.from_numpy(y[:full].view(np.int64).reshape(-1, bs).min(axis=1))
+    if full < n:
+        mins = torch.cat([mins, torch.tensor([int(y[full:].min())], dtype=torch.int64)])
+    f1, f2 = close_filters_from_zonemap(mins, base // 4, base // 2, base, 64)
+    t0 = time.perf_counter()
+    pos = oracle.filter_close_column(stream, offs, n, bs, np.uint64, f1.numpy().view(np.uint64),
+                                     f2.numpy().view(np.uint64), base)
+    out["filter_range_close_gbs"] = round(n * 8 / (time.perf_counter() - t0) / 1e9, 3)
+    r = y % np.uint64(base)
+    assert len(pos) == int(np.count_nonzero((r > base // 4) & (r < base // 2)))
+    for codec in ("for", "delta"):
+        st, of = oracle.base_encode_column(codec, y, bs)
+        t0 = time.perf_counter()
+        dec = oracle.base_decode_column(codec, st, of, n, bs, np.uint64)
+        out[f"{codec}_decode_gbs"] = round(n * 8 / (time.perf_counter() - t0) / 1e9, 3)
+        assert np.array_equal(dec, y)
+    return out
+
+
 def baselines_section(torch, vals, bs: int, raw: int, leco_bytes: int, leco_ms: float, steps: int) -> dict:
     """FOR_int and Delta_int (the reference's fixed-partition baselines) on the
     same column and block size: compression ratio, encode / decode GB/s,
@@ -458,6 +501,8 @@ def main() -> int:
                "all_cores_value": round(gbsn, 3), "all_cores": nthr,
                "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
                "filter_range_value": round(gbsf, 3)}
+        log("cpu baseline: encode, lookups, filter_range_close, FOR / Delta decode on the same sample, 1 thread")
+        cpu["rows_1_thread"] = cpu_rows(vals, bs, sample)
 
     traffic, traffic_src = None, None
     pmc_file = os.path.join(ROOT, "profiles", f"decode_pwl{n // 1_000_000}M_u64_bs{bs}_pmc.json")
