@@ -9,8 +9,12 @@ encoded once by the GPU encoder, verified (decode == input, lookups ==
 input[idx]) and then decoded `--steps` times between HIP events recorded on the
 stream the kernels run on.
 
-Also reported: GPU encode GB/s, 64M uniform random lookups/s, and the CPU
-oracle's single-thread decode on a bounded sample of the same column.
+Also reported: GPU encode GB/s, 64M uniform random lookups/s, the latency
+of a single random access (p50 / p99, one index per launch), the FOR / Delta
+baselines, filters, files, piecewise_fix_op_max, the batched, served (p50 /
+p99 per request kind) and churn workloads, and the CPU oracle's figures on a
+bounded sample of the same column (decode on 1 thread and on all cores; the
+other rows on 1 thread).
 
     python bench.py [--steps K] [--warmup W]
 """
@@ -429,6 +433,19 @@ def main() -> int:
 
     log("lookup timing")
     ms_lk, _ = timed(torch, lambda: col.lookup(idx, out=lk), 5, 1)
+    # random-access latency: one index per call, launch to result on the host (plain path)
+    one = lk[:1].clone()
+    for q in range(50):  # warm-up
+        col.lookup(idx[q:q + 1], out=one)
+    torch.cuda.synchronize()
+    lat_us = []
+    for q in range(2000):
+        t0 = time.perf_counter()
+        col.lookup(idx[q:q + 1], out=one)
+        torch.cuda.synchronize()
+        lat_us.append((time.perf_counter() - t0) * 1e6)
+    lat_us.sort()
+    ra_p50, ra_p99 = lat_us[len(lat_us) // 2], lat_us[int(len(lat_us) * 0.99)]
 
     base = None
     if not args.no_baselines:
@@ -538,6 +555,8 @@ def main() -> int:
             "lookups": args.lookups,
             "lookups_per_s": round(args.lookups / (ms_lk * 1e-3), 0),
             "lookup_batch_ms": round(ms_lk, 3),
+            "random_access_single_p50_us": round(ra_p50, 1),
+            "random_access_single_p99_us": round(ra_p99, 1),
             "decode_wall_ms_per_step": round(wall_dec * 1e3 / args.steps, 4),
             "decode_value_parallel_ms": round(ms_dec_values, 4),
             "device_copy_gbs": round(copy_gbs, 1) if copy_gbs else None,

@@ -156,13 +156,20 @@ def run_batched(cols, reps: int = 5, log=print) -> dict:
     return out
 
 
-def _latency_stats(res, reqs, kinds=(srvmod.REQ_LOOKUP, srvmod.REQ_RANGE)) -> dict:
+def _latency_stats(res, reqs, kinds=(srvmod.REQ_LOOKUP, srvmod.REQ_RANGE), split: bool = True) -> dict:
+    """p50 / p99 latency of the completed requests of `kinds`; with split,
+    also of the point-lookup (random-access) and range requests apart."""
     sel = np.isin(reqs["kind"], kinds) & (res["status"] == 0)
     lat = res["latency_s"][sel]
-    if lat.size == 0:
-        return {"p50_us": None, "p99_us": None}
-    return {"p50_us": round(float(np.percentile(lat, 50)) * 1e6, 1),
-            "p99_us": round(float(np.percentile(lat, 99)) * 1e6, 1)}
+    out = {"p50_us": None, "p99_us": None}
+    if lat.size:
+        out = {"p50_us": round(float(np.percentile(lat, 50)) * 1e6, 1),
+               "p99_us": round(float(np.percentile(lat, 99)) * 1e6, 1)}
+    if split:
+        for name, kind in (("lookup", srvmod.REQ_LOOKUP), ("range", srvmod.REQ_RANGE)):
+            part = _latency_stats(res, reqs, (kind,), split=False)
+            out[f"{name}_p50_us"], out[f"{name}_p99_us"] = part["p50_us"], part["p99_us"]
+    return out
 
 
 def run_serving(col, rates=(1e5, 1e6, 1e7), nreq: int = 60_000, plain_nreq: int = 6_000, seed: int = 4,
