@@ -44,6 +44,7 @@ constexpr int kWaves = kThreads / kWave;
 constexpr int kEmitPer = 8;  // rounds of kThreads candidates per compaction step when writing positions
 constexpr uint32_t kCloseRounds = 64;  // scan rounds of a tile whose ballots pass 1 parks for pass 2
 constexpr uint32_t kCloseWin = 512;    // periods whose windows pass 1 keeps for the scan
+constexpr uint32_t kCloseStage = 8192; // bytes of a block's fields the scan sweep stages through LDS
 
 struct FilterShared {
     uint64_t wsum[kWaves];
@@ -359,8 +360,9 @@ __device__ __forceinline__ bool close_scan_hit(const CloseBlock &c, uint32_t K, 
 }
 
 template <typename T, int W, bool WRITE>
-__device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &sh, uint64_t b, uint32_t m,
-                                            uint32_t slot, uint32_t bstart, uint64_t &acc, uint32_t &rounds_used) {
+__device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &sh, uint64_t *__restrict__ stage,
+                                            uint64_t b, uint32_t m, uint32_t slot, uint32_t bstart, uint64_t &acc,
+                                            uint32_t &rounds_used) {
     constexpr uint64_t wmask = W == 32 ? 0xFFFFFFFFull : ~0ull;
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     CloseBlock c;
@@ -404,25 +406,45 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
         // decrease (the next value from the next lane; the last lane decodes it), and,
         // while the tile has room, the round ballots for pass 2
         const bool park = rounds_used + nr <= kCloseRounds;
+        // the block's fields, [offs[b] + 17, + ceil(m*l/8)), through LDS when they fit (with
+        // one word of over-read); model-only blocks (l = 0) have none
+        const uint64_t g0 = (uint64_t)(c.blk - a.stream) + 17, a0 = g0 & ~15ull;
+        const uint32_t n16 = (uint32_t)((g0 + (((uint64_t)m * c.l + 7) >> 3) - a0 + 15) / 16);
+        const bool staged = c.l > 0 && (uint64_t)n16 * 16 + 16 <= kCloseStage;
+        if (staged)
+            for (uint32_t u = tid; u < n16; u += kThreads)
+                *(uint4 *)(stage + 2 * u) = *(const uint4 *)(a.stream + a0 + 16ull * u);
+        __syncthreads();
+        const uint32_t fbit = (uint32_t)(8 * (g0 - a0));
         uint64_t cnt = 0;
         bool sorted = true;
-        for (uint32_t r = 0; r < nr; ++r) {
-            const uint32_t i = r * kThreads + tid;
-            uint64_t v = 0;
-            bool h = false;
-            if (i < mp) {
-                v = close_value(c, i);
-                h = close_scan_hit<W>(c, K, i, v, a.base, sh, K < kCloseWin ? K : kCloseWin);
+        // the sweep, instantiated once per field source (each keeps only its own registers)
+        auto sweep = [&](auto value) {
+            for (uint32_t r = 0; r < nr; ++r) {
+                const uint32_t i = r * kThreads + tid;
+                uint64_t v = 0;
+                bool h = false;
+                if (i < mp) {
+                    v = value(i);
+                    h = close_scan_hit<W>(c, K, i, v, a.base, sh, K < kCloseWin ? K : kCloseWin);
+                }
+                uint64_t u = (uint64_t)__shfl_down((long long)v, 1, kWave);
+                if (lane == kWave - 1 && i + 1 < mp) u = value(i + 1);
+                if (i + 1 < mp) sorted &= W == 64 ? v <= u : (int64_t)v <= (int64_t)u;
+                cnt += h;
+                if (park) {
+                    const uint64_t bal = __ballot(h);
+                    if (lane == 0) sh.rb[(rounds_used + r) * kWaves + wid] = bal;
+                }
             }
-            uint64_t u = (uint64_t)__shfl_down((long long)v, 1, kWave);
-            if (lane == kWave - 1 && i + 1 < mp) u = close_value(c, i + 1);
-            if (i + 1 < mp) sorted &= W == 64 ? v <= u : (int64_t)v <= (int64_t)u;
-            cnt += h;
-            if (park) {
-                const uint64_t bal = __ballot(h);
-                if (lane == 0) sh.rb[(rounds_used + r) * kWaves + wid] = bal;
-            }
-        }
+        };
+        if (staged)
+            sweep([&](uint32_t i) -> uint64_t {
+                return apply_field(predict(c.t0, c.t1, i), lds_field<W>(stage, fbit + i * (uint32_t)c.l, c.l),
+                                   c.l);
+            });
+        else
+            sweep([&](uint32_t i) -> uint64_t { return close_value(c, i); });
         const uint64_t bad = __ballot(!sorted);
         if (lane == 0) sh.wfail[wid] = bad ? 1u : 0u;
         __syncthreads();
@@ -512,6 +534,8 @@ __device__ __forceinline__ void for_block(const FilterArgs &a, FilterShared &sh,
 template <typename T, int W, int MODE>
 __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     __shared__ FilterShared sh;
+    // the LeCo close scan's field staging (a placeholder word for FOR)
+    __shared__ __align__(16) uint64_t stage[MODE == 1 ? kCloseStage / 8 : 2];
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     if (tid == 0) sh.tile = atomicAdd((unsigned long long *)a.work, 1ull);  // work[0] is the ticket
     __syncthreads();
@@ -530,7 +554,7 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     uint32_t rounds_used = 0;
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
-        if (MODE == 1) ok &= close_block<T, W, false>(a, sh, b, block_len(b), k, 0, mine, rounds_used);
+        if (MODE == 1) ok &= close_block<T, W, false>(a, sh, stage, b, block_len(b), k, 0, mine, rounds_used);
         else for_block<T, W, true, false>(a, sh, b, block_len(b), 0, mine);
     }
     for (int o = kWave / 2; o > 0; o >>= 1) mine += (uint64_t)__shfl_xor((long long)mine, o, kWave);
@@ -554,7 +578,7 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
         const uint32_t bstart = a.pos_base + (uint32_t)(b * a.bs);
-        if (MODE == 1) close_block<T, W, true>(a, sh, b, block_len(b), k, bstart, pos, rounds_used);
+        if (MODE == 1) close_block<T, W, true>(a, sh, stage, b, block_len(b), k, bstart, pos, rounds_used);
         else for_block<T, W, true, true>(a, sh, b, block_len(b), bstart, pos);
     }
 }
