@@ -633,10 +633,9 @@ LECO_BASE_ABI(u64, uint64_t)
 
 uint64_t leco_fix_filter_workspace_bytes(uint64_t n, uint32_t block_size) {
     if (n == 0 || block_size == 0) return 0;
-    // filter_range_close: a ticket and a look-back word per tile; filter_range: a count
-    // and a kFilterTile-bit bitmap per tile
-    const uint64_t close_b = 8 * (leco::filter_tiles(n, block_size) + 1);
-    const uint64_t gt_b = leco::gt_workspace_bytes(n, block_size);
+    // group sums, counts and match bitmaps of the larger of the two filters' tile sets
+    const uint64_t close_b = leco::filter_workspace_for(leco::filter_tiles(n, block_size));
+    const uint64_t gt_b = leco::filter_workspace_for(leco::gt_tiles(n, block_size));
     return close_b > gt_b ? close_b : gt_b;
 }
 

@@ -530,16 +530,38 @@ __device__ __forceinline__ void for_block(const FilterArgs &a, FilterShared &sh,
                         acc);
 }
 
-// The close filters, MODE 1 (LeCo filter_range_close) and 3 (FOR filter_range_close).
+// Match bitmaps of the filters' tiles (kFilterTile positions at most) and the
+// group sums that place each tile's output.
+constexpr uint32_t kGtWords = kFilterTile / 32;  // 32-bit words of a tile's bitmap
+static_assert(kGtWords == kThreads, "one bitmap word per thread");
+constexpr uint32_t kGtGroup = kGtGroupTiles;     // tiles per group sum
+__host__ __device__ inline uint64_t gt_groups(uint64_t tiles) { return (tiles + kGtGroup - 1) / kGtGroup; }
+
+// The close filters, MODE 1 (LeCo filter_range_close) and 3 (FOR
+// filter_range_close), in two launches with no serial dependence between
+// tiles (a tile is filter_tile_blocks() whole blocks):
+//   mark: pass 1 of each tile, in workgroup index order -- the tile's count
+//         goes to the workspace and to the sum of its group of kGtGroupTiles
+//         tiles.  A tile whose blocks were all swept in scan mode with their
+//         ballots parked (LeCo blocks with fields, sorted values, disjoint
+//         periods: the reference's period order is position order there) and
+//         that spans at most kFilterTile positions also writes its match
+//         bitmap, built from those ballots.
+//   emit: a tile's offset is the sum of the groups before it plus the counts
+//         before it in its group.  A bitmap tile expands its bitmap; any
+//         other tile runs pass 1 again (LeCo: the blocks' period counts and
+//         modes) and pass 2, the reference's order, at that offset.
+// Workspace: the group sums, the tiles' counts (bit 63: the tile has a
+// bitmap), then a kGtWords-word bitmap per tile.
+constexpr uint64_t kBitmapTile = 1ull << 63;
+
 template <typename T, int W, int MODE>
-__global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
+__global__ __launch_bounds__(kThreads) void k_filter_close_mark(FilterArgs a) {
     __shared__ FilterShared sh;
-    // the LeCo close scan's field staging (a placeholder word for FOR)
+    // the LeCo close scan's field staging, then the tile's bitmap (a placeholder for FOR)
     __shared__ __align__(16) uint64_t stage[MODE == 1 ? kCloseStage / 8 : 2];
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
-    if (tid == 0) sh.tile = atomicAdd((unsigned long long *)a.work, 1ull);  // work[0] is the ticket
-    __syncthreads();
-    const uint64_t t = sh.tile;
+    const uint64_t t = blockIdx.x;
     const uint64_t nb = (a.n + a.bs - 1) / a.bs;
     const uint64_t b0 = t * a.kb;
     const uint32_t kbt = (uint32_t)(nb - b0 < a.kb ? nb - b0 : a.kb);
@@ -547,8 +569,6 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
         const uint64_t s = b * a.bs;
         return (uint32_t)(a.n - s < a.bs ? a.n - s : a.bs);
     };
-
-    // pass 1: count
     uint64_t mine = 0;
     bool ok = true;
     uint32_t rounds_used = 0;
@@ -560,21 +580,108 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
     for (int o = kWave / 2; o > 0; o >>= 1) mine += (uint64_t)__shfl_xor((long long)mine, o, kWave);
     if (lane == 0) sh.wsum[wid] = mine;
     __syncthreads();
-    if (wid == 0) {
-        uint64_t total = 0;
+    uint64_t total = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) total += sh.wsum[w];
-        const uint64_t excl = look_back(t, total, a.work);
-        if (lane == 0) {
-            sh.base = excl;
-            if (t + 1 == a.tiles) a.result[0] = excl + total;
-            if (!ok) atomicOr((unsigned long long *)&a.result[1], 1ull);
-        }
+    for (int w = 0; w < kWaves; ++w) total += sh.wsum[w];
+    // a bitmap tile: every block has fields (0 < l < W) and either has no periods or was
+    // swept in scan mode with its ballots parked
+    bool bm = MODE == 1 && (uint64_t)kbt * a.bs <= kFilterTile;
+    for (uint32_t k = 0; k < kbt && bm; ++k) {
+        const int l = a.stream[a.offs[b0 + k]];
+        bm = l > 0 && l < W && (sh.bK[k] == 0 || sh.bscan[k] == 2);
     }
-    __syncthreads();
+    const uint64_t groups = gt_groups(a.tiles);
+    if constexpr (MODE == 1) {
+      if (bm) {
+        uint32_t *bits = (uint32_t *)stage;  // pass 1 is done with the staged fields
+        bits[tid] = 0;
+        __syncthreads();
+        for (uint32_t k = 0; k < kbt; ++k) {
+            if (sh.bK[k] == 0) continue;
+            const uint32_t nr = (block_len(b0 + k) + kThreads - 1) / kThreads, r0 = sh.bround[k];
+            for (uint32_t r = 0; r < nr; ++r) {
+                // the wave's 64 positions start at tile bit q: lanes 0-2 OR the ballot, shifted
+                // by q % 32, into words q/32 .. q/32+2 (nonzero parts stay inside the tile)
+                const uint64_t bal = sh.rb[(r0 + r) * kWaves + wid];
+                const uint32_t q = k * a.bs + r * kThreads + (uint32_t)wid * kWave, sft = q & 31;
+                if (bal && lane < 3) {
+                    const uint32_t part = lane == 0   ? (uint32_t)(bal << sft)
+                                          : lane == 1 ? (uint32_t)((bal << sft) >> 32)
+                                                      : (sft ? (uint32_t)(bal >> (64 - sft)) : 0u);
+                    if (part) atomicOr(&bits[(q >> 5) + lane], part);
+                }
+            }
+        }
+        __syncthreads();
+        ((uint32_t *)(a.work + groups + a.tiles))[t * kGtWords + tid] = bits[tid];
+      }
+    }
+    if (tid == 0) {
+        a.work[groups + t] = total | (bm ? kBitmapTile : 0);
+        if (total) atomicAdd((unsigned long long *)&a.work[t / kGtGroup], (unsigned long long)total);
+        if (!ok) atomicOr((unsigned long long *)&a.result[1], 1ull);
+    }
+}
 
-    // pass 2: the same candidates, positions written in order
-    uint64_t pos = sh.base;
+template <typename T, int W, int MODE>
+__global__ __launch_bounds__(kThreads) void k_filter_close_emit(FilterArgs a) {
+    __shared__ FilterShared sh;
+    // the LeCo close scan's field staging, or a bitmap tile's words and their prefixes
+    __shared__ __align__(16) uint64_t stage[MODE == 1 ? kCloseStage / 8 : 2];
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    const uint64_t t = blockIdx.x;
+    const uint64_t nb = (a.n + a.bs - 1) / a.bs;
+    const uint64_t b0 = t * a.kb;
+    const uint32_t kbt = (uint32_t)(nb - b0 < a.kb ? nb - b0 : a.kb);
+    auto block_len = [&](uint64_t b) -> uint32_t {
+        const uint64_t s = b * a.bs;
+        return (uint32_t)(a.n - s < a.bs ? a.n - s : a.bs);
+    };
+    const uint64_t groups = gt_groups(a.tiles);
+    // offset: the groups before t's group, then the tiles before t in its group
+    const uint64_t gi = t / kGtGroup;
+    uint64_t part = 0;
+    for (uint64_t i = tid; i < gi; i += kThreads) part += a.work[i];
+    if (gi * kGtGroup + tid < t) part += a.work[groups + gi * kGtGroup + tid] & ~kBitmapTile;
+    for (int o = kWave / 2; o > 0; o >>= 1) part += (uint64_t)__shfl_xor((long long)part, o, kWave);
+    if (lane == 0) sh.wsum[wid] = part;
+    __syncthreads();
+    uint64_t base = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) base += sh.wsum[w];
+    const uint64_t cw = a.work[groups + t], count = cw & ~kBitmapTile;
+    if (t + 1 == a.tiles && tid == 0) a.result[0] = base + count;
+    __syncthreads();  // wsum is read; pass 1 below reuses the shared state
+    if (!count) return;
+    if constexpr (MODE == 1) {
+      if (cw & kBitmapTile) {
+        uint32_t *bits = (uint32_t *)stage, *pre = bits + kGtWords;
+        const uint32_t word = ((const uint32_t *)(a.work + groups + a.tiles))[t * kGtWords + tid];
+        uint32_t all;
+        const uint32_t mine = wg_exclusive_scan(sh, (uint32_t)__popc(word), &all);
+        bits[tid] = word;
+        pre[tid] = mine;
+        __syncthreads();
+        const uint32_t start = a.pos_base + (uint32_t)(b0 * a.bs);  // uint32 position arithmetic
+        const uint64_t below = (1ull << lane) - 1;
+        for (uint32_t r = 0; r < kGtWords / 2 / kWaves; ++r) {  // rounds of kThreads positions
+            const uint32_t w64 = r * kWaves + wid;               // the wave's 64 positions
+            const uint64_t bw = (uint64_t)bits[2 * w64] | (uint64_t)bits[2 * w64 + 1] << 32;
+            if ((bw >> lane) & 1) {
+                const uint64_t at = base + pre[2 * w64] + (uint64_t)__popcll(bw & below);
+                if (at < a.cap) a.out[at] = start + w64 * kWave + (uint32_t)lane;
+            }
+        }
+        return;
+      }
+    }
+    uint64_t pos = base;
+    uint32_t rounds_used = 0;
+    if (MODE == 1) {  // the blocks' period counts and sweep modes, as mark found them
+        uint64_t cnt = 0;
+        for (uint32_t k = 0; k < kbt; ++k)
+            (void)close_block<T, W, false>(a, sh, stage, b0 + k, block_len(b0 + k), k, 0, cnt, rounds_used);
+    }
     for (uint32_t k = 0; k < kbt; ++k) {
         const uint64_t b = b0 + k;
         const uint32_t bstart = a.pos_base + (uint32_t)(b * a.bs);
@@ -604,9 +711,6 @@ __global__ __launch_bounds__(kThreads) void k_filter(FilterArgs a) {
 // The workspace holds the group sums (zeroed first), the tiles' counts, then
 // the tiles' bitmaps.
 constexpr uint32_t kGtStage = 16384;
-constexpr uint32_t kGtGroup = kGtGroupTiles;  // tiles per group sum
-constexpr uint32_t kGtWords = kFilterTile / 32;  // 32-bit words of a tile's bitmap
-static_assert(kGtWords == kThreads, "one bitmap word per thread");
 
 // The tile's first position and length: blocks [b0, b0 + kbt) whole, or
 // values [c0, c1) of block b0.
@@ -641,7 +745,7 @@ struct GtShared {
     uint32_t bits[kGtWords];
 };
 
-__host__ __device__ inline uint64_t gt_groups(uint64_t tiles) { return (tiles + kGtGroup - 1) / kGtGroup; }
+
 
 template <typename T, int W, bool FOR>
 __global__ __launch_bounds__(kThreads) void k_filter_gt_mark(FilterArgs a) {
@@ -809,6 +913,15 @@ void launch_gt(const FilterArgs &a, hipStream_t st) {
     k_filter_gt_emit<<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
 }
 
+template <typename T, int W, int MODE>
+void launch_close(const FilterArgs &a, hipStream_t st) {
+    const uint64_t words = gt_groups(a.tiles);  // the group sums start at 0
+    k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(a.work, words,
+                                                                                                    a.result);
+    k_filter_close_mark<T, W, MODE><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+    k_filter_close_emit<T, W, MODE><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+}
+
 }  // namespace
 
 template <typename T>
@@ -819,10 +932,7 @@ hipError_t launch_filter(const uint8_t *stream, const uint64_t *offs, uint64_t n
     FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), close ? filter_tiles(n, bs) : gt_tiles(n, bs), filter,
                  0, f1, f2, base, pos_base, out, cap, result, work};
     if (close) {
-        const uint64_t words = a.tiles + 1;  // the ticket and the look-back words start at 0
-        k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
-                                                                                                        result);
-        k_filter<T, W, 1><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+        launch_close<T, W, 1>(a, st);
     } else {
         launch_gt<T, W, false>(a, st);
     }
@@ -837,10 +947,7 @@ hipError_t launch_for_filter(const uint8_t *stream, const uint64_t *offs, uint64
     FilterArgs a{stream, offs, n, bs, filter_tile_blocks(bs), close ? filter_tiles(n, bs) : gt_tiles(n, bs), f1, f2,
                  nullptr, nullptr, base, pos_base, out, cap, result, work};
     if (close) {
-        const uint64_t words = a.tiles + 1;  // the ticket and the look-back words start at 0
-        k_filter_clear<<<dim3((unsigned)((words + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(work, words,
-                                                                                                        result);
-        k_filter<T, W, 3><<<dim3((unsigned)a.tiles), dim3(kThreads), 0, st>>>(a);
+        launch_close<T, W, 3>(a, st);
     } else {
         launch_gt<T, W, true>(a, st);
     }

@@ -166,12 +166,11 @@ __host__ __device__ inline uint64_t gt_tiles(uint64_t n, uint32_t bs) {
     if (bs <= kFilterTile) return filter_tiles(n, bs);
     return (n + bs - 1) / bs * ((bs + kFilterTile - 1) / kFilterTile);
 }
-// filter_range's workspace: a sum per kGtGroupTiles tiles, then a count and a
-// kFilterTile-bit match bitmap per tile.
+// A filter launch over `tiles` tiles uses a sum per kGtGroupTiles tiles, then
+// a count and a kFilterTile-bit match bitmap per tile.
 constexpr uint32_t kGtGroupTiles = 256;
-__host__ __device__ inline uint64_t gt_workspace_bytes(uint64_t n, uint32_t bs) {
-    const uint64_t t = gt_tiles(n, bs);
-    return 8 * ((t + kGtGroupTiles - 1) / kGtGroupTiles) + (8 + kFilterTile / 8) * t;
+__host__ __device__ inline uint64_t filter_workspace_for(uint64_t tiles) {
+    return 8 * ((tiles + kGtGroupTiles - 1) / kGtGroupTiles) + (8 + kFilterTile / 8) * tiles;
 }
 
 // Each launcher enqueues on `st` and returns the hipGetLastError() of the launch.
