@@ -158,8 +158,9 @@ int leco_fix_random_decode_u64(const uint8_t *h_in, uint32_t to_find, uint64_t *
  * d_result[0] receives the total count and d_result[1] flags (bit 0: a block
  * of filter_range_close needed more than 65536 periods and was cut short).
  * d_work: leco_fix_filter_workspace_bytes(n, block_size) bytes of device
- * memory (filter_range: a count and a match bitmap per tile of 8192
- * positions, about n/8 bytes; filter_range_close: look-back status words).  Asynchronous on `stream`; n <= 2^32.
+ * memory (a count and a match bitmap per tile of at most 8192 positions, and
+ * a sum per 256 tiles: about n/8 bytes).  Calls that share one d_work must
+ * not overlap.  Asynchronous on `stream`; n <= 2^32.
  * Raw blocks (bits >= width) are filtered by their values: the reference reads
  * theta from their raw bytes, which is undefined. */
 uint64_t leco_fix_filter_workspace_bytes(uint64_t n, uint32_t block_size);

@@ -3,16 +3,15 @@
 // reference, with read_all_bit_fix_range / read_all_bit_fix_range_close,
 // headers/bit_read.h:387-497) over every block of a column.
 //
-// filter_range_close: one launch, one workgroup per tile of
-// filter_tile_blocks() consecutive blocks, in ticket order.  Pass 1 counts
-// the tile's matches; a decoupled look-back over the tiles' status words
-// (caller workspace) gives the tile's first output slot; pass 2 evaluates the
-// same windows again and writes the positions, compacted in order with wave
-// ballots.  filter_range: mark / scan / emit launches with a match bitmap per
-// tile instead of the look-back (below).  Output order is the reference's:
-// block by block, period by period (filter_range_close), position by
-// position.  Only the windows the reference scans are read: positions from
-// the model-derived threshold on.
+// Both filters run as a mark launch and an emit launch over tiles with no
+// serial dependence between them.  mark counts each tile's matches (and,
+// where it can, keeps them as a match bitmap), adding the count to the sum of
+// its group of tiles; emit places the tile's output after the groups and
+// tiles before it and writes its positions, from the bitmap or by evaluating
+// the tile's windows again, compacted in order with wave ballots.  Output
+// order is the reference's: block by block, period by period
+// (filter_range_close), position by position.  Only the windows the reference
+// scans are read: positions from the model-derived threshold on.
 #include "leco_kernels.h"
 
 namespace leco {
@@ -84,8 +83,7 @@ struct FilterArgs {
     uint32_t *out;
     uint64_t cap;
     uint64_t *result;       // [0] = count, [1] = flags
-    uint64_t *work;         // close: [0] = ticket, [1 + t] = tile t's look-back status;
-                            // range: [t] = tile t's count (then offset), then the tiles' bitmaps
+    uint64_t *work;         // the group sums, the tiles' counts, then the tiles' bitmaps
 };
 
 // Does position i of the block at blk pass?

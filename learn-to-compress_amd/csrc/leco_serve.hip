@@ -15,11 +15,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "leco_internal.h"
@@ -33,8 +37,104 @@ constexpr uint32_t kMaxColumns = 1u << 16;
 constexpr uint64_t kIdxCap = 1ull << 22;  // staged lookup indices per batch
 constexpr uint64_t kItemCap = 1ull << 18; // work items per batch
 constexpr uint32_t kServeGrid = 4096;     // workgroups per served launch at most
+constexpr unsigned kCopyHelpers = 7;      // threads beside the server loop for index copies (at most)
 
 uint64_t up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+// Host copies of a batch's lookup indices into its pinned staging buffer.  A
+// batch of 4M indices is 32 MB; one thread copies ~10 GB/s, which caps
+// lookup-heavy serving well below what the GPU sustains, so copies of a batch
+// above kSplitBytes are split by byte range over `helpers` threads plus the
+// caller.  The helpers sleep on a condition variable between batches.
+class CopyPool {
+public:
+    static constexpr size_t kSplitBytes = 256 << 10;
+
+    bool start(unsigned helpers) {
+        try {
+            for (unsigned h = 0; h < helpers; ++h) threads_.emplace_back([this, h] { work(h + 1); });
+        } catch (...) {
+            stop();
+            return false;
+        }
+        return true;
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            quit_ = true;
+        }
+        cv_.notify_all();
+        for (std::thread &t : threads_) t.join();
+        threads_.clear();
+    }
+    void add(void *dst, const void *src, size_t bytes) {
+        jobs_.push_back(Job{(uint8_t *)dst, (const uint8_t *)src, total_});
+        total_ += bytes;
+    }
+    // Performs every recorded copy and returns when all are done.
+    void run() {
+        if (jobs_.empty()) return;
+        const unsigned parts = total_ >= kSplitBytes ? (unsigned)threads_.size() + 1 : 1;
+        if (parts > 1) {
+            {
+                std::lock_guard<std::mutex> g(m_);
+                parts_ = parts;
+                left_.store(parts - 1, std::memory_order_relaxed);
+                ++gen_;
+            }
+            cv_.notify_all();
+        }
+        copy_part(0, parts);
+        if (parts > 1)
+            while (left_.load(std::memory_order_acquire)) std::this_thread::yield();
+        jobs_.clear();
+        total_ = 0;
+    }
+
+private:
+    struct Job {
+        uint8_t *dst;
+        const uint8_t *src;
+        size_t at;  // offset of this copy in the batch's concatenated bytes
+    };
+    // Bytes [p*total/parts, (p+1)*total/parts) of the concatenated copies.
+    void copy_part(unsigned p, unsigned parts) {
+        const size_t lo = total_ * p / parts, hi = total_ * (p + 1) / parts;
+        size_t j = std::upper_bound(jobs_.begin(), jobs_.end(), lo, [](size_t v, const Job &x) { return v < x.at; }) -
+                   jobs_.begin() - 1;
+        for (; j < jobs_.size() && jobs_[j].at < hi; ++j) {
+            const size_t end = j + 1 < jobs_.size() ? jobs_[j + 1].at : total_;
+            const size_t a = std::max(lo, jobs_[j].at), b = std::min(hi, end);
+            if (b > a) std::memcpy(jobs_[j].dst + (a - jobs_[j].at), jobs_[j].src + (a - jobs_[j].at), b - a);
+        }
+    }
+    void work(unsigned p) {
+        uint64_t seen = 0;
+        for (;;) {
+            unsigned parts;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [&] { return quit_ || gen_ != seen; });
+                if (quit_) return;
+                seen = gen_;
+                parts = parts_;
+            }
+            if (p < parts) copy_part(p, parts);
+            left_.fetch_sub(1, std::memory_order_release);
+        }
+    }
+
+    std::vector<std::thread> threads_;
+    std::vector<Job> jobs_;
+    size_t total_ = 0;
+    std::mutex m_;
+    std::condition_variable cv_;
+    uint64_t gen_ = 0;
+    unsigned parts_ = 1;
+    bool quit_ = false;
+    std::atomic<unsigned> left_{0};
+};
 
 // Ring over [0, cap): allocations are made in order and reclaimed once every
 // older allocation has been released too.
@@ -155,6 +255,7 @@ struct leco_fix_server {
     FreeList colfree;
     std::vector<Slot> slots;
     hipStream_t enc = nullptr;
+    CopyPool copies;
 };
 
 namespace {
@@ -167,6 +268,7 @@ int upload_column(leco_fix_server *s, uint32_t c, hipStream_t st) {
 }
 
 void destroy_server(leco_fix_server *s) {
+    s->copies.stop();
     (void)hipDeviceSynchronize();
     for (Slot &sl : s->slots) {
         if (sl.done) (void)hipEventDestroy(sl.done);
@@ -291,6 +393,11 @@ int leco_fix_server_create(const leco_fix_server_config *cfg_in, leco_fix_server
              hipMalloc(&sl.d_items, kItemCap * sizeof(leco::ServeItem)) == hipSuccess;
     }
     if (!ok) { destroy_server(s); return LECO_FIX_ENOMEM; }
+    const unsigned hw = std::thread::hardware_concurrency();
+    if (!s->copies.start(hw > 1 ? std::min(kCopyHelpers, hw - 1) : 0)) {
+        destroy_server(s);
+        return LECO_FIX_ENOMEM;
+    }
     s->ring.init(s->result_cap);
     s->colfree.init(s->column_cap);
     *out = s;
@@ -459,11 +566,15 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
     } else {
         std::deque<uint64_t> queue;  // arrived, not yet admitted, in arrival order
         const size_t window = 4 * (size_t)s->cfg.max_batch_requests;  // queue entries examined per pass
+        size_t controls = 0;  // insert / evict requests in the queue
         std::vector<uint32_t> pending_inserts;
         while (done < nreq) {
             bool progressed = false;
             double t = now_s();
-            while (next < nreq && reqs[next].arrival_s <= t) queue.push_back(next++);
+            for (; next < nreq && reqs[next].arrival_s <= t; ++next) {
+                queue.push_back(next);
+                controls += reqs[next].kind > LECO_FIX_REQ_RANGE;
+            }
 
             // completions
             for (Slot &sl : s->slots) {
@@ -504,11 +615,15 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                 uint64_t nitems = 0, nidx = 0, nvals = 0;
                 bool full = free_slot == nullptr;
                 std::vector<uint64_t> admitted;
-                for (size_t qi = 0; qi < scan; ++qi) {
+                // Once the batch is full, every data request is held; the scan
+                // goes on only for insert / evict requests further back.
+                size_t ctl_ahead = controls, ctl_kept = 0, qi = 0;
+                for (; qi < scan && !(full && ctl_ahead == 0); ++qi) {
                     const uint64_t ri = queue[qi];
                     const leco_fix_request &r = reqs[ri];
                     ColumnSlot &c = s->cols[r.column];
                     t = now_s();
+                    if (r.kind > LECO_FIX_REQ_RANGE) ctl_ahead--;
                     if (r.kind == LECO_FIX_REQ_INSERT) {
                         if (c.present || !ensure_events(c)) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
                         if (insert_layout(r).total > s->column_cap) { complete(ri, t, LECO_FIX_ENOMEM); progressed = true; continue; }
@@ -518,6 +633,7 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                             progressed = true;
                         } else {
                             keep.push_back(ri);  // waits for column memory
+                            ctl_kept++;
                         }
                         continue;
                     }
@@ -526,7 +642,7 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                         for (uint64_t k : keep)
                             if (reqs[k].column == r.column && reqs[k].kind <= LECO_FIX_REQ_RANGE) earlier = true;
                         if (!c.present) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
-                        if (earlier || c.evicting) { keep.push_back(ri); continue; }
+                        if (earlier || c.evicting) { keep.push_back(ri); ctl_kept++; continue; }
                         c.evicting = true;
                         c.evict_req = ri;
                         res[ri].queued_s = t - r.arrival_s;
@@ -566,7 +682,7 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                     uint8_t *out = s->result_pool + off;
                     const uint32_t bw = c.col.width / 8;
                     if (r.kind == LECO_FIX_REQ_LOOKUP) {
-                        std::memcpy(sl.h_idx + nidx, r.h_idx, r.count * sizeof(uint64_t));
+                        s->copies.add(sl.h_idx + nidx, r.h_idx, r.count * sizeof(uint64_t));
                         for (uint64_t q0 = 0; q0 < r.count; q0 += kLookupChunk) {
                             leco::ServeItem &w = sl.h_items[nitems++];
                             w = leco::ServeItem{leco::kServeLookup, r.column,
@@ -603,8 +719,10 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                         sl.reqs.push_back(UINT64_MAX);
                     }
                 }
-                for (size_t k = 0; k < scan; ++k) queue.pop_front();
+                controls = ctl_ahead + ctl_kept;  // those not reached, and those held
+                for (size_t k = 0; k < qi; ++k) queue.pop_front();
                 for (size_t k = keep.size(); k-- > 0;) queue.push_front(keep[k]);
+                s->copies.run();
                 if (!admitted.empty()) {
                     Slot &sl = *free_slot;
                     hipError_t e = hipMemcpyAsync(sl.d_items, sl.h_items, nitems * sizeof(leco::ServeItem),

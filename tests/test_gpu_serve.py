@@ -69,6 +69,32 @@ def test_served_results_equal_alone_and_truth(served_columns):
     srv.close()
 
 
+def test_lookup_batch_with_split_index_copies(served_columns):
+    """One burst of lookups with odd counts: the batch's index bytes (well
+    above the size where the host copy is split over threads) are cut into
+    parts inside requests and inside 8-byte indices."""
+    from leco_fix.server import REQUEST_DTYPE, Server
+    cols, values = served_columns
+    srv = Server(slots=2, reserve_bytes=0, result_pool_bytes=1 * GB, column_pool_bytes=64 << 20)
+    srv.attach(0, cols[0])
+    srv.attach(1, cols[1])
+    rng = np.random.default_rng(17)
+    counts = rng.integers(1, 20_001, 96) | 1
+    col = np.arange(96) % 2
+    idx = [rng.integers(0, len(values[c]), int(k)).astype(np.uint64) for c, k in zip(col, counts)]
+    outs = [np.zeros(int(k), np.uint64 if c == 0 else np.uint32) for c, k in zip(col, counts)]
+    reqs = np.zeros(96, REQUEST_DTYPE)
+    reqs["kind"], reqs["column"], reqs["count"] = 0, col, counts
+    for i in range(96):
+        reqs["h_idx"][i], reqs["h_out"][i] = idx[i].ctypes.data, outs[i].ctypes.data
+    say(f"burst of 96 lookups, {int(counts.sum()) * 8 >> 10} KiB of indices")
+    res, stats = srv.run(reqs)
+    assert (res["status"] == 0).all() and stats["max_batch_requests"] > 1
+    for i in range(96):
+        assert np.array_equal(outs[i], values[int(col[i])][idx[i]]), i
+    srv.close()
+
+
 def test_admission_holds_requests_when_results_pool_is_small(served_columns):
     from leco_fix.server import REQUEST_DTYPE, Server
     cols, values = served_columns
