@@ -43,16 +43,23 @@ uint64_t up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
 // Host copies of a batch's lookup indices into its pinned staging buffer.  A
 // batch of 4M indices is 32 MB; one thread copies ~10 GB/s, which caps
-// lookup-heavy serving well below what the GPU sustains, so copies of a batch
-// above kSplitBytes are split by byte range over `helpers` threads plus the
-// caller.  The helpers sleep on a condition variable between batches.
+// lookup-heavy serving well below what the GPU sustains, so a batch's copies
+// above kSplitBytes are cut into parts of at least kPartBytes that the caller
+// and up to `helpers` threads claim one at a time.  The caller copies every
+// part nobody has claimed and then waits only for parts already being copied,
+// so a helper that is slow to wake delays nothing.  A claim ticket carries the
+// generation (one per run()), so a helper that wakes after its generation is
+// over claims nothing.  The helpers sleep on a condition variable between
+// batches.
 class CopyPool {
 public:
     static constexpr size_t kSplitBytes = 256 << 10;
+    static constexpr size_t kPartBytes = 64 << 10;
+    static constexpr uint32_t kMaxParts = 32;
 
     bool start(unsigned helpers) {
         try {
-            for (unsigned h = 0; h < helpers; ++h) threads_.emplace_back([this, h] { work(h + 1); });
+            for (unsigned h = 0; h < helpers; ++h) threads_.emplace_back([this] { work(); });
         } catch (...) {
             stop();
             return false;
@@ -75,19 +82,22 @@ public:
     // Performs every recorded copy and returns when all are done.
     void run() {
         if (jobs_.empty()) return;
-        const unsigned parts = total_ >= kSplitBytes ? (unsigned)threads_.size() + 1 : 1;
-        if (parts > 1) {
+        if (threads_.empty() || total_ < kSplitBytes) {
+            copy_range(0, total_);
+        } else {
+            const uint32_t parts = (uint32_t)std::min<size_t>(kMaxParts, total_ / kPartBytes);
+            uint64_t gen;
             {
                 std::lock_guard<std::mutex> g(m_);
+                gen = ++gen_;
                 parts_ = parts;
-                left_.store(parts - 1, std::memory_order_relaxed);
-                ++gen_;
+                done_.store(0, std::memory_order_relaxed);
+                ticket_.store(gen << 32, std::memory_order_release);
             }
             cv_.notify_all();
+            copy_parts(gen, parts);
+            while (done_.load(std::memory_order_acquire) != parts) std::this_thread::yield();
         }
-        copy_part(0, parts);
-        if (parts > 1)
-            while (left_.load(std::memory_order_acquire)) std::this_thread::yield();
         jobs_.clear();
         total_ = 0;
     }
@@ -98,9 +108,8 @@ private:
         const uint8_t *src;
         size_t at;  // offset of this copy in the batch's concatenated bytes
     };
-    // Bytes [p*total/parts, (p+1)*total/parts) of the concatenated copies.
-    void copy_part(unsigned p, unsigned parts) {
-        const size_t lo = total_ * p / parts, hi = total_ * (p + 1) / parts;
+    // Bytes [lo, hi) of the concatenated copies.
+    void copy_range(size_t lo, size_t hi) {
         size_t j = std::upper_bound(jobs_.begin(), jobs_.end(), lo, [](size_t v, const Job &x) { return v < x.at; }) -
                    jobs_.begin() - 1;
         for (; j < jobs_.size() && jobs_[j].at < hi; ++j) {
@@ -109,10 +118,23 @@ private:
             if (b > a) std::memcpy(jobs_[j].dst + (a - jobs_[j].at), jobs_[j].src + (a - jobs_[j].at), b - a);
         }
     }
-    void work(unsigned p) {
+    // Claims and copies parts of generation gen until none is left.
+    void copy_parts(uint64_t gen, uint32_t parts) {
+        uint64_t v = ticket_.load(std::memory_order_acquire);
+        for (;;) {
+            if ((v >> 32) != gen || (uint32_t)v >= parts) return;
+            if (!ticket_.compare_exchange_weak(v, v + 1, std::memory_order_acq_rel, std::memory_order_acquire))
+                continue;
+            const uint32_t p = (uint32_t)v;
+            copy_range(total_ * p / parts, total_ * (p + 1) / parts);
+            done_.fetch_add(1, std::memory_order_release);
+            v = ticket_.load(std::memory_order_acquire);
+        }
+    }
+    void work() {
         uint64_t seen = 0;
         for (;;) {
-            unsigned parts;
+            uint32_t parts;
             {
                 std::unique_lock<std::mutex> g(m_);
                 cv_.wait(g, [&] { return quit_ || gen_ != seen; });
@@ -120,8 +142,7 @@ private:
                 seen = gen_;
                 parts = parts_;
             }
-            if (p < parts) copy_part(p, parts);
-            left_.fetch_sub(1, std::memory_order_release);
+            copy_parts(seen, parts);
         }
     }
 
@@ -131,9 +152,10 @@ private:
     std::mutex m_;
     std::condition_variable cv_;
     uint64_t gen_ = 0;
-    unsigned parts_ = 1;
+    uint32_t parts_ = 1;
     bool quit_ = false;
-    std::atomic<unsigned> left_{0};
+    std::atomic<uint64_t> ticket_{0};  // generation << 32 | next part
+    std::atomic<uint32_t> done_{0};    // parts of the current generation copied
 };
 
 // Ring over [0, cap): allocations are made in order and reclaimed once every
