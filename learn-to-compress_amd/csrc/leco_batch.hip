@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <new>
+#include <memory>
 #include <vector>
 
 #include "leco_internal.h"
@@ -88,285 +89,321 @@ int enqueue(leco_fix_batch *b, int op, hipStream_t st) {
 extern "C" {
 
 int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco_fix_batch **out) {
-    if (!cols || !out || ncols == 0) return LECO_FIX_EINVAL;
-    for (uint32_t c = 0; c < ncols; ++c)
-        if (!valid_desc(cols[c])) return LECO_FIX_EINVAL;
-    leco_fix_batch *b = new (std::nothrow) leco_fix_batch;
-    if (!b) return LECO_FIX_ENOMEM;
-    b->ncols = ncols;
-    b->h_cols.resize(ncols);
-    b->h_first.resize(ncols + 1);
-    b->h_fit_first.resize(ncols + 1);
-    std::vector<uint64_t> tile_first(ncols + 1), unit_first(ncols + 1);
-    uint64_t arena = 0;
-    for (uint32_t c = 0; c < ncols; ++c) {
-        const leco_fix_column_desc &d = cols[c];
-        b->h_cols[c] = BatchColumn{d.d_in, d.d_out, d.n, d.block_size, d.width_bits};
-        b->h_first[c] = b->total;
-        b->h_fit_first[c] = b->fit_groups;
-        const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
-        b->fit_groups += (nb + 63) / 64;
-        unit_first[c] = b->units;
-        b->units += leco::encode_units(nb, d.block_size);
-        const uint64_t last = d.n - (nb - 1) * d.block_size;
-        arena += (nb - 1) * leco::block_bound(d.width_bits, d.block_size) +
-                 leco::block_bound(d.width_bits, (uint32_t)last);
-        b->total += nb;
-    }
-    b->h_first[ncols] = b->total;
-    b->h_fit_first[ncols] = b->fit_groups;
-    unit_first[ncols] = b->units;
-    // decode tiles: every staged column's first, then every multi column's (column order within each)
-    std::vector<uint32_t> ord;
-    std::vector<uint64_t> ord_first;
-    ord.reserve(ncols);
-    ord_first.reserve(ncols + 1);
-    for (int staged = 1; staged >= 0; --staged) {
-        for (uint32_t c = 0; c < ncols; ++c) {
-            const leco_fix_column_desc &d = cols[c];
-            if (leco::staged_decode(d.block_size) != (staged == 1)) continue;
-            const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
-            const uint32_t kb = leco::multi_blocks(d.block_size, d.width_bits);
-            ord.push_back(c);
+    return leco::abi_guard([&]() -> int {
+        if (!cols || !out || ncols == 0) return LECO_FIX_EINVAL;
+        for (uint32_t c = 0; c < ncols; ++c)
+            if (!valid_desc(cols[c])) return LECO_FIX_EINVAL;
+        leco_fix_batch *b = new (std::nothrow) leco_fix_batch;
+        if (!b) return LECO_FIX_ENOMEM;
+        b->ncols = ncols;
+        // host tables (a failed allocation frees the batch)
+        std::vector<uint64_t> tile_first, unit_first;
+        std::vector<uint32_t> ord;
+        std::vector<uint64_t> ord_first;
+        uint64_t arena = 0;
+        try {
+            b->h_cols.resize(ncols);
+            b->h_first.resize(ncols + 1);
+            b->h_fit_first.resize(ncols + 1);
+            tile_first.resize(ncols + 1);
+            unit_first.resize(ncols + 1);
+            for (uint32_t c = 0; c < ncols; ++c) {
+                const leco_fix_column_desc &d = cols[c];
+                b->h_cols[c] = BatchColumn{d.d_in, d.d_out, d.n, d.block_size, d.width_bits};
+                b->h_first[c] = b->total;
+                b->h_fit_first[c] = b->fit_groups;
+                const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
+                b->fit_groups += (nb + 63) / 64;
+                unit_first[c] = b->units;
+                b->units += leco::encode_units(nb, d.block_size);
+                const uint64_t last = d.n - (nb - 1) * d.block_size;
+                arena += (nb - 1) * leco::block_bound(d.width_bits, d.block_size) +
+                         leco::block_bound(d.width_bits, (uint32_t)last);
+                b->total += nb;
+            }
+            b->h_first[ncols] = b->total;
+            b->h_fit_first[ncols] = b->fit_groups;
+            unit_first[ncols] = b->units;
+            // decode tiles: every staged column's first, then every multi column's (column order within each)
+            ord.reserve(ncols);
+            ord_first.reserve(ncols + 1);
+            for (int staged = 1; staged >= 0; --staged) {
+                for (uint32_t c = 0; c < ncols; ++c) {
+                    const leco_fix_column_desc &d = cols[c];
+                    if (leco::staged_decode(d.block_size) != (staged == 1)) continue;
+                    const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
+                    const uint32_t kb = leco::multi_blocks(d.block_size, d.width_bits);
+                    ord.push_back(c);
+                    ord_first.push_back(b->tiles);
+                    tile_first[c] = b->tiles;
+                    b->tiles += staged ? nb : (nb + kb - 1) / kb;
+                }
+                if (staged) b->staged_tiles = b->tiles;
+            }
             ord_first.push_back(b->tiles);
-            tile_first[c] = b->tiles;
-            b->tiles += staged ? nb : (nb + kb - 1) / kb;
+            tile_first[ncols] = b->tiles;
+        } catch (...) {
+            free_batch(b);
+            return LECO_FIX_ENOMEM;
         }
-        if (staged) b->staged_tiles = b->tiles;
-    }
-    ord_first.push_back(b->tiles);
-    tile_first[ncols] = b->tiles;
-    b->arena_cap = arena + LECO_FIX_TAIL_PAD;
-    uint32_t *d_ord = nullptr;
-    uint64_t *d_ord_first = nullptr;
-    bool ok = hipMalloc(&b->d_cols, ncols * sizeof(BatchColumn)) == hipSuccess &&
-              hipMalloc(&b->d_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
-              hipMalloc(&b->d_fit_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
-              hipMalloc(&b->d_tile_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
-              hipMalloc(&b->d_tile_col, b->tiles * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc(&b->d_arena, b->arena_cap) == hipSuccess &&
-              hipMalloc(&b->d_offs, (b->total + 1) * sizeof(uint64_t)) == hipSuccess &&
-              hipMalloc(&b->d_prm, b->total * sizeof(leco_fix_block_param)) == hipSuccess &&
-              hipMalloc(&b->d_widths, b->total * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc(&b->d_unit_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
-              hipMalloc(&b->d_unit_col, b->units * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc(&d_ord, ncols * sizeof(uint32_t)) == hipSuccess &&
-              hipMalloc(&d_ord_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess;
-    auto free_ord = [&]() {
-        (void)hipFree(d_ord);
-        (void)hipFree(d_ord_first);
-    };
-    if (!ok) {
+        b->arena_cap = arena + LECO_FIX_TAIL_PAD;
+        uint32_t *d_ord = nullptr;
+        uint64_t *d_ord_first = nullptr;
+        bool ok = hipMalloc(&b->d_cols, ncols * sizeof(BatchColumn)) == hipSuccess &&
+                  hipMalloc(&b->d_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
+                  hipMalloc(&b->d_fit_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
+                  hipMalloc(&b->d_tile_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
+                  hipMalloc(&b->d_tile_col, b->tiles * sizeof(uint32_t)) == hipSuccess &&
+                  hipMalloc(&b->d_arena, b->arena_cap) == hipSuccess &&
+                  hipMalloc(&b->d_offs, (b->total + 1) * sizeof(uint64_t)) == hipSuccess &&
+                  hipMalloc(&b->d_prm, b->total * sizeof(leco_fix_block_param)) == hipSuccess &&
+                  hipMalloc(&b->d_widths, b->total * sizeof(uint32_t)) == hipSuccess &&
+                  hipMalloc(&b->d_unit_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess &&
+                  hipMalloc(&b->d_unit_col, b->units * sizeof(uint32_t)) == hipSuccess &&
+                  hipMalloc(&d_ord, ncols * sizeof(uint32_t)) == hipSuccess &&
+                  hipMalloc(&d_ord_first, (ncols + 1) * sizeof(uint64_t)) == hipSuccess;
+        auto free_ord = [&]() {
+            (void)hipFree(d_ord);
+            (void)hipFree(d_ord_first);
+        };
+        if (!ok) {
+            free_ord();
+            free_batch(b);
+            return LECO_FIX_ENOMEM;
+        }
+        if (hipMemcpy(b->d_cols, b->h_cols.data(), ncols * sizeof(BatchColumn), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(b->d_first, b->h_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(b->d_fit_first, b->h_fit_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(b->d_tile_first, tile_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(b->d_unit_first, unit_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(d_ord, ord.data(), ncols * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(d_ord_first, ord_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemset(b->d_arena + arena, 0, LECO_FIX_TAIL_PAD) != hipSuccess ||
+            leco::launch_tile_columns(d_ord_first, d_ord, ncols, b->tiles, b->d_tile_col, nullptr) != hipSuccess ||
+            leco::launch_tile_columns(b->d_unit_first, nullptr, ncols, b->units, b->d_unit_col, nullptr) !=
+                hipSuccess ||
+            leco::launch_batch_blocks(b->d_cols, b->d_first, ncols, b->total, b->d_widths, nullptr) != hipSuccess ||
+            hipDeviceSynchronize() != hipSuccess) {
+            free_ord();
+            free_batch(b);
+            return LECO_FIX_EHIP;
+        }
         free_ord();
-        free_batch(b);
-        return LECO_FIX_ENOMEM;
-    }
-    if (hipMemcpy(b->d_cols, b->h_cols.data(), ncols * sizeof(BatchColumn), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(b->d_first, b->h_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(b->d_fit_first, b->h_fit_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(b->d_tile_first, tile_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(b->d_unit_first, unit_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemcpy(d_ord, ord.data(), ncols * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(d_ord_first, ord_first.data(), (ncols + 1) * sizeof(uint64_t), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        hipMemset(b->d_arena + arena, 0, LECO_FIX_TAIL_PAD) != hipSuccess ||
-        leco::launch_tile_columns(d_ord_first, d_ord, ncols, b->tiles, b->d_tile_col, nullptr) != hipSuccess ||
-        leco::launch_tile_columns(b->d_unit_first, nullptr, ncols, b->units, b->d_unit_col, nullptr) !=
-            hipSuccess ||
-        leco::launch_batch_blocks(b->d_cols, b->d_first, ncols, b->total, b->d_widths, nullptr) != hipSuccess ||
-        hipDeviceSynchronize() != hipSuccess) {
-        free_ord();
-        free_batch(b);
-        return LECO_FIX_EHIP;
-    }
-    free_ord();
-    *out = b;
-    return LECO_FIX_OK;
+        *out = b;
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_batch_destroy(leco_fix_batch *b) {
-    if (!b) return LECO_FIX_EINVAL;
-    free_batch(b);
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!b) return LECO_FIX_EINVAL;
+        free_batch(b);
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_batch_encode(leco_fix_batch *b, void *stream) {
-    if (!b) return LECO_FIX_EINVAL;
-    for (const BatchColumn &c : b->h_cols)
-        if (!c.in) return LECO_FIX_EINVAL;
-    return enqueue(b, 0, (hipStream_t)stream);
+    return leco::abi_guard([&]() -> int {
+        if (!b) return LECO_FIX_EINVAL;
+        for (const BatchColumn &c : b->h_cols)
+            if (!c.in) return LECO_FIX_EINVAL;
+        return enqueue(b, 0, (hipStream_t)stream);
+    });
 }
 
 int leco_fix_batch_decode(leco_fix_batch *b, void *stream) {
-    if (!b) return LECO_FIX_EINVAL;
-    for (const BatchColumn &c : b->h_cols)
-        if (!c.out) return LECO_FIX_EINVAL;
-    return enqueue(b, 1, (hipStream_t)stream);
+    return leco::abi_guard([&]() -> int {
+        if (!b) return LECO_FIX_EINVAL;
+        for (const BatchColumn &c : b->h_cols)
+            if (!c.out) return LECO_FIX_EINVAL;
+        return enqueue(b, 1, (hipStream_t)stream);
+    });
 }
 
 int leco_fix_batch_graph_launch(leco_fix_batch *b, int op, void *stream) {
-    if (!b || (op != 0 && op != 1)) return LECO_FIX_EINVAL;
-    for (const BatchColumn &c : b->h_cols)
-        if (op == 0 ? !c.in : !c.out) return LECO_FIX_EINVAL;
-    if (!b->graph[op]) {
-        if (!b->cap_stream && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess)
-            return LECO_FIX_EHIP;
-        hipGraph_t g = nullptr;
-        if (hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeRelaxed) != hipSuccess) return LECO_FIX_EHIP;
-        int rc = enqueue(b, op, b->cap_stream);
-        hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
-        if (rc != LECO_FIX_OK || e != hipSuccess) {
-            if (g) (void)hipGraphDestroy(g);
-            return LECO_FIX_EHIP;
+    return leco::abi_guard([&]() -> int {
+        if (!b || (op != 0 && op != 1)) return LECO_FIX_EINVAL;
+        for (const BatchColumn &c : b->h_cols)
+            if (op == 0 ? !c.in : !c.out) return LECO_FIX_EINVAL;
+        if (!b->graph[op]) {
+            if (!b->cap_stream && hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking) != hipSuccess)
+                return LECO_FIX_EHIP;
+            hipGraph_t g = nullptr;
+            if (hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeRelaxed) != hipSuccess) return LECO_FIX_EHIP;
+            int rc = enqueue(b, op, b->cap_stream);
+            hipError_t e = hipStreamEndCapture(b->cap_stream, &g);
+            if (rc != LECO_FIX_OK || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return LECO_FIX_EHIP;
+            }
+            e = hipGraphInstantiate(&b->graph[op], g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                b->graph[op] = nullptr;
+                return LECO_FIX_EHIP;
+            }
         }
-        e = hipGraphInstantiate(&b->graph[op], g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            b->graph[op] = nullptr;
-            return LECO_FIX_EHIP;
-        }
-    }
-    return hip_status(hipGraphLaunch(b->graph[op], (hipStream_t)stream));
+        return hip_status(hipGraphLaunch(b->graph[op], (hipStream_t)stream));
+    });
 }
 
 int leco_fix_batch_decode_per_column(leco_fix_batch *b, void *stream) {
-    if (!b) return LECO_FIX_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
-    for (uint32_t c = 0; c < b->ncols; ++c) {
-        const BatchColumn &col = b->h_cols[c];
-        if (!col.out) return LECO_FIX_EINVAL;
-        const uint64_t *offs = b->d_offs + b->h_first[c];
-        const leco_fix_block_param *prm = b->d_prm + b->h_first[c];
-        hipError_t e;
-        (void)prm;
-        e = col.width == 64 ? leco::launch_decode<uint64_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
-                                                             (uint64_t *)col.out, st)
-                            : leco::launch_decode<uint32_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
-                                                             (uint32_t *)col.out, st);
-        if (e != hipSuccess) return LECO_FIX_EHIP;
-    }
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!b) return LECO_FIX_EINVAL;
+        hipStream_t st = (hipStream_t)stream;
+        for (uint32_t c = 0; c < b->ncols; ++c) {
+            const BatchColumn &col = b->h_cols[c];
+            if (!col.out) return LECO_FIX_EINVAL;
+            const uint64_t *offs = b->d_offs + b->h_first[c];
+            const leco_fix_block_param *prm = b->d_prm + b->h_first[c];
+            hipError_t e;
+            (void)prm;
+            e = col.width == 64 ? leco::launch_decode<uint64_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
+                                                                 (uint64_t *)col.out, st)
+                                : leco::launch_decode<uint32_t>(b->d_arena, offs, col.n, col.bs, 0, col.n,
+                                                                 (uint32_t *)col.out, st);
+            if (e != hipSuccess) return LECO_FIX_EHIP;
+        }
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_batch_encode_per_column(leco_fix_batch *b, void *stream) {
-    if (!b) return LECO_FIX_EINVAL;
-    uint64_t need = 0, nbmax = 0;
-    for (const BatchColumn &c : b->h_cols) {
-        if (!c.in) return LECO_FIX_EINVAL;
-        need = std::max(need, leco_fix_stream_bound(c.n, c.bs, c.width));
-        nbmax = std::max(nbmax, (c.n + c.bs - 1) / c.bs);
-    }
-    const uint64_t offs_at = (need + 255) / 256 * 256, prm_at = offs_at + ((nbmax + 1) * 8 + 255) / 256 * 256;
-    const uint64_t total = prm_at + nbmax * sizeof(leco_fix_block_param);
-    if (total > b->scratch_cap) {
-        (void)hipFree(b->d_scratch);
-        b->d_scratch = nullptr;
-        b->scratch_cap = 0;
-        if (hipMalloc(&b->d_scratch, total) != hipSuccess) return LECO_FIX_ENOMEM;
-        b->scratch_cap = total;
-    }
-    hipStream_t st = (hipStream_t)stream;
-    uint64_t *offs = (uint64_t *)(b->d_scratch + offs_at);
-    leco_fix_block_param *prm = (leco_fix_block_param *)(b->d_scratch + prm_at);
-    for (const BatchColumn &c : b->h_cols) {
-        hipError_t e = c.width == 64
-                           ? leco::launch_encode<uint64_t>((const uint64_t *)c.in, c.n, c.bs, b->d_scratch, offs, prm, st)
-                           : leco::launch_encode<uint32_t>((const uint32_t *)c.in, c.n, c.bs, b->d_scratch, offs, prm, st);
-        if (e != hipSuccess) return LECO_FIX_EHIP;
-    }
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!b) return LECO_FIX_EINVAL;
+        uint64_t need = 0, nbmax = 0;
+        for (const BatchColumn &c : b->h_cols) {
+            if (!c.in) return LECO_FIX_EINVAL;
+            need = std::max(need, leco_fix_stream_bound(c.n, c.bs, c.width));
+            nbmax = std::max(nbmax, (c.n + c.bs - 1) / c.bs);
+        }
+        const uint64_t offs_at = (need + 255) / 256 * 256, prm_at = offs_at + ((nbmax + 1) * 8 + 255) / 256 * 256;
+        const uint64_t total = prm_at + nbmax * sizeof(leco_fix_block_param);
+        if (total > b->scratch_cap) {
+            (void)hipFree(b->d_scratch);
+            b->d_scratch = nullptr;
+            b->scratch_cap = 0;
+            if (hipMalloc(&b->d_scratch, total) != hipSuccess) return LECO_FIX_ENOMEM;
+            b->scratch_cap = total;
+        }
+        hipStream_t st = (hipStream_t)stream;
+        uint64_t *offs = (uint64_t *)(b->d_scratch + offs_at);
+        leco_fix_block_param *prm = (leco_fix_block_param *)(b->d_scratch + prm_at);
+        for (const BatchColumn &c : b->h_cols) {
+            hipError_t e = c.width == 64
+                               ? leco::launch_encode<uint64_t>((const uint64_t *)c.in, c.n, c.bs, b->d_scratch, offs,
+                                                               prm, st)
+                               : leco::launch_encode<uint32_t>((const uint32_t *)c.in, c.n, c.bs, b->d_scratch, offs,
+                                                               prm, st);
+            if (e != hipSuccess) return LECO_FIX_EHIP;
+        }
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_batch_set_output(leco_fix_batch *b, uint32_t c, void *d_out) {
-    if (!b || c >= b->ncols) return LECO_FIX_EINVAL;
-    b->h_cols[c].out = d_out;
-    return hip_status(hipMemcpy(&b->d_cols[c], &b->h_cols[c], sizeof(BatchColumn), hipMemcpyHostToDevice));
+    return leco::abi_guard([&]() -> int {
+        if (!b || c >= b->ncols) return LECO_FIX_EINVAL;
+        b->h_cols[c].out = d_out;
+        return hip_status(hipMemcpy(&b->d_cols[c], &b->h_cols[c], sizeof(BatchColumn), hipMemcpyHostToDevice));
+    });
 }
 
 int leco_fix_batch_set_decode_grid(leco_fix_batch *b, uint32_t wgs) {
-    if (!b) return LECO_FIX_EINVAL;
-    b->decode_grid = wgs;  // 0: one workgroup per tile
-    if (b->graph[1]) {  // the captured launch has the old grid baked in
-        (void)hipGraphExecDestroy(b->graph[1]);
-        b->graph[1] = nullptr;
-    }
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!b) return LECO_FIX_EINVAL;
+        b->decode_grid = wgs;  // 0: one workgroup per tile
+        if (b->graph[1]) {  // the captured launch has the old grid baked in
+            (void)hipGraphExecDestroy(b->graph[1]);
+            b->graph[1] = nullptr;
+        }
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_batch_info(const leco_fix_batch *b, uint64_t *total, uint64_t *cap, uint8_t **arena, uint64_t **offs,
                         leco_fix_block_param **prm) {
-    if (!b) return LECO_FIX_EINVAL;
-    if (total) *total = b->total;
-    if (cap) *cap = b->arena_cap;
-    if (arena) *arena = b->d_arena;
-    if (offs) *offs = b->d_offs;
-    if (prm) *prm = b->d_prm;
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!b) return LECO_FIX_EINVAL;
+        if (total) *total = b->total;
+        if (cap) *cap = b->arena_cap;
+        if (arena) *arena = b->d_arena;
+        if (offs) *offs = b->d_offs;
+        if (prm) *prm = b->d_prm;
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_batch_column(const leco_fix_batch *b, uint32_t c, uint64_t *first, uint64_t *nb) {
-    if (!b || c >= b->ncols) return LECO_FIX_EINVAL;
-    if (first) *first = b->h_first[c];
-    if (nb) *nb = b->h_first[c + 1] - b->h_first[c];
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!b || c >= b->ncols) return LECO_FIX_EINVAL;
+        if (first) *first = b->h_first[c];
+        if (nb) *nb = b->h_first[c + 1] - b->h_first[c];
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_autotune_block_sizes(const leco_fix_column_desc *cols, uint32_t ncols, const uint32_t *cand,
                                   uint32_t ncand, uint32_t windows, uint32_t window, double tol, uint32_t *out_bs,
                                   uint64_t *out_bytes, void *stream) {
-    if (!cols || !cand || !out_bs || ncols == 0 || ncand == 0 || windows == 0 || window == 0) return LECO_FIX_EINVAL;
-    for (uint32_t j = 0; j < ncand; ++j)
-        if (cand[j] == 0 || cand[j] > leco::kMaxBlockSize || window % cand[j]) return LECO_FIX_EINVAL;
-    // tuning batch: column c, window k, candidate j -> one entry
-    std::vector<leco_fix_column_desc> tune;
-    std::vector<uint32_t> owner;
-    for (uint32_t c = 0; c < ncols; ++c) {
-        const leco_fix_column_desc &d = cols[c];
-        if (!d.d_in || !valid_desc(d)) return LECO_FIX_EINVAL;
-        const uint64_t slots = d.n / window;  // whole windows, aligned to every candidate
-        const uint32_t nw = slots == 0 ? 1 : (uint32_t)std::min<uint64_t>(windows, slots);
-        for (uint32_t k = 0; k < nw; ++k) {
-            const uint64_t start = slots == 0 ? 0 : (slots * k / nw) * window;
-            const uint64_t len = slots == 0 ? d.n : window;
-            for (uint32_t j = 0; j < ncand; ++j) {
-                tune.push_back({(const uint8_t *)d.d_in + start * (d.width_bits / 8), nullptr, len, cand[j],
-                                d.width_bits});
-                owner.push_back(c * ncand + j);
+    return leco::abi_guard([&]() -> int {
+        if (!cols || !cand || !out_bs || ncols == 0 || ncand == 0 || windows == 0 || window == 0)
+            return LECO_FIX_EINVAL;
+        for (uint32_t j = 0; j < ncand; ++j)
+            if (cand[j] == 0 || cand[j] > leco::kMaxBlockSize || window % cand[j]) return LECO_FIX_EINVAL;
+        // tuning batch: column c, window k, candidate j -> one entry
+        std::vector<leco_fix_column_desc> tune;
+        std::vector<uint32_t> owner;
+        for (uint32_t c = 0; c < ncols; ++c) {
+            const leco_fix_column_desc &d = cols[c];
+            if (!d.d_in || !valid_desc(d)) return LECO_FIX_EINVAL;
+            const uint64_t slots = d.n / window;  // whole windows, aligned to every candidate
+            const uint32_t nw = slots == 0 ? 1 : (uint32_t)std::min<uint64_t>(windows, slots);
+            for (uint32_t k = 0; k < nw; ++k) {
+                const uint64_t start = slots == 0 ? 0 : (slots * k / nw) * window;
+                const uint64_t len = slots == 0 ? d.n : window;
+                for (uint32_t j = 0; j < ncand; ++j) {
+                    tune.push_back({(const uint8_t *)d.d_in + start * (d.width_bits / 8), nullptr, len, cand[j],
+                                    d.width_bits});
+                    owner.push_back(c * ncand + j);
+                }
             }
         }
-    }
-    leco_fix_batch *b = nullptr;
-    int rc = leco_fix_batch_create(tune.data(), (uint32_t)tune.size(), &b);
-    if (rc) return rc;
-    hipStream_t st = (hipStream_t)stream;
-    rc = leco_fix_batch_encode(b, st);
-    std::vector<uint64_t> offs(b->total + 1);
-    if (rc == LECO_FIX_OK &&
-        (hipMemcpyAsync(offs.data(), b->d_offs, offs.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st) !=
-             hipSuccess ||
-         hipStreamSynchronize(st) != hipSuccess))
-        rc = LECO_FIX_EHIP;
-    if (rc == LECO_FIX_OK) {
-        std::vector<uint64_t> bytes((size_t)ncols * ncand, 0);
-        for (size_t e = 0; e < tune.size(); ++e) bytes[owner[e]] += offs[b->h_first[e + 1]] - offs[b->h_first[e]];
-        for (uint32_t c = 0; c < ncols; ++c) {
-            uint64_t best = UINT64_MAX;
-            for (uint32_t j = 0; j < ncand; ++j) best = std::min(best, bytes[(size_t)c * ncand + j]);
-            uint32_t pick = 0;
-            for (uint32_t j = 0; j < ncand; ++j)
-                if ((double)bytes[(size_t)c * ncand + j] <= (double)best * (1.0 + tol) && cand[j] > pick)
-                    pick = cand[j];
-            out_bs[c] = pick;
-            if (out_bytes)
-                for (uint32_t j = 0; j < ncand; ++j) out_bytes[(size_t)c * ncand + j] = bytes[(size_t)c * ncand + j];
+        leco_fix_batch *b = nullptr;
+        int rc = leco_fix_batch_create(tune.data(), (uint32_t)tune.size(), &b);
+        if (rc) return rc;
+        const std::unique_ptr<leco_fix_batch, void (*)(leco_fix_batch *)> hold(b, free_batch);  // freed on any exit
+        hipStream_t st = (hipStream_t)stream;
+        rc = leco_fix_batch_encode(b, st);
+        std::vector<uint64_t> offs(b->total + 1);
+        if (rc == LECO_FIX_OK &&
+            (hipMemcpyAsync(offs.data(), b->d_offs, offs.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st) !=
+                 hipSuccess ||
+             hipStreamSynchronize(st) != hipSuccess))
+            rc = LECO_FIX_EHIP;
+        if (rc == LECO_FIX_OK) {
+            std::vector<uint64_t> bytes((size_t)ncols * ncand, 0);
+            for (size_t e = 0; e < tune.size(); ++e) bytes[owner[e]] += offs[b->h_first[e + 1]] - offs[b->h_first[e]];
+            for (uint32_t c = 0; c < ncols; ++c) {
+                uint64_t best = UINT64_MAX;
+                for (uint32_t j = 0; j < ncand; ++j) best = std::min(best, bytes[(size_t)c * ncand + j]);
+                uint32_t pick = 0;
+                for (uint32_t j = 0; j < ncand; ++j)
+                    if ((double)bytes[(size_t)c * ncand + j] <= (double)best * (1.0 + tol) && cand[j] > pick)
+                        pick = cand[j];
+                out_bs[c] = pick;
+                if (out_bytes)
+                    for (uint32_t j = 0; j < ncand; ++j)
+                        out_bytes[(size_t)c * ncand + j] = bytes[(size_t)c * ncand + j];
+            }
         }
-    }
-    free_batch(b);
-    return rc;
+        return rc;
+    });
 }
 
 }  // extern "C"

@@ -5,9 +5,28 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <new>
+#include <stdexcept>
+
 #include "leco_fix.h"
 
 namespace leco {
+
+// C ABI entry points whose host code allocates (std containers) run their
+// bodies through this: a C++ exception must not unwind into a C caller.
+// Allocation failures (and sizes past a container's max_size) are ENOMEM.
+template <class F>
+inline int abi_guard(F &&body) noexcept {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        return LECO_FIX_ENOMEM;
+    } catch (const std::length_error &) {
+        return LECO_FIX_ENOMEM;
+    } catch (...) {
+        return LECO_FIX_EINVAL;
+    }
+}
 
 // Largest block size the fit supports: sum(i) and sum(i*i) over a block are
 // exact doubles up to here, so they are formed in closed form.

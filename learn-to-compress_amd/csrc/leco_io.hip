@@ -290,133 +290,149 @@ int read_header(int fd, FileHeader *h) {
 extern "C" {
 
 int leco_fix_sosd_info(const char *path, uint32_t width_bits, uint64_t *count) {
-    if (!path || !count || (width_bits != 32 && width_bits != 64)) return LECO_FIX_EINVAL;
-    Fd f;
-    f.fd = open(path, O_RDONLY | O_CLOEXEC);
-    if (f.fd < 0) return LECO_FIX_EINVAL;
-    uint64_t c = 0;
-    struct stat sb;
-    if (!read_full(f.fd, &c, 8, 0) || fstat(f.fd, &sb) != 0) return LECO_FIX_EINVAL;
-    if ((uint64_t)sb.st_size < 8 || ((uint64_t)sb.st_size - 8) / (width_bits / 8) < c) return LECO_FIX_EINVAL;
-    *count = c;
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!path || !count || (width_bits != 32 && width_bits != 64)) return LECO_FIX_EINVAL;
+        Fd f;
+        f.fd = open(path, O_RDONLY | O_CLOEXEC);
+        if (f.fd < 0) return LECO_FIX_EINVAL;
+        uint64_t c = 0;
+        struct stat sb;
+        if (!read_full(f.fd, &c, 8, 0) || fstat(f.fd, &sb) != 0) return LECO_FIX_EINVAL;
+        if ((uint64_t)sb.st_size < 8 || ((uint64_t)sb.st_size - 8) / (width_bits / 8) < c) return LECO_FIX_EINVAL;
+        *count = c;
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_sosd_load(const char *path, uint32_t width_bits, void *d_out, uint64_t capacity, uint64_t *count,
                        void *stream) {
-    uint64_t c = 0;
-    int rc = leco_fix_sosd_info(path, width_bits, &c);
-    if (rc) return rc;
-    if (!d_out || c > capacity) return LECO_FIX_EINVAL;
-    Fd f;
-    f.fd = open(path, O_RDONLY | O_CLOEXEC);
-    if (f.fd < 0) return LECO_FIX_EINVAL;
-    rc = file_to_device(f.fd, 8, (uint8_t *)d_out, c * (width_bits / 8), (hipStream_t)stream);
-    if (rc == LECO_FIX_OK && count) *count = c;
-    return rc;
+    return leco::abi_guard([&]() -> int {
+        uint64_t c = 0;
+        int rc = leco_fix_sosd_info(path, width_bits, &c);
+        if (rc) return rc;
+        if (!d_out || c > capacity) return LECO_FIX_EINVAL;
+        Fd f;
+        f.fd = open(path, O_RDONLY | O_CLOEXEC);
+        if (f.fd < 0) return LECO_FIX_EINVAL;
+        rc = file_to_device(f.fd, 8, (uint8_t *)d_out, c * (width_bits / 8), (hipStream_t)stream);
+        if (rc == LECO_FIX_OK && count) *count = c;
+        return rc;
+    });
 }
 
 int leco_fix_sosd_save(const char *path, uint32_t width_bits, const void *d_in, uint64_t count, void *stream) {
-    if (!path || (width_bits != 32 && width_bits != 64) || (count && !d_in)) return LECO_FIX_EINVAL;
-    Fd f;
-    f.fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
-    if (f.fd < 0) return LECO_FIX_EINVAL;
-    if (!write_full(f.fd, &count, 8, 0)) return LECO_FIX_EINVAL;
-    return device_to_file((const uint8_t *)d_in, count * (width_bits / 8), f.fd, 8, (hipStream_t)stream);
+    return leco::abi_guard([&]() -> int {
+        if (!path || (width_bits != 32 && width_bits != 64) || (count && !d_in)) return LECO_FIX_EINVAL;
+        Fd f;
+        f.fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+        if (f.fd < 0) return LECO_FIX_EINVAL;
+        if (!write_full(f.fd, &count, 8, 0)) return LECO_FIX_EINVAL;
+        return device_to_file((const uint8_t *)d_in, count * (width_bits / 8), f.fd, 8, (hipStream_t)stream);
+    });
 }
 
 int leco_fix_text_info(const char *path, uint32_t width_bits, uint64_t *count) {
-    return text_pass(path, width_bits, nullptr, 0, count, nullptr);
+    return leco::abi_guard([&]() -> int {
+        return text_pass(path, width_bits, nullptr, 0, count, nullptr);
+    });
 }
 
 int leco_fix_text_load(const char *path, uint32_t width_bits, void *d_out, uint64_t capacity, uint64_t *count,
                        void *stream) {
-    if (!d_out || !count) return LECO_FIX_EINVAL;
-    return text_pass(path, width_bits, (uint8_t *)d_out, capacity, count, (hipStream_t)stream);
+    return leco::abi_guard([&]() -> int {
+        if (!d_out || !count) return LECO_FIX_EINVAL;
+        return text_pass(path, width_bits, (uint8_t *)d_out, capacity, count, (hipStream_t)stream);
+    });
 }
 
 int leco_fix_column_file_info(const char *path, leco_fix_file_info *info) {
-    if (!path || !info) return LECO_FIX_EINVAL;
-    Fd f;
-    f.fd = open(path, O_RDONLY | O_CLOEXEC);
-    if (f.fd < 0) return LECO_FIX_EINVAL;
-    FileHeader h;
-    int rc = read_header(f.fd, &h);
-    if (rc) return rc;
-    info->n = h.n;
-    info->num_blocks = h.num_blocks;
-    info->stream_bytes = h.stream_bytes;
-    info->block_size = h.block_size;
-    info->width_bits = h.width_bits;
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!path || !info) return LECO_FIX_EINVAL;
+        Fd f;
+        f.fd = open(path, O_RDONLY | O_CLOEXEC);
+        if (f.fd < 0) return LECO_FIX_EINVAL;
+        FileHeader h;
+        int rc = read_header(f.fd, &h);
+        if (rc) return rc;
+        info->n = h.n;
+        info->num_blocks = h.num_blocks;
+        info->stream_bytes = h.stream_bytes;
+        info->block_size = h.block_size;
+        info->width_bits = h.width_bits;
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_column_save(const char *path, const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
                          uint32_t block_size, uint32_t width_bits, void *stream) {
-    if (!path || !d_stream || !d_offsets || n == 0 || block_size == 0 || block_size > leco::kMaxBlockSize ||
-        (width_bits != 32 && width_bits != 64))
-        return LECO_FIX_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
-    const uint64_t nb = (n + block_size - 1) / block_size;
-    std::vector<uint64_t> offs(nb + 1);
-    if (hipMemcpyAsync(offs.data(), d_offsets, (nb + 1) * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return LECO_FIX_EHIP;
-    // offsets may be absolute (a column inside a batch arena): store them from 0
-    const uint64_t base = offs[0];
-    for (uint64_t b = 0; b <= nb; ++b) {
-        if (offs[b] < base || (b && offs[b] < offs[b - 1])) return LECO_FIX_EINVAL;
-        offs[b] -= base;
-    }
-    FileHeader h{};
-    memcpy(h.magic, kMagic, 8);
-    h.version = kFileVersion;
-    h.width_bits = width_bits;
-    h.n = n;
-    h.block_size = block_size;
-    h.num_blocks = nb;
-    h.stream_bytes = offs[nb];
-    h.check = fnv1a(&h, 48);
-    Fd f;
-    f.fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
-    if (f.fd < 0) return LECO_FIX_EINVAL;
-    const uint64_t start = stream_start(nb);
-    std::vector<uint8_t> pad(start - 64 - 8 * (nb + 1), 0);
-    if (!write_full(f.fd, &h, sizeof(h), 0) || !write_full(f.fd, offs.data(), 8 * (nb + 1), 64) ||
-        (!pad.empty() && !write_full(f.fd, pad.data(), pad.size(), 64 + 8 * (nb + 1))))
-        return LECO_FIX_EINVAL;
-    return device_to_file(d_stream + base, h.stream_bytes, f.fd, start, st);
+    return leco::abi_guard([&]() -> int {
+        if (!path || !d_stream || !d_offsets || n == 0 || block_size == 0 || block_size > leco::kMaxBlockSize ||
+            (width_bits != 32 && width_bits != 64))
+            return LECO_FIX_EINVAL;
+        hipStream_t st = (hipStream_t)stream;
+        const uint64_t nb = (n + block_size - 1) / block_size;
+        std::vector<uint64_t> offs(nb + 1);
+        if (hipMemcpyAsync(offs.data(), d_offsets, (nb + 1) * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return LECO_FIX_EHIP;
+        // offsets may be absolute (a column inside a batch arena): store them from 0
+        const uint64_t base = offs[0];
+        for (uint64_t b = 0; b <= nb; ++b) {
+            if (offs[b] < base || (b && offs[b] < offs[b - 1])) return LECO_FIX_EINVAL;
+            offs[b] -= base;
+        }
+        FileHeader h{};
+        memcpy(h.magic, kMagic, 8);
+        h.version = kFileVersion;
+        h.width_bits = width_bits;
+        h.n = n;
+        h.block_size = block_size;
+        h.num_blocks = nb;
+        h.stream_bytes = offs[nb];
+        h.check = fnv1a(&h, 48);
+        Fd f;
+        f.fd = open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
+        if (f.fd < 0) return LECO_FIX_EINVAL;
+        const uint64_t start = stream_start(nb);
+        std::vector<uint8_t> pad(start - 64 - 8 * (nb + 1), 0);
+        if (!write_full(f.fd, &h, sizeof(h), 0) || !write_full(f.fd, offs.data(), 8 * (nb + 1), 64) ||
+            (!pad.empty() && !write_full(f.fd, pad.data(), pad.size(), 64 + 8 * (nb + 1))))
+            return LECO_FIX_EINVAL;
+        return device_to_file(d_stream + base, h.stream_bytes, f.fd, start, st);
+    });
 }
 
 int leco_fix_column_load(const char *path, uint8_t *d_stream, uint64_t stream_capacity, uint64_t *d_offsets,
                          leco_fix_block_param *d_params, void *stream) {
-    if (!path || !d_stream || !d_offsets) return LECO_FIX_EINVAL;
-    Fd f;
-    f.fd = open(path, O_RDONLY | O_CLOEXEC);
-    if (f.fd < 0) return LECO_FIX_EINVAL;
-    FileHeader h;
-    int rc = read_header(f.fd, &h);
-    if (rc) return rc;
-    if (stream_capacity < h.stream_bytes + LECO_FIX_TAIL_PAD) return LECO_FIX_EINVAL;
-    const uint64_t nb = h.num_blocks;
-    std::vector<uint64_t> offs(nb + 1);
-    if (!read_full(f.fd, offs.data(), 8 * (nb + 1), 64)) return LECO_FIX_EINVAL;
-    if (offs[0] != 0 || offs[nb] != h.stream_bytes) return LECO_FIX_EINVAL;
-    for (uint64_t b = 1; b <= nb; ++b)
-        if (offs[b] < offs[b - 1]) return LECO_FIX_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
-    if (hipMemcpyAsync(d_offsets, offs.data(), 8 * (nb + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemsetAsync(d_stream + h.stream_bytes, 0, LECO_FIX_TAIL_PAD, st) != hipSuccess)
-        return LECO_FIX_EHIP;
-    rc = file_to_device(f.fd, stream_start(nb), d_stream, h.stream_bytes, st);
-    if (rc) return rc;
-    if (d_params) {
-        if (leco::launch_build_params(d_stream, d_offsets, h.n, h.block_size, h.width_bits, d_params, st) !=
-            hipSuccess)
+    return leco::abi_guard([&]() -> int {
+        if (!path || !d_stream || !d_offsets) return LECO_FIX_EINVAL;
+        Fd f;
+        f.fd = open(path, O_RDONLY | O_CLOEXEC);
+        if (f.fd < 0) return LECO_FIX_EINVAL;
+        FileHeader h;
+        int rc = read_header(f.fd, &h);
+        if (rc) return rc;
+        if (stream_capacity < h.stream_bytes + LECO_FIX_TAIL_PAD) return LECO_FIX_EINVAL;
+        const uint64_t nb = h.num_blocks;
+        std::vector<uint64_t> offs(nb + 1);
+        if (!read_full(f.fd, offs.data(), 8 * (nb + 1), 64)) return LECO_FIX_EINVAL;
+        if (offs[0] != 0 || offs[nb] != h.stream_bytes) return LECO_FIX_EINVAL;
+        for (uint64_t b = 1; b <= nb; ++b)
+            if (offs[b] < offs[b - 1]) return LECO_FIX_EINVAL;
+        hipStream_t st = (hipStream_t)stream;
+        if (hipMemcpyAsync(d_offsets, offs.data(), 8 * (nb + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemsetAsync(d_stream + h.stream_bytes, 0, LECO_FIX_TAIL_PAD, st) != hipSuccess)
             return LECO_FIX_EHIP;
-        rc = hip_status(hipStreamSynchronize(st));
-    }
-    return rc;
+        rc = file_to_device(f.fd, stream_start(nb), d_stream, h.stream_bytes, st);
+        if (rc) return rc;
+        if (d_params) {
+            if (leco::launch_build_params(d_stream, d_offsets, h.n, h.block_size, h.width_bits, d_params, st) !=
+                hipSuccess)
+                return LECO_FIX_EHIP;
+            rc = hip_status(hipStreamSynchronize(st));
+        }
+        return rc;
+    });
 }
 
 }  // extern "C"

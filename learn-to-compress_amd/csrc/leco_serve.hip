@@ -381,422 +381,453 @@ void free_column(leco_fix_server *s, uint32_t ci) {
 extern "C" {
 
 int leco_fix_server_create(const leco_fix_server_config *cfg_in, leco_fix_server **out) {
-    if (!out) return LECO_FIX_EINVAL;
-    leco_fix_server *s = new (std::nothrow) leco_fix_server;
-    if (!s) return LECO_FIX_ENOMEM;
-    if (cfg_in) s->cfg = *cfg_in;
-    leco_fix_server_config &cfg = s->cfg;
-    if (!cfg.slots) cfg.slots = 4;
-    if (!cfg.max_batch_requests) cfg.max_batch_requests = 8192;
-    if (!cfg.max_batch_values) cfg.max_batch_values = 1ull << 26;
-    size_t freeb = 0, totalb = 0;
-    if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) { destroy_server(s); return LECO_FIX_EHIP; }
-    const uint64_t staging = (uint64_t)cfg.slots * (kIdxCap * 8 + kItemCap * sizeof(leco::ServeItem)) + (64ull << 20);
-    if (freeb < cfg.reserve_bytes + staging) { destroy_server(s); return LECO_FIX_ENOMEM; }
-    const uint64_t avail = freeb - cfg.reserve_bytes - staging;
-    if (!cfg.result_pool_bytes) cfg.result_pool_bytes = cfg.column_pool_bytes ? avail - std::min(avail, cfg.column_pool_bytes) : avail / 2;
-    if (!cfg.column_pool_bytes) cfg.column_pool_bytes = avail - std::min(avail, cfg.result_pool_bytes);
-    if (cfg.result_pool_bytes + cfg.column_pool_bytes > avail) { destroy_server(s); return LECO_FIX_ENOMEM; }
-    s->result_cap = cfg.result_pool_bytes;
-    s->column_cap = cfg.column_pool_bytes;
-    s->cols.resize(kMaxColumns);
-    bool ok = hipMalloc(&s->d_cols, kMaxColumns * sizeof(leco::ServeColumn)) == hipSuccess &&
-              (s->result_cap == 0 || hipMalloc(&s->result_pool, s->result_cap) == hipSuccess) &&
-              (s->column_cap == 0 || hipMalloc(&s->column_pool, s->column_cap) == hipSuccess) &&
-              hipStreamCreateWithFlags(&s->enc, hipStreamNonBlocking) == hipSuccess;
-    s->slots.resize(cfg.slots);
-    for (Slot &sl : s->slots) {
-        ok = ok && hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
-             hipHostMalloc((void **)&sl.h_idx, kIdxCap * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess &&
-             hipHostMalloc((void **)&sl.h_items, kItemCap * sizeof(leco::ServeItem), hipHostMallocDefault) ==
-                 hipSuccess &&
-             hipMalloc(&sl.d_idx, kIdxCap * sizeof(uint64_t)) == hipSuccess &&
-             hipMalloc(&sl.d_items, kItemCap * sizeof(leco::ServeItem)) == hipSuccess;
-    }
-    if (!ok) { destroy_server(s); return LECO_FIX_ENOMEM; }
-    const unsigned hw = std::thread::hardware_concurrency();
-    if (!s->copies.start(hw > 1 ? std::min(kCopyHelpers, hw - 1) : 0)) {
-        destroy_server(s);
-        return LECO_FIX_ENOMEM;
-    }
-    s->ring.init(s->result_cap);
-    s->colfree.init(s->column_cap);
-    *out = s;
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!out) return LECO_FIX_EINVAL;
+        leco_fix_server *s = new (std::nothrow) leco_fix_server;
+        if (!s) return LECO_FIX_ENOMEM;
+        if (cfg_in) s->cfg = *cfg_in;
+        leco_fix_server_config &cfg = s->cfg;
+        if (!cfg.slots) cfg.slots = 4;
+        if (!cfg.max_batch_requests) cfg.max_batch_requests = 8192;
+        if (!cfg.max_batch_values) cfg.max_batch_values = 1ull << 26;
+        size_t freeb = 0, totalb = 0;
+        if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) { destroy_server(s); return LECO_FIX_EHIP; }
+        const uint64_t staging =
+            (uint64_t)cfg.slots * (kIdxCap * 8 + kItemCap * sizeof(leco::ServeItem)) + (64ull << 20);
+        if (freeb < cfg.reserve_bytes + staging) { destroy_server(s); return LECO_FIX_ENOMEM; }
+        const uint64_t avail = freeb - cfg.reserve_bytes - staging;
+        if (!cfg.result_pool_bytes)
+            cfg.result_pool_bytes = cfg.column_pool_bytes ? avail - std::min(avail, cfg.column_pool_bytes) : avail / 2;
+        if (!cfg.column_pool_bytes) cfg.column_pool_bytes = avail - std::min(avail, cfg.result_pool_bytes);
+        if (cfg.result_pool_bytes + cfg.column_pool_bytes > avail) { destroy_server(s); return LECO_FIX_ENOMEM; }
+        s->result_cap = cfg.result_pool_bytes;
+        s->column_cap = cfg.column_pool_bytes;
+        try {
+            s->cols.resize(kMaxColumns);
+            s->slots.resize(cfg.slots);
+        } catch (...) {
+            destroy_server(s);
+            return LECO_FIX_ENOMEM;
+        }
+        bool ok = hipMalloc(&s->d_cols, kMaxColumns * sizeof(leco::ServeColumn)) == hipSuccess &&
+                  (s->result_cap == 0 || hipMalloc(&s->result_pool, s->result_cap) == hipSuccess) &&
+                  (s->column_cap == 0 || hipMalloc(&s->column_pool, s->column_cap) == hipSuccess) &&
+                  hipStreamCreateWithFlags(&s->enc, hipStreamNonBlocking) == hipSuccess;
+        for (Slot &sl : s->slots) {
+            ok = ok && hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
+                 hipHostMalloc((void **)&sl.h_idx, kIdxCap * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess &&
+                 hipHostMalloc((void **)&sl.h_items, kItemCap * sizeof(leco::ServeItem), hipHostMallocDefault) ==
+                     hipSuccess &&
+                 hipMalloc(&sl.d_idx, kIdxCap * sizeof(uint64_t)) == hipSuccess &&
+                 hipMalloc(&sl.d_items, kItemCap * sizeof(leco::ServeItem)) == hipSuccess;
+        }
+        if (!ok) { destroy_server(s); return LECO_FIX_ENOMEM; }
+        const unsigned hw = std::thread::hardware_concurrency();
+        if (!s->copies.start(hw > 1 ? std::min(kCopyHelpers, hw - 1) : 0)) {
+            destroy_server(s);
+            return LECO_FIX_ENOMEM;
+        }
+        s->ring.init(s->result_cap);
+        s->colfree.init(s->column_cap);
+        *out = s;
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_server_destroy(leco_fix_server *s) {
-    if (!s) return LECO_FIX_EINVAL;
-    destroy_server(s);
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!s) return LECO_FIX_EINVAL;
+        destroy_server(s);
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_server_attach_column(leco_fix_server *s, uint32_t c, const uint8_t *st, const uint64_t *offs,
                                   const leco_fix_block_param *prm, uint64_t n, uint32_t bs, uint32_t w) {
-    if (!s || c >= kMaxColumns || !st || !offs || !prm || n == 0 || bs == 0 || bs > leco::kMaxBlockSize ||
-        (w != 32 && w != 64) || s->cols[c].present)
-        return LECO_FIX_EINVAL;
-    ColumnSlot &cs = s->cols[c];
-    cs.col = leco::ServeColumn{st, offs, prm, n, bs, w};
-    cs.present = cs.ready = true;
-    cs.owned = false;
-    int rc = upload_column(s, c, s->enc);
-    if (rc) return rc;
-    return hipStreamSynchronize(s->enc) == hipSuccess ? LECO_FIX_OK : LECO_FIX_EHIP;
+    return leco::abi_guard([&]() -> int {
+        if (!s || c >= kMaxColumns || !st || !offs || !prm || n == 0 || bs == 0 || bs > leco::kMaxBlockSize ||
+            (w != 32 && w != 64) || s->cols[c].present)
+            return LECO_FIX_EINVAL;
+        ColumnSlot &cs = s->cols[c];
+        cs.col = leco::ServeColumn{st, offs, prm, n, bs, w};
+        cs.present = cs.ready = true;
+        cs.owned = false;
+        int rc = upload_column(s, c, s->enc);
+        if (rc) return rc;
+        return hipStreamSynchronize(s->enc) == hipSuccess ? LECO_FIX_OK : LECO_FIX_EHIP;
+    });
 }
 
 int leco_fix_server_column(const leco_fix_server *s, uint32_t c, const uint8_t **st, const uint64_t **offs,
                            const leco_fix_block_param **prm, uint64_t *n, uint32_t *bs, uint32_t *w,
                            uint64_t *bytes) {
-    if (!s || c >= kMaxColumns || !s->cols[c].present || !s->cols[c].ready) return LECO_FIX_EINVAL;
-    const ColumnSlot &cs = s->cols[c];
-    if (st) *st = cs.col.stream;
-    if (offs) *offs = cs.col.offsets;
-    if (prm) *prm = cs.col.params;
-    if (n) *n = cs.col.n;
-    if (bs) *bs = cs.col.bs;
-    if (w) *w = cs.col.width;
-    if (bytes) *bytes = cs.stream_bytes;
-    return LECO_FIX_OK;
+    return leco::abi_guard([&]() -> int {
+        if (!s || c >= kMaxColumns || !s->cols[c].present || !s->cols[c].ready) return LECO_FIX_EINVAL;
+        const ColumnSlot &cs = s->cols[c];
+        if (st) *st = cs.col.stream;
+        if (offs) *offs = cs.col.offsets;
+        if (prm) *prm = cs.col.params;
+        if (n) *n = cs.col.n;
+        if (bs) *bs = cs.col.bs;
+        if (w) *w = cs.col.width;
+        if (bytes) *bytes = cs.stream_bytes;
+        return LECO_FIX_OK;
+    });
 }
 
 int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64_t nreq, int mode,
                         leco_fix_request_result *res, leco_fix_server_stats *stats) {
-    if (!s || (!reqs && nreq) || !res || (mode != LECO_FIX_SERVE_BATCHED && mode != LECO_FIX_SERVE_PLAIN))
-        return LECO_FIX_EINVAL;
-    for (uint64_t i = 0; i < nreq; ++i) {
-        const leco_fix_request &r = reqs[i];
-        if (r.column >= kMaxColumns || r.kind > LECO_FIX_REQ_EVICT || (i && r.arrival_s < reqs[i - 1].arrival_s))
+    return leco::abi_guard([&]() -> int {
+        if (!s || (!reqs && nreq) || !res || (mode != LECO_FIX_SERVE_BATCHED && mode != LECO_FIX_SERVE_PLAIN))
             return LECO_FIX_EINVAL;
-        if (r.kind == LECO_FIX_REQ_LOOKUP && (!r.h_idx || r.count == 0 || r.count > kIdxCap)) return LECO_FIX_EINVAL;
-        if (r.kind == LECO_FIX_REQ_RANGE && (r.count == 0 || r.count > s->cfg.max_batch_values)) return LECO_FIX_EINVAL;
-        if (r.kind == LECO_FIX_REQ_INSERT && !valid_insert(r)) return LECO_FIX_EINVAL;
-        res[i] = leco_fix_request_result{0, 0, 0, -1, 0};
-    }
-    leco_fix_server_stats st{};
-    const Clock::time_point t0 = Clock::now();
-    auto now_s = [&]() { return secs(t0, Clock::now()); };
-    uint64_t done = 0, next = 0, batch_id = 0;
-    double last_done = 0;
-
-    auto complete = [&](uint64_t ri, double t, int status) {
-        res[ri].latency_s = t - reqs[ri].arrival_s;
-        res[ri].status = status;
-        if (status == LECO_FIX_OK) st.completed++;
-        else st.failed++;
-        last_done = std::max(last_done, t);
-        ++done;
-    };
-    auto ensure_events = [&](ColumnSlot &c) -> bool {
-        if (!c.e0 && hipEventCreate(&c.e0) != hipSuccess) return false;
-        if (!c.e1 && hipEventCreate(&c.e1) != hipSuccess) return false;
-        if (!c.h_total && hipHostMalloc((void **)&c.h_total, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
-            return false;
-        return true;
-    };
-    auto finish_insert = [&](uint32_t ci, double t) {
-        ColumnSlot &c = s->cols[ci];
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, c.e0, c.e1);
-        st.encode_gpu_s += ms * 1e-3;
-        st.encoded_values += c.col.n;
-        st.encoded_bytes += c.col.n * (c.col.width / 8);
-        c.stream_bytes = *c.h_total;
-        // shrink the column's storage to what the stream needs
-        const uint64_t keep = up(c.prefix + c.stream_bytes + LECO_FIX_TAIL_PAD, kAlign);
-        if (keep < c.pool_len) {
-            s->colfree.release(c.pool_off + keep, c.pool_len - keep);
-            c.pool_len = keep;
-        }
-        c.ready = true;
-        complete(c.insert_req, t, LECO_FIX_OK);
-        c.insert_req = UINT64_MAX;
-    };
-    auto try_evict = [&](uint32_t ci, double t) {
-        ColumnSlot &c = s->cols[ci];
-        if (c.evicting && c.refs == 0 && c.ready) {
-            const uint64_t ri = c.evict_req;
-            free_column(s, ci);
-            c.evict_req = UINT64_MAX;
-            complete(ri, t, LECO_FIX_OK);
-        }
-    };
-
-    if (mode == LECO_FIX_SERVE_PLAIN) {
-        Slot &sl = s->slots[0];
         for (uint64_t i = 0; i < nreq; ++i) {
             const leco_fix_request &r = reqs[i];
-            while (now_s() < r.arrival_s) {
-            }
-            const double ts = now_s();
-            res[i].queued_s = ts - r.arrival_s;
-            res[i].batch = batch_id++;
-            ColumnSlot &c = s->cols[r.column];
-            int status = LECO_FIX_OK;
-            if (r.kind == LECO_FIX_REQ_INSERT) {
-                if (c.present || !ensure_events(c) || !start_insert(s, r, i)) { complete(i, now_s(), LECO_FIX_ENOMEM); continue; }
-                (void)hipStreamSynchronize(s->enc);
-                finish_insert(r.column, now_s());
-                continue;
-            }
-            if (r.kind == LECO_FIX_REQ_EVICT) {
-                if (!c.present) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
-                c.evicting = true;
-                c.evict_req = i;
-                try_evict(r.column, now_s());
-                continue;
-            }
-            if (!c.present || !c.ready || c.evicting) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
-            const uint64_t bytes = result_bytes(s, r);
-            if (bytes > s->result_cap) { complete(i, now_s(), LECO_FIX_ENOMEM); continue; }
-            void *out = s->result_pool;
-            hipError_t e;
-            if (r.kind == LECO_FIX_REQ_LOOKUP) {
-                std::memcpy(sl.h_idx, r.h_idx, r.count * sizeof(uint64_t));
-                e = hipMemcpyAsync(sl.d_idx, sl.h_idx, r.count * sizeof(uint64_t), hipMemcpyHostToDevice, sl.st);
-                if (e == hipSuccess)
-                    e = c.col.width == 64
-                            ? leco::launch_lookup<uint64_t>(c.col.stream, c.col.params, c.col.n, c.col.bs, sl.d_idx,
-                                                            r.count, (uint64_t *)out, sl.st)
-                            : leco::launch_lookup<uint32_t>(c.col.stream, c.col.params, c.col.n, c.col.bs, sl.d_idx,
-                                                            r.count, (uint32_t *)out, sl.st);
-            } else {
-                if (r.begin + r.count > c.col.n) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
-                if (leco::staged_decode(c.col.bs))
-                    e = c.col.width == 64
-                            ? leco::launch_decode<uint64_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
-                                                            r.begin + r.count, (uint64_t *)out, sl.st)
-                            : leco::launch_decode<uint32_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
-                                                            r.begin + r.count, (uint32_t *)out, sl.st);
-                else
-                    e = c.col.width == 64
-                            ? leco::launch_decode_values<uint64_t>(c.col.stream, c.col.params, c.col.bs, r.begin,
-                                                                   r.begin + r.count, (uint64_t *)out, sl.st)
-                            : leco::launch_decode_values<uint32_t>(c.col.stream, c.col.params, c.col.bs, r.begin,
-                                                                   r.begin + r.count, (uint32_t *)out, sl.st);
-            }
-            if (e == hipSuccess && r.h_out) e = hipMemcpyAsync(r.h_out, out, bytes, hipMemcpyDeviceToHost, sl.st);
-            if (e == hipSuccess) e = hipStreamSynchronize(sl.st);
-            if (e != hipSuccess) status = LECO_FIX_EHIP;
-            st.values_out += r.count;
-            st.bytes_out += bytes;
-            st.batches++;
-            st.max_batch_requests = 1;
-            complete(i, now_s(), status);
+            if (r.column >= kMaxColumns || r.kind > LECO_FIX_REQ_EVICT || (i && r.arrival_s < reqs[i - 1].arrival_s))
+                return LECO_FIX_EINVAL;
+            if (r.kind == LECO_FIX_REQ_LOOKUP && (!r.h_idx || r.count == 0 || r.count > kIdxCap))
+                return LECO_FIX_EINVAL;
+            if (r.kind == LECO_FIX_REQ_RANGE && (r.count == 0 || r.count > s->cfg.max_batch_values))
+                return LECO_FIX_EINVAL;
+            if (r.kind == LECO_FIX_REQ_INSERT && !valid_insert(r)) return LECO_FIX_EINVAL;
+            res[i] = leco_fix_request_result{0, 0, 0, -1, 0};
         }
-    } else {
-        std::deque<uint64_t> queue;  // arrived, not yet admitted, in arrival order
-        const size_t window = 4 * (size_t)s->cfg.max_batch_requests;  // queue entries examined per pass
-        size_t controls = 0;  // insert / evict requests in the queue
-        std::vector<uint32_t> pending_inserts;
-        while (done < nreq) {
-            bool progressed = false;
-            double t = now_s();
-            for (; next < nreq && reqs[next].arrival_s <= t; ++next) {
-                queue.push_back(next);
-                controls += reqs[next].kind > LECO_FIX_REQ_RANGE;
-            }
+        leco_fix_server_stats st{};
+        const Clock::time_point t0 = Clock::now();
+        auto now_s = [&]() { return secs(t0, Clock::now()); };
+        uint64_t done = 0, next = 0, batch_id = 0;
+        double last_done = 0;
 
-            // completions
-            for (Slot &sl : s->slots) {
-                if (!sl.busy || hipEventQuery(sl.done) != hipSuccess) continue;
-                t = now_s();
-                for (uint64_t ri : sl.reqs) {
-                    s->cols[reqs[ri].column].refs--;
-                    complete(ri, t, LECO_FIX_OK);
-                }
-                for (uint64_t id : sl.ring_ids) s->ring.release(id);
-                for (uint64_t ri : sl.reqs) try_evict(reqs[ri].column, t);
-                sl.reqs.clear();
-                sl.ring_ids.clear();
-                sl.busy = false;
-                progressed = true;
+        auto complete = [&](uint64_t ri, double t, int status) {
+            res[ri].latency_s = t - reqs[ri].arrival_s;
+            res[ri].status = status;
+            if (status == LECO_FIX_OK) st.completed++;
+            else st.failed++;
+            last_done = std::max(last_done, t);
+            ++done;
+        };
+        auto ensure_events = [&](ColumnSlot &c) -> bool {
+            if (!c.e0 && hipEventCreate(&c.e0) != hipSuccess) return false;
+            if (!c.e1 && hipEventCreate(&c.e1) != hipSuccess) return false;
+            if (!c.h_total && hipHostMalloc((void **)&c.h_total, sizeof(uint64_t), hipHostMallocDefault) != hipSuccess)
+                return false;
+            return true;
+        };
+        auto finish_insert = [&](uint32_t ci, double t) {
+            ColumnSlot &c = s->cols[ci];
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, c.e0, c.e1);
+            st.encode_gpu_s += ms * 1e-3;
+            st.encoded_values += c.col.n;
+            st.encoded_bytes += c.col.n * (c.col.width / 8);
+            c.stream_bytes = *c.h_total;
+            // shrink the column's storage to what the stream needs
+            const uint64_t keep = up(c.prefix + c.stream_bytes + LECO_FIX_TAIL_PAD, kAlign);
+            if (keep < c.pool_len) {
+                s->colfree.release(c.pool_off + keep, c.pool_len - keep);
+                c.pool_len = keep;
             }
-            for (size_t k = 0; k < pending_inserts.size();) {
-                ColumnSlot &c = s->cols[pending_inserts[k]];
-                if (hipEventQuery(c.e1) == hipSuccess) {
-                    t = now_s();
-                    finish_insert(pending_inserts[k], t);
-                    try_evict(pending_inserts[k], t);
-                    pending_inserts[k] = pending_inserts.back();
-                    pending_inserts.pop_back();
-                    progressed = true;
-                } else {
-                    ++k;
-                }
+            c.ready = true;
+            complete(c.insert_req, t, LECO_FIX_OK);
+            c.insert_req = UINT64_MAX;
+        };
+        auto try_evict = [&](uint32_t ci, double t) {
+            ColumnSlot &c = s->cols[ci];
+            if (c.evicting && c.refs == 0 && c.ready) {
+                const uint64_t ri = c.evict_req;
+                free_column(s, ci);
+                c.evict_req = UINT64_MAX;
+                complete(ri, t, LECO_FIX_OK);
             }
+        };
 
-            // admission: control requests, then one batch into a free slot
-            Slot *free_slot = nullptr;
-            for (Slot &sl : s->slots)
-                if (!sl.busy) { free_slot = &sl; break; }
-            if (!queue.empty()) {
-                std::vector<uint64_t> keep;
-                const size_t scan = std::min(queue.size(), window);
-                uint64_t nitems = 0, nidx = 0, nvals = 0;
-                bool full = free_slot == nullptr;
-                std::vector<uint64_t> admitted;
-                // Once the batch is full, every data request is held; the scan
-                // goes on only for insert / evict requests further back.
-                size_t ctl_ahead = controls, ctl_kept = 0, qi = 0;
-                for (; qi < scan && !(full && ctl_ahead == 0); ++qi) {
-                    const uint64_t ri = queue[qi];
-                    const leco_fix_request &r = reqs[ri];
-                    ColumnSlot &c = s->cols[r.column];
-                    t = now_s();
-                    if (r.kind > LECO_FIX_REQ_RANGE) ctl_ahead--;
-                    if (r.kind == LECO_FIX_REQ_INSERT) {
-                        if (c.present || !ensure_events(c)) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
-                        if (insert_layout(r).total > s->column_cap) { complete(ri, t, LECO_FIX_ENOMEM); progressed = true; continue; }
-                        if (start_insert(s, r, ri)) {
-                            res[ri].queued_s = t - r.arrival_s;
-                            pending_inserts.push_back(r.column);
-                            progressed = true;
-                        } else {
-                            keep.push_back(ri);  // waits for column memory
-                            ctl_kept++;
-                        }
-                        continue;
-                    }
-                    if (r.kind == LECO_FIX_REQ_EVICT) {
-                        bool earlier = false;
-                        for (uint64_t k : keep)
-                            if (reqs[k].column == r.column && reqs[k].kind <= LECO_FIX_REQ_RANGE) earlier = true;
-                        if (!c.present) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
-                        if (earlier || c.evicting) { keep.push_back(ri); ctl_kept++; continue; }
-                        c.evicting = true;
-                        c.evict_req = ri;
-                        res[ri].queued_s = t - r.arrival_s;
-                        try_evict(r.column, t);
-                        progressed = true;
-                        continue;
-                    }
-                    if (!c.present || c.evicting) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
-                    if (full || !c.ready) { keep.push_back(ri); continue; }
-                    if ((r.kind == LECO_FIX_REQ_RANGE && r.begin + r.count > c.col.n) ||
-                        up(result_bytes(s, r), kAlign) > s->result_cap) {
-                        complete(ri, t, r.kind == LECO_FIX_REQ_RANGE && r.begin + r.count > c.col.n ? LECO_FIX_EINVAL
-                                                                                                  : LECO_FIX_ENOMEM);
-                        progressed = true;
-                        continue;
-                    }
-                    const bool staged = leco::staged_decode(c.col.bs);
-                    const uint64_t items = r.kind == LECO_FIX_REQ_LOOKUP
-                                               ? (r.count + kLookupChunk - 1) / kLookupChunk
-                                           : staged ? (r.begin + r.count - 1) / c.col.bs - r.begin / c.col.bs + 1
-                                                    : (r.count + leco::kValueTile - 1) / leco::kValueTile;
-                    const uint64_t idx = r.kind == LECO_FIX_REQ_LOOKUP ? r.count : 0;
-                    if (admitted.size() >= s->cfg.max_batch_requests || nitems + items > kItemCap ||
-                        nidx + idx > kIdxCap || (nvals + r.count > s->cfg.max_batch_values && !admitted.empty())) {
-                        full = true;
-                        keep.push_back(ri);
-                        continue;
-                    }
-                    uint64_t off, id;
-                    if (!s->ring.alloc(result_bytes(s, r), &off, &id)) {  // memory admission
-                        full = true;
-                        keep.push_back(ri);
-                        continue;
-                    }
-                    // stage the request's work items
-                    Slot &sl = *free_slot;
-                    uint8_t *out = s->result_pool + off;
-                    const uint32_t bw = c.col.width / 8;
-                    if (r.kind == LECO_FIX_REQ_LOOKUP) {
-                        s->copies.add(sl.h_idx + nidx, r.h_idx, r.count * sizeof(uint64_t));
-                        for (uint64_t q0 = 0; q0 < r.count; q0 += kLookupChunk) {
-                            leco::ServeItem &w = sl.h_items[nitems++];
-                            w = leco::ServeItem{leco::kServeLookup, r.column,
-                                                (uint32_t)std::min<uint64_t>(kLookupChunk, r.count - q0), 0,
-                                                sl.d_idx + nidx + q0, out + q0 * bw, 0, 0, 0};
-                        }
-                        nidx += r.count;
-                    } else if (staged) {
-                        const uint64_t hi = r.begin + r.count;
-                        for (uint64_t b = r.begin / c.col.bs; b <= (hi - 1) / c.col.bs; ++b) {
-                            leco::ServeItem &w = sl.h_items[nitems++];
-                            w = leco::ServeItem{leco::kServeRange, r.column, 0, 0, nullptr, out, b, r.begin, hi};
-                        }
-                    } else {
-                        for (uint64_t q0 = 0; q0 < r.count; q0 += leco::kValueTile) {
-                            leco::ServeItem &w = sl.h_items[nitems++];
-                            w = leco::ServeItem{leco::kServeRangeValues, r.column,
-                                                (uint32_t)std::min<uint64_t>(leco::kValueTile, r.count - q0), 0,
-                                                nullptr, out + q0 * bw, 0, r.begin + q0, 0};
-                        }
-                    }
-                    nvals += r.count;
-                    c.refs++;
-                    admitted.push_back(ri);
-                    sl.ring_ids.push_back(id);
-                    res[ri].queued_s = t - r.arrival_s;
-                    res[ri].batch = batch_id;
-                    if (r.h_out) {
-                        // recorded now, issued after the launch below
-                        sl.reqs.push_back(ri);
-                        sl.reqs.push_back(off);
-                    } else {
-                        sl.reqs.push_back(ri);
-                        sl.reqs.push_back(UINT64_MAX);
-                    }
+        if (mode == LECO_FIX_SERVE_PLAIN) {
+            Slot &sl = s->slots[0];
+            for (uint64_t i = 0; i < nreq; ++i) {
+                const leco_fix_request &r = reqs[i];
+                while (now_s() < r.arrival_s) {
                 }
-                controls = ctl_ahead + ctl_kept;  // those not reached, and those held
-                for (size_t k = 0; k < qi; ++k) queue.pop_front();
-                for (size_t k = keep.size(); k-- > 0;) queue.push_front(keep[k]);
-                s->copies.run();
-                if (!admitted.empty()) {
-                    Slot &sl = *free_slot;
-                    hipError_t e = hipMemcpyAsync(sl.d_items, sl.h_items, nitems * sizeof(leco::ServeItem),
-                                                  hipMemcpyHostToDevice, sl.st);
-                    if (e == hipSuccess && nidx)
-                        e = hipMemcpyAsync(sl.d_idx, sl.h_idx, nidx * sizeof(uint64_t), hipMemcpyHostToDevice, sl.st);
+                const double ts = now_s();
+                res[i].queued_s = ts - r.arrival_s;
+                res[i].batch = batch_id++;
+                ColumnSlot &c = s->cols[r.column];
+                int status = LECO_FIX_OK;
+                if (r.kind == LECO_FIX_REQ_INSERT) {
+                    if (c.present || !ensure_events(c) || !start_insert(s, r, i)) {
+                        complete(i, now_s(), LECO_FIX_ENOMEM);
+                        continue;
+                    }
+                    (void)hipStreamSynchronize(s->enc);
+                    finish_insert(r.column, now_s());
+                    continue;
+                }
+                if (r.kind == LECO_FIX_REQ_EVICT) {
+                    if (!c.present) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
+                    c.evicting = true;
+                    c.evict_req = i;
+                    try_evict(r.column, now_s());
+                    continue;
+                }
+                if (!c.present || !c.ready || c.evicting) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
+                const uint64_t bytes = result_bytes(s, r);
+                if (bytes > s->result_cap) { complete(i, now_s(), LECO_FIX_ENOMEM); continue; }
+                void *out = s->result_pool;
+                hipError_t e;
+                if (r.kind == LECO_FIX_REQ_LOOKUP) {
+                    std::memcpy(sl.h_idx, r.h_idx, r.count * sizeof(uint64_t));
+                    e = hipMemcpyAsync(sl.d_idx, sl.h_idx, r.count * sizeof(uint64_t), hipMemcpyHostToDevice, sl.st);
                     if (e == hipSuccess)
-                        e = leco::launch_serve(s->d_cols, sl.d_items, (uint32_t)nitems,
-                                               (uint32_t)std::min<uint64_t>(nitems, kServeGrid), sl.st);
-                    std::vector<uint64_t> ids;
-                    for (size_t k = 0; k < sl.reqs.size(); k += 2) {
-                        const uint64_t ri = sl.reqs[k], off = sl.reqs[k + 1];
-                        ids.push_back(ri);
-                        const uint64_t bytes = result_bytes(s, reqs[ri]);
-                        st.values_out += reqs[ri].count;
-                        st.bytes_out += bytes;
-                        if (off != UINT64_MAX && e == hipSuccess)
-                            e = hipMemcpyAsync(reqs[ri].h_out, s->result_pool + off, bytes, hipMemcpyDeviceToHost,
-                                               sl.st);
+                        e = c.col.width == 64
+                                ? leco::launch_lookup<uint64_t>(c.col.stream, c.col.params, c.col.n, c.col.bs, sl.d_idx,
+                                                                r.count, (uint64_t *)out, sl.st)
+                                : leco::launch_lookup<uint32_t>(c.col.stream, c.col.params, c.col.n, c.col.bs, sl.d_idx,
+                                                                r.count, (uint32_t *)out, sl.st);
+                } else {
+                    if (r.begin + r.count > c.col.n) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
+                    if (leco::staged_decode(c.col.bs))
+                        e = c.col.width == 64
+                                ? leco::launch_decode<uint64_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
+                                                                r.begin + r.count, (uint64_t *)out, sl.st)
+                                : leco::launch_decode<uint32_t>(c.col.stream, c.col.offsets, c.col.n, c.col.bs, r.begin,
+                                                                r.begin + r.count, (uint32_t *)out, sl.st);
+                    else
+                        e = c.col.width == 64
+                                ? leco::launch_decode_values<uint64_t>(c.col.stream, c.col.params, c.col.bs, r.begin,
+                                                                       r.begin + r.count, (uint64_t *)out, sl.st)
+                                : leco::launch_decode_values<uint32_t>(c.col.stream, c.col.params, c.col.bs, r.begin,
+                                                                       r.begin + r.count, (uint32_t *)out, sl.st);
+                }
+                if (e == hipSuccess && r.h_out) e = hipMemcpyAsync(r.h_out, out, bytes, hipMemcpyDeviceToHost, sl.st);
+                if (e == hipSuccess) e = hipStreamSynchronize(sl.st);
+                if (e != hipSuccess) status = LECO_FIX_EHIP;
+                st.values_out += r.count;
+                st.bytes_out += bytes;
+                st.batches++;
+                st.max_batch_requests = 1;
+                complete(i, now_s(), status);
+            }
+        } else {
+            std::deque<uint64_t> queue;  // arrived, not yet admitted, in arrival order
+            const size_t window = 4 * (size_t)s->cfg.max_batch_requests;  // queue entries examined per pass
+            size_t controls = 0;  // insert / evict requests in the queue
+            std::vector<uint32_t> pending_inserts;
+            while (done < nreq) {
+                bool progressed = false;
+                double t = now_s();
+                for (; next < nreq && reqs[next].arrival_s <= t; ++next) {
+                    queue.push_back(next);
+                    controls += reqs[next].kind > LECO_FIX_REQ_RANGE;
+                }
+
+                // completions
+                for (Slot &sl : s->slots) {
+                    if (!sl.busy || hipEventQuery(sl.done) != hipSuccess) continue;
+                    t = now_s();
+                    for (uint64_t ri : sl.reqs) {
+                        s->cols[reqs[ri].column].refs--;
+                        complete(ri, t, LECO_FIX_OK);
                     }
-                    sl.reqs.swap(ids);
-                    if (e == hipSuccess) e = hipEventRecord(sl.done, sl.st);
-                    if (e != hipSuccess) return LECO_FIX_EHIP;
-                    sl.busy = true;
-                    sl.batch = batch_id++;
-                    st.batches++;
-                    st.max_batch_requests = std::max<uint64_t>(st.max_batch_requests, admitted.size());
+                    for (uint64_t id : sl.ring_ids) s->ring.release(id);
+                    for (uint64_t ri : sl.reqs) try_evict(reqs[ri].column, t);
+                    sl.reqs.clear();
+                    sl.ring_ids.clear();
+                    sl.busy = false;
                     progressed = true;
                 }
-                st.peak_result_bytes = std::max(st.peak_result_bytes, s->ring.used());
-                st.peak_column_bytes = std::max(st.peak_column_bytes, s->colfree.used());
-            }
-            // Nothing in flight, nothing arriving, nothing admitted: what is
-            // queued waits for memory that nothing will free, so it fails.
-            if (!progressed && next == nreq && pending_inserts.empty() && !queue.empty()) {
-                bool busy = false;
-                for (Slot &sl : s->slots) busy = busy || sl.busy;
-                if (!busy) {
-                    t = now_s();
-                    for (uint64_t ri : queue) {
-                        if (reqs[ri].kind == LECO_FIX_REQ_EVICT && s->cols[reqs[ri].column].evicting) continue;
-                        complete(ri, t, LECO_FIX_ENOMEM);
+                for (size_t k = 0; k < pending_inserts.size();) {
+                    ColumnSlot &c = s->cols[pending_inserts[k]];
+                    if (hipEventQuery(c.e1) == hipSuccess) {
+                        t = now_s();
+                        finish_insert(pending_inserts[k], t);
+                        try_evict(pending_inserts[k], t);
+                        pending_inserts[k] = pending_inserts.back();
+                        pending_inserts.pop_back();
+                        progressed = true;
+                    } else {
+                        ++k;
                     }
-                    queue.clear();
+                }
+
+                // admission: control requests, then one batch into a free slot
+                Slot *free_slot = nullptr;
+                for (Slot &sl : s->slots)
+                    if (!sl.busy) { free_slot = &sl; break; }
+                if (!queue.empty()) {
+                    std::vector<uint64_t> keep;
+                    const size_t scan = std::min(queue.size(), window);
+                    uint64_t nitems = 0, nidx = 0, nvals = 0;
+                    bool full = free_slot == nullptr;
+                    std::vector<uint64_t> admitted;
+                    // Once the batch is full, every data request is held; the scan
+                    // goes on only for insert / evict requests further back.
+                    size_t ctl_ahead = controls, ctl_kept = 0, qi = 0;
+                    for (; qi < scan && !(full && ctl_ahead == 0); ++qi) {
+                        const uint64_t ri = queue[qi];
+                        const leco_fix_request &r = reqs[ri];
+                        ColumnSlot &c = s->cols[r.column];
+                        t = now_s();
+                        if (r.kind > LECO_FIX_REQ_RANGE) ctl_ahead--;
+                        if (r.kind == LECO_FIX_REQ_INSERT) {
+                            if (c.present || !ensure_events(c)) {
+                                complete(ri, t, LECO_FIX_EINVAL);
+                                progressed = true;
+                                continue;
+                            }
+                            if (insert_layout(r).total > s->column_cap) {
+                                complete(ri, t, LECO_FIX_ENOMEM);
+                                progressed = true;
+                                continue;
+                            }
+                            if (start_insert(s, r, ri)) {
+                                res[ri].queued_s = t - r.arrival_s;
+                                pending_inserts.push_back(r.column);
+                                progressed = true;
+                            } else {
+                                keep.push_back(ri);  // waits for column memory
+                                ctl_kept++;
+                            }
+                            continue;
+                        }
+                        if (r.kind == LECO_FIX_REQ_EVICT) {
+                            bool earlier = false;
+                            for (uint64_t k : keep)
+                                if (reqs[k].column == r.column && reqs[k].kind <= LECO_FIX_REQ_RANGE) earlier = true;
+                            if (!c.present) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
+                            if (earlier || c.evicting) { keep.push_back(ri); ctl_kept++; continue; }
+                            c.evicting = true;
+                            c.evict_req = ri;
+                            res[ri].queued_s = t - r.arrival_s;
+                            try_evict(r.column, t);
+                            progressed = true;
+                            continue;
+                        }
+                        if (!c.present || c.evicting) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
+                        if (full || !c.ready) { keep.push_back(ri); continue; }
+                        if ((r.kind == LECO_FIX_REQ_RANGE && r.begin + r.count > c.col.n) ||
+                            up(result_bytes(s, r), kAlign) > s->result_cap) {
+                            const bool past = r.kind == LECO_FIX_REQ_RANGE && r.begin + r.count > c.col.n;
+                            complete(ri, t, past ? LECO_FIX_EINVAL : LECO_FIX_ENOMEM);
+                            progressed = true;
+                            continue;
+                        }
+                        const bool staged = leco::staged_decode(c.col.bs);
+                        const uint64_t items = r.kind == LECO_FIX_REQ_LOOKUP
+                                                   ? (r.count + kLookupChunk - 1) / kLookupChunk
+                                               : staged ? (r.begin + r.count - 1) / c.col.bs - r.begin / c.col.bs + 1
+                                                        : (r.count + leco::kValueTile - 1) / leco::kValueTile;
+                        const uint64_t idx = r.kind == LECO_FIX_REQ_LOOKUP ? r.count : 0;
+                        if (admitted.size() >= s->cfg.max_batch_requests || nitems + items > kItemCap ||
+                            nidx + idx > kIdxCap || (nvals + r.count > s->cfg.max_batch_values && !admitted.empty())) {
+                            full = true;
+                            keep.push_back(ri);
+                            continue;
+                        }
+                        uint64_t off, id;
+                        if (!s->ring.alloc(result_bytes(s, r), &off, &id)) {  // memory admission
+                            full = true;
+                            keep.push_back(ri);
+                            continue;
+                        }
+                        // stage the request's work items
+                        Slot &sl = *free_slot;
+                        uint8_t *out = s->result_pool + off;
+                        const uint32_t bw = c.col.width / 8;
+                        if (r.kind == LECO_FIX_REQ_LOOKUP) {
+                            s->copies.add(sl.h_idx + nidx, r.h_idx, r.count * sizeof(uint64_t));
+                            for (uint64_t q0 = 0; q0 < r.count; q0 += kLookupChunk) {
+                                leco::ServeItem &w = sl.h_items[nitems++];
+                                w = leco::ServeItem{leco::kServeLookup, r.column,
+                                                    (uint32_t)std::min<uint64_t>(kLookupChunk, r.count - q0), 0,
+                                                    sl.d_idx + nidx + q0, out + q0 * bw, 0, 0, 0};
+                            }
+                            nidx += r.count;
+                        } else if (staged) {
+                            const uint64_t hi = r.begin + r.count;
+                            for (uint64_t b = r.begin / c.col.bs; b <= (hi - 1) / c.col.bs; ++b) {
+                                leco::ServeItem &w = sl.h_items[nitems++];
+                                w = leco::ServeItem{leco::kServeRange, r.column, 0, 0, nullptr, out, b, r.begin, hi};
+                            }
+                        } else {
+                            for (uint64_t q0 = 0; q0 < r.count; q0 += leco::kValueTile) {
+                                leco::ServeItem &w = sl.h_items[nitems++];
+                                w = leco::ServeItem{leco::kServeRangeValues, r.column,
+                                                    (uint32_t)std::min<uint64_t>(leco::kValueTile, r.count - q0), 0,
+                                                    nullptr, out + q0 * bw, 0, r.begin + q0, 0};
+                            }
+                        }
+                        nvals += r.count;
+                        c.refs++;
+                        admitted.push_back(ri);
+                        sl.ring_ids.push_back(id);
+                        res[ri].queued_s = t - r.arrival_s;
+                        res[ri].batch = batch_id;
+                        if (r.h_out) {
+                            // recorded now, issued after the launch below
+                            sl.reqs.push_back(ri);
+                            sl.reqs.push_back(off);
+                        } else {
+                            sl.reqs.push_back(ri);
+                            sl.reqs.push_back(UINT64_MAX);
+                        }
+                    }
+                    controls = ctl_ahead + ctl_kept;  // those not reached, and those held
+                    for (size_t k = 0; k < qi; ++k) queue.pop_front();
+                    for (size_t k = keep.size(); k-- > 0;) queue.push_front(keep[k]);
+                    s->copies.run();
+                    if (!admitted.empty()) {
+                        Slot &sl = *free_slot;
+                        hipError_t e = hipMemcpyAsync(sl.d_items, sl.h_items, nitems * sizeof(leco::ServeItem),
+                                                      hipMemcpyHostToDevice, sl.st);
+                        if (e == hipSuccess && nidx)
+                            e = hipMemcpyAsync(sl.d_idx, sl.h_idx, nidx * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                               sl.st);
+                        if (e == hipSuccess)
+                            e = leco::launch_serve(s->d_cols, sl.d_items, (uint32_t)nitems,
+                                                   (uint32_t)std::min<uint64_t>(nitems, kServeGrid), sl.st);
+                        std::vector<uint64_t> ids;
+                        for (size_t k = 0; k < sl.reqs.size(); k += 2) {
+                            const uint64_t ri = sl.reqs[k], off = sl.reqs[k + 1];
+                            ids.push_back(ri);
+                            const uint64_t bytes = result_bytes(s, reqs[ri]);
+                            st.values_out += reqs[ri].count;
+                            st.bytes_out += bytes;
+                            if (off != UINT64_MAX && e == hipSuccess)
+                                e = hipMemcpyAsync(reqs[ri].h_out, s->result_pool + off, bytes, hipMemcpyDeviceToHost,
+                                                   sl.st);
+                        }
+                        sl.reqs.swap(ids);
+                        if (e == hipSuccess) e = hipEventRecord(sl.done, sl.st);
+                        if (e != hipSuccess) return LECO_FIX_EHIP;
+                        sl.busy = true;
+                        sl.batch = batch_id++;
+                        st.batches++;
+                        st.max_batch_requests = std::max<uint64_t>(st.max_batch_requests, admitted.size());
+                        progressed = true;
+                    }
+                    st.peak_result_bytes = std::max(st.peak_result_bytes, s->ring.used());
+                    st.peak_column_bytes = std::max(st.peak_column_bytes, s->colfree.used());
+                }
+                // Nothing in flight, nothing arriving, nothing admitted: what is
+                // queued waits for memory that nothing will free, so it fails.
+                if (!progressed && next == nreq && pending_inserts.empty() && !queue.empty()) {
+                    bool busy = false;
+                    for (Slot &sl : s->slots) busy = busy || sl.busy;
+                    if (!busy) {
+                        t = now_s();
+                        for (uint64_t ri : queue) {
+                            if (reqs[ri].kind == LECO_FIX_REQ_EVICT && s->cols[reqs[ri].column].evicting) continue;
+                            complete(ri, t, LECO_FIX_ENOMEM);
+                        }
+                        queue.clear();
+                    }
                 }
             }
         }
-    }
-    st.wall_s = nreq ? last_done - reqs[0].arrival_s : 0;
-    st.peak_column_bytes = std::max(st.peak_column_bytes, s->colfree.used());
-    if (stats) *stats = st;
-    return LECO_FIX_OK;
+        st.wall_s = nreq ? last_done - reqs[0].arrival_s : 0;
+        st.peak_column_bytes = std::max(st.peak_column_bytes, s->colfree.used());
+        if (stats) *stats = st;
+        return LECO_FIX_OK;
+    });
 }
 
 }  // extern "C"

@@ -79,3 +79,33 @@ def test_host_mirror_surface():
     assert c.get_block_nums() == 7 and c.block_size == 1000
     with pytest.raises(TypeError):
         Leco_int(np.int64)
+
+
+def test_host_allocation_failure_is_an_error_code(tmp_path):
+    """A host allocation that fails inside an entry point comes back as
+    LECO_FIX_ENOMEM instead of an exception unwinding into the C caller (which
+    would abort it).  leco_fix_text_info is host only and allocates a 16 MB
+    read buffer; a child process caps its address space just above what it
+    already uses, then calls it."""
+    import subprocess
+    import sys
+    p = tmp_path / "v.txt"
+    p.write_text("1 2 3\n")
+    code = f"""
+import ctypes, resource, sys
+sys.path.insert(0, {os.path.join(ROOT, "learn-to-compress_amd")!r})
+from leco_fix import _lib
+L = _lib.lib()
+cnt = ctypes.c_uint64(0)
+assert L.leco_fix_text_info({str(p)!r}.encode(), 64, ctypes.byref(cnt)) == 0 and cnt.value == 3
+used = 0
+for line in open("/proc/self/status"):
+    if line.startswith("VmSize:"):
+        used = int(line.split()[1]) * 1024
+resource.setrlimit(resource.RLIMIT_AS, (used + (4 << 20), resource.RLIM_INFINITY))
+rc = L.leco_fix_text_info({str(p)!r}.encode(), 64, ctypes.byref(cnt))
+print("rc", rc)
+"""
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "rc 3" in r.stdout, r.stdout
