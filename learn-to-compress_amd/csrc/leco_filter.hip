@@ -248,7 +248,20 @@ struct CloseBlock {
     int64_t delta;
     uint64_t f1, f2, step;  // period 0's filters, base as T
     uint32_t m;
+    double inv_base;        // 1 / base (scan mode: base >= 1)
 };
+
+// n / base for n < K * base, 1 <= base < 2^31, K <= 2^16 (so n < 2^47 is an
+// exact double and the quotient is below 2^16).  n * (1/base) carries a
+// relative error of a few 2^-53, far less than 1/base, the least distance
+// from a non-integer n / base to the next integer; so its truncation is the
+// quotient, or one less when n / base is an integer that the rounding left
+// just below.  One remainder test corrects that case.
+__device__ __forceinline__ uint32_t period_quotient(uint64_t n, uint64_t base, double inv_base) {
+    uint32_t q = (uint32_t)((double)n * inv_base);
+    if (n - (uint64_t)q * base >= base) ++q;
+    return q;
+}
 
 // K: the periods the reference's loop runs (at most kFilterMaxPeriods);
 // *cut: the loop was stopped by the limit with the next period still passing.
@@ -342,8 +355,9 @@ __device__ __forceinline__ bool close_scan_hit(const CloseBlock &c, uint32_t K, 
         if ((int64_t)v <= (int64_t)c.f1) return false;
         d = (uint64_t)((int64_t)v - (int64_t)c.f1);
     }
-    const uint64_t k = (d - 1) / (uint64_t)base;  // the last period whose f1 lies below v
-    if (k >= K) return false;
+    // k: the last period whose f1 lies below v (periods from K on never pass)
+    if (d - 1 >= (uint64_t)K * (uint64_t)base) return false;
+    const uint64_t k = period_quotient(d - 1, (uint64_t)base, c.inv_base);
     const uint64_t f1 = c.f1 + k * (uint64_t)base, f2 = c.f2 + k * (uint64_t)base;
     if (!(W == 64 ? v < f2 : (int64_t)v < (int64_t)f2)) return false;
     int32_t thre1, thre2;
@@ -378,6 +392,7 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
     c.t1 = bits_to_double(load_u64_any(c.blk + 9));
     c.delta = c.l ? (int64_t)1 << (c.l - 1) : 0;
     c.step = (uint64_t)(int64_t)a.base & wmask;
+    c.inv_base = a.base > 0 ? 1.0 / (double)a.base : 0.0;
     bool cut = false;
     uint32_t K;
     bool scan;
