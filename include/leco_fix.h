@@ -494,6 +494,9 @@ typedef struct leco_fix_server_stats {
 
 typedef struct leco_fix_server leco_fix_server;
 
+/* Carves the pools and starts up to 7 host helper threads, which copy large
+ * batches of lookup indices into pinned staging.  A server is driven by one
+ * thread at a time. */
 int leco_fix_server_create(const leco_fix_server_config *cfg, leco_fix_server **out);
 int leco_fix_server_destroy(leco_fix_server *srv);
 /* Makes an encoded column (in caller memory) servable in slot `column`. */
