@@ -966,9 +966,16 @@ __global__ __launch_bounds__(kThreads) void k_batch_blocks(const BatchColumn *__
 __global__ __launch_bounds__(kThreads) void k_serve(const ServeColumn *__restrict__ cols,
                                                     const ServeItem *__restrict__ items, uint32_t nitems) {
     __shared__ __align__(16) uint64_t buf[kLdsBytes / 8];
-    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const ServeItem w = items[it];
+    uint32_t it = blockIdx.x;
+    if (it >= nitems) return;
+    ServeItem w = items[it];
+    for (;;) {
         const ServeColumn col = cols[w.column];
+        // the next item loads while this one is served
+        const uint32_t nx = it + gridDim.x;
+        const bool more = nx < nitems;
+        ServeItem wn{};
+        if (more) wn = items[nx];
         if (w.kind == kServeLookup) {
             for (uint32_t q = threadIdx.x; q < w.count; q += kThreads) {
                 const uint64_t i = w.idx[q];
@@ -997,6 +1004,9 @@ __global__ __launch_bounds__(kThreads) void k_serve(const ServeColumn *__restric
                                                 (uint32_t *)w.out - w.lo + s, buf);
         }
         __syncthreads();
+        if (!more) break;
+        it = nx;
+        w = wn;
     }
 }
 
