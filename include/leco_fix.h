@@ -37,6 +37,15 @@ extern "C" {
 #define LECO_FIX_EHIP 2     /* a HIP runtime call or kernel launch failed */
 #define LECO_FIX_ENOMEM 3   /* device or host allocation failed */
 #define LECO_FIX_ELIMIT 4   /* a filter_range_close block needed more than 65536 periods */
+#define LECO_FIX_EDEVICE 6  /* a kernel found a broken invariant (an encoder's bounded look-back wait gave up) */
+
+/* d_offsets[nb] after an encode whose device-side look-back wait reached its
+ * bound (a unit never published its size: a broken invariant, not expected
+ * in any run).  The stream and offsets of that encode are not valid and must
+ * not be decoded.  The column and batch encoders are asynchronous, so they
+ * cannot return LECO_FIX_EDEVICE themselves; leco_fix_encode_result() reads
+ * d_offsets[nb] and does, as do the synchronous calls that read it. */
+#define LECO_FIX_OFFSETS_POISON UINT64_MAX
 
 #define LECO_FIX_TAIL_PAD 64
 
@@ -122,6 +131,20 @@ int leco_fix_lookup_u32(const uint8_t *d_stream, const leco_fix_block_param *d_p
 int leco_fix_lookup_u64(const uint8_t *d_stream, const leco_fix_block_param *d_params,
                         uint64_t n, uint32_t block_size, const uint64_t *d_idx, uint64_t nq,
                         uint64_t *d_out, void *stream);
+
+/* Waits for `stream` and reads the encoded size d_offsets[num_blocks] of an
+ * encode enqueued on it (any codec and width): LECO_FIX_OK with
+ * *stream_bytes set, or LECO_FIX_EDEVICE when the encoder poisoned the
+ * offsets (LECO_FIX_OFFSETS_POISON).  No counterpart in the reference, whose
+ * encoder is synchronous and cannot fail. */
+int leco_fix_encode_result(const uint64_t *d_offsets, uint64_t num_blocks, uint64_t *stream_bytes, void *stream);
+
+/* Test only: sets the encoders' look-back wait bound (0 = the default of 2^26
+ * sleeps) and a unit (block ticket, or ~0 for none) that never publishes its
+ * status, so that a test drives the poisoned-offsets path with every kernel
+ * exiting normally.  The settings hold for later launches until reset with
+ * (0, ~0). */
+int leco_fix_test_lookback_hook(uint32_t spin_limit, uint64_t withhold_unit);
 
 /* Rebuilds d_params from block headers, for a stream written elsewhere. */
 int leco_fix_build_params(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,

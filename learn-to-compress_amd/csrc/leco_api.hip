@@ -104,6 +104,7 @@ int encode_block_host(const T *h, uint64_t len, uint8_t *res, uint64_t rcap, uin
     uint64_t offs[2];
     if (hipMemcpyAsync(offs, d_off, sizeof(offs), hipMemcpyDeviceToHost, s) != hipSuccess) return LECO_FIX_EHIP;
     if (hipStreamSynchronize(s) != hipSuccess) return LECO_FIX_EHIP;
+    if (offs[1] == LECO_FIX_OFFSETS_POISON) return LECO_FIX_EDEVICE;
     if (offs[1] > rcap) return LECO_FIX_EINVAL;
     if (hipMemcpyAsync(res, d_st, offs[1], hipMemcpyDeviceToHost, s) != hipSuccess) return LECO_FIX_EHIP;
     if (hipStreamSynchronize(s) != hipSuccess) return LECO_FIX_EHIP;
@@ -285,6 +286,7 @@ int base_encode_block_host(int codec, const T *h, uint64_t len, uint8_t *res, ui
     if (hipMemcpyAsync(offs, d_off, sizeof(offs), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return LECO_FIX_EHIP;
+    if (offs[1] == LECO_FIX_OFFSETS_POISON) return LECO_FIX_EDEVICE;
     if (offs[1] > rcap) return LECO_FIX_EINVAL;
     if (hipMemcpyAsync(res, d_st, offs[1], hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
@@ -415,6 +417,7 @@ int opmax_encode_block_host(const uint32_t *h, uint64_t len, uint8_t *res, uint6
     if (hipMemcpyAsync(offs, d_off, sizeof(offs), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return LECO_FIX_EHIP;
+    if (offs[1] == LECO_FIX_OFFSETS_POISON) return LECO_FIX_EDEVICE;
     if (offs[1] > rcap) return LECO_FIX_EINVAL;
     if (hipMemcpyAsync(res, d_st, offs[1], hipMemcpyDeviceToHost, s) != hipSuccess ||
         (h_fix && hipMemcpyAsync(h_fix, d_fix, len, hipMemcpyDeviceToHost, s) != hipSuccess) ||
@@ -684,6 +687,26 @@ int leco_fix_filter_close_block_u64(const uint8_t *h_in, uint64_t len, uint32_t 
     return filter_block_host<uint64_t>(h_in, len, true, 0, f1, f2, base, h_out, cap, block_id, block_size, count);
 }
 
+int leco_fix_encode_result(const uint64_t *d_offsets, uint64_t nb, uint64_t *stream_bytes, void *stream) {
+    if (!d_offsets || !stream_bytes) return LECO_FIX_EINVAL;
+    uint64_t end = 0;
+    const hipStream_t st = (hipStream_t)stream;
+    if (hipMemcpyAsync(&end, d_offsets + nb, sizeof(end), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return LECO_FIX_EHIP;
+    if (end == LECO_FIX_OFFSETS_POISON) return LECO_FIX_EDEVICE;
+    *stream_bytes = end;
+    return LECO_FIX_OK;
+}
+
+int leco_fix_test_lookback_hook(uint32_t spin_limit, uint64_t withhold) {
+    if (leco::set_lookback_hook_leco(spin_limit, withhold) != hipSuccess ||
+        leco::set_lookback_hook_base(spin_limit, withhold) != hipSuccess ||
+        leco::set_lookback_hook_opmax(spin_limit, withhold) != hipSuccess)
+        return LECO_FIX_EHIP;
+    return hip_status(hipDeviceSynchronize());
+}
+
 const char *leco_fix_strerror(int code) {
     switch (code) {
     case LECO_FIX_OK: return "ok";
@@ -691,6 +714,7 @@ const char *leco_fix_strerror(int code) {
     case LECO_FIX_EHIP: return "HIP runtime error";
     case LECO_FIX_ENOMEM: return "out of memory";
     case LECO_FIX_ELIMIT: return "a filter_range_close block needed more than 65536 periods";
+    case LECO_FIX_EDEVICE: return "a kernel found a broken invariant (look-back wait gave up); the output is not valid";
     default: return "unknown error";
     }
 }

@@ -14,6 +14,8 @@
 
 namespace leco {
 
+LECO_LOOKBACK_HOOK_SETTER(set_lookback_hook_base)
+
 namespace {
 
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) { return ~wave_max_u64(~v); }
@@ -37,7 +39,7 @@ __global__ __launch_bounds__(kThreads) void k_base_encode(const T *__restrict__ 
     __shared__ EncodeShared sh;
     __shared__ uint64_t s_b;
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
-    if (tid == 0) s_b = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+    if (tid == 0) s_b = take_ticket(offs);  // offs[0] is the ticket
     __syncthreads();
     const uint64_t b = s_b;
     const uint64_t s = b * bs;
@@ -120,12 +122,17 @@ __global__ __launch_bounds__(kThreads) void k_base_clear(uint64_t *__restrict__ 
     if (i < words) offs[i] = 0;
 }
 
-// Block end offsets from the look-back words (flags cleared), the ticket back to 0.
+// Block end offsets from the look-back words (flags cleared), the ticket back
+// to 0; offs[nb] is poisoned if a look-back wait gave up.
 __global__ __launch_bounds__(kThreads) void k_base_fixup(uint64_t *__restrict__ offs, uint64_t nb) {
     const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (b >= nb) return;
-    offs[b + 1] &= kStatusValue;
-    if (b == 0) offs[0] = 0;
+    uint64_t end = offs[b + 1] & kStatusValue;
+    if (b + 1 == nb) {
+        if (offs[0] & kTicketError) end = LECO_FIX_OFFSETS_POISON;
+        offs[0] = 0;
+    }
+    offs[b + 1] = end;
 }
 
 // FOR values [j0, j0 + jn) of a run staged in LDS, field j at bit

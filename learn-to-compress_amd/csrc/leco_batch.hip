@@ -386,6 +386,7 @@ int leco_fix_autotune_block_sizes(const leco_fix_column_desc *cols, uint32_t nco
                  hipSuccess ||
              hipStreamSynchronize(st) != hipSuccess))
             rc = LECO_FIX_EHIP;
+        if (rc == LECO_FIX_OK && offs[b->total] == LECO_FIX_OFFSETS_POISON) rc = LECO_FIX_EDEVICE;
         if (rc == LECO_FIX_OK) {
             std::vector<uint64_t> bytes((size_t)ncols * ncand, 0);
             for (size_t e = 0; e < tune.size(); ++e) bytes[owner[e]] += offs[b->h_first[e + 1]] - offs[b->h_first[e]];

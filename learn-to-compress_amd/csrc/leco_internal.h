@@ -192,6 +192,12 @@ __host__ __device__ inline uint64_t filter_workspace_for(uint64_t tiles) {
     return 8 * ((tiles + kGtGroupTiles - 1) / kGtGroupTiles) + (8 + kFilterTile / 8) * tiles;
 }
 
+// Look-back test hooks, one per translation unit with an encoder (see
+// LookbackHook in leco_kernels.h).
+hipError_t set_lookback_hook_leco(uint32_t spin_limit, uint64_t withhold);
+hipError_t set_lookback_hook_base(uint32_t spin_limit, uint64_t withhold);
+hipError_t set_lookback_hook_opmax(uint32_t spin_limit, uint64_t withhold);
+
 // Each launcher enqueues on `st` and returns the hipGetLastError() of the launch.
 template <typename T>
 hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t *offs,

@@ -32,6 +32,32 @@ constexpr uint64_t kStatusInclusive = 2ull << 62;   // value = bytes up to and i
 constexpr uint64_t kStatusMask = 3ull << 62;
 constexpr uint64_t kStatusValue = (1ull << 62) - 1;
 constexpr uint32_t kSpinLimit = 1u << 26;              // look-back wait bound (s_sleep 1 each)
+// Bit 63 of the ticket word offs[0]: a look-back wait of this launch reached
+// its bound (a broken invariant: the unit it waited for never published), so
+// the offsets it produced are wrong.  The fix-up turns it into
+// offs[nb] = LECO_FIX_OFFSETS_POISON; tickets mask it off.
+constexpr uint64_t kTicketError = 1ull << 63;
+
+// Test hook of the look-back wait, one copy per translation unit (set through
+// the setter that LECO_LOOKBACK_HOOK_SETTER defines there): the wait bound,
+// and a unit that never publishes its status, so that a test can drive the
+// bound's error path with every kernel still exiting.  The defaults leave the
+// encoders as they are.
+struct LookbackHook {
+    uint32_t spin_limit;
+    uint32_t reserved;
+    uint64_t withhold;  // unit index, or ~0
+};
+static __device__ LookbackHook g_lookback_hook = {kSpinLimit, 0, ~0ull};
+#define LECO_LOOKBACK_HOOK_SETTER(fn)                                                  \
+    hipError_t fn(uint32_t spin_limit, uint64_t withhold) {                             \
+        const LookbackHook h{spin_limit ? spin_limit : kSpinLimit, 0, withhold};        \
+        return hipMemcpyToSymbol(HIP_SYMBOL(g_lookback_hook), &h, sizeof(h));            \
+    }
+
+__device__ __forceinline__ uint64_t take_ticket(uint64_t *offs) {
+    return atomicAdd((unsigned long long *)offs, 1ull) & ~kTicketError;
+}
 
 __device__ __forceinline__ void store_status(uint64_t *p, uint64_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -66,17 +92,21 @@ struct EncodeShared {
 // prefix.  Every lower unit belongs to a wave
 // that is already running and publishes its size before it waits, so the
 // waits end; the spin bound (~seconds) only keeps a broken invariant from
-// hanging the GPU.
+// hanging the GPU.  A wait that reaches the bound counts the predecessor as
+// empty and sets kTicketError in offs[0], which poisons the launch's offsets.
 template <int PER = kLookPer>
 __device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_t *__restrict__ offs) {
     constexpr int kWin = kWave * PER;
     const int lane = threadIdx.x % kWave;
+    const LookbackHook hook = g_lookback_hook;
+    const bool publish = b != hook.withhold;
     if (b == 0) {
-        if (lane == 0) store_status(&offs[1], kStatusInclusive | size);
+        if (lane == 0 && publish) store_status(&offs[1], kStatusInclusive | size);
         return 0;
     }
-    if (lane == 0) store_status(&offs[b + 1], kStatusAggregate | size);
+    if (lane == 0 && publish) store_status(&offs[b + 1], kStatusAggregate | size);
     uint64_t excl = 0;
+    bool gave_up = false;
     for (int64_t top = (int64_t)b - 1;; top -= kWin) {
         uint64_t stw[PER];
 #pragma unroll
@@ -93,12 +123,14 @@ __device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_
             if (!done) {
                 const int64_t p = top - (k * kWave + lane);
                 uint32_t spins = 0;
-                while ((stw[k] & kStatusMask) == 0 && ++spins < kSpinLimit) {
+                while ((stw[k] & kStatusMask) == 0 && ++spins < hook.spin_limit) {
                     __builtin_amdgcn_s_sleep(1);
                     stw[k] = load_status(&offs[p + 1]);
                 }
                 const uint64_t incl = __ballot((stw[k] & kStatusMask) == kStatusInclusive);
                 const int stop = incl ? __ffsll((long long)incl) - 1 : kWave;
+                // only a predecessor this unit counts matters (lanes past the stop are not added)
+                gave_up = gave_up || (lane <= stop && (stw[k] & kStatusMask) == 0);
                 uint64_t part = lane <= stop ? stw[k] & kStatusValue : 0;
                 for (int o = kWave / 2; o > 0; o >>= 1) part += (uint64_t)__shfl_xor((long long)part, o, kWave);
                 excl += part;
@@ -107,7 +139,8 @@ __device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_
         }
         if (done) break;
     }
-    if (lane == 0) store_status(&offs[b + 1], kStatusInclusive | (excl + size));
+    if (lane == 0 && publish) store_status(&offs[b + 1], kStatusInclusive | (excl + size));
+    if (__ballot(gave_up) && lane == 0) atomicOr((unsigned long long *)offs, (unsigned long long)kTicketError);
     return excl;
 }
 

@@ -14,6 +14,8 @@
 
 namespace leco {
 
+LECO_LOOKBACK_HOOK_SETTER(set_lookback_hook_leco)
+
 // ---------------------------------------------------------------------------
 // K1 fit: lr_int_T<T>::caltheta (lr.h:66-85).  The sums over y and i*y are
 // sequential FP64 chains in the reference, so each block's chain is replayed
@@ -391,7 +393,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_encode(Rows rows, leco_fix_bloc
                                                         uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
     __shared__ EncodeShared sh;
     __shared__ uint64_t s_u;
-    if (threadIdx.x == 0) s_u = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+    if (threadIdx.x == 0) s_u = take_ticket(offs);  // offs[0] is the ticket
     __syncthreads();
     const uint64_t u = s_u;
     const EncodeUnit un = rows.unit(u);
@@ -417,7 +419,7 @@ __global__ __launch_bounds__(kThreads) void k_encode_waves(PlainRows rows, leco_
                                                            uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
     __shared__ EncodeShared sh;
     __shared__ uint64_t s_u;
-    if (threadIdx.x == 0) s_u = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+    if (threadIdx.x == 0) s_u = take_ticket(offs);  // offs[0] is the ticket
     __syncthreads();
     const uint64_t u = s_u;
     const EncodeUnit un = rows.unit(u);
@@ -434,7 +436,7 @@ __global__ __launch_bounds__(kThreads) void k_encode_large(PlainRows rows, leco_
                                                            uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
     __shared__ EncodeShared sh;
     __shared__ uint64_t s_u;
-    if (threadIdx.x == 0) s_u = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+    if (threadIdx.x == 0) s_u = take_ticket(offs);  // offs[0] is the ticket
     __syncthreads();
     const uint64_t u = s_u;
     const EncodeUnit un = rows.unit(u);
@@ -446,7 +448,8 @@ __global__ __launch_bounds__(kThreads) void k_encode_large(PlainRows rows, leco_
 
 // K3: block end offsets offs[b+1] from the param table (the unit status
 // words that shared offs during the encode are overwritten), and offs[0]
-// (the ticket) back to 0.
+// (the ticket) back to 0.  The last block's lane also reads the ticket's
+// error bit: if a look-back wait gave up, offs[nb] becomes the poison value.
 __global__ __launch_bounds__(kThreads) void k_encode_fixup(const leco_fix_block_param *__restrict__ prm,
                                                            uint64_t *__restrict__ offs, uint64_t nb,
                                                            const uint32_t *__restrict__ widths, uint32_t width) {
@@ -454,8 +457,12 @@ __global__ __launch_bounds__(kThreads) void k_encode_fixup(const leco_fix_block_
     if (b >= nb) return;
     const uint32_t w = widths ? widths[b] : width;
     const leco_fix_block_param p = prm[b];
-    offs[b + 1] = p.payload_offset - (p.bits >= w ? 1 : 17) + block_bytes(p.bits, w, p.length);
-    if (b == 0) offs[0] = 0;
+    uint64_t end = p.payload_offset - (p.bits >= w ? 1 : 17) + block_bytes(p.bits, w, p.length);
+    if (b + 1 == nb) {
+        if (offs[0] & kTicketError) end = LECO_FIX_OFFSETS_POISON;
+        offs[0] = 0;
+    }
+    offs[b + 1] = end;
 }
 
 // ---------------------------------------------------------------------------

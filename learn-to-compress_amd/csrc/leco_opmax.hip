@@ -26,6 +26,8 @@
 
 namespace leco {
 
+LECO_LOOKBACK_HOOK_SETTER(set_lookback_hook_opmax)
+
 namespace {
 
 // (float)d as x86 CVTSD2SS: round to nearest even; a NaN keeps its sign and
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(kThreads) void k_opmax_encode(const uint32_t *__res
     __shared__ uint64_t s_u;
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     constexpr uint32_t kWaves = kThreads / kWave;
-    if (tid == 0) s_u = atomicAdd((unsigned long long *)offs, 1ull);  // offs[0] is the ticket
+    if (tid == 0) s_u = take_ticket(offs);  // offs[0] is the ticket
     __syncthreads();
     const uint64_t u = s_u;
     const uint64_t nb = (n + bs - 1) / bs;
@@ -228,14 +230,19 @@ __global__ __launch_bounds__(kThreads) void k_opmax_encode(const uint32_t *__res
     }
 }
 
-// Block end offsets offs[b+1] from the param table; the ticket back to 0.
+// Block end offsets offs[b+1] from the param table; the ticket back to 0;
+// offs[nb] is poisoned if a look-back wait gave up.
 __global__ __launch_bounds__(kThreads) void k_opmax_fixup(const leco_fix_block_param *__restrict__ prm,
                                                           uint64_t *__restrict__ offs, uint64_t nb) {
     const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
     if (b >= nb) return;
     const leco_fix_block_param p = prm[b];
-    offs[b + 1] = p.payload_offset - kOpmaxHdr + opmax_block_bytes(p.bits, p.length);
-    if (b == 0) offs[0] = 0;
+    uint64_t end = p.payload_offset - kOpmaxHdr + opmax_block_bytes(p.bits, p.length);
+    if (b + 1 == nb) {
+        if (offs[0] & kTicketError) end = LECO_FIX_OFFSETS_POISON;
+        offs[0] = 0;
+    }
+    offs[b + 1] = end;
 }
 
 // ---------------------------------------------------------------------------

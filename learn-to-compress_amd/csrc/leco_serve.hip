@@ -518,6 +518,14 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
             float ms = 0;
             (void)hipEventElapsedTime(&ms, c.e0, c.e1);
             st.encode_gpu_s += ms * 1e-3;
+            if (*c.h_total == LECO_FIX_OFFSETS_POISON) {  // the encoder found a broken invariant
+                const uint64_t ri = c.insert_req, ev = c.evict_req;
+                free_column(s, ci);
+                c.insert_req = c.evict_req = UINT64_MAX;
+                complete(ri, t, LECO_FIX_EDEVICE);
+                if (ev != UINT64_MAX) complete(ev, t, LECO_FIX_OK);  // an evict waiting for the insert: nothing left
+                return;
+            }
             st.encoded_values += c.col.n;
             st.encoded_bytes += c.col.n * (c.col.width / 8);
             c.stream_bytes = *c.h_total;

@@ -19,6 +19,8 @@ c_int = ctypes.c_int
 
 LECO_FIX_OK = 0
 LECO_FIX_ELIMIT = 4
+LECO_FIX_EDEVICE = 6
+LECO_FIX_OFFSETS_POISON = (1 << 64) - 1
 LECO_FIX_TAIL_PAD = 64
 
 
@@ -65,6 +67,8 @@ for _w in ("u32", "u64"):
                                                       ctypes.POINTER(c_u64)]),
     })
 SIGNATURES["leco_fix_filter_workspace_bytes"] = (c_u64, [c_u64, c_u32])
+SIGNATURES["leco_fix_encode_result"] = (c_int, [c_ptr, c_u64, ctypes.POINTER(c_u64), c_ptr])
+SIGNATURES["leco_fix_test_lookback_hook"] = (c_int, [c_u32, c_u64])
 
 
 LECO_FIX_CODEC_FOR, LECO_FIX_CODEC_DELTA = 1, 2
@@ -159,12 +163,28 @@ SIGNATURES.update({
 _lib = None
 
 
+def _load_torch_runtime_first() -> None:
+    """torch ships its own HIP runtime (torch/lib/libamdhip64.so, soname
+    libamdhip64.so.7).  The library links libamdhip64.so.7 by soname, so when
+    torch is imported first the library binds to torch's runtime and the
+    process has one.  Loaded the other way round, /opt/rocm's runtime comes in
+    first and torch then loads its own copy as well: two HIP runtimes in one
+    process, where device pointers and kernels of one are invalid in the
+    other (torch reports no GPU, or the library's launches fail).  So torch,
+    when it is installed, is imported before the library is loaded."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib() -> ctypes.CDLL:
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run `python learn-to-compress_amd/build.py` "
                               "(or __graft_entry__.build()) first")
+        _load_torch_runtime_first()
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
@@ -184,3 +204,11 @@ def strerror(code: int) -> str:
 def check(code: int, what: str) -> None:
     if code != LECO_FIX_OK:
         raise LecoError(code, what)
+
+
+def encoded_size(end: int, what: str) -> int:
+    """The encoded size d_offsets[nb] read back as an int64; raises
+    LECO_FIX_EDEVICE if the encoder poisoned it (LECO_FIX_OFFSETS_POISON)."""
+    if end & LECO_FIX_OFFSETS_POISON == LECO_FIX_OFFSETS_POISON:
+        raise LecoError(LECO_FIX_EDEVICE, what)
+    return end

@@ -12,7 +12,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import LECO_FIX_CODEC_DELTA, LECO_FIX_CODEC_FOR, LECO_FIX_TAIL_PAD, check, lib
+from ._lib import LECO_FIX_CODEC_DELTA, LECO_FIX_CODEC_FOR, LECO_FIX_TAIL_PAD, check, encoded_size, lib
 from .column import _STORAGE, _as_storage, _stream_handle, width_of
 
 CODECS = {"for": LECO_FIX_CODEC_FOR, "delta": LECO_FIX_CODEC_DELTA}
@@ -48,7 +48,7 @@ class BaseColumn:
         fn = getattr(L, f"leco_fix_base_encode_column_{'u32' if w == 32 else 'u64'}")
         check(fn(CODECS[codec], values.data_ptr(), n, block_size, st.data_ptr(), bound, offs.data_ptr(),
                  _stream_handle(stream)), f"{codec} encode_column")
-        return cls(codec, st, offs, n, block_size, w, int(offs[-1].item()))
+        return cls(codec, st, offs, n, block_size, w, encoded_size(int(offs[-1].item()), "base_encode_column"))
 
     def stream_bytes(self) -> bytes:
         return self.stream[:self.nbytes].cpu().numpy().tobytes()

@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import ColumnDesc, check, lib
+from ._lib import ColumnDesc, check, encoded_size, lib
 from .column import PARAM_BYTES, Column, _as_storage, _stream_handle, width_of
 
 AUTOTUNE_CANDIDATES = (256, 512, 1024, 2048, 4096, 8192, 16384)
@@ -124,7 +124,7 @@ class Batch:
         check(lib().leco_fix_batch_set_decode_grid(self._h, workgroups), "batch_set_decode_grid")
 
     def compressed_bytes(self) -> int:
-        return int(self.offsets[self.total_blocks].item())
+        return encoded_size(int(self.offsets[self.total_blocks].item()), "batch_encode")
 
     def column(self, c: int) -> Column:
         """Column c as a Column (arena, offsets + first, params + first)."""

@@ -15,7 +15,7 @@ import ctypes
 
 import torch
 
-from ._lib import LECO_FIX_TAIL_PAD, BlockParam, check, lib
+from ._lib import LECO_FIX_TAIL_PAD, BlockParam, check, encoded_size, lib
 
 _STORAGE = {32: torch.int32, 64: torch.int64}
 DECODE_AUTO, DECODE_STAGED, DECODE_VALUES = 0, 1, 2
@@ -81,7 +81,7 @@ class Column:
         fn = L.leco_fix_encode_column_u32 if w == 32 else L.leco_fix_encode_column_u64
         check(fn(values.data_ptr(), n, block_size, st.data_ptr(), bound, offs.data_ptr(), prm.data_ptr(),
                  _stream_handle(stream)), "encode_column")
-        nbytes = int(offs[nb].item())
+        nbytes = encoded_size(int(offs[nb].item()), "encode_column")
         col = cls(st, offs, prm, n, block_size, w, nbytes)
         if compact:
             col.compact()

@@ -19,7 +19,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import BlockParam, LECO_FIX_TAIL_PAD, check, lib
+from ._lib import BlockParam, LECO_FIX_TAIL_PAD, check, encoded_size, lib
 from .column import _stream_handle
 
 PARAM_BYTES = ctypes.sizeof(BlockParam)
@@ -61,7 +61,7 @@ class OpMaxColumn:
         check(L.leco_fix_opmax_encode_column(values.data_ptr(), n, block_size, st.data_ptr(), bound, offs.data_ptr(),
                                              prm.data_ptr(), fix.data_ptr() if fix is not None else None,
                                              _stream_handle(stream)), "opmax encode_column")
-        return cls(st, offs, prm, fix, n, block_size, int(offs[-1].item()))
+        return cls(st, offs, prm, fix, n, block_size, encoded_size(int(offs[-1].item()), "opmax_encode_column"))
 
     @classmethod
     def from_stream(cls, stream_bytes: bytes, offsets, n: int, block_size: int, fix=None,
