@@ -37,6 +37,7 @@ extern "C" {
 #define LECO_FIX_EHIP 2     /* a HIP runtime call or kernel launch failed */
 #define LECO_FIX_ENOMEM 3   /* device or host allocation failed */
 #define LECO_FIX_ELIMIT 4   /* a filter_range_close block needed more than 65536 periods */
+#define LECO_FIX_ESCRATCH 5 /* the input needs more than the static scratch of leco_fix_filter_range_noalloc */
 #define LECO_FIX_EDEVICE 6  /* a kernel found a broken invariant (an encoder's bounded look-back wait gave up) */
 
 /* d_offsets[nb] after an encode whose device-side look-back wait reached its
@@ -207,6 +208,21 @@ int leco_fix_filter_range_close_u64(const uint8_t *d_stream, const uint64_t *d_o
                                     uint32_t block_size, const uint64_t *d_filter1, const uint64_t *d_filter2,
                                     int32_t base, uint32_t *d_out, uint64_t out_capacity, uint64_t *d_result,
                                     void *d_work, uint64_t work_bytes, void *stream);
+/* leco_fix_filter_range_u32 / _u64 for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Same arguments and
+ * results as leco_fix_filter_range_u<width_bits>, except that there is no
+ * d_work: the workspace is the library's static device array of
+ * LECO_FIX_NOALLOC_SCRATCH_BYTES (64 MiB), held by every GPU the library runs
+ * on until the process ends.  The call allocates nothing.  An input whose
+ * workspace would exceed it (about 520M values at block sizes that divide
+ * 8192) returns LECO_FIX_ESCRATCH before any HIP call.  All calls on one GPU
+ * share that GPU's array: calls in a row are fine, calls that overlap on one
+ * GPU (concurrent streams or threads) race and give wrong results. */
+#define LECO_FIX_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int leco_fix_filter_range_noalloc(uint32_t width_bits, const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n,
+                                  uint32_t block_size, uint64_t filter, uint32_t *d_out, uint64_t out_capacity,
+                                  uint64_t *d_result, void *stream);
 /* The two member functions on one block in host memory (filter_range(in,
  * length, filter, out, block_id) and filter_range_close(in, length, out,
  * block_id, filter1, filter2, base)): positions block_id * block_size + i.

@@ -715,6 +715,7 @@ const char *leco_fix_strerror(int code) {
     case LECO_FIX_ENOMEM: return "out of memory";
     case LECO_FIX_ELIMIT: return "a filter_range_close block needed more than 65536 periods";
     case LECO_FIX_EDEVICE: return "a kernel found a broken invariant (look-back wait gave up); the output is not valid";
+    case LECO_FIX_ESCRATCH: return "the input needs more workspace than the library's static scratch holds";
     default: return "unknown error";
     }
 }

@@ -12,6 +12,8 @@
 // order is the reference's: block by block, period by period
 // (filter_range_close), position by position.  Only the windows the reference
 // scans are read: positions from the model-derived threshold on.
+#include <atomic>
+
 #include "leco_kernels.h"
 
 namespace leco {
@@ -982,3 +984,57 @@ template hipError_t launch_filter<uint64_t>(const uint8_t *, const uint64_t *, u
                                             uint64_t, uint64_t *, uint64_t *, hipStream_t);
 
 }  // namespace leco
+
+// ---- filter_range with the library's static scratch (allocates nothing) ----
+// The workspace of filter_range (group sums, tile counts, match bitmaps) lives
+// in `scratch`, a file-scope device array that every GPU the library runs on
+// holds until the process ends.  Host code never passes `scratch` itself (its
+// host name is only a handle for the symbol): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current, and is kept
+// per device.  Every call on one GPU shares that GPU's array; the clear kernel
+// at the start of each call zeroes what must start at zero (the group sums and
+// the result), so calls in a row need nothing else, but calls that overlap on
+// one GPU race on it.
+namespace {
+
+constexpr int kScratchMaxDevices = 64;
+__device__ __align__(256) uint64_t scratch[LECO_FIX_NOALLOC_SCRATCH_BYTES / 8];
+std::atomic<void *> g_scratch_ptr[kScratchMaxDevices];
+
+// The current device's address of `scratch`, or null if the runtime fails.
+void *scratch_address(int *rc) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { *rc = LECO_FIX_EHIP; return nullptr; }
+    if (dev < 0 || dev >= kScratchMaxDevices) { *rc = LECO_FIX_EINVAL; return nullptr; }
+    void *p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (p) return p;
+    // the same address whichever thread asks first, so a race only repeats the call
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) { *rc = LECO_FIX_EHIP; return nullptr; }
+    g_scratch_ptr[dev].store(p, std::memory_order_release);
+    return p;
+}
+
+}  // namespace
+
+extern "C" int leco_fix_filter_range_noalloc(uint32_t width_bits, const uint8_t *d_stream, const uint64_t *d_offsets,
+                                             uint64_t n, uint32_t block_size, uint64_t filter, uint32_t *d_out,
+                                             uint64_t out_capacity, uint64_t *d_result, void *stream) {
+    // every argument check comes before the first HIP call
+    if ((width_bits != 32 && width_bits != 64) || n == 0 || n > (1ull << 32) || block_size == 0 ||
+        block_size > leco::kMaxBlockSize || !d_stream || !d_offsets || !d_result || (out_capacity && !d_out) ||
+        (width_bits == 32 && filter > 0xFFFFFFFFull))
+        return LECO_FIX_EINVAL;
+    const uint64_t need = leco::filter_workspace_for(leco::gt_tiles(n, block_size));
+    if (need > LECO_FIX_NOALLOC_SCRATCH_BYTES) return LECO_FIX_ESCRATCH;
+    int rc = LECO_FIX_OK;
+    uint64_t *work = (uint64_t *)scratch_address(&rc);
+    if (!work) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    const hipError_t e =
+        width_bits == 64
+            ? leco::launch_filter<uint64_t>(d_stream, d_offsets, n, block_size, false, filter, nullptr, nullptr, 0, 0,
+                                            d_out, out_capacity, d_result, work, st)
+            : leco::launch_filter<uint32_t>(d_stream, d_offsets, n, block_size, false, filter, nullptr, nullptr, 0, 0,
+                                            d_out, out_capacity, d_result, work, st);
+    return e == hipSuccess ? LECO_FIX_OK : LECO_FIX_EHIP;
+}

@@ -19,8 +19,10 @@ c_int = ctypes.c_int
 
 LECO_FIX_OK = 0
 LECO_FIX_ELIMIT = 4
+LECO_FIX_ESCRATCH = 5
 LECO_FIX_EDEVICE = 6
 LECO_FIX_OFFSETS_POISON = (1 << 64) - 1
+LECO_FIX_NOALLOC_SCRATCH_BYTES = 64 << 20
 LECO_FIX_TAIL_PAD = 64
 
 
@@ -69,6 +71,8 @@ for _w in ("u32", "u64"):
 SIGNATURES["leco_fix_filter_workspace_bytes"] = (c_u64, [c_u64, c_u32])
 SIGNATURES["leco_fix_encode_result"] = (c_int, [c_ptr, c_u64, ctypes.POINTER(c_u64), c_ptr])
 SIGNATURES["leco_fix_test_lookback_hook"] = (c_int, [c_u32, c_u64])
+SIGNATURES["leco_fix_filter_range_noalloc"] = (c_int, [c_u32, c_ptr, c_ptr, c_u64, c_u32, c_u64, c_ptr, c_u64, c_ptr,
+                                                     c_ptr])
 
 
 LECO_FIX_CODEC_FOR, LECO_FIX_CODEC_DELTA = 1, 2
