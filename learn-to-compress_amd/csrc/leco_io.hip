@@ -113,18 +113,35 @@ struct PinnedPair {
 };
 thread_local PinnedPair g_pinned;
 
-// File bytes [off, off + len) -> device dst; piece i+1 is read while piece i is copied.
-int file_to_device(int fd, uint64_t off, uint8_t *dst, uint64_t len, hipStream_t st) {
+// Checks the bytes of each piece while it is in host memory, before it is
+// copied: check(piece, pos, len) sees file bytes [off + pos, off + pos + len).
+struct PieceCheck {
+    virtual bool operator()(const uint8_t *piece, uint64_t pos, size_t len) = 0;
+};
+
+// Returns rc once the copies already enqueued on st have finished: an error
+// return must not leave a copy still writing the caller's buffer (or reading
+// a pinned buffer the next call refills).
+int drain(hipStream_t st, int rc) {
+    const hipError_t e = hipStreamSynchronize(st);
+    return rc != LECO_FIX_OK ? rc : hip_status(e);
+}
+
+// File bytes [off, off + len) -> device dst; piece i+1 is read while piece i
+// is copied.  A piece that fails `check` is not copied and the call returns
+// LECO_FIX_EINVAL once the earlier copies are done.
+int file_to_device(int fd, uint64_t off, uint8_t *dst, uint64_t len, hipStream_t st, PieceCheck *check = nullptr) {
     int rc = g_pinned.init();
     if (rc) return rc;
     for (uint64_t pos = 0, i = 0; pos < len; pos += kChunk, ++i) {
         const int k = (int)(i & 1);
         const size_t piece = (size_t)(len - pos < kChunk ? len - pos : kChunk);
-        if (i >= 2 && hipEventSynchronize(g_pinned.ev[k]) != hipSuccess) return LECO_FIX_EHIP;
-        if (!read_full(fd, g_pinned.buf[k], piece, off + pos)) return LECO_FIX_EINVAL;
+        if (i >= 2 && hipEventSynchronize(g_pinned.ev[k]) != hipSuccess) return drain(st, LECO_FIX_EHIP);
+        if (!read_full(fd, g_pinned.buf[k], piece, off + pos)) return drain(st, LECO_FIX_EINVAL);
+        if (check && !(*check)((const uint8_t *)g_pinned.buf[k], pos, piece)) return drain(st, LECO_FIX_EINVAL);
         if (hipMemcpyAsync(dst + pos, g_pinned.buf[k], piece, hipMemcpyHostToDevice, st) != hipSuccess ||
             hipEventRecord(g_pinned.ev[k], st) != hipSuccess)
-            return LECO_FIX_EHIP;
+            return drain(st, LECO_FIX_EHIP);
     }
     return hip_status(hipStreamSynchronize(st));
 }
@@ -260,13 +277,16 @@ int text_pass(const char *path, uint32_t width_bits, uint8_t *d_out, uint64_t ca
     for (uint64_t off = 0;;) {
         const ssize_t r = pread(f.fd, buf.data(), buf.size(), (off_t)off);
         if (r < 0 && errno == EINTR) continue;
-        if (r < 0) return LECO_FIX_EINVAL;
+        if (r < 0) {
+            rc = rc != LECO_FIX_OK ? rc : LECO_FIX_EINVAL;
+            break;
+        }
         if (r == 0 || !parser.feed(buf.data(), (size_t)r, emit)) break;
         off += (uint64_t)r;
     }
     if (d_out) {
         flush();
-        if (rc == LECO_FIX_OK && hipStreamSynchronize(st) != hipSuccess) rc = LECO_FIX_EHIP;
+        rc = drain(st, rc);  // copies already enqueued finish before any return
     }
     if (rc == LECO_FIX_OK) *count = n;
     return rc;
@@ -284,6 +304,45 @@ int read_header(int fd, FileHeader *h) {
         return LECO_FIX_EINVAL;
     return LECO_FIX_OK;
 }
+
+uint32_t block_len(const FileHeader &h, uint64_t b) {
+    const uint64_t s = b * h.block_size;
+    return (uint32_t)(h.n - s < h.block_size ? h.n - s : h.block_size);
+}
+
+// Bytes of a block of m values whose header byte is l, or 0 if no decoder
+// accepts l: 0..W (W = raw), and the one- and two-value forms 127 and 126 of
+// decodeArray8 (piecewise_fix_integer_template.h:152-170).
+uint64_t bytes_for_header(uint32_t l, uint32_t w, uint32_t m) {
+    if (l <= w) return leco::block_bytes(l, w, m);
+    if (l == 127 && m == 1) return 1 + w / 8;
+    if (l == 126 && m == 2) return 1 + 2 * (w / 8);
+    return 0;
+}
+
+bool span_possible(uint64_t span, uint32_t w, uint32_t m) {
+    for (uint32_t l = 0; l <= w; ++l)
+        if (leco::block_bytes(l, w, m) == span) return true;
+    return bytes_for_header(126, w, m) == span || bytes_for_header(127, w, m) == span;
+}
+
+// Header byte of each block against its span, piece by piece (offsets are
+// sorted, so a cursor walks the blocks whose first byte lies in the piece).
+struct HeaderCheck : PieceCheck {
+    const std::vector<uint64_t> &offs;
+    const FileHeader &h;
+    uint64_t next = 0;
+    HeaderCheck(const std::vector<uint64_t> &o, const FileHeader &hd) : offs(o), h(hd) {}
+    bool operator()(const uint8_t *piece, uint64_t pos, size_t len) override {
+        const uint64_t nb = h.num_blocks;
+        for (; next < nb && offs[next] < pos + len; ++next) {
+            if (offs[next] < pos) return false;
+            const uint64_t span = offs[next + 1] - offs[next];
+            if (bytes_for_header(piece[offs[next] - pos], h.width_bits, block_len(h, next)) != span) return false;
+        }
+        return true;
+    }
+};
 
 }  // namespace
 
@@ -417,14 +476,23 @@ int leco_fix_column_load(const char *path, uint8_t *d_stream, uint64_t stream_ca
         std::vector<uint64_t> offs(nb + 1);
         if (!read_full(f.fd, offs.data(), 8 * (nb + 1), 64)) return LECO_FIX_EINVAL;
         if (offs[0] != 0 || offs[nb] != h.stream_bytes) return LECO_FIX_EINVAL;
-        for (uint64_t b = 1; b <= nb; ++b)
-            if (offs[b] < offs[b - 1]) return LECO_FIX_EINVAL;
+        // every block's span must be a size its header byte can give (a shifted
+        // or altered offset fails here, before any HIP call) ...
+        for (uint64_t b = 0; b < nb; ++b)
+            if (offs[b + 1] < offs[b] || !span_possible(offs[b + 1] - offs[b], h.width_bits, block_len(h, b)))
+                return LECO_FIX_EINVAL;
+        // ... and, as each piece of the stream passes through host memory, the
+        // block's header byte must give exactly its span: the decoders, lookups
+        // and filters read a block by its header byte, so a mismatch would make
+        // them read past the stream
+        HeaderCheck check(offs, h);
         hipStream_t st = (hipStream_t)stream;
+        rc = file_to_device(f.fd, stream_start(nb), d_stream, h.stream_bytes, st, &check);
+        if (rc) return rc;
+        if (check.next != nb) return LECO_FIX_EINVAL;
         if (hipMemcpyAsync(d_offsets, offs.data(), 8 * (nb + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemsetAsync(d_stream + h.stream_bytes, 0, LECO_FIX_TAIL_PAD, st) != hipSuccess)
-            return LECO_FIX_EHIP;
-        rc = file_to_device(f.fd, stream_start(nb), d_stream, h.stream_bytes, st);
-        if (rc) return rc;
+            return drain(st, LECO_FIX_EHIP);
         if (d_params) {
             if (leco::launch_build_params(d_stream, d_offsets, h.n, h.block_size, h.width_bits, d_params, st) !=
                 hipSuccess)

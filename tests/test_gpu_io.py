@@ -100,6 +100,57 @@ def test_column_load_rejects_small_buffers(tmp_path):
         io.load_sosd(tmp_path / "missing.bin", 64)
 
 
+@pytest.mark.parametrize("width,bs", [(64, 7), (64, 4096), (32, 1000)])
+def test_column_load_checks_header_bytes_against_spans(tmp_path, oracle_mod, width, bs):
+    """A block whose header byte does not give exactly its span (a flipped
+    width byte, or offsets shifted by a multiple of 8 so that every span is
+    still a possible size) is refused while the stream is copied, before the
+    param table is built; the buffers are not touched by any kernel.  (A
+    width that gives the same span, e.g. 6 and 7 bits for 7 values, cannot be
+    told apart and is skipped: every read stays inside the block.)  The
+    unaltered file loads and decodes."""
+    import torch
+    from test_io import column_file
+
+    from leco_fix import datagen, io
+    from leco_fix._lib import lib
+    n = 3000 * bs // 7 + 5 if bs == 7 else 30 * bs + 5
+    vals = datagen.sorted_pwl_u64(n, seed=6) if width == 64 else datagen.linear_noise_u32(n, seed=6)
+    st, off = oracle_mod.encode_column(vals, bs)
+    off = np.asarray(off, np.uint64)
+    p = tmp_path / "col.leco"
+    dst = torch.zeros(len(st) + 64, dtype=torch.uint8, device="cuda")
+    offs = torch.zeros(len(off), dtype=torch.int64, device="cuda")
+    prm = torch.zeros(32 * (len(off) - 1), dtype=torch.uint8, device="cuda")
+
+    def load(stream_bytes, offsets):
+        p.write_bytes(column_file(n, bs, width, stream_bytes, offsets))
+        return lib().leco_fix_column_load(bytes(p), dst.data_ptr(), dst.numel(), offs.data_ptr(), prm.data_ptr(),
+                                          None)
+    nb = len(off) - 1
+    for b in (0, nb // 2, nb - 1):
+        for flip in (1, 0x40, 0x7F, 0xFF):
+            bad = bytearray(st.tobytes())
+            bad[int(off[b])] ^= flip
+            l = bad[int(off[b])]
+            m = min(bs, n - b * bs)
+            if l <= width and oracle_mod.block_bytes(l, width, m) == int(off[b + 1] - off[b]):
+                continue  # this width gives the same span: a stream the check cannot tell apart
+            say(f"column load: width={width} bs={bs} block {b} header {st[int(off[b])]} -> {l}")
+            assert load(bytes(bad), off) == 1, (b, flip)
+    if bs == 7:
+        # an offset moved by 8: both neighbouring spans are still possible sizes
+        moved = off.copy()
+        widths = st[off[:-1].astype(np.int64)]
+        k = next(i for i in range(1, nb) if 0 < widths[i - 1] < width - 1 and 0 < widths[i] < width - 1)
+        moved[k] = np.uint64(int(moved[k]) + 8)
+        say(f"column load: offset {k} moved by 8")
+        assert load(st.tobytes(), moved) == 1
+    assert load(st.tobytes(), off) == 0
+    col = io.load_column(p)
+    assert np.array_equal(col.decode().cpu().numpy().view(vals.dtype), vals)
+
+
 @pytest.mark.parametrize("width", [32, 64])
 def test_text_load_values(tmp_path, istream_values, width):
     """leco_fix_text_load's values against libstdc++'s extraction loop

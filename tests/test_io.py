@@ -60,3 +60,38 @@ def test_column_file_info_and_rejections(tmp_path):
     p.write_bytes(column_file(n, bs, 64, bytes(170), offsets)[:64])
     with pytest.raises(LecoError):
         io.column_file_info(p)
+
+
+def _load_rc(path, cap):
+    """leco_fix_column_load with host stand-ins for the device buffers: a
+    file that fails the host-side checks is refused before any HIP call, so
+    the stand-ins are never touched."""
+    import ctypes
+    from leco_fix import lib
+    buf = (ctypes.c_uint8 * 64)()
+    return lib().leco_fix_column_load(str(path).encode(), buf, cap, buf, None, None)
+
+
+@pytest.mark.parametrize("width,bs", [(64, 4096), (32, 1000), (64, 7)])
+def test_column_load_refuses_spans_no_header_byte_gives(tmp_path, oracle_mod, width, bs):
+    """A shifted or altered block offset leaves a block whose span no header
+    byte can give (no width 0..W, raw, or the 126/127 forms): refused with
+    EINVAL before anything is copied or launched.  (A shift by a multiple of
+    8 can leave possible spans; the header-byte check while the stream is
+    copied catches those, tests/test_gpu_io.py.)"""
+    from leco_fix import datagen
+    n = 20 * bs + 3
+    vals = datagen.sorted_pwl_u64(n, seed=4) if width == 64 else datagen.linear_noise_u32(n, seed=4)
+    st, off = oracle_mod.encode_column(vals, bs)
+    off = np.asarray(off, np.uint64)
+    p = tmp_path / "col"
+    cap = len(st) + 64
+    for b, delta in ((3, 5), (7, -1), (1, 1), (19, 3)):
+        bad = off.copy()
+        bad[b] = np.uint64(int(bad[b]) + delta)
+        p.write_bytes(column_file(n, bs, width, st.tobytes(), bad))
+        assert _load_rc(p, cap) == 1, (b, delta)
+    bad = off.copy()
+    bad[5], bad[6] = bad[6], bad[5]               # not monotone
+    p.write_bytes(column_file(n, bs, width, st.tobytes(), bad))
+    assert _load_rc(p, cap) == 1
