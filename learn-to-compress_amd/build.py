@@ -71,18 +71,22 @@ OPMAX_DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "opmax_driver.cpp")
 OPMAX_DRIVER = os.path.join(ROOT, "tests", "cpp", "opmax_driver")
 CHAIN_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "chain_check.cpp")
 CHAIN_CHECK = os.path.join(ROOT, "tests", "cpp", "chain_check")
+PERIOD_CHECK_SRC = os.path.join(ROOT, "tests", "cpp", "period_check.cpp")
+PERIOD_CHECK = os.path.join(ROOT, "tests", "cpp", "period_check")
 
 
 def build_driver() -> str:
     """The C++ drivers over include/leco_int.hpp, leco_baselines.hpp and
-    leco_opmax.hpp (host code: g++ links the library), and the host-side check
-    of the op_max chain evaluation (leco_chain.h is HIP host+device code, so
-    hipcc compiles it; only its host code runs)."""
+    leco_opmax.hpp (host code: g++ links the library), and the host-side checks
+    of the op_max chain evaluation and of the close filter's period index
+    (leco_chain.h and leco_period.h are HIP host+device code, so hipcc compiles
+    them; only their host code runs)."""
     for src, exe in ((DRIVER_SRC, DRIVER), (BASE_DRIVER_SRC, BASE_DRIVER), (OPMAX_DRIVER_SRC, OPMAX_DRIVER)):
         _run(["g++", "-std=c++17", "-O2", "-Wall", "-I", os.path.join(ROOT, "include"), src,
               "-o", exe, "-L", LIBDIR, "-lleco_fix", "-Wl,-rpath,$ORIGIN/../../learn-to-compress_amd/lib"])
-    _run([HIPCC, "-x", "hip", f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-ffp-contract=off", "-I", CSRC,
-          CHAIN_CHECK_SRC, "-o", CHAIN_CHECK])
+    for src, exe in ((CHAIN_CHECK_SRC, CHAIN_CHECK), (PERIOD_CHECK_SRC, PERIOD_CHECK)):
+        _run([HIPCC, "-x", "hip", f"--offload-arch={ARCH}", "-O2", "-std=c++17", "-ffp-contract=off", "-I", CSRC,
+              src, "-o", exe])
     return DRIVER
 
 

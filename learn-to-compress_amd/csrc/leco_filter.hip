@@ -15,6 +15,7 @@
 #include <atomic>
 
 #include "leco_kernels.h"
+#include "leco_period.h"
 
 namespace leco {
 
@@ -253,18 +254,6 @@ struct CloseBlock {
     double inv_base;        // 1 / base (scan mode: base >= 1)
 };
 
-// n / base for n < K * base, 1 <= base < 2^31, K <= 2^16 (so n < 2^47 is an
-// exact double and the quotient is below 2^16).  n * (1/base) carries a
-// relative error of a few 2^-53, far less than 1/base, the least distance
-// from a non-integer n / base to the next integer; so its truncation is the
-// quotient, or one less when n / base is an integer that the rounding left
-// just below.  One remainder test corrects that case.
-__device__ __forceinline__ uint32_t period_quotient(uint64_t n, uint64_t base, double inv_base) {
-    uint32_t q = (uint32_t)((double)n * inv_base);
-    if (n - (uint64_t)q * base >= base) ++q;
-    return q;
-}
-
 // K: the periods the reference's loop runs (at most kFilterMaxPeriods);
 // *cut: the loop was stopped by the limit with the next period still passing.
 template <int W>
@@ -394,7 +383,7 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
     c.t1 = bits_to_double(load_u64_any(c.blk + 9));
     c.delta = c.l ? (int64_t)1 << (c.l - 1) : 0;
     c.step = (uint64_t)(int64_t)a.base & wmask;
-    c.inv_base = a.base > 0 ? 1.0 / (double)a.base : 0.0;
+    c.inv_base = a.base > 0 ? period_inverse((uint64_t)a.base) : 0.0;
     bool cut = false;
     uint32_t K;
     bool scan;

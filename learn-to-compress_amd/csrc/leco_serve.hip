@@ -627,13 +627,16 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                     controls += reqs[next].kind > LECO_FIX_REQ_RANGE;
                 }
 
-                // completions
+                // completions (a query that fails for another reason than "not
+                // ready" fails the slot's requests, so the loop still ends)
                 for (Slot &sl : s->slots) {
-                    if (!sl.busy || hipEventQuery(sl.done) != hipSuccess) continue;
+                    if (!sl.busy) continue;
+                    const hipError_t q = hipEventQuery(sl.done);
+                    if (q == hipErrorNotReady) continue;
                     t = now_s();
                     for (uint64_t ri : sl.reqs) {
                         s->cols[reqs[ri].column].refs--;
-                        complete(ri, t, LECO_FIX_OK);
+                        complete(ri, t, q == hipSuccess ? LECO_FIX_OK : LECO_FIX_EHIP);
                     }
                     for (uint64_t id : sl.ring_ids) s->ring.release(id);
                     for (uint64_t ri : sl.reqs) try_evict(reqs[ri].column, t);
@@ -644,9 +647,18 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                 }
                 for (size_t k = 0; k < pending_inserts.size();) {
                     ColumnSlot &c = s->cols[pending_inserts[k]];
-                    if (hipEventQuery(c.e1) == hipSuccess) {
+                    const hipError_t q = hipEventQuery(c.e1);
+                    if (q != hipErrorNotReady) {
                         t = now_s();
-                        finish_insert(pending_inserts[k], t);
+                        if (q == hipSuccess) {
+                            finish_insert(pending_inserts[k], t);
+                        } else {  // the encode's stream failed: the insert fails, its column is dropped
+                            const uint64_t ri = c.insert_req, ev = c.evict_req;
+                            free_column(s, pending_inserts[k]);
+                            c.insert_req = c.evict_req = UINT64_MAX;
+                            complete(ri, t, LECO_FIX_EHIP);
+                            if (ev != UINT64_MAX) complete(ev, t, LECO_FIX_OK);
+                        }
                         try_evict(pending_inserts[k], t);
                         pending_inserts[k] = pending_inserts.back();
                         pending_inserts.pop_back();
@@ -822,9 +834,12 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                     for (Slot &sl : s->slots) busy = busy || sl.busy;
                     if (!busy) {
                         t = now_s();
+                        // every queued request completes: one that waits for memory fails with ENOMEM,
+                        // an evict of a column another evict is already dropping with EINVAL
                         for (uint64_t ri : queue) {
-                            if (reqs[ri].kind == LECO_FIX_REQ_EVICT && s->cols[reqs[ri].column].evicting) continue;
-                            complete(ri, t, LECO_FIX_ENOMEM);
+                            const bool dup_evict =
+                                reqs[ri].kind == LECO_FIX_REQ_EVICT && s->cols[reqs[ri].column].evicting;
+                            complete(ri, t, dup_evict ? LECO_FIX_EINVAL : LECO_FIX_ENOMEM);
                         }
                         queue.clear();
                     }
