@@ -500,7 +500,7 @@ def main() -> int:
             torch.cuda.empty_cache()
             log(f"batched small columns: {small}")
     if not args.no_serving:
-        serving = workloads.run_serving(col, nreq=args.serve_nreq, log=log)
+        serving = workloads.run_serving(col, vals, nreq=args.serve_nreq, log=log)
         log(f"serving: {serving}")
     if not args.no_churn:
         churn = workloads.run_churn(ncols=args.churn_cols, log=log)

@@ -502,7 +502,11 @@ typedef struct leco_fix_request {
     uint64_t begin;          /* range start */
     const uint64_t *h_idx;   /* lookup indices (host) */
     void *h_out;             /* if set, results are copied here (host) before the request completes */
-    const void *d_values;    /* insert: values in HBM */
+    union {
+        const void *d_values; /* insert: values in HBM */
+        void *d_out;          /* lookup / range: if set, results are written here in HBM (count values of
+                                 the column's width) instead of the result pool; no copy, no pool memory */
+    };
     uint32_t block_size;     /* insert */
     uint32_t width_bits;     /* insert */
     double arrival_s;        /* arrival time, seconds after the run starts (non-decreasing) */
@@ -546,6 +550,25 @@ int leco_fix_server_attach_column(leco_fix_server *srv, uint32_t column, const u
  * fills one result per request; results[i] follows reqs[i]. */
 int leco_fix_server_run(leco_fix_server *srv, const leco_fix_request *reqs, uint64_t nreq, int mode,
                         leco_fix_request_result *results, leco_fix_server_stats *stats);
+/* The slowest iterations of the last batched run's admission loop (at most
+ * LECO_FIX_SERVER_TRACE of them, slowest first), with the time each phase
+ * took: where a latency stall comes from.  Returns the number written. */
+#define LECO_FIX_SERVER_TRACE 16
+typedef struct leco_fix_server_iteration {
+    double start_s;          /* seconds after the run's start */
+    float between_us;        /* since the previous iteration ended */
+    float arrivals_us;       /* queueing the requests that arrived */
+    float completions_us;    /* finished batches and inserts */
+    float admission_us;      /* the admission scan: checks, result memory, work items */
+    float copies_us;         /* lookup indices into pinned staging */
+    float launch_us;         /* staging copies, the launch, the completion event */
+    float rest_us;           /* the rest of the iteration */
+    uint32_t admitted;       /* requests admitted in this iteration */
+    uint32_t queued;         /* requests waiting when it started */
+    uint32_t reserved;
+} leco_fix_server_iteration;
+int leco_fix_server_slowest_iterations(const leco_fix_server *srv, leco_fix_server_iteration *out, uint32_t cap,
+                                       uint32_t *count);
 /* Device pointers of an inserted column (valid until it is evicted). */
 int leco_fix_server_column(const leco_fix_server *srv, uint32_t column, const uint8_t **d_stream,
                            const uint64_t **d_offsets, const leco_fix_block_param **d_params, uint64_t *n,

@@ -265,8 +265,30 @@ double secs(Clock::time_point a, Clock::time_point b) { return std::chrono::dura
 
 }  // namespace
 
+// The LECO_FIX_SERVER_TRACE slowest iterations of a run's admission loop,
+// kept in a fixed array (no allocation inside the loop).
+struct IterationTrace {
+    leco_fix_server_iteration slow[LECO_FIX_SERVER_TRACE];
+    uint32_t count = 0;
+    void reset() { count = 0; }
+    void add(const leco_fix_server_iteration &it) {
+        auto total = [](const leco_fix_server_iteration &x) {
+            return x.arrivals_us + x.completions_us + x.admission_us + x.copies_us + x.launch_us + x.rest_us;
+        };
+        if (count < LECO_FIX_SERVER_TRACE) {
+            slow[count++] = it;
+            return;
+        }
+        uint32_t lo = 0;
+        for (uint32_t k = 1; k < count; ++k)
+            if (total(slow[k]) < total(slow[lo])) lo = k;
+        if (total(it) > total(slow[lo])) slow[lo] = it;
+    }
+};
+
 struct leco_fix_server {
     leco_fix_server_config cfg{};
+    IterationTrace trace;
     std::vector<ColumnSlot> cols;
     leco::ServeColumn *d_cols = nullptr;
     uint8_t *result_pool = nullptr;
@@ -423,6 +445,26 @@ int leco_fix_server_create(const leco_fix_server_config *cfg_in, leco_fix_server
                  hipMalloc(&sl.d_items, kItemCap * sizeof(leco::ServeItem)) == hipSuccess;
         }
         if (!ok) { destroy_server(s); return LECO_FIX_ENOMEM; }
+        // The runtime sets a stream up on its first command, which blocked the
+        // admission loop for ~7 ms the first time a run needed a third or
+        // fourth batch in flight (the loop's phase trace put the stall in the
+        // launch step of a 1-request batch, once per slot, in the first run
+        // only).  Each slot's stream and the insert stream therefore run an
+        // empty served launch, a staging copy and the completion event here.
+        for (Slot &sl : s->slots) {
+            if (hipMemcpyAsync(sl.d_items, sl.h_items, sizeof(leco::ServeItem), hipMemcpyHostToDevice, sl.st) !=
+                    hipSuccess ||
+                leco::launch_serve(s->d_cols, sl.d_items, 0, 1, sl.st) != hipSuccess ||
+                hipEventRecord(sl.done, sl.st) != hipSuccess || hipStreamSynchronize(sl.st) != hipSuccess) {
+                destroy_server(s);
+                return LECO_FIX_EHIP;
+            }
+        }
+        if (leco::launch_serve(s->d_cols, s->slots[0].d_items, 0, 1, s->enc) != hipSuccess ||
+            hipStreamSynchronize(s->enc) != hipSuccess) {
+            destroy_server(s);
+            return LECO_FIX_EHIP;
+        }
         const unsigned hw = std::thread::hardware_concurrency();
         if (!s->copies.start(hw > 1 ? std::min(kCopyHelpers, hw - 1) : 0)) {
             destroy_server(s);
@@ -457,6 +499,22 @@ int leco_fix_server_attach_column(leco_fix_server *s, uint32_t c, const uint8_t 
         if (rc) return rc;
         return hipStreamSynchronize(s->enc) == hipSuccess ? LECO_FIX_OK : LECO_FIX_EHIP;
     });
+}
+
+int leco_fix_server_slowest_iterations(const leco_fix_server *s, leco_fix_server_iteration *out, uint32_t cap,
+                                       uint32_t *count) {
+    if (!s || !count || (cap && !out)) return LECO_FIX_EINVAL;
+    leco_fix_server_iteration sorted[LECO_FIX_SERVER_TRACE];
+    const uint32_t n = s->trace.count;
+    std::copy(s->trace.slow, s->trace.slow + n, sorted);
+    auto total = [](const leco_fix_server_iteration &x) {
+        return x.arrivals_us + x.completions_us + x.admission_us + x.copies_us + x.launch_us + x.rest_us;
+    };
+    std::sort(sorted, sorted + n, [&](const auto &a, const auto &b) { return total(a) > total(b); });
+    const uint32_t k = std::min(n, cap);
+    std::copy(sorted, sorted + k, out);
+    *count = k;
+    return LECO_FIX_OK;
 }
 
 int leco_fix_server_column(const leco_fix_server *s, uint32_t c, const uint8_t **st, const uint64_t **offs,
@@ -578,8 +636,8 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                 }
                 if (!c.present || !c.ready || c.evicting) { complete(i, now_s(), LECO_FIX_EINVAL); continue; }
                 const uint64_t bytes = result_bytes(s, r);
-                if (bytes > s->result_cap) { complete(i, now_s(), LECO_FIX_ENOMEM); continue; }
-                void *out = s->result_pool;
+                if (!r.d_out && bytes > s->result_cap) { complete(i, now_s(), LECO_FIX_ENOMEM); continue; }
+                void *out = r.d_out ? r.d_out : s->result_pool;
                 hipError_t e;
                 if (r.kind == LECO_FIX_REQ_LOOKUP) {
                     std::memcpy(sl.h_idx, r.h_idx, r.count * sizeof(uint64_t));
@@ -619,13 +677,23 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
             const size_t window = 4 * (size_t)s->cfg.max_batch_requests;  // queue entries examined per pass
             size_t controls = 0;  // insert / evict requests in the queue
             std::vector<uint32_t> pending_inserts;
+            s->trace.reset();
+            Clock::time_point prev_end = Clock::now();
             while (done < nreq) {
                 bool progressed = false;
-                double t = now_s();
+                // phase marks of this iteration (the trace keeps the slowest ones)
+                const Clock::time_point p0 = Clock::now();
+                Clock::time_point p1, p2, p3 = p0, p4 = p0, p5 = p0;
+                leco_fix_server_iteration it{};
+                it.start_s = secs(t0, p0);
+                it.between_us = (float)(secs(prev_end, p0) * 1e6);
+                it.queued = (uint32_t)queue.size();
+                double t = it.start_s;
                 for (; next < nreq && reqs[next].arrival_s <= t; ++next) {
                     queue.push_back(next);
                     controls += reqs[next].kind > LECO_FIX_REQ_RANGE;
                 }
+                p1 = Clock::now();
 
                 // completions (a query that fails for another reason than "not
                 // ready" fails the slot's requests, so the loop still ends)
@@ -668,6 +736,8 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                     }
                 }
 
+                p2 = Clock::now();
+                p3 = p4 = p5 = p2;
                 // admission: control requests, then one batch into a free slot
                 Slot *free_slot = nullptr;
                 for (Slot &sl : s->slots)
@@ -724,7 +794,7 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                         if (!c.present || c.evicting) { complete(ri, t, LECO_FIX_EINVAL); progressed = true; continue; }
                         if (full || !c.ready) { keep.push_back(ri); continue; }
                         if ((r.kind == LECO_FIX_REQ_RANGE && r.begin + r.count > c.col.n) ||
-                            up(result_bytes(s, r), kAlign) > s->result_cap) {
+                            (!r.d_out && up(result_bytes(s, r), kAlign) > s->result_cap)) {
                             const bool past = r.kind == LECO_FIX_REQ_RANGE && r.begin + r.count > c.col.n;
                             complete(ri, t, past ? LECO_FIX_EINVAL : LECO_FIX_ENOMEM);
                             progressed = true;
@@ -742,15 +812,15 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                             keep.push_back(ri);
                             continue;
                         }
-                        uint64_t off, id;
-                        if (!s->ring.alloc(result_bytes(s, r), &off, &id)) {  // memory admission
+                        uint64_t off = 0, id = 0;
+                        if (!r.d_out && !s->ring.alloc(result_bytes(s, r), &off, &id)) {  // memory admission
                             full = true;
                             keep.push_back(ri);
                             continue;
                         }
                         // stage the request's work items
                         Slot &sl = *free_slot;
-                        uint8_t *out = s->result_pool + off;
+                        uint8_t *out = r.d_out ? (uint8_t *)r.d_out : s->result_pool + off;
                         const uint32_t bw = c.col.width / 8;
                         if (r.kind == LECO_FIX_REQ_LOOKUP) {
                             s->copies.add(sl.h_idx + nidx, r.h_idx, r.count * sizeof(uint64_t));
@@ -778,22 +848,20 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                         nvals += r.count;
                         c.refs++;
                         admitted.push_back(ri);
-                        sl.ring_ids.push_back(id);
+                        if (!r.d_out) sl.ring_ids.push_back(id);
                         res[ri].queued_s = t - r.arrival_s;
                         res[ri].batch = batch_id;
-                        if (r.h_out) {
-                            // recorded now, issued after the launch below
-                            sl.reqs.push_back(ri);
-                            sl.reqs.push_back(off);
-                        } else {
-                            sl.reqs.push_back(ri);
-                            sl.reqs.push_back(UINT64_MAX);
-                        }
+                        // where the host copy (h_out), recorded now and issued after the launch, reads from
+                        sl.reqs.push_back(ri);
+                        sl.reqs.push_back(r.h_out ? (uint64_t)(uintptr_t)out : 0);
                     }
                     controls = ctl_ahead + ctl_kept;  // those not reached, and those held
                     for (size_t k = 0; k < qi; ++k) queue.pop_front();
                     for (size_t k = keep.size(); k-- > 0;) queue.push_front(keep[k]);
+                    p3 = Clock::now();
                     s->copies.run();
+                    p4 = p5 = Clock::now();
+                    it.admitted = (uint32_t)admitted.size();
                     if (!admitted.empty()) {
                         Slot &sl = *free_slot;
                         hipError_t e = hipMemcpyAsync(sl.d_items, sl.h_items, nitems * sizeof(leco::ServeItem),
@@ -806,14 +874,14 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                                                    (uint32_t)std::min<uint64_t>(nitems, kServeGrid), sl.st);
                         std::vector<uint64_t> ids;
                         for (size_t k = 0; k < sl.reqs.size(); k += 2) {
-                            const uint64_t ri = sl.reqs[k], off = sl.reqs[k + 1];
+                            const uint64_t ri = sl.reqs[k], src = sl.reqs[k + 1];
                             ids.push_back(ri);
                             const uint64_t bytes = result_bytes(s, reqs[ri]);
                             st.values_out += reqs[ri].count;
                             st.bytes_out += bytes;
-                            if (off != UINT64_MAX && e == hipSuccess)
-                                e = hipMemcpyAsync(reqs[ri].h_out, s->result_pool + off, bytes, hipMemcpyDeviceToHost,
-                                                   sl.st);
+                            if (src && e == hipSuccess)
+                                e = hipMemcpyAsync(reqs[ri].h_out, (const void *)(uintptr_t)src, bytes,
+                                                   hipMemcpyDeviceToHost, sl.st);
                         }
                         sl.reqs.swap(ids);
                         if (e == hipSuccess) e = hipEventRecord(sl.done, sl.st);
@@ -823,6 +891,7 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                         st.batches++;
                         st.max_batch_requests = std::max<uint64_t>(st.max_batch_requests, admitted.size());
                         progressed = true;
+                        p5 = Clock::now();
                     }
                     st.peak_result_bytes = std::max(st.peak_result_bytes, s->ring.used());
                     st.peak_column_bytes = std::max(st.peak_column_bytes, s->colfree.used());
@@ -844,6 +913,14 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                         queue.clear();
                     }
                 }
+                prev_end = Clock::now();
+                it.arrivals_us = (float)(secs(p0, p1) * 1e6);
+                it.completions_us = (float)(secs(p1, p2) * 1e6);
+                it.admission_us = (float)(secs(p2, p3) * 1e6);
+                it.copies_us = (float)(secs(p3, p4) * 1e6);
+                it.launch_us = (float)(secs(p4, p5) * 1e6);
+                it.rest_us = (float)(secs(p5, prev_end) * 1e6);
+                s->trace.add(it);
             }
         }
         st.wall_s = nreq ? last_done - reqs[0].arrival_s : 0;

@@ -148,6 +148,8 @@ SIGNATURES.update({
                                        ctypes.POINTER(c_u32), ctypes.POINTER(c_u64)]),
 })
 
+SIGNATURES["leco_fix_server_slowest_iterations"] = (c_int, [c_ptr, c_ptr, c_u32, ctypes.POINTER(c_u32)])
+
 SIGNATURES.update({
     "leco_fix_opmax_stream_bound": (c_u64, [c_u64, c_u32]),
     "leco_fix_opmax_encode_column": (c_int, [c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr, c_ptr, c_ptr]),

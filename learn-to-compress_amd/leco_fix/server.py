@@ -26,7 +26,13 @@ RESULT_DTYPE = np.dtype([("latency_s", "<f8"), ("queued_s", "<f8"), ("batch", "<
 STATS_FIELDS = ["wall_s", "batches", "completed", "failed", "max_batch_requests", "values_out", "bytes_out",
                 "encoded_values", "encoded_bytes", "encode_gpu_s", "peak_result_bytes", "peak_column_bytes"]
 STATS_DTYPE = np.dtype([(f, "<f8" if f.endswith("_s") else "<u8") for f in STATS_FIELDS], align=True)
+ITERATION_FIELDS = ["between_us", "arrivals_us", "completions_us", "admission_us", "copies_us", "launch_us",
+                    "rest_us"]
+ITERATION_DTYPE = np.dtype([("start_s", "<f8")] + [(f, "<f4") for f in ITERATION_FIELDS] +
+                           [("admitted", "<u4"), ("queued", "<u4"), ("reserved", "<u4")], align=True)
+TRACE_CAP = 16
 assert REQUEST_DTYPE.itemsize == 64 and RESULT_DTYPE.itemsize == 32 and STATS_DTYPE.itemsize == 96
+assert ITERATION_DTYPE.itemsize == 48
 
 
 class ServerConfig(ctypes.Structure):
@@ -69,6 +75,21 @@ class Server:
               "server_column")
         return {"stream": st.value, "offsets": offs.value, "params": prm.value, "n": n.value,
                 "block_size": bs.value, "width": w.value, "stream_bytes": nbytes.value}
+
+    def slowest_iterations(self) -> list[dict]:
+        """The last batched run's slowest admission-loop iterations, slowest
+        first, with the microseconds each phase took."""
+        out = np.zeros(TRACE_CAP, ITERATION_DTYPE)
+        cnt = ctypes.c_uint32()
+        check(lib().leco_fix_server_slowest_iterations(self._h, out.ctypes.data, TRACE_CAP, ctypes.byref(cnt)),
+              "server_slowest_iterations")
+        rows = []
+        for r in out[:cnt.value]:
+            row = {"start_s": round(float(r["start_s"]), 6)}
+            row.update({f: round(float(r[f]), 1) for f in ITERATION_FIELDS})
+            row.update({"admitted": int(r["admitted"]), "queued": int(r["queued"])})
+            rows.append(row)
+        return rows
 
     def run(self, reqs: np.ndarray, mode: int = SERVE_BATCHED):
         """Serves the trace in real time; returns (results, stats dict)."""
@@ -117,3 +138,43 @@ def make_trace(rng: np.random.Generator, columns: list[tuple[int, int, int]], ra
         for i, o in enumerate(outs):
             reqs["h_out"][i] = o.ctypes.data
     return reqs, idx, outs
+
+
+def device_outputs(reqs: np.ndarray, select: np.ndarray, width: int):
+    """One HBM buffer for the results of the selected lookup / range requests
+    (all on columns of `width` bits): their d_out (the d_values field) points
+    at consecutive slices of it, so the server writes those results straight
+    into HBM.  Returns (buffer, start of each request's slice)."""
+    import torch
+    counts = np.where(select, reqs["count"], 0).astype(np.int64)
+    starts = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int64)
+    total = int(counts.sum())
+    buf = torch.empty(max(total, 1), dtype=torch.int64 if width == 64 else torch.int32, device="cuda")
+    esize = width // 8
+    sel = np.asarray(select, bool)
+    reqs["d_values"][sel] = (buf.data_ptr() + esize * starts[sel]).astype(np.uint64)
+    return buf, starts
+
+
+def check_device_outputs(reqs: np.ndarray, idx: np.ndarray, select: np.ndarray, buf, starts, values) -> int:
+    """Mismatching values among the selected requests' device results:
+    lookups against values[idx], ranges against values[begin:begin + count]
+    of the column (`values`: slot -> device tensor of the raw column),
+    compared on the device, one synchronization at the end."""
+    import torch
+    idx_dev = torch.from_numpy(idx.view(np.int64)).to("cuda") if len(idx) else None
+    base = idx.ctypes.data
+    bad = torch.zeros((), dtype=torch.int64, device="cuda")
+    for i in np.nonzero(select)[0]:
+        r = reqs[i]
+        c, s = int(r["count"]), int(starts[i])
+        got = buf[s:s + c]
+        v = values[int(r["column"])]
+        if r["kind"] == REQ_LOOKUP:
+            q = (int(r["h_idx"]) - base) // 8
+            want = v[idx_dev[q:q + c]]
+        else:
+            b = int(r["begin"])
+            want = v[b:b + c]
+        bad += (got != want).sum()
+    return int(bad.item())

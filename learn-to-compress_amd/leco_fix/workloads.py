@@ -172,13 +172,26 @@ def _latency_stats(res, reqs, kinds=(srvmod.REQ_LOOKUP, srvmod.REQ_RANGE), split
     return out
 
 
-def run_serving(col, rates=(1e5, 1e6, 1e7), nreq: int = 60_000, plain_nreq: int = 6_000, seed: int = 4,
-                reserve_bytes: int = 64 << 30, log=print) -> dict:
+def _verify_sample(rng, reqs, per_kind: int):
+    """A seeded sample of per_kind lookups and per_kind ranges of a trace."""
+    sel = np.zeros(len(reqs), bool)
+    for kind in (srvmod.REQ_LOOKUP, srvmod.REQ_RANGE):
+        ids = np.nonzero(reqs["kind"] == kind)[0]
+        sel[rng.choice(ids, min(per_kind, len(ids)), replace=False)] = True
+    return sel
+
+
+def run_serving(col, values=None, rates=(1e5, 1e6, 1e7), nreq: int = 60_000, plain_nreq: int = 6_000,
+                seed: int = 4, reserve_bytes: int = 64 << 30, verify_per_kind: int = 200, log=print) -> dict:
     """Poisson traffic on one resident column: 80% lookups of U[1, 4096]
     indices, 20% ranges of log-uniform length in [1e2, 1e7].  One untimed
     run of 2000 requests first, so that the measured runs see a warm server
     (the first launch of a kernel loads its code object, which otherwise
-    stalls the first run's queue for milliseconds)."""
+    stalls the first run's queue for milliseconds).  Every run checks that
+    all requests completed with status 0; with `values` (the raw column in
+    HBM), a seeded sample of verify_per_kind lookups and ranges of each
+    measured batched run writes its results to HBM (d_out), and they are
+    compared with the column on the device."""
     srv = srvmod.Server(slots=4, reserve_bytes=reserve_bytes)
     srv.attach(0, col)
     specs = [(0, col.n, col.width)]
@@ -187,15 +200,28 @@ def run_serving(col, rates=(1e5, 1e6, 1e7), nreq: int = 60_000, plain_nreq: int 
     srv.run(warm, srvmod.SERVE_BATCHED)  # the requests point into warm_idx, alive until here
     srv.run(warm[:200], srvmod.SERVE_PLAIN)
     del warm, warm_idx
+    verified, checked, statuses_ok = True, 0, True
     for rate in rates:
         rng = np.random.default_rng(seed)
         reqs, idx, _ = srvmod.make_trace(rng, specs, rate, nreq)
+        sel = buf = None
+        if values is not None:
+            sel = _verify_sample(np.random.default_rng(seed + 7), reqs, verify_per_kind)
+            buf, starts = srvmod.device_outputs(reqs, sel, col.width)
         log(f"serving: batched at {rate:.0e} req/s, {nreq} requests")
         res, st = srv.run(reqs, srvmod.SERVE_BATCHED)
+        slow = srv.slowest_iterations()[:3]
+        statuses_ok = statuses_ok and bool((res["status"] == 0).all())
+        if sel is not None:
+            bad = srvmod.check_device_outputs(reqs, idx, sel, buf, starts, {0: values})
+            verified = verified and bad == 0
+            checked += int(sel.sum())
+            del buf
         row = {"offered_req_s": rate, "requests": nreq, "completed": st["completed"],
                "throughput_req_s": round(st["completed"] / st["wall_s"], 1),
                "gbs": round(st["bytes_out"] / st["wall_s"] / 1e9, 1), "batches": st["batches"],
-               "max_batch_requests": st["max_batch_requests"], **_latency_stats(res, reqs)}
+               "max_batch_requests": st["max_batch_requests"], **_latency_stats(res, reqs),
+               "slowest_loop_iterations": slow}
         out["rates"].append(row)
         del idx
     # the same kind of trace, one launch + sync per request
@@ -204,10 +230,12 @@ def run_serving(col, rates=(1e5, 1e6, 1e7), nreq: int = 60_000, plain_nreq: int 
     reqs, idx, _ = srvmod.make_trace(rng, specs, rate, plain_nreq)
     log(f"serving: plain per-request path at {rate:.0e} req/s, {plain_nreq} requests")
     res, st = srv.run(reqs, srvmod.SERVE_PLAIN)
+    statuses_ok = statuses_ok and bool((res["status"] == 0).all())
     plain = {"offered_req_s": rate, "requests": plain_nreq,
              "throughput_req_s": round(st["completed"] / st["wall_s"], 1),
              "gbs": round(st["bytes_out"] / st["wall_s"] / 1e9, 1), **_latency_stats(res, reqs)}
     res2, st2 = srv.run(reqs, srvmod.SERVE_BATCHED)
+    statuses_ok = statuses_ok and bool((res2["status"] == 0).all())
     same = {"throughput_req_s": round(st2["completed"] / st2["wall_s"], 1),
             "gbs": round(st2["bytes_out"] / st2["wall_s"] / 1e9, 1), **_latency_stats(res2, reqs)}
     out["plain_same_trace"] = plain
@@ -217,8 +245,9 @@ def run_serving(col, rates=(1e5, 1e6, 1e7), nreq: int = 60_000, plain_nreq: int 
     sat = reqs.copy()
     sat["arrival_s"] = 0.0
     log(f"serving: saturation, {plain_nreq} requests at t=0, plain then batched")
-    _, st_p = srv.run(sat, srvmod.SERVE_PLAIN)
-    _, st_b = srv.run(sat, srvmod.SERVE_BATCHED)
+    res_p, st_p = srv.run(sat, srvmod.SERVE_PLAIN)
+    res_b, st_b = srv.run(sat, srvmod.SERVE_BATCHED)
+    statuses_ok = statuses_ok and bool((res_p["status"] == 0).all()) and bool((res_b["status"] == 0).all())
     out["saturation"] = {
         "requests": plain_nreq,
         "plain_req_s": round(st_p["completed"] / st_p["wall_s"], 1),
@@ -227,6 +256,9 @@ def run_serving(col, rates=(1e5, 1e6, 1e7), nreq: int = 60_000, plain_nreq: int 
         "batched_gbs": round(st_b["bytes_out"] / st_b["wall_s"] / 1e9, 1),
         "gbs_ratio": round((st_b["bytes_out"] / st_b["wall_s"]) / (st_p["bytes_out"] / st_p["wall_s"]), 2),
     }
+    out["all_status_ok"] = statuses_ok
+    out["verified"] = verified if values is not None else None
+    out["verified_requests"] = checked
     srv.close()
     return out
 

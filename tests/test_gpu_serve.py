@@ -181,3 +181,52 @@ def test_churn_insert_evict_while_serving(oracle_mod):
         want, _ = oracle_mod.encode_column(data[s], bss[s])
         assert got == want.tobytes(), s
     srv.close()
+
+
+def test_config4_shape_served_results_equal_the_column():
+    """The serving workload at its full shape: the resident 500M-key column
+    (sorted piecewise-linear uint64, block size 4096), Poisson arrivals at
+    10^6 req/s, 80% lookups of 1-4096 random indices and 20% ranges of
+    log-uniform length in [10^2, 10^7].  Every result is written to HBM by
+    the server (d_out) and compared with the input column on the device; a
+    subset of requests, run again alone on the plain path, gives the same
+    values as in the mix."""
+    import torch
+    from leco_fix import Column, datagen
+    from leco_fix.server import (REQ_LOOKUP, REQ_RANGE, SERVE_PLAIN, Server, check_device_outputs,
+                                 device_outputs, make_trace)
+    n = 500_000_000
+    say("config-4 shape: generate and encode the 500M-key column")
+    vals = datagen.sorted_pwl_u64(n, seed=2, device="cuda")
+    col = Column.encode(vals, 4096)
+    srv = Server(slots=4, reserve_bytes=0, result_pool_bytes=1 * GB, column_pool_bytes=64 << 20)
+    srv.attach(0, col)
+    rng = np.random.default_rng(44)
+    reqs, idx, _ = make_trace(rng, [(0, n, 64)], rate=1e6, nreq=3000, max_lookup=4096, range_hi=1e7)
+    kinds = reqs["kind"]
+    assert (kinds == REQ_RANGE).sum() > 400 and reqs["count"][kinds == REQ_RANGE].max() > 1_000_000
+    everything = np.ones(len(reqs), bool)
+    buf, starts = device_outputs(reqs, everything, 64)
+    say(f"config-4 shape: batched run of {len(reqs)} requests, {int(reqs['count'].sum())} values out")
+    res, stats = srv.run(reqs)
+    assert (res["status"] == 0).all() and stats["completed"] == len(reqs)
+    assert stats["max_batch_requests"] > 1
+    bad = check_device_outputs(reqs, idx, everything, buf, starts, {0: vals})
+    assert bad == 0
+    # alone = in the mix: 24 lookups and 24 ranges (the longest included), one request at a time
+    lk = np.nonzero(kinds == REQ_LOOKUP)[0][:24]
+    rg = np.nonzero(kinds == REQ_RANGE)[0]
+    rg = np.unique(np.concatenate([rg[:23], [rg[np.argmax(reqs['count'][rg])]]]))
+    sub = np.sort(np.concatenate([lk, rg]))
+    alone = reqs[sub].copy()
+    alone["arrival_s"] = np.arange(len(sub)) * 1e-4
+    abuf, astarts = device_outputs(alone, np.ones(len(sub), bool), 64)
+    say(f"config-4 shape: {len(sub)} requests alone on the plain path")
+    res2, _ = srv.run(alone, SERVE_PLAIN)
+    assert (res2["status"] == 0).all()
+    for k, i in enumerate(sub):
+        c = int(reqs["count"][i])
+        assert torch.equal(abuf[int(astarts[k]):int(astarts[k]) + c], buf[int(starts[i]):int(starts[i]) + c]), i
+    srv.close()
+    del buf, abuf, col, vals
+    torch.cuda.empty_cache()

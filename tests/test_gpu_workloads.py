@@ -37,9 +37,11 @@ def test_serving_and_churn_drivers_small():
     from leco_fix import Column, workloads
     vals = workloads.gen_column("sorted_uniform", 3_000_000, 64, 9)
     col = Column.encode(vals, 4096)
-    out = workloads.run_serving(col, rates=(1e5, 1e6), nreq=3000, plain_nreq=400, reserve_bytes=8 << 30, log=say)
+    out = workloads.run_serving(col, vals, rates=(1e5, 1e6), nreq=3000, plain_nreq=400, reserve_bytes=8 << 30,
+                                log=say)
     for row in out["rates"]:
         assert row["completed"] == 3000 and row["p99_us"] is not None
+    assert out["all_status_ok"] and out["verified"] and out["verified_requests"] == 2 * 400
     assert out["plain_same_trace"]["throughput_req_s"] > 0
     ch = workloads.run_churn(ncols=40, sources=4, resident=6, insert_rate=2000, req_rate=5e4, lo=1e4, hi=2e6,
                              reserve_bytes=8 << 30, log=say)
