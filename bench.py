@@ -66,8 +66,86 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def config1_section(torch, cpu: bool) -> dict:
+    """The reference's own workload (experiments/leco_int_fix_template.cpp:98-213):
+    10M uint32 y_i = clamp(round(1000 i + N(0, 50^2)), 0, 2^32-1), seed 1,
+    block size 10,000 (1000 blocks).  On the GPU: compression ratio, encode,
+    full decode over 10 repeats, 1M uniform lookups, round trip and the stream
+    against the oracle's bytes.  On the host (cpu=True): the oracle's encode,
+    the refill decode loop (the shape of read_all_bit_fix, bit_read.h:222-275)
+    over 10 repeats, and 1M lookups, on 1 thread and on all cores."""
+    import numpy as np
+
+    from leco_fix import Column, datagen
+    from leco_fix._lib import lib
+
+    sys.path.insert(0, ROOT)
+    from oracle import oracle
+
+    n, bs = 10_000_000, 10_000
+    y = datagen.linear_noise_u32(n, seed=1)
+    raw = 4 * n
+    dev = torch.from_numpy(y.view(np.int32)).to("cuda")
+    col = Column.encode(dev, bs)
+    out = torch.empty_like(dev)
+    col.decode(out=out)
+    ok = bool(torch.equal(out, dev))
+    idx_h = np.random.default_rng(1).integers(0, n, 1_000_000, dtype=np.uint64)
+    idx = torch.from_numpy(idx_h.view(np.int64)).to("cuda")
+    got = col.lookup(idx)
+    ok = ok and bool(torch.equal(got, dev[idx]))
+    nthr = min(16, os.cpu_count() or 1)
+    want, offs = oracle.encode_column(y, bs, threads=nthr)
+    same_stream = col.stream_bytes() == want.tobytes()
+
+    def enc():
+        lib().leco_fix_encode_column_u32(dev.data_ptr(), n, bs, col.stream.data_ptr(), col.stream.numel(),
+                                         col.offsets.data_ptr(), col.params.data_ptr(),
+                                         torch.cuda.current_stream().cuda_stream)
+    ms_enc, _ = timed(torch, enc, 10, 2)
+    ms_dec, _ = timed(torch, lambda: col.decode(out=out), 10, 2)
+    ms_lk, _ = timed(torch, lambda: col.lookup(idx, out=got), 10, 2)
+    col.decode(out=out)
+    ok = ok and bool(torch.equal(out, dev))
+    res = {"workload": "10M uint32, y_i = clamp(round(1000 i + N(0, 50^2))), seed 1, block size 10000",
+           "compression_ratio": round(col.nbytes / raw, 5), "compressed_bytes": col.nbytes,
+           "round_trip_exact": ok, "stream_equals_oracle": same_stream,
+           "gpu": {"encode_gbs": round(raw / (ms_enc * 1e-3) / 1e9, 1), "encode_ms": round(ms_enc, 4),
+                   "decode_gbs": round(raw / (ms_dec * 1e-3) / 1e9, 1), "decode_ms": round(ms_dec, 4),
+                   "lookups": len(idx_h), "lookup_batch_ms": round(ms_lk, 4),
+                   "lookups_per_s": round(len(idx_h) / (ms_lk * 1e-3), 0),
+                   "ns_per_lookup_in_batch": round(ms_lk * 1e6 / len(idx_h), 4)}}
+    if cpu:
+        host = {"threads_all": nthr, "cpu_model": cpu_model()}
+        for tag, thr in (("1_thread", 1), ("all_cores", nthr)):
+            t0 = time.perf_counter()
+            st, of = oracle.encode_column(y, bs, threads=thr)
+            t_enc = time.perf_counter() - t0
+            dec = np.empty_like(y)
+            oracle.decode_column(st, of, n, bs, np.uint32, threads=thr, out=dec, refill=True)  # warm-up
+            t0 = time.perf_counter()
+            for _ in range(10):
+                oracle.decode_column(st, of, n, bs, np.uint32, threads=thr, out=dec, refill=True)
+            t_dec = (time.perf_counter() - t0) / 10
+            t0 = time.perf_counter()
+            lk = oracle.lookup_column(st, of, n, bs, idx_h, np.uint32, threads=thr)
+            t_lk = time.perf_counter() - t0
+            host[tag] = {"encode_gbs": round(raw / t_enc / 1e9, 3), "decode_gbs": round(raw / t_dec / 1e9, 3),
+                         "decode_ns_per_value": round(t_dec * 1e9 / n, 3),
+                         "ns_per_lookup": round(t_lk * 1e9 / len(idx_h), 2),
+                         "round_trip_exact": bool(np.array_equal(dec, y)) and bool(np.array_equal(lk, y[idx_h])),
+                         "stream_equals_gpu": st.tobytes() == want.tobytes()}
+        res["cpu"] = host
+    del dev, out, idx, got, col
+    torch.cuda.empty_cache()
+    return res
+
+
 def cpu_baseline(vals_dev, bs: int, sample: int, repeat: int):
-    """Oracle decode (1 thread) on the first `sample` values of the column."""
+    """Decode of the first `sample` values of the column on the host: the
+    oracle's refill loop (the loop shape of the reference's read_all_bit_fix,
+    bit_read.h:222-275) on 1 thread and on all cores, and the byte-window
+    decode of the checker on 1 thread."""
     import numpy as np
 
     sys.path.insert(0, ROOT)
@@ -77,19 +155,23 @@ def cpu_baseline(vals_dev, bs: int, sample: int, repeat: int):
     stream, offs = oracle.encode_column(y, bs, threads=min(16, os.cpu_count() or 1))
     padded = np.concatenate([stream, np.zeros(64, np.uint8)])
     out = np.empty(len(y), np.uint64)
-    oracle.decode_column(padded, offs, len(y), bs, np.uint64, threads=1, out=out)  # warm-up
+    oracle.decode_column(padded, offs, len(y), bs, np.uint64, threads=1, out=out, refill=True)  # warm-up
     t0 = time.perf_counter()
     for _ in range(repeat):
-        oracle.decode_column(padded, offs, len(y), bs, np.uint64, threads=1, out=out)
+        oracle.decode_column(padded, offs, len(y), bs, np.uint64, threads=1, out=out, refill=True)
     t1 = time.perf_counter()
     assert np.array_equal(out, y)
     gbs1 = len(y) * 8 * repeat / (t1 - t0) / 1e9
     nthr = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
     for _ in range(repeat):
-        oracle.decode_column(padded, offs, len(y), bs, np.uint64, threads=nthr, out=out)
+        oracle.decode_column(padded, offs, len(y), bs, np.uint64, threads=nthr, out=out, refill=True)
     t1 = time.perf_counter()
     gbsn = len(y) * 8 * repeat / (t1 - t0) / 1e9
+    assert np.array_equal(out, y)
+    t0 = time.perf_counter()
+    oracle.decode_column(padded, offs, len(y), bs, np.uint64, threads=1, out=out)
+    gbs_window = len(y) * 8 / (time.perf_counter() - t0) / 1e9
     # filter_range at the sample's median, 1 thread
     f = int(y[len(y) // 2])
     oracle.filter_column(padded, offs, len(y), bs, np.uint64, f)
@@ -99,7 +181,7 @@ def cpu_baseline(vals_dev, bs: int, sample: int, repeat: int):
     t1 = time.perf_counter()
     assert len(got) == int(np.count_nonzero(y > np.uint64(f)))
     gbsf = len(y) * 8 * max(1, repeat // 2) / (t1 - t0) / 1e9
-    return gbs1, gbsn, nthr, gbsf
+    return gbs1, gbsn, nthr, gbsf, gbs_window
 
 
 def cpu_rows(vals_dev, bs: int, sample: int) -> dict:
@@ -361,6 +443,8 @@ def main() -> int:
     ap.add_argument("--cpu-sample", type=int, default=64_000_000)
     ap.add_argument("--cpu-repeat", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the reference's own workload (10M uint32, block size 10000)")
     ap.add_argument("--copy-reps", type=int, default=3,
                     help="plain device copies of the decoded column (a known-size HBM reference kernel)")
     ap.add_argument("--no-filter", action="store_true", help="skip the filter_range / filter_range_close section")
@@ -447,6 +531,12 @@ def main() -> int:
     lat_us.sort()
     ra_p50, ra_p99 = lat_us[len(lat_us) // 2], lat_us[int(len(lat_us) * 0.99)]
 
+    config1 = None
+    if not args.no_config1:
+        log("config 1: 10M uint32, block size 10000, GPU and host")
+        config1 = config1_section(torch, not args.no_cpu)
+        log(f"config1: {config1}")
+
     base = None
     if not args.no_baselines:
         log("baselines: FOR_int / Delta_int on the same column")
@@ -511,12 +601,15 @@ def main() -> int:
     if not args.no_cpu:
         sample = min(args.cpu_sample, n)
         log(f"cpu baseline: oracle decode of the first {sample} values, {args.cpu_repeat} repeats")
-        gbs1, gbsn, nthr, gbsf = cpu_baseline(vals, bs, sample, args.cpu_repeat)
+        gbs1, gbsn, nthr, gbsf, gbsw = cpu_baseline(vals, bs, sample, args.cpu_repeat)
         cpu = {"value": round(gbs1, 3), "unit": "GB/s", "cores": 1, "kind": "port",
                "sample": f"first {sample} values of the column ({(sample + bs - 1) // bs} blocks), "
-                         f"decoded {args.cpu_repeat}x by the C oracle, 1 thread",
+                         f"decoded {args.cpu_repeat}x, 1 thread, by the oracle's refill loop (the loop shape of "
+                         "read_all_bit_fix, bit_read.h:222-275: 64-bit words into a 128-bit register, "
+                         "fma + CVTTSD2SI per value)",
                "all_cores_value": round(gbsn, 3), "all_cores": nthr,
                "host_cpus": os.cpu_count(), "cpu_model": cpu_model(),
+               "checker_byte_window_decode_value": round(gbsw, 3),
                "filter_range_value": round(gbsf, 3)}
         log("cpu baseline: encode, lookups, filter_range_close, FOR / Delta decode on the same sample, 1 thread")
         cpu["rows_1_thread"] = cpu_rows(vals, bs, sample)
@@ -573,6 +666,7 @@ def main() -> int:
             "bytes_per_launch": comp + raw_bytes,
         },
         "cpu_baseline": cpu,
+        "config1": config1,
         "baselines": base,
         "opmax": opmax,
         "filter": filt,

@@ -561,8 +561,10 @@ typedef struct leco_fix_server_iteration {
     float completions_us;    /* finished batches and inserts */
     float admission_us;      /* the admission scan: checks, result memory, work items */
     float copies_us;         /* lookup indices into pinned staging */
-    float launch_us;         /* staging copies, the launch, the completion event */
+    float launch_us;         /* the launch, result copies to h_out, the completion event */
     float rest_us;           /* the rest of the iteration */
+    float kernel_us;         /* of launch_us: the kernel launch */
+    float event_us;          /* of launch_us: result copies to h_out and the completion event */
     uint32_t admitted;       /* requests admitted in this iteration */
     uint32_t queued;         /* requests waiting when it started */
     uint32_t reserved;

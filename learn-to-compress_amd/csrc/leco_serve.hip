@@ -8,7 +8,10 @@
 // a ring-allocated pool; columns inserted while traffic runs are encoded on a
 // separate stream into a first-fit column pool and evicted once no admitted
 // request uses them.  The pools are carved once from free device memory minus
-// a reserve, which is the memory admission.
+// a reserve, which is the memory admission.  A batch's work items and lookup
+// indices are written into its slot's pinned, coherent host staging, which
+// the kernel reads over the host link: the loop enqueues no host-to-device
+// copy per batch (such copies occasionally blocked the host call for ~10 ms).
 //
 // The plain mode serves the same trace with one launch and one synchronize
 // per request (the per-request launch path), for comparison.
@@ -248,7 +251,7 @@ struct Slot {
     std::vector<uint64_t> reqs;
     std::vector<uint64_t> ring_ids;
     uint64_t *h_idx = nullptr, *d_idx = nullptr;
-    leco::ServeItem *h_items = nullptr, *d_items = nullptr;
+    leco::ServeItem *h_items = nullptr;
 };
 
 struct ColumnSlot {
@@ -320,7 +323,6 @@ void destroy_server(leco_fix_server *s) {
         if (sl.h_idx) (void)hipHostFree(sl.h_idx);
         if (sl.h_items) (void)hipHostFree(sl.h_items);
         (void)hipFree(sl.d_idx);
-        (void)hipFree(sl.d_items);
     }
     for (ColumnSlot &c : s->cols) {
         if (c.e0) (void)hipEventDestroy(c.e0);
@@ -438,33 +440,12 @@ int leco_fix_server_create(const leco_fix_server_config *cfg_in, leco_fix_server
         for (Slot &sl : s->slots) {
             ok = ok && hipStreamCreateWithFlags(&sl.st, hipStreamNonBlocking) == hipSuccess &&
                  hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) == hipSuccess &&
-                 hipHostMalloc((void **)&sl.h_idx, kIdxCap * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess &&
-                 hipHostMalloc((void **)&sl.h_items, kItemCap * sizeof(leco::ServeItem), hipHostMallocDefault) ==
+                 hipHostMalloc((void **)&sl.h_idx, kIdxCap * sizeof(uint64_t), hipHostMallocCoherent) == hipSuccess &&
+                 hipHostMalloc((void **)&sl.h_items, kItemCap * sizeof(leco::ServeItem), hipHostMallocCoherent) ==
                      hipSuccess &&
-                 hipMalloc(&sl.d_idx, kIdxCap * sizeof(uint64_t)) == hipSuccess &&
-                 hipMalloc(&sl.d_items, kItemCap * sizeof(leco::ServeItem)) == hipSuccess;
+                 hipMalloc(&sl.d_idx, kIdxCap * sizeof(uint64_t)) == hipSuccess;
         }
         if (!ok) { destroy_server(s); return LECO_FIX_ENOMEM; }
-        // The runtime sets a stream up on its first command, which blocked the
-        // admission loop for ~7 ms the first time a run needed a third or
-        // fourth batch in flight (the loop's phase trace put the stall in the
-        // launch step of a 1-request batch, once per slot, in the first run
-        // only).  Each slot's stream and the insert stream therefore run an
-        // empty served launch, a staging copy and the completion event here.
-        for (Slot &sl : s->slots) {
-            if (hipMemcpyAsync(sl.d_items, sl.h_items, sizeof(leco::ServeItem), hipMemcpyHostToDevice, sl.st) !=
-                    hipSuccess ||
-                leco::launch_serve(s->d_cols, sl.d_items, 0, 1, sl.st) != hipSuccess ||
-                hipEventRecord(sl.done, sl.st) != hipSuccess || hipStreamSynchronize(sl.st) != hipSuccess) {
-                destroy_server(s);
-                return LECO_FIX_EHIP;
-            }
-        }
-        if (leco::launch_serve(s->d_cols, s->slots[0].d_items, 0, 1, s->enc) != hipSuccess ||
-            hipStreamSynchronize(s->enc) != hipSuccess) {
-            destroy_server(s);
-            return LECO_FIX_EHIP;
-        }
         const unsigned hw = std::thread::hardware_concurrency();
         if (!s->copies.start(hw > 1 ? std::min(kCopyHelpers, hw - 1) : 0)) {
             destroy_server(s);
@@ -677,6 +658,11 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
             const size_t window = 4 * (size_t)s->cfg.max_batch_requests;  // queue entries examined per pass
             size_t controls = 0;  // insert / evict requests in the queue
             std::vector<uint32_t> pending_inserts;
+            // per-iteration lists, kept across iterations so the loop does not allocate once they have grown
+            std::vector<uint64_t> keep, admitted, ids;
+            keep.reserve(window);
+            admitted.reserve(s->cfg.max_batch_requests);
+            ids.reserve(2 * (size_t)s->cfg.max_batch_requests);
             s->trace.reset();
             Clock::time_point prev_end = Clock::now();
             while (done < nreq) {
@@ -743,11 +729,11 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                 for (Slot &sl : s->slots)
                     if (!sl.busy) { free_slot = &sl; break; }
                 if (!queue.empty()) {
-                    std::vector<uint64_t> keep;
+                    keep.clear();
                     const size_t scan = std::min(queue.size(), window);
                     uint64_t nitems = 0, nidx = 0, nvals = 0;
                     bool full = free_slot == nullptr;
-                    std::vector<uint64_t> admitted;
+                    admitted.clear();
                     // Once the batch is full, every data request is held; the scan
                     // goes on only for insert / evict requests further back.
                     size_t ctl_ahead = controls, ctl_kept = 0, qi = 0;
@@ -828,7 +814,7 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                                 leco::ServeItem &w = sl.h_items[nitems++];
                                 w = leco::ServeItem{leco::kServeLookup, r.column,
                                                     (uint32_t)std::min<uint64_t>(kLookupChunk, r.count - q0), 0,
-                                                    sl.d_idx + nidx + q0, out + q0 * bw, 0, 0, 0};
+                                                    sl.h_idx + nidx + q0, out + q0 * bw, 0, 0, 0};
                             }
                             nidx += r.count;
                         } else if (staged) {
@@ -864,15 +850,14 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                     it.admitted = (uint32_t)admitted.size();
                     if (!admitted.empty()) {
                         Slot &sl = *free_slot;
-                        hipError_t e = hipMemcpyAsync(sl.d_items, sl.h_items, nitems * sizeof(leco::ServeItem),
-                                                      hipMemcpyHostToDevice, sl.st);
-                        if (e == hipSuccess && nidx)
-                            e = hipMemcpyAsync(sl.d_idx, sl.h_idx, nidx * sizeof(uint64_t), hipMemcpyHostToDevice,
-                                               sl.st);
-                        if (e == hipSuccess)
-                            e = leco::launch_serve(s->d_cols, sl.d_items, (uint32_t)nitems,
-                                                   (uint32_t)std::min<uint64_t>(nitems, kServeGrid), sl.st);
-                        std::vector<uint64_t> ids;
+                        // the kernel reads the work items and lookup indices straight from the
+                        // slot's pinned (fine-grained, coherent) staging over the host link: no
+                        // host-to-device copy is enqueued per batch
+                        hipError_t e = leco::launch_serve(s->d_cols, sl.h_items, (uint32_t)nitems,
+                                                          (uint32_t)std::min<uint64_t>(nitems, kServeGrid), sl.st);
+                        const Clock::time_point q2 = Clock::now();
+                        it.kernel_us = (float)(secs(p4, q2) * 1e6);
+                        ids.clear();
                         for (size_t k = 0; k < sl.reqs.size(); k += 2) {
                             const uint64_t ri = sl.reqs[k], src = sl.reqs[k + 1];
                             ids.push_back(ri);
@@ -892,6 +877,7 @@ int leco_fix_server_run(leco_fix_server *s, const leco_fix_request *reqs, uint64
                         st.max_batch_requests = std::max<uint64_t>(st.max_batch_requests, admitted.size());
                         progressed = true;
                         p5 = Clock::now();
+                        it.event_us = (float)(secs(q2, p5) * 1e6);
                     }
                     st.peak_result_bytes = std::max(st.peak_result_bytes, s->ring.used());
                     st.peak_column_bytes = std::max(st.peak_column_bytes, s->colfree.used());

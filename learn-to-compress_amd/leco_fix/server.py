@@ -27,12 +27,12 @@ STATS_FIELDS = ["wall_s", "batches", "completed", "failed", "max_batch_requests"
                 "encoded_values", "encoded_bytes", "encode_gpu_s", "peak_result_bytes", "peak_column_bytes"]
 STATS_DTYPE = np.dtype([(f, "<f8" if f.endswith("_s") else "<u8") for f in STATS_FIELDS], align=True)
 ITERATION_FIELDS = ["between_us", "arrivals_us", "completions_us", "admission_us", "copies_us", "launch_us",
-                    "rest_us"]
+                    "rest_us", "kernel_us", "event_us"]
 ITERATION_DTYPE = np.dtype([("start_s", "<f8")] + [(f, "<f4") for f in ITERATION_FIELDS] +
                            [("admitted", "<u4"), ("queued", "<u4"), ("reserved", "<u4")], align=True)
 TRACE_CAP = 16
 assert REQUEST_DTYPE.itemsize == 64 and RESULT_DTYPE.itemsize == 32 and STATS_DTYPE.itemsize == 96
-assert ITERATION_DTYPE.itemsize == 48
+assert ITERATION_DTYPE.itemsize == 56
 
 
 class ServerConfig(ctypes.Structure):
@@ -58,7 +58,10 @@ class Server:
         self._h = None
 
     def __del__(self):
-        self.close()
+        try:
+            self.close()
+        except TypeError:  # interpreter shutdown: the module's globals are already gone
+            pass
 
     def attach(self, slot: int, col) -> None:
         """Serves a Column (or batch column view) in `slot`."""

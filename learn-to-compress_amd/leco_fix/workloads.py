@@ -265,16 +265,26 @@ def run_serving(col, values=None, rates=(1e5, 1e6, 1e7), nreq: int = 60_000, pla
 
 def run_churn(ncols: int = 2000, sources: int = 16, resident: int = 48, insert_rate: float = 2000.0,
               req_rate: float = 1e5, lo: float = 1e4, hi: float = 1e8, seed: int = 5,
-              reserve_bytes: int = 64 << 30, log=print) -> dict:
+              reserve_bytes: int = 64 << 30, verify_per_kind: int = 200, log=print) -> dict:
     """Columns inserted (GPU encode) and evicted while lookup/range traffic
-    runs: `ncols` inserts of uint64 drift columns (lengths log-uniform in
-    [lo, hi], drawn from `sources` resident raw buffers), at most `resident`
-    columns live, traffic on the live ones."""
+    runs.  Each of the `ncols` inserts draws its own length, log-uniform in
+    [lo, hi]; its values are a window of that length, at a random offset, of
+    one of `sources` uint64 drift columns of `hi` values (a*i + b*i^2 +
+    N(0, 2^10), a ~ U[1, 2^16], b ~ logU[1e-6, 1e-3], each source its own
+    a, b and noise) generated in HBM before the run: a bounded pool of
+    sources * hi * 8 bytes.  A window of a drift column is a drift column (the
+    same a and b, shifted).  At most `resident` columns live; traffic (80%
+    lookups of 1-4096 indices, 20% ranges of log-uniform length in
+    [1e2, 1e7]) goes to live columns; a seeded sample of verify_per_kind
+    lookups and ranges writes to HBM and is compared with its window."""
     rng = np.random.default_rng(seed)
-    lens = log_uniform_lengths(rng, sources, lo, hi)
-    log(f"churn: {sources} raw sources ({lens.sum() * 8 / 1e9:.1f} GB), block sizes by autotune")
-    raws = [gen_column("drift", int(n), 64, seed * 1000 + s) for s, n in enumerate(lens)]
-    bss, _ = autotune_block_sizes(raws, candidates=(1024, 2048, 4096, 8192), window=8192)
+    lens = log_uniform_lengths(rng, ncols, lo, hi)
+    src_of = np.arange(ncols) % sources
+    offs = (rng.random(ncols) * (int(hi) - lens + 1)).astype(np.int64)
+    log(f"churn: {sources} pool sources of {int(hi)} values ({sources * int(hi) * 8 / 1e9:.1f} GB), "
+        f"{ncols} inserts totalling {lens.sum() * 8 / 1e9:.1f} GB raw; block sizes by autotune")
+    pool = [gen_column("drift", int(hi), 64, seed * 1000 + s) for s in range(sources)]
+    bss, _ = autotune_block_sizes(pool, candidates=(1024, 2048, 4096, 8192), window=8192)
     R = srvmod.REQUEST_DTYPE
     rows = []
     t = 0.0
@@ -283,12 +293,13 @@ def run_churn(ncols: int = 2000, sources: int = 16, resident: int = 48, insert_r
     idx_bufs = []
     live = []
     for i in range(ncols):
-        s = i % sources
+        s, L = int(src_of[i]), int(lens[i])
         r = np.zeros(1, R)[0]
-        r["kind"], r["column"], r["count"] = srvmod.REQ_INSERT, i, lens[s]
-        r["d_values"], r["block_size"], r["width_bits"], r["arrival_s"] = raws[s].data_ptr(), bss[s], 64, t
+        r["kind"], r["column"], r["count"] = srvmod.REQ_INSERT, i, L
+        r["d_values"] = pool[s].data_ptr() + 8 * int(offs[i])
+        r["block_size"], r["width_bits"], r["arrival_s"] = bss[s], 64, t
         rows.append(r)
-        live.append((i, int(lens[s])))
+        live.append((i, L))
         if len(live) > resident:
             ev, _ = live.pop(0)
             r = np.zeros(1, R)[0]
@@ -310,19 +321,48 @@ def run_churn(ncols: int = 2000, sources: int = 16, resident: int = 48, insert_r
             rows.append(r)
         t += dt_ins
     reqs = np.array(rows, R)
+    # the sampled requests' lookup indices, gathered into one buffer for the check
+    sel = _verify_sample(np.random.default_rng(seed + 3), reqs, verify_per_kind) if verify_per_kind else None
+    buf = starts = None
+    if sel is not None:
+        buf, starts = srvmod.device_outputs(reqs, sel, 64)
     srv = srvmod.Server(slots=4, reserve_bytes=reserve_bytes)
     log(f"churn: {len(reqs)} requests ({ncols} inserts) over {t:.2f} s")
     res, st = srv.run(reqs, srvmod.SERVE_BATCHED)
     srv.close()
+    verified = None
+    if sel is not None:
+        cols = {int(c): None for c in reqs["column"][sel]}
+        values = {c: pool[int(src_of[c])][int(offs[c]):int(offs[c]) + int(lens[c])] for c in cols}
+        allidx = np.concatenate(idx_bufs) if idx_bufs else np.zeros(0, np.uint64)
+        # check_device_outputs reads lookup indices at h_idx relative to one buffer: rebase them
+        chk = reqs.copy()
+        at = 0
+        for i in np.nonzero(chk["kind"] == srvmod.REQ_LOOKUP)[0]:
+            chk["h_idx"][i] = allidx.ctypes.data + 8 * at
+            at += int(chk["count"][i])
+        bad = srvmod.check_device_outputs(chk, allidx, sel, buf, starts, values)
+        verified = bad == 0 and bool((res["status"][sel] == 0).all())
+        del buf
     ins = reqs["kind"] == srvmod.REQ_INSERT
     ok_ins = int((res["status"][ins] == 0).sum())
     traffic = _latency_stats(res, reqs)
+    edges = [10 ** k for k in range(int(math.log10(lo)), int(math.ceil(math.log10(hi))) + 1)]
+    hist = {f"[{edges[k]:.0e},{edges[k + 1]:.0e})": int(((lens >= edges[k]) & (lens < edges[k + 1])).sum())
+            for k in range(len(edges) - 1)}
+    hist[f"[{edges[-2]:.0e},{edges[-1]:.0e})"] += int((lens >= edges[-1]).sum())
+    del pool
     return {
         "inserts": ncols, "inserts_ok": ok_ins, "requests": int((~ins & (reqs["kind"] <= 1)).sum()),
         "failed": st["failed"], "wall_s": round(st["wall_s"], 3),
+        "insert_lengths": {"distribution": f"log-uniform in [{lo:.0e}, {hi:.0e}], one draw per insert",
+                           "histogram": hist, "min": int(lens.min()), "max": int(lens.max()),
+                           "pool_sources": sources, "pool_gb": round(sources * int(hi) * 8 / 1e9, 2)},
+        "realized_raw_gb": round(float(lens.sum()) * 8 / 1e9, 2),
         "encode_gbs": round(st["encoded_bytes"] / st["encode_gpu_s"] / 1e9, 1) if st["encode_gpu_s"] else None,
-        "encoded_gb": round(st["encoded_bytes"] / 1e9, 1),
+        "encoded_gb": round(st["encoded_bytes"] / 1e9, 2),
         "decode_gbs": round(st["bytes_out"] / st["wall_s"] / 1e9, 1),
         "insert_p99_ms": round(float(np.percentile(res["latency_s"][ins], 99)) * 1e3, 2),
         **traffic, "peak_column_pool_gb": round(st["peak_column_bytes"] / 1e9, 2),
+        "verified": verified, "verified_requests": int(sel.sum()) if sel is not None else 0,
     }

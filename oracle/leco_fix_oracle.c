@@ -33,6 +33,7 @@
  * cvtt64 is x86 CVTTSD2SI r64: truncate toward zero, NaN or out-of-range
  * gives INT64_MIN.
  */
+#include <emmintrin.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
@@ -244,6 +245,60 @@ ORACLE_API void leco_oracle_decode_block(const uint8_t *in, int width, uint32_t 
     }
 }
 
+/* The same block decode in the loop shape of read_all_bit_fix<T>
+   (bit_read.h:222-275), for timing the reference's CPU path: the packed
+   region is consumed as consecutive unaligned 64-bit words shifted into a
+   128-bit register, each step takes the register's low l bits as the next
+   field, and the prediction is one fma and one CVTTSD2SI (the instruction
+   the reference's (long long) cast compiles to on x86, with its NaN and
+   out-of-range result INT64_MIN), stored as T.  One copy per T, like the
+   template.  Output equals leco_oracle_decode_block(in, width, m, out, 0)
+   (a l == width-1 block is decoded, as there); the tests hold the two
+   against each other on the golden vectors and on blocks of every width. */
+#define REFILL_BLOCK(NAME, T)                                                              \
+    static void NAME(const uint8_t *in, uint32_t m, T *out)                                \
+    {                                                                                      \
+        const int l = in[0];                                                               \
+        double t0, t1;                                                                     \
+        memcpy(&t0, in + 1, 8);                                                            \
+        memcpy(&t1, in + 9, 8);                                                            \
+        if (l == 0) {                                                                      \
+            for (uint32_t i = 0; i < m; i++)                                               \
+                out[i] = (T)_mm_cvttsd_si64(_mm_set_sd(fma((double)i, t1, t0)));           \
+            return;                                                                        \
+        }                                                                                  \
+        const uint8_t *word = in + 17;                                                     \
+        const uint64_t mask = ((uint64_t)1 << l) - 1; /* l <= 63 here */                   \
+        const uint64_t low = ((uint64_t)1 << (l - 1)) - 1;                                 \
+        unsigned __int128 reg = 0;                                                         \
+        int held = 0;                                                                      \
+        for (uint32_t i = 0; i < m; i++) {                                                 \
+            if (held < l) {                                                                \
+                uint64_t w;                                                                \
+                memcpy(&w, word, 8);                                                       \
+                word += 8;                                                                 \
+                reg |= (unsigned __int128)w << held;                                       \
+                held += 64;                                                                \
+            }                                                                              \
+            const uint64_t field = (uint64_t)reg & mask;                                   \
+            reg >>= l;                                                                     \
+            held -= l;                                                                     \
+            const int64_t p = _mm_cvttsd_si64(_mm_set_sd(fma((double)i, t1, t0)));         \
+            const uint64_t mag = field & low;                                              \
+            out[i] = (T)((field >> (l - 1)) ? (uint64_t)p + mag : (uint64_t)p - mag);      \
+        }                                                                                  \
+    }
+REFILL_BLOCK(refill_block_u32, uint32_t)
+REFILL_BLOCK(refill_block_u64, uint64_t)
+
+ORACLE_API void leco_oracle_decode_block_refill(const uint8_t *in, int width, uint32_t m, void *out)
+{
+    int l = in[0];
+    if (l >= width || l == 126 || l == 127) { leco_oracle_decode_block(in, width, m, out, 0); return; }
+    if (width == 32) refill_block_u32(in, m, (uint32_t *)out);
+    else refill_block_u64(in, m, (uint64_t *)out);
+}
+
 /* Leco_int<T>::randomdecodeArray8, :305-337, with read_bit_fix_int_wo_round
    (bit_read.h:1170-1199).  For l == 0 the reference converts the double
    straight to T (:334); this returns (T)cvtt64 like the full decoder, which
@@ -272,6 +327,7 @@ typedef struct {
     uint8_t *stream; const uint64_t *offsets; uint64_t *sizes;
     void *out; const uint64_t *idx; uint64_t nq;
     uint64_t lo, hi;
+    int refill; /* decode: 1 = the refill loop (leco_oracle_decode_block_refill) */
 } job_t;
 
 static void *enc_worker(void *arg)
@@ -293,8 +349,9 @@ static void *dec_worker(void *arg)
     job_t *j = (job_t *)arg;
     for (uint64_t b = j->lo; b < j->hi; b++) {
         uint64_t s = b * j->bs, m = j->n - s < j->bs ? j->n - s : j->bs;
-        leco_oracle_decode_block(j->stream + j->offsets[b], j->width, (uint32_t)m,
-                                 (uint8_t *)j->out + s * (j->width / 8), 0);
+        uint8_t *o = (uint8_t *)j->out + s * (j->width / 8);
+        if (j->refill) leco_oracle_decode_block_refill(j->stream + j->offsets[b], j->width, (uint32_t)m, o);
+        else leco_oracle_decode_block(j->stream + j->offsets[b], j->width, (uint32_t)m, o, 0);
     }
     return NULL;
 }
@@ -363,12 +420,22 @@ ORACLE_API uint64_t leco_oracle_encode_column(const void *y, int width, uint64_t
     return offsets[nb];
 }
 
-/* The timed decode loop :142-153 (without the in-loop verification). */
+/* The timed decode loop :142-153 (without the in-loop verification);
+   refill = 1 decodes each block with the refill loop. */
 ORACLE_API void leco_oracle_decode_column(const uint8_t *stream, const uint64_t *offsets, int width,
                                           uint64_t n, uint32_t bs, void *out, int threads)
 {
     job_t p = {0};
     p.width = width; p.n = n; p.bs = bs; p.stream = (uint8_t *)stream; p.offsets = offsets; p.out = out;
+    run_jobs(dec_worker, p, (n + bs - 1) / bs, threads);
+}
+
+ORACLE_API void leco_oracle_decode_column_refill(const uint8_t *stream, const uint64_t *offsets, int width,
+                                                 uint64_t n, uint32_t bs, void *out, int threads)
+{
+    job_t p = {0};
+    p.width = width; p.n = n; p.bs = bs; p.stream = (uint8_t *)stream; p.offsets = offsets; p.out = out;
+    p.refill = 1;
     run_jobs(dec_worker, p, (n + bs - 1) / bs, threads);
 }
 

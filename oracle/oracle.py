@@ -110,7 +110,12 @@ def lib() -> ctypes.CDLL:
                  [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
                   ctypes.c_void_p]),
                 ("leco_oracle_opmax_decode_column", None,
-                 [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p])):
+                 [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p]),
+                ("leco_oracle_decode_block_refill", None,
+                 [ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_void_p]),
+                ("leco_oracle_decode_column_refill", None,
+                 [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_uint32, ctypes.c_void_p,
+                  ctypes.c_int])):
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
@@ -188,12 +193,16 @@ def encode_column(y: np.ndarray, bs: int, threads: int = 1) -> tuple[np.ndarray,
 
 
 def decode_column(stream: np.ndarray, offs: np.ndarray, n: int, bs: int, dtype, threads: int = 1,
-                  out: np.ndarray | None = None) -> np.ndarray:
+                  out: np.ndarray | None = None, refill: bool = False) -> np.ndarray:
+    """Every block decoded; refill=True uses the loop shape of the reference's
+    read_all_bit_fix (64-bit words into a 128-bit register) instead of the
+    byte-window field reads of the checker."""
     dtype = np.dtype(dtype)
     padded = stream if len(stream) >= int(offs[-1]) + 64 else np.concatenate([stream, np.zeros(64, np.uint8)])
     if out is None:
         out = np.empty(n, dtype)
-    lib().leco_oracle_decode_column(_ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs, _ptr(out), threads)
+    fn = lib().leco_oracle_decode_column_refill if refill else lib().leco_oracle_decode_column
+    fn(_ptr(padded), _ptr(offs), 8 * dtype.itemsize, n, bs, _ptr(out), threads)
     return out
 
 

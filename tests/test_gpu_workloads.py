@@ -46,3 +46,6 @@ def test_serving_and_churn_drivers_small():
     ch = workloads.run_churn(ncols=40, sources=4, resident=6, insert_rate=2000, req_rate=5e4, lo=1e4, hi=2e6,
                              reserve_bytes=8 << 30, log=say)
     assert ch["inserts_ok"] == 40 and ch["failed"] == 0 and ch["encode_gbs"] > 0
+    assert ch["verified"] and ch["verified_requests"] > 0
+    assert sum(ch["insert_lengths"]["histogram"].values()) == 40
+    assert abs(ch["encoded_gb"] - ch["realized_raw_gb"]) <= 0.01

@@ -37,6 +37,37 @@ def test_oracle_decode_and_lookup_golden(oracle_mod, name):
     assert np.array_equal(got, g["lookup_val"])
 
 
+@pytest.mark.parametrize("name", CASES)
+def test_refill_decode_loop_matches_golden(oracle_mod, name):
+    """The timing loop shaped like read_all_bit_fix (bit_read.h:222-275):
+    64-bit words into a 128-bit register.  It must decode every golden
+    stream to its values (the checker's byte-window decode does the same)."""
+    g = load_golden(name)
+    vals, bs = g["values"], int(g["bs"])
+    out = oracle_mod.decode_column(g["stream"], g["offsets"], len(vals), bs, vals.dtype, refill=True)
+    assert np.array_equal(out, vals)
+
+
+def test_refill_decode_loop_matches_window_decode_on_every_width(oracle_mod):
+    """Blocks forced to every width l = 0..W (a block's residual range sets
+    l), odd lengths, so fields straddle the 64-bit words in every way."""
+    rng = np.random.default_rng(12)
+    for width in (32, 64):
+        dt = np.uint32 if width == 32 else np.uint64
+        for l in range(0, width + 1):
+            for n in (1, 3, 64, 65, 1023):
+                base = np.arange(n, dtype=np.float64) * 97.0 + 1e6
+                amp = 0 if l <= 1 else float(2 ** (l - 2))
+                noise = rng.integers(-int(amp), int(amp) + 1, n) if amp else np.zeros(n, np.int64)
+                v = np.clip(base + noise, 0, float((1 << width) - 1)).astype(np.uint64).astype(dt)
+                if l >= width - 1:
+                    v = rng.integers(0, (1 << width) - 1, n, dtype=np.uint64, endpoint=True).astype(dt)
+                st, off = oracle_mod.encode_column(v, n)
+                a = oracle_mod.decode_column(st, off, n, n, dt)
+                b = oracle_mod.decode_column(st, off, n, n, dt, refill=True)
+                assert np.array_equal(a, v) and np.array_equal(b, v), (width, l, n)
+
+
 def _random_block(rng, width, n, kind):
     top = (1 << width) - 1
     if kind == "uniform":
@@ -196,6 +227,8 @@ def test_reference_cpu_workload(oracle_mod):
     # ~429 blocks at l = 9 (11,274 B each), ~570 constant blocks of 17 B
     assert 0.11 < cr < 0.13
     out = oracle_mod.decode_column(stream, offs, n, bs, np.uint32, threads=8)
+    assert np.array_equal(out, y)
+    out = oracle_mod.decode_column(stream, offs, n, bs, np.uint32, threads=8, refill=True)
     assert np.array_equal(out, y)
     idx = np.random.default_rng(1).integers(0, n, 1_000_000, dtype=np.uint64)
     got = oracle_mod.lookup_column(stream, offs, n, bs, idx, np.uint32, threads=8)
