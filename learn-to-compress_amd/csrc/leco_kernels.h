@@ -304,14 +304,19 @@ struct FitGroup {
     uint64_t gb0;         // its index in the param table / offsets
 };
 
+// Fit groups g0, g0 + 1, ... of one column (a launch may cover a slice of
+// the column's groups); the launch with g0 = 0 also clears the ticket.
 struct PlainFit {
     const uint8_t *in;
     uint64_t n;
     uint32_t bs;
     uint32_t width;
+    uint64_t g0;
     __device__ __forceinline__ FitGroup group(uint32_t wg) const {
-        return {in, n, bs, width, (uint64_t)wg * kFitRows, (uint64_t)wg * kFitRows};
+        const uint64_t b0 = (g0 + wg) * kFitRows;
+        return {in, n, bs, width, b0, b0};
     }
+    __device__ __forceinline__ bool clears_ticket(uint32_t wg) const { return g0 == 0 && wg == 0; }
 };
 
 // Largest c with first[c] <= it (first[0] = 0 <= it < first[ncols]).
@@ -331,6 +336,7 @@ struct BatchFit {
     const uint64_t *first;
     const uint64_t *fit_first;
     uint32_t ncols;
+    __device__ __forceinline__ bool clears_ticket(uint32_t wg) const { return wg == 0; }
     __device__ __forceinline__ FitGroup group(uint32_t wg) const {
         const uint32_t c = find_column(fit_first, ncols, wg);
         const BatchColumn col = cols[c];

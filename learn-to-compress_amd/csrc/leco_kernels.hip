@@ -113,7 +113,7 @@ template <class Src>
 __global__ __launch_bounds__(kFitThreads) void k_fit(Src src, leco_fix_block_param *__restrict__ prm,
                                                      uint64_t *__restrict__ offs) {
     __shared__ double tile[kFitRows][kFitTK + 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) offs[0] = 0;  // ticket
+    if (src.clears_ticket(blockIdx.x) && threadIdx.x == 0) offs[0] = 0;  // ticket
     const FitGroup g = src.group(blockIdx.x);
     if (g.width == 64) fit_group<uint64_t, Src>(g, prm, offs, tile);
     else fit_group<uint32_t, Src>(g, prm, offs, tile);
@@ -425,6 +425,24 @@ __global__ __launch_bounds__(kThreads) void k_encode_waves(PlainRows rows, leco_
     const EncodeUnit un = rows.unit(u);
     if (!un.count) return;
     encode_unit_waves<T, W>(un, u, prm, offs, out, sh);
+}
+
+// A plain column of blocks of kWaveBlock < bs <= kLargeEncodeBlock values:
+// the workgroup body alone (two cached chunks), without the wave-unit body
+// that k_encode also carries for batches of mixed block sizes.
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_encode_block(PlainRows rows, leco_fix_block_param *__restrict__ prm,
+                                                           uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
+    __shared__ EncodeShared sh;
+    __shared__ uint64_t s_u;
+    if (threadIdx.x == 0) s_u = take_ticket(offs);  // offs[0] is the ticket
+    __syncthreads();
+    const uint64_t u = s_u;
+    const EncodeUnit un = rows.unit(u);
+    if (!un.count) return;
+    const uint64_t s = un.cb0 * un.bs;
+    const uint32_t m = (uint32_t)(un.n - s < un.bs ? un.n - s : un.bs);
+    encode_block_body<kThreads, 2, T, W>((const T *)un.base + s, m, prm, un.gb0, u, offs, out, sh);
 }
 
 // A plain column of blocks of more than kLargeEncodeBlock values: the
@@ -1020,20 +1038,21 @@ __global__ __launch_bounds__(kThreads) void k_serve(const ServeColumn *__restric
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+
 template <typename T>
 hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t *offs,
                          leco_fix_block_param *prm, hipStream_t st) {
     constexpr int W = 8 * sizeof(T);
     const uint64_t nb = (n + bs - 1) / bs;
     PlainRows rows{(const uint8_t *)in, n, bs, W, nb};
-    PlainFit fit{(const uint8_t *)in, n, bs, W};
+    PlainFit fit{(const uint8_t *)in, n, bs, W, 0};
     k_fit<PlainFit><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
     if (bs > kLargeEncodeBlock)  // one block per unit
         k_encode_large<T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     else if (bs <= kWaveBlock)
         k_encode_waves<T, W><<<dim3((unsigned)encode_units(nb, bs)), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     else
-        k_encode<PlainRows><<<dim3((unsigned)encode_units(nb, bs)), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
+        k_encode_block<T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     k_encode_fixup<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs, nb,
                                                                                                 nullptr, W);
     return hipGetLastError();
@@ -1042,7 +1061,7 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
 hipError_t launch_fit(const void *in, uint64_t n, uint32_t bs, uint32_t width, leco_fix_block_param *prm,
                       uint64_t *offs, hipStream_t st) {
     const uint64_t nb = (n + bs - 1) / bs;
-    PlainFit fit{(const uint8_t *)in, n, bs, width};
+    PlainFit fit{(const uint8_t *)in, n, bs, width, 0};
     k_fit<PlainFit><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
     return hipGetLastError();
 }
