@@ -577,6 +577,9 @@ def main() -> int:
     if not args.no_batched:
         cols = workloads.batched_columns(args.batched_cols, 1e3, args.batched_max)
         batched = workloads.run_batched(cols, log=log)
+        torch.cuda.empty_cache()
+        # the same columns with the bytes-only tuner, for comparison
+        batched["bytes_only_tuner"] = workloads.run_batched(cols, tuner="bytes", plain=False, log=log)
         del cols
         torch.cuda.empty_cache()
         log(f"batched: {batched}")

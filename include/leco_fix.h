@@ -428,7 +428,10 @@ typedef struct leco_fix_column_desc {
     void *d_out;
     uint64_t n;
     uint32_t block_size;
-    uint32_t width_bits; /* 32 or 64 */
+    uint32_t width_bits;     /* 32 or 64 */
+    uint32_t tile_positions; /* blocks below 2048 values: positions one decode workgroup takes (the tile is
+                                tile_positions / block_size consecutive blocks, 1..128); 0 = 8192 */
+    uint32_t reserved;
 } leco_fix_column_desc;
 
 /* A batch owns a device block table over all blocks of its columns (column
@@ -475,6 +478,29 @@ int leco_fix_batch_column(const leco_fix_batch *batch, uint32_t c, uint64_t *fir
 int leco_fix_autotune_block_sizes(const leco_fix_column_desc *cols, uint32_t ncols, const uint32_t *candidates,
                                   uint32_t ncand, uint32_t windows, uint32_t window, double tolerance,
                                   uint32_t *out_bs, uint64_t *out_bytes, void *stream);
+
+/* The autotuner with a decode-cost term and the tile shape.  For each
+ * column: (1) every candidate block size is encoded on sample windows, as
+ * above; (2) 3/4, 5/4 and 3/2 of the best candidate are tried too (a refinement
+ * around the best size, after search_blocksize.h:68-110, which refines by
+ * +-100 around its best); (3) the sizes whose encoded bytes are within
+ * `tolerance` (relative) of the column's fewest are kept, each with its decode
+ * shapes: staged for 2048 values and more, otherwise tiles of 8192 or 16384
+ * positions; (4) every (width, size, shape) kept for any column is timed: one
+ * batched decode of the sample windows of all columns of that width (five
+ * repeats, HIP events); (5) each column starts from its fewest-bytes size and
+ * shape, and takes another kept one only if its measured decode time per
+ * value is more than 3% lower (the least such).  No counterpart in the
+ * reference beyond the block-size search.  Synchronous; allocates. */
+typedef struct leco_fix_autotune_choice {
+    uint32_t block_size;
+    uint32_t tile_positions;     /* for leco_fix_column_desc (0: staged decode) */
+    double bytes_per_value;      /* encoded bytes per value on the sample windows */
+    double decode_ns_per_value;  /* measured decode time per value of the chosen shape */
+} leco_fix_autotune_choice;
+int leco_fix_autotune_columns(const leco_fix_column_desc *cols, uint32_t ncols, const uint32_t *candidates,
+                              uint32_t ncand, uint32_t windows, uint32_t window, double tolerance,
+                              leco_fix_autotune_choice *out, void *stream);
 
 /* ---- request server: resident columns, continuous batching ------------- */
 

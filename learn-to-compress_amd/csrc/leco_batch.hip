@@ -9,8 +9,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <new>
+#include <cmath>
+#include <map>
 #include <memory>
+#include <new>
 #include <vector>
 
 #include "leco_internal.h"
@@ -109,7 +111,8 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
             unit_first.resize(ncols + 1);
             for (uint32_t c = 0; c < ncols; ++c) {
                 const leco_fix_column_desc &d = cols[c];
-                b->h_cols[c] = BatchColumn{d.d_in, d.d_out, d.n, d.block_size, d.width_bits};
+                b->h_cols[c] = BatchColumn{d.d_in, d.d_out, d.n, d.block_size, d.width_bits,
+                                           leco::multi_blocks(d.block_size, d.width_bits, d.tile_positions), 0};
                 b->h_first[c] = b->total;
                 b->h_fit_first[c] = b->fit_groups;
                 const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
@@ -132,7 +135,7 @@ int leco_fix_batch_create(const leco_fix_column_desc *cols, uint32_t ncols, leco
                     const leco_fix_column_desc &d = cols[c];
                     if (leco::staged_decode(d.block_size) != (staged == 1)) continue;
                     const uint64_t nb = (d.n + d.block_size - 1) / d.block_size;
-                    const uint32_t kb = leco::multi_blocks(d.block_size, d.width_bits);
+                    const uint32_t kb = b->h_cols[c].kb;
                     ord.push_back(c);
                     ord_first.push_back(b->tiles);
                     tile_first[c] = b->tiles;
@@ -408,3 +411,209 @@ int leco_fix_autotune_block_sizes(const leco_fix_column_desc *cols, uint32_t nco
 }
 
 }  // extern "C"
+
+// ---- autotuner with refinement and a measured decode-cost term -----------
+namespace {
+
+// Sample windows of column d for block size bs: up to `windows` windows of
+// floor(window / bs) * bs values at evenly spaced multiples of `window` (one
+// window of the whole column if it is shorter than `window`).
+void sample_windows(const leco_fix_column_desc &d, uint32_t bs, uint32_t windows, uint32_t window,
+                    std::vector<std::pair<uint64_t, uint64_t>> *out) {
+    out->clear();
+    const uint64_t slots = d.n / window;
+    if (slots == 0) {
+        out->push_back({0, d.n});
+        return;
+    }
+    const uint64_t len = std::max<uint64_t>(bs, (uint64_t)(window / bs) * bs);
+    const uint32_t nw = (uint32_t)std::min<uint64_t>(windows, slots);
+    for (uint32_t k = 0; k < nw; ++k) {
+        const uint64_t start = (slots * k / nw) * window;
+        out->push_back({start, std::min<uint64_t>(len, d.n - start)});
+    }
+}
+
+// Encoded bytes per value of column c at each of its candidates cand[c],
+// from one batched encode of every column's sample windows.
+int sample_bytes(const leco_fix_column_desc *cols, uint32_t ncols, const std::vector<std::vector<uint32_t>> &cand,
+                 uint32_t windows, uint32_t window, hipStream_t st, std::vector<std::vector<double>> *bpv) {
+    std::vector<leco_fix_column_desc> tune;
+    std::vector<std::pair<uint32_t, uint32_t>> owner;
+    std::vector<std::pair<uint64_t, uint64_t>> win;
+    for (uint32_t c = 0; c < ncols; ++c)
+        for (uint32_t j = 0; j < cand[c].size(); ++j) {
+            sample_windows(cols[c], cand[c][j], windows, window, &win);
+            for (const auto &w : win) {
+                tune.push_back({(const uint8_t *)cols[c].d_in + w.first * (cols[c].width_bits / 8), nullptr, w.second,
+                                cand[c][j], cols[c].width_bits, 0, 0});
+                owner.push_back({c, j});
+            }
+        }
+    leco_fix_batch *b = nullptr;
+    int rc = leco_fix_batch_create(tune.data(), (uint32_t)tune.size(), &b);
+    if (rc) return rc;
+    const std::unique_ptr<leco_fix_batch, void (*)(leco_fix_batch *)> hold(b, free_batch);
+    rc = leco_fix_batch_encode(b, st);
+    std::vector<uint64_t> offs(b->total + 1);
+    if (rc == LECO_FIX_OK &&
+        (hipMemcpyAsync(offs.data(), b->d_offs, offs.size() * sizeof(uint64_t), hipMemcpyDeviceToHost, st) !=
+             hipSuccess ||
+         hipStreamSynchronize(st) != hipSuccess))
+        rc = LECO_FIX_EHIP;
+    if (rc == LECO_FIX_OK && offs[b->total] == LECO_FIX_OFFSETS_POISON) rc = LECO_FIX_EDEVICE;
+    if (rc) return rc;
+    std::vector<std::vector<uint64_t>> bytes(ncols), values(ncols);
+    for (uint32_t c = 0; c < ncols; ++c) {
+        bytes[c].assign(cand[c].size(), 0);
+        values[c].assign(cand[c].size(), 0);
+    }
+    for (size_t e = 0; e < tune.size(); ++e) {
+        bytes[owner[e].first][owner[e].second] += offs[b->h_first[e + 1]] - offs[b->h_first[e]];
+        values[owner[e].first][owner[e].second] += tune[e].n;
+    }
+    bpv->assign(ncols, {});
+    for (uint32_t c = 0; c < ncols; ++c)
+        for (uint32_t j = 0; j < cand[c].size(); ++j) (*bpv)[c].push_back((double)bytes[c][j] / values[c][j]);
+    return LECO_FIX_OK;
+}
+
+// Decode nanoseconds per value of every column of width w (`members`) at
+// block size bs and tile shape `tile` (0 = staged), timed over three batched
+// decodes of their sample windows into `scratch`.
+int sample_decode_ns(const leco_fix_column_desc *cols, const std::vector<uint32_t> &members, uint32_t bs, uint32_t tile,
+                     uint32_t windows, uint32_t window, uint8_t *scratch, hipStream_t st, double *ns_per_value) {
+    std::vector<leco_fix_column_desc> descs;
+    std::vector<std::pair<uint64_t, uint64_t>> win;
+    uint64_t at = 0, values = 0;
+    for (uint32_t c : members) {
+        sample_windows(cols[c], bs, windows, window, &win);
+        const uint32_t vb = cols[c].width_bits / 8;
+        for (const auto &w : win) {
+            descs.push_back({(const uint8_t *)cols[c].d_in + w.first * vb, scratch + at, w.second, bs,
+                             cols[c].width_bits, tile, 0});
+            at += (w.second * vb + 255) / 256 * 256;
+            values += w.second;
+        }
+    }
+    leco_fix_batch *b = nullptr;
+    int rc = leco_fix_batch_create(descs.data(), (uint32_t)descs.size(), &b);
+    if (rc) return rc;
+    const std::unique_ptr<leco_fix_batch, void (*)(leco_fix_batch *)> hold(b, free_batch);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return LECO_FIX_EHIP;
+    }
+    constexpr int kReps = 5;
+    rc = leco_fix_batch_encode(b, st);
+    if (rc == LECO_FIX_OK) rc = leco_fix_batch_decode(b, st);  // warm-up
+    if (rc == LECO_FIX_OK && hipEventRecord(e0, st) != hipSuccess) rc = LECO_FIX_EHIP;
+    for (int r = 0; r < kReps && rc == LECO_FIX_OK; ++r) rc = leco_fix_batch_decode(b, st);
+    float ms = 0;
+    if (rc == LECO_FIX_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        rc = LECO_FIX_EHIP;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == LECO_FIX_OK) *ns_per_value = (double)ms * 1e6 / kReps / (double)values;
+    return rc;
+}
+
+}  // namespace
+
+extern "C" int leco_fix_autotune_columns(const leco_fix_column_desc *cols, uint32_t ncols, const uint32_t *cand,
+                                         uint32_t ncand, uint32_t windows, uint32_t window, double tol,
+                                         leco_fix_autotune_choice *out, void *stream) {
+    return leco::abi_guard([&]() -> int {
+        if (!cols || !cand || !out || ncols == 0 || ncand == 0 || windows == 0 || window == 0 || tol < 0)
+            return LECO_FIX_EINVAL;
+        for (uint32_t j = 0; j < ncand; ++j)
+            if (cand[j] == 0 || cand[j] > leco::kMaxBlockSize || cand[j] > window) return LECO_FIX_EINVAL;
+        for (uint32_t c = 0; c < ncols; ++c)
+            if (!cols[c].d_in || !valid_desc(cols[c])) return LECO_FIX_EINVAL;
+        hipStream_t st = (hipStream_t)stream;
+        // 1. coarse: every candidate for every column
+        std::vector<std::vector<uint32_t>> sizes(ncols, std::vector<uint32_t>(cand, cand + ncand));
+        std::vector<std::vector<double>> bpv;
+        int rc = sample_bytes(cols, ncols, sizes, windows, window, st, &bpv);
+        if (rc) return rc;
+        // 2. refinement around each column's best size (search_blocksize.h:68-110 refines by +-100 around
+        //    its best of 200..10000; here 3/4, 5/4 and 3/2 of the best power of two)
+        std::vector<std::vector<uint32_t>> refine(ncols);
+        for (uint32_t c = 0; c < ncols; ++c) {
+            const uint32_t jb = (uint32_t)(std::min_element(bpv[c].begin(), bpv[c].end()) - bpv[c].begin());
+            const uint32_t best = sizes[c][jb];
+            for (uint32_t r : {best - best / 4, best + best / 4, best + best / 2})
+                if (r >= 1 && r <= leco::kMaxBlockSize && r <= window &&
+                    std::find(sizes[c].begin(), sizes[c].end(), r) == sizes[c].end() &&
+                    std::find(refine[c].begin(), refine[c].end(), r) == refine[c].end())
+                    refine[c].push_back(r);
+        }
+        std::vector<std::vector<double>> rbpv;
+        rc = sample_bytes(cols, ncols, refine, windows, window, st, &rbpv);
+        if (rc) return rc;
+        for (uint32_t c = 0; c < ncols; ++c) {
+            sizes[c].insert(sizes[c].end(), refine[c].begin(), refine[c].end());
+            bpv[c].insert(bpv[c].end(), rbpv[c].begin(), rbpv[c].end());
+        }
+        // 3. feasible sizes: within tol of the column's fewest bytes, each with its decode shapes
+        static const uint32_t kTiles[] = {8192, 16384};
+        struct Shape { uint32_t width, bs, tile; };
+        auto key = [](const Shape &s) { return ((uint64_t)s.width << 56) | ((uint64_t)s.bs << 24) | s.tile; };
+        std::map<uint64_t, double> cost;  // shape -> ns per value
+        std::vector<std::vector<std::pair<uint32_t, Shape>>> feasible(ncols);
+        for (uint32_t c = 0; c < ncols; ++c) {
+            const double best = *std::min_element(bpv[c].begin(), bpv[c].end());
+            for (uint32_t j = 0; j < sizes[c].size(); ++j) {
+                if (bpv[c][j] > best * (1.0 + tol)) continue;
+                const uint32_t bs = sizes[c][j], r = bs;
+                if (leco::staged_decode(bs)) {
+                    feasible[c].push_back({j, Shape{cols[c].width_bits, r, 0}});
+                } else {
+                    for (uint32_t t : kTiles) feasible[c].push_back({j, Shape{cols[c].width_bits, r, t}});
+                }
+                for (const auto &f : feasible[c]) cost.emplace(key(f.second), 0.0);
+            }
+        }
+        // 4. one timed batched decode per shape, over the columns of its width
+        uint64_t scratch_bytes = 0;
+        std::vector<std::vector<uint32_t>> of_width(2);
+        for (uint32_t c = 0; c < ncols; ++c) {
+            of_width[cols[c].width_bits == 64].push_back(c);
+            scratch_bytes += (uint64_t)windows * (window + 256) * (cols[c].width_bits / 8) + cols[c].n % 256 + 256;
+        }
+        uint8_t *scratch = nullptr;
+        if (hipMalloc(&scratch, scratch_bytes) != hipSuccess) return LECO_FIX_ENOMEM;
+        const std::unique_ptr<uint8_t, hipError_t (*)(void *)> hold(scratch, hipFree);
+        for (auto &kv : cost) {
+            const uint32_t width = (uint32_t)(kv.first >> 56), bs = (uint32_t)((kv.first >> 24) & 0xFFFFFFFF),
+                           tile = (uint32_t)(kv.first & 0xFFFFFF);
+            rc = sample_decode_ns(cols, of_width[width == 64], bs, tile, windows, window, scratch, st, &kv.second);
+            if (rc) return rc;
+        }
+        // 5. per column: start from the fewest bytes; another feasible size or shape replaces it only
+        //    when its measured decode time per value is lower by more than kGain (timing noise on the
+        //    sample must not trade bytes for nothing)
+        constexpr double kGain = 0.03;
+        for (uint32_t c = 0; c < ncols; ++c) {
+            size_t bi = 0;
+            for (size_t i = 1; i < feasible[c].size(); ++i) {
+                const auto &f = feasible[c][i], &b = feasible[c][bi];
+                if (bpv[c][f.first] < bpv[c][b.first] ||
+                    (bpv[c][f.first] == bpv[c][b.first] && cost[key(f.second)] < cost[key(b.second)]))
+                    bi = i;
+            }
+            double bc = cost[key(feasible[c][bi].second)];
+            size_t pick = bi;
+            for (size_t i = 0; i < feasible[c].size(); ++i) {
+                const double k = cost[key(feasible[c][i].second)];
+                if (k < bc * (1 - kGain) && k < cost[key(feasible[c][pick].second)]) pick = i;
+            }
+            const auto &f = feasible[c][pick];
+            out[c] = leco_fix_autotune_choice{sizes[c][f.first], f.second.tile, bpv[c][f.first],
+                                              cost[key(f.second)]};
+        }
+        return LECO_FIX_OK;
+    });
+}

@@ -111,6 +111,8 @@ struct BatchColumn {
     uint64_t n;
     uint32_t bs;
     uint32_t width;   // 32 or 64
+    uint32_t kb;      // blocks per multi-block decode tile (blocks below kStagedMinBlock values)
+    uint32_t reserved;
 };
 
 // A resident column of the request server.
@@ -158,8 +160,8 @@ constexpr uint32_t kMultiPositions = 8192;
 constexpr uint32_t kMultiMax = 128;
 constexpr uint32_t kMultiLds = 2048 * 8 + 32;
 static_assert(kMultiLds >= 1 + (kStagedMinBlock - 1) * 8 + 15, "a small block fits the staging buffer");
-__host__ __device__ inline uint32_t multi_blocks(uint32_t bs, uint32_t /*w*/) {
-    uint32_t k = kMultiPositions / bs;
+__host__ __device__ inline uint32_t multi_blocks(uint32_t bs, uint32_t /*w*/, uint32_t positions = kMultiPositions) {
+    uint32_t k = (positions ? positions : kMultiPositions) / bs;
     if (k > kMultiMax) k = kMultiMax;
     return k ? k : 1;
 }

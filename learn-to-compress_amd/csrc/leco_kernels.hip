@@ -952,7 +952,7 @@ __global__ __launch_bounds__(kThreads) void k_batch_decode_multi(const BatchColu
     for (uint64_t t = t0 + blockIdx.x; t < t1; t += gridDim.x) {
         const uint32_t c = tile_col[t];
         const BatchColumn col = cols[c];
-        const uint32_t kb = multi_blocks(col.bs, col.width);
+        const uint32_t kb = col.kb;
         const uint64_t b0 = (t - tile_first[c]) * kb;
         if (col.width == 64)
             decode_multi_body<kCodecLeco, uint64_t, 64>(arena, offs + first[c], col.n, col.bs, b0, kb, 0, kb * col.bs,

@@ -116,10 +116,17 @@ SIGNATURES.update({
 
 
 
+class AutotuneChoice(ctypes.Structure):
+    """leco_fix_autotune_choice (include/leco_fix.h)."""
+    _fields_ = [("block_size", ctypes.c_uint32), ("tile_positions", ctypes.c_uint32),
+                ("bytes_per_value", ctypes.c_double), ("decode_ns_per_value", ctypes.c_double)]
+
+
 class ColumnDesc(ctypes.Structure):
     """leco_fix_column_desc (include/leco_fix.h)."""
     _fields_ = [("d_in", ctypes.c_void_p), ("d_out", ctypes.c_void_p), ("n", ctypes.c_uint64),
-                ("block_size", ctypes.c_uint32), ("width_bits", ctypes.c_uint32)]
+                ("block_size", ctypes.c_uint32), ("width_bits", ctypes.c_uint32),
+                ("tile_positions", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 _pp = ctypes.POINTER(ctypes.c_void_p)
@@ -148,6 +155,8 @@ SIGNATURES.update({
                                        ctypes.POINTER(c_u32), ctypes.POINTER(c_u64)]),
 })
 
+SIGNATURES["leco_fix_autotune_columns"] = (c_int, [c_ptr, c_u32, c_ptr, c_u32, c_u32, c_u32, ctypes.c_double, c_ptr,
+                                                 c_ptr])
 SIGNATURES["leco_fix_server_slowest_iterations"] = (c_int, [c_ptr, c_ptr, c_u32, ctypes.POINTER(c_u32)])
 
 SIGNATURES.update({

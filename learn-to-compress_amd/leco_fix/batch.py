@@ -15,7 +15,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import ColumnDesc, check, encoded_size, lib
+from ._lib import AutotuneChoice, ColumnDesc, check, encoded_size, lib
 from .column import PARAM_BYTES, Column, _as_storage, _stream_handle, width_of
 
 AUTOTUNE_CANDIDATES = (256, 512, 1024, 2048, 4096, 8192, 16384)
@@ -33,14 +33,14 @@ def _view(ptr: int, count: int, typestr: str) -> torch.Tensor:
     return torch.as_tensor(_DeviceArray(ptr, count, typestr), device="cuda")
 
 
-def _descs(values, outs, block_sizes):
+def _descs(values, outs, block_sizes, tiles=None):
     arr = (ColumnDesc * len(block_sizes))()
     for c, bs in enumerate(block_sizes):
         v = values[c] if values is not None else None
         o = outs[c] if outs is not None else None
         ref = v if v is not None else o
         arr[c] = ColumnDesc(v.data_ptr() if v is not None else None, o.data_ptr() if o is not None else None,
-                            ref.numel(), int(bs), width_of(ref))
+                            ref.numel(), int(bs), width_of(ref), int(tiles[c]) if tiles is not None else 0, 0)
     return arr
 
 
@@ -62,13 +62,36 @@ def autotune_block_sizes(values, candidates=AUTOTUNE_CANDIDATES, windows: int = 
     return list(out), nbytes
 
 
+def autotune_columns(values, candidates=AUTOTUNE_CANDIDATES, windows: int = 8, window: int | None = None,
+                     tolerance: float = 0.02, stream=None):
+    """Block size and decode tile shape per column (leco_fix_autotune_columns):
+    candidate sizes and a refinement around the best one (after
+    search_bsize::search_block_size, headers/search_blocksize.h:14-114) are
+    encoded on sample windows; among the sizes within `tolerance` of the
+    fewest bytes, the size and tile shape of least measured batched-decode
+    time per value win.  Returns (block sizes, tile positions, choices) where
+    choices holds the sample bytes and decode ns per value of each pick."""
+    values = [_as_storage(v.contiguous()) for v in values]
+    window = window or max(candidates)
+    cand = (ctypes.c_uint32 * len(candidates))(*candidates)
+    descs = _descs(values, None, [candidates[0]] * len(values))
+    out = (AutotuneChoice * len(values))()
+    check(lib().leco_fix_autotune_columns(descs, len(values), cand, len(candidates), windows, window, tolerance,
+                                          out, _stream_handle(stream)), "autotune_columns")
+    return [c.block_size for c in out], [c.tile_positions for c in out], \
+        [(c.bytes_per_value, c.decode_ns_per_value) for c in out]
+
+
 class Batch:
-    def __init__(self, values, block_sizes, outs=None):
-        """values: device tensors (int32/int64 storage); outs: decode targets (default: new tensors)."""
+    def __init__(self, values, block_sizes, outs=None, tile_positions=None):
+        """values: device tensors (int32/int64 storage); outs: decode targets (default: new tensors);
+        tile_positions: per column, the positions one decode workgroup takes for blocks below 2048
+        values (0 or None: 8192)."""
         self.values = [_as_storage(v.contiguous()) for v in values]
         self.block_sizes = [int(b) for b in block_sizes]
+        self.tile_positions = [int(t) for t in tile_positions] if tile_positions is not None else None
         self.outs = outs if outs is not None else [torch.empty_like(v) for v in self.values]
-        descs = _descs(self.values, self.outs, self.block_sizes)
+        descs = _descs(self.values, self.outs, self.block_sizes, self.tile_positions)
         h = ctypes.c_void_p()
         check(lib().leco_fix_batch_create(descs, len(self.values), ctypes.byref(h)), "batch_create")
         self._h = h

@@ -20,7 +20,7 @@ import numpy as np
 import torch
 
 from . import server as srvmod
-from .batch import AUTOTUNE_CANDIDATES, Batch, autotune_block_sizes
+from .batch import AUTOTUNE_CANDIDATES, Batch, autotune_block_sizes, autotune_columns
 
 U32_MAX = (1 << 32) - 1
 DISTS = ("sorted_uniform", "sorted_zipf", "linear_sigma", "stepwise")
@@ -112,15 +112,22 @@ def _timed(fn, reps: int, warmup: int = 1) -> float:
     return e0.elapsed_time(e1) / reps / 1e3
 
 
-def run_batched(cols, reps: int = 5, log=print) -> dict:
-    """Batched (one persistent launch, graph replay) vs plain per-column launches."""
+def run_batched(cols, reps: int = 5, tuner: str = "shape", plain: bool = True, log=print) -> dict:
+    """Batched (one persistent launch, graph replay) vs plain per-column launches.
+    tuner "shape": block size and decode tile per column by measured decode
+    cost within 2% of the fewest bytes (autotune_columns); "bytes": the
+    fewest encoded bytes, the largest size within 1% (autotune_block_sizes)."""
     values = [c[0] for c in cols]
     raw = sum(v.numel() * v.element_size() for v in values)
-    log(f"batched: {len(values)} columns, {raw / 1e9:.2f} GB raw; autotune block sizes")
+    log(f"batched: {len(values)} columns, {raw / 1e9:.2f} GB raw; autotune ({tuner})")
     t0 = time.perf_counter()
-    bss, _ = autotune_block_sizes(values)
+    if tuner == "shape":
+        bss, tiles, _ = autotune_columns(values)
+    else:
+        bss, _ = autotune_block_sizes(values)
+        tiles = [0] * len(bss)
     t_tune = time.perf_counter() - t0
-    batch = Batch(values, bss)
+    batch = Batch(values, bss, tile_positions=tiles)
     log("batched: encode")
     batch.encode(graph=True)
     torch.cuda.synchronize()
@@ -135,23 +142,32 @@ def run_batched(cols, reps: int = 5, log=print) -> dict:
     batch.set_decode_grid(8 * cus)
     t_pers = _timed(lambda: batch.decode(graph=True), reps)
     batch.set_decode_grid(0)
-    # plain path: one decode launch / one column encode per column, from a host loop
-    for o in batch.outs:
-        o.zero_()
-    t_pdec = _timed(batch.decode_per_column, reps)
-    ok = ok and all(torch.equal(o, v) for o, v in zip(batch.outs, values))
-    t_penc = _timed(batch.encode_per_column, max(1, reps // 2))
     hist = {int(b): bss.count(b) for b in sorted(set(bss))}
+    shapes = {}
+    for b, t in zip(bss, tiles):
+        k = "staged" if b >= 2048 else f"tile{t or 8192}"
+        shapes[k] = shapes.get(k, 0) + 1
     out = {
         "columns": len(values), "raw_bytes": raw, "compressed_bytes": comp, "compression_ratio": round(comp / raw, 5),
-        "block_sizes": hist, "autotune_s": round(t_tune, 3), "verified_bit_exact": ok,
-        "batched_decode_gbs": round(raw / t_bdec / 1e9, 1), "plain_decode_gbs": round(raw / t_pdec / 1e9, 1),
+        "tuner": tuner, "block_sizes": hist, "decode_shapes": shapes, "autotune_s": round(t_tune, 3),
+        "batched_decode_gbs": round(raw / t_bdec / 1e9, 1),
+        "batched_decode_algorithmic_gbs": round((raw + comp) / t_bdec / 1e9, 1),
+        "batched_decode_hbm_frac": round((raw + comp) / t_bdec / 1e9 / 8000.0, 4),
         "persistent_decode_gbs": round(raw / t_pers / 1e9, 1),
-        "decode_speedup": round(t_pdec / t_bdec, 3),
-        "batched_encode_gbs": round(raw / t_benc / 1e9, 1), "plain_encode_gbs": round(raw / t_penc / 1e9, 1),
-        "encode_speedup": round(t_penc / t_benc, 3),
-        "batched_decode_ms": round(t_bdec * 1e3, 3), "plain_decode_ms": round(t_pdec * 1e3, 3),
+        "batched_encode_gbs": round(raw / t_benc / 1e9, 1),
+        "batched_decode_ms": round(t_bdec * 1e3, 3),
     }
+    if plain:
+        # plain path: one decode launch / one column encode per column, from a host loop
+        for o in batch.outs:
+            o.zero_()
+        t_pdec = _timed(batch.decode_per_column, reps)
+        ok = ok and all(torch.equal(o, v) for o, v in zip(batch.outs, values))
+        t_penc = _timed(batch.encode_per_column, max(1, reps // 2))
+        out.update({"plain_decode_gbs": round(raw / t_pdec / 1e9, 1), "decode_speedup": round(t_pdec / t_bdec, 3),
+                    "plain_encode_gbs": round(raw / t_penc / 1e9, 1), "encode_speedup": round(t_penc / t_benc, 3),
+                    "plain_decode_ms": round(t_pdec * 1e3, 3)})
+    out["verified_bit_exact"] = ok
     batch.close()
     return out
 

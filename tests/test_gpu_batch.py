@@ -108,3 +108,71 @@ def test_autotune_bytes_match_oracle(oracle_mod):
             want = sum(len(oracle_mod.encode_column(v[s:s + ln], cb)[0]) for s, ln in zip(starts, lens))
             assert int(nbytes[c, j]) == want, (c, cb)
         assert bs[c] == cand[int(np.argmin(nbytes[c]))] or nbytes[c][cand.index(bs[c])] == nbytes[c].min()
+
+
+def test_batch_tile_shapes_decode_exactly(oracle_mod):
+    """Per-column decode tiles of small blocks: 4096, 8192 (default) and 16384
+    positions per workgroup (at most 128 blocks), against the oracle's bytes
+    and the input."""
+    import torch
+    from leco_fix.batch import Batch
+    cols = mixed_columns(4, count=12)
+    vals = [to_dev(torch, v) for v, _ in cols]
+    tiles = [(4096, 0, 16384)[c % 3] for c in range(len(cols))]
+    say(f"batch with tile shapes {tiles}")
+    b = Batch(vals, [bs for _, bs in cols], tile_positions=tiles)
+    b.encode()
+    outs = b.decode()
+    torch.cuda.synchronize()
+    for c, (v, bs) in enumerate(cols):
+        assert b.column_bytes(c) == oracle_mod.encode_column(v, bs)[0].tobytes(), c
+        assert torch.equal(outs[c], vals[c]), (c, bs, tiles[c])
+    for o in b.outs:
+        o.zero_()
+    b.decode(graph=True)
+    torch.cuda.synchronize()
+    assert all(torch.equal(o, v) for o, v in zip(b.outs, vals))
+    b.close()
+
+
+def _windows(n, bs, windows, window):
+    slots = n // window
+    if slots == 0:
+        return [(0, n)]
+    ln = max(bs, window // bs * bs)
+    nw = min(windows, slots)
+    return [((slots * k // nw) * window, min(ln, n - (slots * k // nw) * window)) for k in range(nw)]
+
+
+def test_autotune_columns_picks_within_tolerance(oracle_mod):
+    """autotune_columns: each pick's sample bytes per value equals the
+    oracle's on the same windows and is within the tolerance of the best
+    coarse candidate; tile shapes are staged exactly for 2048+ values; the
+    batch built from the picks decodes exactly."""
+    import torch
+    from leco_fix import datagen
+    from leco_fix.batch import Batch, autotune_columns
+    cand = (256, 512, 1024, 2048, 4096, 8192)
+    window, windows, tol = 8192, 3, 0.02
+    cols = [datagen.sorted_uniform(60_000, 64, 5), datagen.stepwise(40_000, 32, 6),
+            datagen.linear_sigma(5_000, 32, 7), datagen.sorted_pwl_u64(70_000, seed=8),
+            datagen.sorted_zipf_gaps(50_000, 32, 9)]
+    dev = [to_dev(torch, v) for v in cols]
+    bss, tiles, info = autotune_columns(dev, cand, windows, window, tolerance=tol)
+    say(f"autotune picks {list(zip(bss, tiles))}")
+
+    def bpv(v, bs):
+        ws = _windows(len(v), bs, windows, window)
+        return sum(len(oracle_mod.encode_column(v[s:s + ln], bs)[0]) for s, ln in ws) / sum(ln for _, ln in ws)
+    for c, v in enumerate(cols):
+        got = bpv(v, bss[c])
+        assert abs(info[c][0] - got) <= 1e-12 * got, (c, info[c][0], got)
+        assert got <= (1 + tol) * min(bpv(v, cb) for cb in cand) * (1 + 1e-12), c
+        assert (tiles[c] == 0) == (bss[c] >= 2048) and tiles[c] in (0, 8192, 16384)
+        assert info[c][1] > 0
+    b = Batch(dev, bss, tile_positions=tiles)
+    b.encode()
+    outs = b.decode()
+    torch.cuda.synchronize()
+    assert all(torch.equal(o, v) for o, v in zip(outs, dev))
+    b.close()

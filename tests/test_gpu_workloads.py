@@ -16,7 +16,7 @@ def test_batched_driver_small():
     cols = workloads.batched_columns(24, 1e3, 2e5, seed=7)
     out = workloads.run_batched(cols, reps=2, log=say)
     assert out["verified_bit_exact"] and out["columns"] == 24
-    assert set(out["block_sizes"]) <= {256, 512, 1024, 2048, 4096, 8192, 16384}
+    assert all(256 * 3 // 4 <= b <= 16384 * 3 // 2 for b in out["block_sizes"])
     assert out["batched_decode_gbs"] > 0 and out["plain_decode_gbs"] > 0
 
 
