@@ -141,6 +141,54 @@ def config1_section(torch, cpu: bool) -> dict:
     return res
 
 
+def dropin_section(vals, bs: int, blocks: int, col_encode_ms: float, col_decode_ms: float) -> dict:
+    """The per-block host-buffer calls a reference driver reaches through the
+    drop-in Leco_int<T> header (encodeArray8_int / decodeArray8 /
+    randomdecodeArray8, one call per block or lookup, each staging its block
+    through the GPU and synchronizing), timed over the column's first `blocks`
+    blocks, against the column calls on the whole column."""
+    import ctypes
+
+    import numpy as np
+
+    from leco_fix._lib import lib
+
+    L = lib()
+    y = vals[:blocks * bs].cpu().numpy().view(np.uint64)
+    cap = int(L.leco_fix_stream_bound(bs, bs, 64))
+    enc = [np.empty(cap, np.uint8) for _ in range(blocks)]
+    sizes = []
+    written = ctypes.c_uint64()
+    L.leco_fix_encode_block_u64(y.ctypes.data, bs, enc[0].ctypes.data, cap, ctypes.byref(written))  # warm-up
+    t0 = time.perf_counter()
+    for b in range(blocks):
+        L.leco_fix_encode_block_u64(y[b * bs:].ctypes.data, bs, enc[b].ctypes.data, cap, ctypes.byref(written))
+        sizes.append(written.value)
+    t_enc = (time.perf_counter() - t0) / blocks
+    out = np.empty(bs, np.uint64)
+    ok = True
+    t0 = time.perf_counter()
+    for b in range(blocks):
+        L.leco_fix_decode_block_u64(enc[b].ctypes.data, bs, out.ctypes.data)
+    t_dec = (time.perf_counter() - t0) / blocks
+    ok = ok and bool(np.array_equal(out, y[(blocks - 1) * bs:blocks * bs]))
+    rng = np.random.default_rng(3)
+    qs = rng.integers(0, blocks * bs, 2000)
+    v = ctypes.c_uint64()
+    t0 = time.perf_counter()
+    for q in qs:
+        L.leco_fix_random_decode_u64(enc[int(q) // bs].ctypes.data, int(q) % bs, ctypes.byref(v))
+    t_ra = (time.perf_counter() - t0) / len(qs)
+    ok = ok and v.value == int(y[int(qs[-1])])
+    nb = (vals.numel() + bs - 1) // bs
+    return {"blocks_timed": blocks, "encode_block_us": round(t_enc * 1e6, 1), "decode_block_us": round(t_dec * 1e6, 1),
+            "random_decode_us": round(t_ra * 1e6, 1), "verified": ok,
+            "column_by_blocks_encode_s": round(t_enc * nb, 2), "column_by_blocks_decode_s": round(t_dec * nb, 2),
+            "column_call_encode_ms": round(col_encode_ms, 3), "column_call_decode_ms": round(col_decode_ms, 4),
+            "note": "host buffers: each call copies its block to HBM, runs the kernels and synchronizes; the "
+                    "column calls above work on HBM-resident data"}
+
+
 def cpu_baseline(vals_dev, bs: int, sample: int, repeat: int):
     """Decode of the first `sample` values of the column on the host: the
     oracle's refill loop (the loop shape of the reference's read_all_bit_fix,
@@ -531,6 +579,10 @@ def main() -> int:
     lat_us.sort()
     ra_p50, ra_p99 = lat_us[len(lat_us) // 2], lat_us[int(len(lat_us) * 0.99)]
 
+    log("drop-in per-block calls")
+    dropin = dropin_section(vals, bs, 256, ms_enc, ms_dec)
+    log(f"dropin: {dropin}")
+
     config1 = None
     if not args.no_config1:
         log("config 1: 10M uint32, block size 10000, GPU and host")
@@ -670,6 +722,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "config1": config1,
+        "dropin_per_block": dropin,
         "baselines": base,
         "opmax": opmax,
         "filter": filt,

@@ -45,7 +45,7 @@ __device__ __forceinline__ bool below(uint64_t v, uint64_t f) {
 constexpr int kWaves = kThreads / kWave;
 constexpr int kEmitPer = 8;  // rounds of kThreads candidates per compaction step when writing positions
 constexpr uint32_t kCloseRounds = 64;  // scan rounds of a tile whose ballots pass 1 parks for pass 2
-constexpr uint32_t kCloseWin = 512;    // periods whose windows pass 1 keeps for the scan
+constexpr uint32_t kCloseWin = 1024;   // periods whose windows pass 1 keeps for the scan
 constexpr uint32_t kCloseStage = 8192; // bytes of a block's fields the scan sweep stages through LDS
 
 struct FilterShared {
