@@ -35,6 +35,23 @@ __device__ __forceinline__ int64_t predict(double t0, double t1, uint32_t i) {
     return cvtt64(__builtin_fma(t1, (double)i, t0));
 }
 
+// predict() without the range branch: equal to it when
+// |trunc(fma(t1, i, t0))| < 2^51 (see near_range).
+__device__ __forceinline__ int64_t predict_near(double t0, double t1, uint32_t i) {
+    const double t = __builtin_trunc(__builtin_fma(t1, (double)i, t0));
+    return (int64_t)((uint64_t)__double_as_longlong(t + 0x1.8p52) - 0x4338000000000000ull);
+}
+
+// True when predict_near(t0, t1, i) == predict(t0, t1, i) for every i < m
+// (m >= 1).  The exact t1*i + t0 is monotone in i, and rounding to nearest
+// and truncation are monotone, so trunc(fma(t1, i, t0)) lies between its
+// values at i = 0 and i = m - 1; NaN or infinite thetas fail the test.
+__device__ __forceinline__ bool near_range(double t0, double t1, uint32_t m) {
+    const double a = __builtin_trunc(__builtin_fma(t1, 0.0, t0));
+    const double b = __builtin_trunc(__builtin_fma(t1, (double)(m - 1), t0));
+    return __builtin_fabs(a) < 0x1p51 && __builtin_fabs(b) < 0x1p51;
+}
+
 // Low 32 bits of predict(): the fast path's bits(t + 1.5*2^52) already hold
 // them (the subtracted 0x4338000000000000 has zero low word).
 __device__ __forceinline__ uint32_t predict_lo32(double t0, double t1, uint32_t i) {
@@ -114,6 +131,52 @@ __device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
     for (int o = kWave / 2; o > 0; o >>= 1) { int64_t u = __shfl_xor((long long)v, o, kWave); v = u < v ? u : v; }
     return v;
 }
+// A value every lane holds alike (an LDS broadcast, say), marked uniform for
+// the compiler so that branches on it stay scalar.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
+// 64-bit lane permutation within rows of 16 lanes (DPP; every source lane
+// exists for the patterns used here).
+template <int CTRL>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)v, (int)(uint32_t)v, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(v >> 32), (int)(uint32_t)(v >> 32), CTRL, 0xf, 0xf,
+                                               false);
+    return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
+}
+
+// Reduction of a commutative, associative op over each row of 16 lanes;
+// every lane of the row ends with the row's result.  Lane exchanges: xor 1
+// and xor 2 (quad permutations), then the 8-lane and 16-lane mirrors, which
+// pair each quad with the other quad of its half-row and each half-row with
+// the other half; every step combines two disjoint sets of lanes.
+template <class Op>
+__device__ __forceinline__ uint64_t row_reduce_u64(uint64_t v, Op op) {
+    v = op(v, dpp_u64<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = op(v, dpp_u64<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = op(v, dpp_u64<0x141>(v));  // row_half_mirror
+    v = op(v, dpp_u64<0x140>(v));  // row_mirror
+    return v;
+}
+
+// Sum over the wave (all 64 lanes active), returned to every lane: row sums
+// by DPP, then the four rows' totals read from lanes 0, 16, 32 and 48.
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+    v = row_reduce_u64(v, [](uint64_t a, uint64_t c) { return a + c; });
+    uint64_t s = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, 16 * r);
+        const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 16 * r);
+        s += (uint64_t)lo | ((uint64_t)hi << 32);
+    }
+    return s;
+}
+
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
     for (int o = kWave / 2; o > 0; o >>= 1) {
         uint64_t u = (uint64_t)__shfl_xor((long long)v, o, kWave);

@@ -2,6 +2,8 @@
 // the kernels in leco_kernels.hip.
 #pragma once
 
+#include <type_traits>
+
 #include "leco_device.h"
 #include "leco_internal.h"
 
@@ -31,7 +33,7 @@ constexpr uint64_t kStatusAggregate = 1ull << 62;   // value = this block's size
 constexpr uint64_t kStatusInclusive = 2ull << 62;   // value = bytes up to and including this block
 constexpr uint64_t kStatusMask = 3ull << 62;
 constexpr uint64_t kStatusValue = (1ull << 62) - 1;
-constexpr uint32_t kSpinLimit = 1u << 26;              // look-back wait bound (s_sleep 1 each)
+constexpr uint32_t kSpinLimit = 1u << 22;              // look-back wait bound (rounds of s_sleep 1 + reload)
 // Bit 63 of the ticket word offs[0]: a look-back wait of this launch reached
 // its bound (a broken invariant: the unit it waited for never published), so
 // the offsets it produced are wrong.  The fix-up turns it into
@@ -73,6 +75,19 @@ struct EncodeShared {
     alignas(16) uint32_t buf[kLdsBytes / 4];
     int64_t rmax[kEncodeMaxWaves], rmin[kEncodeMaxWaves];
     uint64_t rmag[kEncodeMaxWaves];
+    // block-wide residual max / min and magnitude max, reduced by LDS atomics
+    // (set by init_reductions() before the kernel's first barrier)
+    int64_t amax, amin;
+    uint64_t amag;
+    // workgroup look-back: per wave, the sum of its window part and flags
+    // (bit 0: an inclusive prefix was found, bit 1: a wait gave up)
+    uint64_t lbs[kEncodeMaxWaves];
+    uint32_t lbf[kEncodeMaxWaves];
+    __device__ void init_reductions() {
+        amax = INT64_MIN;
+        amin = INT64_MAX;
+        amag = 0;
+    }
     uint64_t off, end;
     double t0;
     int l;
@@ -82,18 +97,74 @@ struct EncodeShared {
     int ul[kUnitMaxBlocks];
 };
 
+// One wave's part of a look-back window: PER chunks of 64 predecessors'
+// status words (lane l of chunk k holds unit wtop - 64k - l; units below 0
+// read as an inclusive prefix of 0), nearest first.  The part that counts
+// ends at the nearest inclusive prefix (or covers all PER chunks); every
+// word in it must be ready.  Words that are not are loaded again, all of
+// them in flight together, so that each wait costs one round trip however
+// many chunks it spans; the nearest inclusive prefix is looked for afresh
+// after each round.  Returns the sum of the counted words' values (every
+// lane); `found`: an inclusive prefix ends the part; `gave_up`: the wait
+// bound was reached with words of the part still not ready (they count 0).
+template <int PER>
+__device__ __forceinline__ uint64_t resolve_window(uint64_t (&stw)[PER], int64_t wtop,
+                                                   const uint64_t *__restrict__ offs, uint32_t spin_limit,
+                                                   bool &found, bool &gave_up) {
+    const int lane = threadIdx.x % kWave;
+    int ks = PER;               // chunk of the nearest inclusive prefix
+    uint64_t stop_mask = ~0ull; // lanes of chunk ks up to and including it
+    bool ready = true;
+    for (uint32_t round = 0;; ++round) {
+        ks = PER;
+        stop_mask = ~0ull;
+        ready = true;
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            if (k < ks) {
+                const uint64_t incl = __ballot((stw[k] & kStatusMask) == kStatusInclusive);
+                const uint64_t pending = __ballot((stw[k] & kStatusMask) == 0);
+                if (incl) {
+                    const int stop = __ffsll((long long)incl) - 1;
+                    stop_mask = stop == kWave - 1 ? ~0ull : ((2ull << stop) - 1);
+                    ks = k;
+                    if (pending & stop_mask) ready = false;
+                } else if (pending) {
+                    ready = false;
+                }
+            }
+        }
+        if (ready || round + 1 >= spin_limit) break;
+        __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int64_t p = wtop - (k * kWave + lane);
+            if (k <= ks && (stw[k] & kStatusMask) == 0) stw[k] = load_status(&offs[p + 1]);
+        }
+    }
+    found = ks < PER;
+    gave_up = !ready;
+    uint64_t part = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const bool counted = k < ks || (k == ks && ((stop_mask >> lane) & 1));
+        part += counted ? stw[k] & kStatusValue : 0;
+    }
+    return wave_sum_u64(part);
+}
+
 // Decoupled look-back of unit b (one wave; every lane returns the same
 // exclusive prefix) over the status words offs[p+1] of the units before it
 // (offs[0] is the ticket that orders the units).  Publishes the unit's size,
 // then loads kWave*PER predecessors' words per step (PER per lane, all in
-// flight together) and walks them 64 at a time, nearest first, until the
-// nearest one that carries an inclusive prefix; the prefix is its value plus
-// the sizes of the units in between; finally publishes the unit's inclusive
-// prefix.  Every lower unit belongs to a wave
-// that is already running and publishes its size before it waits, so the
-// waits end; the spin bound (~seconds) only keeps a broken invariant from
-// hanging the GPU.  A wait that reaches the bound counts the predecessor as
-// empty and sets kTicketError in offs[0], which poisons the launch's offsets.
+// flight together) and resolves them (resolve_window) up to the nearest one
+// that carries an inclusive prefix; the prefix is its value plus the sizes
+// of the units in between; finally publishes the unit's inclusive prefix.
+// Every lower unit belongs to a wave that is already running and publishes
+// its size before it waits, so the waits end; the spin bound (~seconds) only
+// keeps a broken invariant from hanging the GPU.  A wait that reaches the
+// bound counts the predecessor as empty and sets kTicketError in offs[0],
+// which poisons the launch's offsets.
 template <int PER = kLookPer>
 __device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_t *__restrict__ offs) {
     constexpr int kWin = kWave * PER;
@@ -114,33 +185,93 @@ __device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_
             const int64_t p = top - (k * kWave + lane);
             stw[k] = p >= 0 ? load_status(&offs[p + 1]) : kStatusInclusive;
         }
-        // chunks of 64 predecessors, nearest first: wait for a chunk, add the
-        // sizes up to its nearest inclusive prefix, stop there (predecessors
-        // behind that one are never waited for)
-        bool done = false;
-#pragma unroll
-        for (int k = 0; k < PER; ++k) {
-            if (!done) {
-                const int64_t p = top - (k * kWave + lane);
-                uint32_t spins = 0;
-                while ((stw[k] & kStatusMask) == 0 && ++spins < hook.spin_limit) {
-                    __builtin_amdgcn_s_sleep(1);
-                    stw[k] = load_status(&offs[p + 1]);
-                }
-                const uint64_t incl = __ballot((stw[k] & kStatusMask) == kStatusInclusive);
-                const int stop = incl ? __ffsll((long long)incl) - 1 : kWave;
-                // only a predecessor this unit counts matters (lanes past the stop are not added)
-                gave_up = gave_up || (lane <= stop && (stw[k] & kStatusMask) == 0);
-                uint64_t part = lane <= stop ? stw[k] & kStatusValue : 0;
-                for (int o = kWave / 2; o > 0; o >>= 1) part += (uint64_t)__shfl_xor((long long)part, o, kWave);
-                excl += part;
-                done = incl != 0;
-            }
-        }
-        if (done) break;
+        bool found, gu;
+        excl += resolve_window<PER>(stw, top, offs, hook.spin_limit, found, gu);
+        gave_up = gave_up || gu;
+        if (found) break;
     }
     if (lane == 0 && publish) store_status(&offs[b + 1], kStatusInclusive | (excl + size));
-    if (__ballot(gave_up) && lane == 0) atomicOr((unsigned long long *)offs, (unsigned long long)kTicketError);
+    if (gave_up && lane == 0) atomicOr((unsigned long long *)offs, (unsigned long long)kTicketError);
+    return excl;
+}
+
+// The workgroup look-back in two steps, so that a workgroup can publish a
+// unit's size, do other work, and resolve the unit's prefix later.
+// publish_size: the unit's status word (an inclusive prefix for unit 0, an
+// aggregate otherwise), by thread 0.
+__device__ __forceinline__ void publish_size(uint64_t b, uint64_t size, uint64_t *__restrict__ offs) {
+    if (threadIdx.x == 0 && b != g_lookback_hook.withhold)
+        store_status(&offs[b + 1], (b == 0 ? kStatusInclusive : kStatusAggregate) | size);
+}
+
+// resolve_wg: the exclusive prefix of unit b (every thread of the NT-thread
+// workgroup returns it), then its inclusive prefix published.  Each wave
+// resolves its own kWave*PER predecessors of a window of NT*PER, nearest
+// first by wave, so that one round trip of status loads covers NT/kWave
+// times the predecessors of a one-wave look-back.  The waves' sums and flags
+// meet in LDS (lbs, lbf: one slot per wave); the window ends at the first
+// wave that saw an inclusive prefix.  The first window's words come from
+// lookback_issue(), called earlier so that their round trip overlaps other
+// work.  Waiting and giving up as look_back().
+template <int NT, int PER>
+__device__ __forceinline__ void lookback_issue(uint64_t b, const uint64_t *__restrict__ offs, uint64_t (&stw)[PER]) {
+    constexpr int kWin = kWave * PER;
+    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int64_t wtop = (int64_t)b - 1 - (int64_t)wid * kWin;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int64_t p = wtop - (k * kWave + lane);
+        stw[k] = p >= 0 ? load_status(&offs[p + 1]) : kStatusInclusive;
+    }
+}
+
+template <int NT, int PER>
+__device__ __forceinline__ uint64_t resolve_wg(uint64_t b, uint64_t size, uint64_t *__restrict__ offs,
+                                               uint64_t *lbs, uint32_t *lbf, uint64_t (&first)[PER]) {
+    constexpr int kWaves = NT / kWave;
+    constexpr int kWin = kWave * PER;
+    if (b == 0) return 0;
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    const LookbackHook hook = g_lookback_hook;
+    uint64_t excl = 0;
+    bool gave_up = false;
+    for (int64_t top = (int64_t)b - 1;; top -= (int64_t)kWin * kWaves) {
+        const int64_t wtop = top - (int64_t)wid * kWin;
+        uint64_t stw[PER];
+        if (top == (int64_t)b - 1) {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) stw[k] = first[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < PER; ++k) {
+                const int64_t p = wtop - (k * kWave + lane);
+                stw[k] = p >= 0 ? load_status(&offs[p + 1]) : kStatusInclusive;
+            }
+        }
+        bool done, wgu;
+        const uint64_t wsum = resolve_window<PER>(stw, wtop, offs, hook.spin_limit, done, wgu);
+        if (lane == 0) {
+            lbs[wid] = wsum;
+            lbf[wid] = (done ? 1u : 0u) | (wgu ? 2u : 0u);
+        }
+        __syncthreads();
+        bool found = false;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            if (!found) {
+                const uint32_t f = lbf[w];
+                excl += lbs[w];
+                gave_up = gave_up || (f & 2u);
+                found = (f & 1u) != 0;
+            }
+        }
+        __syncthreads();  // the slots are rewritten by the next window
+        if (found) break;
+    }
+    if (tid == 0) {
+        if (b != hook.withhold) store_status(&offs[b + 1], kStatusInclusive | (excl + size));
+        if (gave_up) atomicOr((unsigned long long *)offs, (unsigned long long)kTicketError);
+    }
     return excl;
 }
 
@@ -163,6 +294,67 @@ struct PackedBlock {
     uint32_t nchunks;
     int lbyte = -1;        // header byte 0 when it differs from the field width l
 };
+
+// Bytes [g0, g1) of the output from the LDS image `buf` of the 16-byte
+// aligned range [a0, a0 + span): 16-byte stores, byte stores at the ragged
+// ends (the neighbouring blocks' bytes are not touched).
+template <int NT>
+__device__ __forceinline__ void store_image(const uint32_t *buf, uint64_t a0, uint64_t g0, uint64_t g1, uint32_t span,
+                                            int tid, uint8_t *__restrict__ out) {
+    const uint8_t *bb = (const uint8_t *)buf;
+    for (uint32_t u = tid; u < span / 16; u += NT) {
+        const uint64_t ga = a0 + 16ull * u;
+        if (ga >= g0 && ga + 16 <= g1) {
+            *(uint4 *)(out + ga) = *(const uint4 *)(bb + 16 * u);
+        } else {
+            for (int k = 0; k < 16; ++k) {
+                const uint64_t gb = ga + k;
+                if (gb >= g0 && gb < g1) out[gb] = bb[16 * u + k];
+            }
+        }
+    }
+}
+
+// Bytes [0, size) of the LDS image `img` (a block packed at lead 0) to
+// out[off, off + size): the 16-byte aligned middle with 16-byte stores, each
+// built from five image words funnel-shifted by the same byte count; the
+// ragged head and tail bytes (fewer than 16 each, shared with neighbouring
+// blocks' segments) with byte stores, one thread each.  Reads image words up
+// to (size + 19) / 4; the image has that room for the blocks stored here.
+template <int NT>
+__device__ __forceinline__ void store_image_shifted(const uint32_t *img, uint64_t off, uint64_t size, int tid,
+                                                    uint8_t *__restrict__ out) {
+    const uint64_t end = off + size;
+    const uint64_t a0 = (off + 15) & ~15ull;  // first 16-byte boundary at or after off
+    const uint64_t hend = a0 < end ? a0 : end;
+    const uint64_t a1 = a0 < end ? end & ~15ull : a0;  // the middle is [a0, a1)
+    auto byte_at = [&](uint64_t g) -> uint32_t {
+        const uint32_t i = (uint32_t)(g - off);
+        return (img[i >> 2] >> (8 * (i & 3))) & 0xFF;
+    };
+    if (tid < 16) {
+        const uint64_t g = off + tid;
+        if (g < hend) out[g] = (uint8_t)byte_at(g);
+    } else if (tid < 32) {
+        const uint64_t g = a1 + (tid - 16);
+        if (g >= a1 && g < end && a0 < end) out[g] = (uint8_t)byte_at(g);
+    }
+    const uint32_t i0 = (uint32_t)(a0 - off);  // image byte of the middle's first byte
+    const uint32_t r = i0 & 3;
+    const uint32_t units = (uint32_t)((a1 - a0) / 16);
+    for (uint32_t u = tid; u < units; u += NT) {
+        const uint32_t q = (i0 + 16 * u) >> 2;
+        uint32_t w[5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) w[j] = img[q + j];
+        uint4 o;
+        o.x = __builtin_amdgcn_alignbyte(w[1], w[0], r);
+        o.y = __builtin_amdgcn_alignbyte(w[2], w[1], r);
+        o.z = __builtin_amdgcn_alignbyte(w[3], w[2], r);
+        o.w = __builtin_amdgcn_alignbyte(w[4], w[3], r);
+        *(uint4 *)(out + a0 + 16ull * u) = o;
+    }
+}
 
 // Chunk c (CH values) of a block, packed by NT threads (thread `tid`) through
 // the LDS image `buf`; sync() orders the group's LDS accesses.  value(kk, i)
@@ -218,18 +410,7 @@ __device__ __forceinline__ void pack_chunk(const PackedBlock &pb, uint32_t c, in
         }
     }
     sync();
-    const uint8_t *bb = (const uint8_t *)buf;
-    for (uint32_t u = tid; u < span / 16; u += NT) {
-        const uint64_t ga = a0 + 16ull * u;
-        if (ga >= g0 && ga + 16 <= g1) {
-            *(uint4 *)(out + ga) = *(const uint4 *)(bb + 16 * u);
-        } else {
-            for (int k = 0; k < 16; ++k) {
-                const uint64_t gb = ga + k;
-                if (gb >= g0 && gb < g1) out[gb] = bb[16 * u + k];
-            }
-        }
-    }
+    store_image<NT>(buf, a0, g0, g1, span, tid, out);
     sync();
 }
 

@@ -7,9 +7,12 @@
 //
 // Encode = K1 fit (one lane per block) -> K2 residuals + bit width + offset
 // (decoupled look-back) + pack, one workgroup per large block or one wave per
-// small block, values kept in registers -> K3 fix-up of payload offsets.  Decode = one workgroup per
-// block; lookup = one lane per query.  The batched variants run the same
-// per-block bodies over a device table of blocks from many columns.
+// small block, values kept in registers -> K3 fix-up of payload offsets.  A
+// uint64 column of 4096-value blocks takes the persistent stream encoder
+// instead of K2 (k_encode_stream, k_encode_deferred, k_encode_fixup_status).
+// Decode = one workgroup per block; lookup = one lane per query.  The
+// batched variants run the same per-block bodies over a device table of
+// blocks from many columns.
 #include "leco_kernels.h"
 
 namespace leco {
@@ -106,7 +109,8 @@ __device__ __forceinline__ void fit_group(const FitGroup &g, leco_fix_block_para
     p->theta0 = t0;
     p->theta1 = t1;
     p->length = m;
-    offs[gb + 1] = 0;  // look-back status: not ready
+    p->payload_offset = 0;  // the stream encoder's ticket counters start here
+    offs[gb + 1] = 0;       // look-back status: not ready
 }
 
 template <class Src>
@@ -134,6 +138,126 @@ __global__ __launch_bounds__(kFitThreads) void k_fit(Src src, leco_fix_block_par
 // smaller blocks by one wave each, four blocks side by side per workgroup.
 // Either way the block's values stay in registers across all passes.
 // ---------------------------------------------------------------------------
+
+// ---------------------------------------------------------------------------
+// Stream encoder pieces for full blocks of VPT*NT values (all register-cached,
+// thread t holding values 2(t + k*NT) and 2(t + k*NT) + 1 in v[2k], v[2k+1])
+// whose predictions stay within +-2^51 (near_range).
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ __forceinline__ uint32_t near_pos(int tid, int k) {
+    return 2 * ((uint32_t)tid + (uint32_t)(k >> 1) * NT) + (uint32_t)(k & 1);
+}
+
+// Passes 1 and 2 (piecewise_fix_integer_template.h:36-64) over the cached
+// values: pass 1 predicts without the range branch, pass 2 too unless the
+// recentred theta0 leaves the range.  Block-wide max / min are row
+// reductions followed by LDS atomics on sh.amax / sh.amin / sh.amag (set by
+// init_reductions() and ordered by barriers before the atomics); amax and
+// amin are set again here once read, amag is left to the caller.  On return
+// v[] holds the magnitudes, `signs` bit k the sign of v[k] (1: above the
+// line), t0 the recentred theta0 and the return value the bit width l.
+template <int NT, int VPT, int W>
+__device__ __forceinline__ int near_passes(int tid, uint64_t (&v)[VPT], double &t0, double t1, uint32_t &signs,
+                                           EncodeShared &sh) {
+    constexpr uint32_t m = VPT * NT;
+    const int lane = threadIdx.x % kWave;
+    const auto max_i64 = [](uint64_t a, uint64_t c) { return (int64_t)a > (int64_t)c ? a : c; };
+    const auto min_i64 = [](uint64_t a, uint64_t c) { return (int64_t)a < (int64_t)c ? a : c; };
+    const auto max_u64 = [](uint64_t a, uint64_t c) { return a > c ? a : c; };
+
+    int64_t mx = INT64_MIN, mn = INT64_MAX;
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+        const int64_t d = (int64_t)(v[k] - (uint64_t)predict_near(t0, t1, near_pos<NT>(tid, k)));
+        mx = d > mx ? d : mx;
+        mn = d < mn ? d : mn;
+    }
+    mx = (int64_t)row_reduce_u64((uint64_t)mx, max_i64);
+    mn = (int64_t)row_reduce_u64((uint64_t)mn, min_i64);
+    if ((lane & 15) == 0) {
+        __hip_atomic_fetch_max(&sh.amax, mx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_min(&sh.amin, mn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    mx = (int64_t)uniform_u64((uint64_t)sh.amax);  // uniform: the branches below stay scalar
+    mn = (int64_t)uniform_u64((uint64_t)sh.amin);
+    // :46 theta0 += (max + min) / 2 (t0 is finite here)
+    t0 = t0 + (double)(int64_t)((uint64_t)mx + (uint64_t)mn) * 0.5;
+
+    const bool near2 = near_range(t0, t1, m);
+    uint64_t mg = 0;
+    signs = 0;
+    // positions recomputed (an opaque copy of tid): holding pass 1's 16
+    // converted positions would take 32 VGPRs.  (`tid` itself is the caller's
+    // opaque copy, so that nothing position-dependent leaves its loop either.)
+    int tid2 = tid;
+    asm volatile("" : "+v"(tid2));
+    auto pass2 = [&](auto near) {
+#pragma unroll
+        for (int k = VPT - 1; k >= 0; --k) {  // signs built by shifts: bit k ends at position k
+            const uint32_t j = near_pos<NT>(tid2, k);
+            uint64_t mag;
+            const bool pos =
+                residual_sign<W>(v[k], decltype(near)::value ? predict_near(t0, t1, j) : predict(t0, t1, j), &mag);
+            v[k] = mag;
+            signs = (signs << 1) | (uint32_t)pos;
+            mg = mag > mg ? mag : mg;
+        }
+    };
+    if (near2) pass2(std::true_type{});
+    else pass2(std::false_type{});
+    mg = row_reduce_u64(mg, max_u64);
+    if ((lane & 15) == 0) __hip_atomic_fetch_max(&sh.amag, mg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __syncthreads();
+    if (tid == 0) {  // every thread has read amax / amin
+        sh.amax = INT64_MIN;
+        sh.amin = INT64_MAX;
+    }
+    mg = uniform_u64(sh.amag);
+    int l = mg ? bitlen64(mg) + 1 : 0;
+    return l > W ? W : l;
+}
+
+// Field k of a block with a width 1 <= l <= 32 (not raw): magnitude |
+// sign << (l-1) (the magnitude is below 2^(l-1)).
+template <int VPT>
+__device__ __forceinline__ void near_fields(const uint64_t (&v)[VPT], uint32_t signs, int l, uint32_t (&f)[VPT]) {
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) f[k] = (uint32_t)v[k] | (((signs >> k) & 1u) << (l - 1));
+}
+
+// The block's bytes with a width l <= 32 (not raw), ORed into a cleared LDS
+// image at byte `lead`: header [l][theta0][theta1], then field j (near_fields)
+// at bit 8*(lead + 17) + j*l, two 32-bit ORs per field (bit_write.h:257-323).
+template <int NT, int VPT>
+__device__ __forceinline__ void near_pack(int tid, const uint32_t (&f)[VPT], int l, double t0, double t1,
+                                          uint32_t lead, uint32_t *img) {
+    static_assert((uint64_t)VPT * NT * 32 / 8 + 17 + 2 * 16 <= kLdsBytes, "a block of 32-bit fields fits the image");
+    if (tid < 17) {
+        uint32_t byte;
+        if (tid == 0) byte = (uint32_t)l;
+        else if (tid < 9) byte = (uint32_t)(double_to_bits(t0) >> (8 * (tid - 1))) & 0xFF;
+        else byte = (uint32_t)(double_to_bits(t1) >> (8 * (tid - 9))) & 0xFF;
+        const uint32_t pos = lead + tid;
+        __hip_atomic_fetch_or(&img[pos >> 2], byte << (8 * (pos & 3)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    if (l == 0) return;
+    const uint32_t q0 = 8 * (lead + 17) + 2 * (uint32_t)tid * (uint32_t)l;
+#pragma unroll
+    for (int k = 0; k < VPT; ++k) {
+        const uint32_t q = q0 + (uint32_t)((k >> 1) * 2 * NT + (k & 1)) * (uint32_t)l;
+        const uint64_t s = (uint64_t)f[k] << (q & 31);
+        __hip_atomic_fetch_or(&img[q >> 5], (uint32_t)s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_or(&img[(q >> 5) + 1], (uint32_t)(s >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void clear_image(uint32_t *img) {
+    uint4 *b4 = (uint4 *)img;
+    for (int i = threadIdx.x; i < kLdsBytes / 16; i += NT) b4[i] = make_uint4(0, 0, 0, 0);
+}
 
 // Workgroup body (NT threads): a block of any size up to kMaxBlockSize; the
 // values of the first CC pack chunks (VPT per thread) are register-cached,
@@ -249,6 +373,300 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
                                      [&](int kk, uint32_t i) { return cached((CC > 1 ? kChunk / NT : 0) + kk, i); });
     for (uint32_t c = CC; c < pb.nchunks; ++c)
         pack_chunk<NT, kChunk, false>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) { return fresh(i); });
+}
+
+// Persistent, software-pipelined encoder for a plain column of 4096-value
+// blocks.  Each workgroup takes tickets in order and, per block: passes 1-2,
+// publishes the block's size, ORs its bytes into a free LDS image (lead 0:
+// the offset is not known yet), starts the loads of its next block, and only
+// then resolves the previous block's look-back and stores that block's image
+// at its offset (store_image_shifted).  The look-back's round trips thus
+// overlap the next block's work instead of idling the workgroup.  Blocks
+// that do not fit this path (not full, input not 2*sizeof(T)-aligned,
+// predictions beyond +-2^51, raw or fields over 32 bits) first drain the
+// pending block, then take the look-back at once (the general body with
+// one cached chunk, or near passes plus chunked packing).  No block waits
+// for a look-back while its own workgroup holds an unresolved lower ticket,
+// so every wait ends (each ticket's size is published without waiting).
+// Bit of leco_fix_block_param::bits marking a block whose bytes the stream
+// encoder left to k_encode_deferred (its size, thetas, width and offset are
+// final); k_encode_deferred clears it.
+constexpr uint32_t kDeferred = 1u << 31;
+
+// Width and recentred theta0 of a block the stream encoder does not cache
+// (any length, any predictions): passes 1-2 of the general body
+// (piecewise_fix_integer_template.h:36-64) over values read from memory,
+// one workgroup reduction per pass.  Raw blocks (l == W) get theta0 = 0.
+struct BlockWidth {
+    double t0;
+    int l;
+};
+template <typename T, int W>
+__device__ __attribute__((noinline)) BlockWidth stream_block_width(const T *__restrict__ y, uint32_t m, double t0,
+                                                                   double t1, EncodeShared &sh) {
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    int64_t mx = INT64_MIN, mn = INT64_MAX;
+    for (uint32_t j = tid; j < m; j += kThreads) {
+        const int64_t d = (int64_t)((uint64_t)y[j] - (uint64_t)predict(t0, t1, j));
+        mx = d > mx ? d : mx;
+        mn = d < mn ? d : mn;
+    }
+    mx = wave_max_i64(mx);
+    mn = wave_min_i64(mn);
+    if (lane == 0) {
+        sh.rmax[wid] = mx;
+        sh.rmin[wid] = mn;
+    }
+    __syncthreads();
+    mx = sh.rmax[0];
+    mn = sh.rmin[0];
+#pragma unroll
+    for (int w = 1; w < kThreads / kWave; ++w) {
+        mx = sh.rmax[w] > mx ? sh.rmax[w] : mx;
+        mn = sh.rmin[w] < mn ? sh.rmin[w] : mn;
+    }
+    // :46 theta0 += (max + min) / 2.  (int64 sum wraps; x * 0.5 is exact)
+    if (!__builtin_isnan(t0)) t0 = t0 + (double)(int64_t)((uint64_t)mx + (uint64_t)mn) * 0.5;
+    uint64_t mg = 0;
+    for (uint32_t j = tid; j < m; j += kThreads) {
+        uint64_t mag;
+        residual_sign<W>((uint64_t)y[j], predict(t0, t1, j), &mag);
+        mg = mag > mg ? mag : mg;
+    }
+    mg = wave_max_u64(mg);
+    if (lane == 0) sh.rmag[wid] = mg;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < kThreads / kWave; ++w) mg = sh.rmag[w] > mg ? sh.rmag[w] : mg;
+    int l = mg ? bitlen64(mg) + 1 : 0;
+    if (l > W) l = W;
+    return {l == W ? 0.0 : t0, l};
+}
+
+// Persistent, software-pipelined encoder for a plain column of 4096-value
+// blocks.  Each workgroup takes tickets in order (stream_counter); per block
+// it runs passes
+// 1-2, publishes the block's size, ORs its bytes into a free LDS image at
+// lead 0 (the offset is not known yet), and starts the loads of its next
+// block; the previous block's look-back, whose first status loads were
+// issued at the start of the iteration, is resolved only then and that
+// block's image stored at its offset (store_image_shifted).  The look-back
+// round trips thus overlap the next block's work instead of idling the
+// workgroup.  A block outside the cached path (not full, input not
+// 2*sizeof(T)-aligned, predictions beyond +-2^51) gets its width from
+// stream_block_width; such a block, and one whose fields exceed 32 bits or
+// are raw values, is marked kDeferred: it takes part in the look-back like
+// any other, and k_encode_deferred writes its bytes afterwards.  No block
+// waits for a look-back while its workgroup holds an unpublished lower
+// ticket (every size is published without waiting), so every wait ends.
+constexpr int kStreamVPT = 2 * kChunk / kThreads;
+constexpr uint32_t kStreamBlock = kStreamVPT * kThreads;
+// Ticket counters: same-address device atomics complete about one per 12 ns
+// (a single counter would cap the encoder near 80 blocks/us), and counters
+// in different 128-byte lines proceed in parallel.  Counter c lives in
+// prm[4c].payload_offset (zeroed by k_fit; the fix-up writes the real
+// payload offsets from the look-back status words); workgroup w draws from
+// counter w % C, and its k-th ticket is block k*C + w % C.
+constexpr uint32_t kStreamCounters = 8;
+__device__ __forceinline__ unsigned long long *stream_counter(leco_fix_block_param *prm, uint32_t c) {
+    return (unsigned long long *)&prm[4 * c].payload_offset;
+}
+inline uint32_t stream_counters(uint64_t nb) {
+    const uint64_t c = (nb + 3) / 4;
+    return (uint32_t)(c < kStreamCounters ? c : kStreamCounters);
+}
+
+struct StreamShared {
+    EncodeShared e;                            // e.buf: image 0
+    alignas(16) uint32_t img1[kLdsBytes / 4];  // image 1
+    uint64_t next_ticket;
+};
+
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads, 4) void k_encode_stream(PlainRows rows, leco_fix_block_param *__restrict__ prm,
+                                                               uint64_t *__restrict__ offs, uint8_t *__restrict__ out,
+                                                               uint32_t nctr) {
+    constexpr int VPT = kStreamVPT;
+    constexpr uint32_t M = kStreamBlock;
+    __shared__ StreamShared ss;
+    EncodeShared &sh = ss.e;
+    const int tid = threadIdx.x;
+    const uint32_t myc = blockIdx.x % nctr;
+    unsigned long long *ctr = stream_counter(prm, myc);
+    auto take = [&]() -> uint64_t { return (uint64_t)atomicAdd(ctr, 1ull) * nctr + myc; };
+    if (tid == 0) {
+        ss.next_ticket = take();
+        sh.init_reductions();
+    }
+    clear_image<kThreads>(sh.buf);
+    clear_image<kThreads>(ss.img1);
+    __syncthreads();
+    uint64_t cur = uniform_u64(ss.next_ticket);
+
+    // the current block: unit, values (when `fast`), thetas
+    const T *y = nullptr;
+    uint32_t m = 0;
+    uint64_t b = 0;
+    bool fast = false;
+    uint64_t v[VPT];
+    double t0 = 0.0, t1 = 0.0;
+    auto fetch = [&](uint64_t u) -> bool {
+        int tl = tid;
+        asm volatile("" : "+v"(tl));
+        const EncodeUnit un = rows.unit(u);
+        if (!un.count) return false;
+        const uint64_t s = un.cb0 * un.bs;
+        m = (uint32_t)(un.n - s < un.bs ? un.n - s : un.bs);
+        y = (const T *)un.base + s;
+        b = un.gb0;
+        fast = m == M && ((uintptr_t)y & (2 * sizeof(T) - 1)) == 0;
+        // thetas by vector loads, issued before the values: their wait then
+        // does not hold up LDS waits (scalar loads share lgkmcnt with LDS)
+        int z = 0;
+        asm volatile("" : "+v"(z));
+        const double2 th = *(const double2 *)&prm[b + (uint32_t)z];
+        t0 = th.x;
+        t1 = th.y;
+        if (fast) {
+#pragma unroll
+            for (int k = 0; k < VPT; k += 2) {
+                const T *p = y + near_pos<kThreads>(tl, k);
+                if constexpr (sizeof(T) == 8) {
+                    const ulonglong2 q = *(const ulonglong2 *)p;
+                    v[k] = q.x;
+                    v[k + 1] = q.y;
+                } else {
+                    const uint2 q = *(const uint2 *)p;
+                    v[k] = q.x;
+                    v[k + 1] = q.y;
+                }
+            }
+        }
+        return true;
+    };
+
+    // the pending block: size published, bytes in image p_img (or deferred)
+    bool pend = false;
+    uint64_t p_u = 0, p_b = 0, p_size = 0;
+    int p_img = -1;
+    uint64_t stw[kLookPer];  // the pending block's first look-back window
+    auto image = [&](int i) -> uint32_t * { return i ? ss.img1 : sh.buf; };
+    auto drain = [&] {
+        const uint64_t off = uniform_u64(resolve_wg<kThreads, kLookPer>(p_u, p_size, offs, sh.lbs, sh.lbf, stw));
+        if (p_img >= 0) {
+            uint32_t *img = image(p_img);
+            store_image_shifted<kThreads>(img, off, p_size, tid, out);
+            __syncthreads();  // the image is read; clear it for its next block
+            clear_image<kThreads>(img);
+        }
+        pend = false;
+    };
+
+    int ci = 0;       // the free image
+    uint64_t tk = 0;  // thread 0: the block after the next one
+    bool have = fetch(cur);
+    while (have) {
+        // an opaque copy of the thread index per block: position arithmetic
+        // is redone per block instead of held in registers across the loop
+        int tl = tid;
+        asm volatile("" : "+v"(tl));
+        if (tid == 0) tk = take();  // read back below
+        if (pend) lookback_issue<kThreads, kLookPer>(p_u, offs, stw);
+        t0 = __longlong_as_double((long long)uniform_u64((uint64_t)__double_as_longlong(t0)));
+        t1 = __longlong_as_double((long long)uniform_u64((uint64_t)__double_as_longlong(t1)));
+        double tr = t0;
+        int l;
+        bool packs = false;
+        uint32_t fld[VPT];
+        if (fast && near_range(t0, t1, M)) {
+            uint32_t signs;
+            l = near_passes<kThreads, VPT, W>(tl, v, tr, t1, signs, sh);
+            if (l == W) tr = 0.0;
+            publish_size(cur, block_bytes((uint32_t)l, W, M), offs);
+            packs = l != W && l <= 32;
+            if (packs && l > 0) near_fields<VPT>(v, signs, l, fld);
+        } else {
+            const BlockWidth r = stream_block_width<T, W>(y, m, t0, t1, sh);
+            l = __builtin_amdgcn_readfirstlane(r.l);
+            tr = __longlong_as_double((long long)uniform_u64((uint64_t)__double_as_longlong(r.t0)));
+            publish_size(cur, block_bytes((uint32_t)l, W, m), offs);
+        }
+        const bool raw = (l == W);
+        if (tid == 0) {
+            prm[b].theta0 = tr;
+            if (raw) prm[b].theta1 = 0.0;
+            prm[b].bits = (uint32_t)l | (packs ? 0u : kDeferred);
+        }
+        const uint64_t size = block_bytes((uint32_t)l, W, m);
+        const uint64_t done_u = cur, done_b = b;
+        const double t1d = t1;
+        if (tid == 0) ss.next_ticket = tk;
+        __syncthreads();
+        if (tid == 0) sh.amag = 0;  // every thread has read it
+        cur = uniform_u64(ss.next_ticket);
+        have = fetch(cur);  // the next block's loads fly during the pack and the drain
+        if (packs) near_pack<kThreads, VPT>(tl, fld, l, tr, t1d, 0, image(ci));
+        if (pend) drain();
+        pend = true;
+        p_u = done_u;
+        p_b = done_b;
+        p_size = size;
+        p_img = packs ? ci : -1;
+        if (packs) ci ^= 1;
+    }
+    if (pend) {
+        lookback_issue<kThreads, kLookPer>(p_u, offs, stw);
+        drain();
+    }
+}
+
+// The bytes of the blocks k_encode_stream marked kDeferred, at their final
+// offsets (from the look-back status words, which end as inclusive
+// prefixes): each workgroup scans kThreads param entries at a time and packs
+// the marked ones chunk by chunk (fields recomputed from the values with
+// the block's final theta0 and width), then clears the mark.
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_encode_deferred(PlainRows rows, leco_fix_block_param *__restrict__ prm,
+                                                              const uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
+    __shared__ EncodeShared sh;
+    __shared__ uint64_t s_mask[kThreads / kWave];
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    for (uint64_t base = (uint64_t)blockIdx.x * kThreads; base < rows.nb; base += (uint64_t)gridDim.x * kThreads) {
+        const uint64_t bb = base + tid;
+        const bool mark = bb < rows.nb && (prm[bb].bits & kDeferred);
+        const uint64_t bal = __ballot(mark);
+        if (lane == 0) s_mask[wid] = bal;
+        __syncthreads();
+        for (int w = 0; w < kThreads / kWave; ++w) {
+            uint64_t mk = uniform_u64(s_mask[w]);
+            while (mk) {
+                const int bit = __ffsll((long long)mk) - 1;
+                mk &= mk - 1;
+                const uint64_t blk = base + (uint64_t)w * kWave + bit;
+                const leco_fix_block_param p = prm[blk];
+                const int l = (int)(p.bits & ~kDeferred);
+                const bool raw = (l == W);
+                const uint64_t s = blk * rows.bs;
+                const uint32_t m = (uint32_t)(rows.n - s < rows.bs ? rows.n - s : rows.bs);
+                const T *y = (const T *)rows.in + s;
+                const uint32_t hdr = raw ? 1u : 17u;
+                const uint64_t end = offs[blk + 1] & kStatusValue;  // the block's inclusive prefix
+                const uint64_t off = end - block_bytes((uint32_t)l, W, m);
+                const double t0 = p.theta0, t1 = p.theta1;
+                const PackedBlock pb{off, end, double_to_bits(t0), double_to_bits(t1), m, l, hdr,
+                                     l == 0 ? 1 : (m + kChunk - 1) / kChunk};
+                auto sync = [] { __syncthreads(); };
+                for (uint32_t c = 0; c < pb.nchunks; ++c)
+                    pack_chunk<kThreads, kChunk, false>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) -> uint64_t {
+                        if (raw) return (uint64_t)y[i];
+                        uint64_t mg2;
+                        return residual_field<W>((uint64_t)y[i], predict(t0, t1, i), l, &mg2);
+                    });
+                if (tid == 0) prm[blk].bits = (uint32_t)l;
+            }
+        }
+        __syncthreads();  // s_mask is rewritten by the next round
+    }
 }
 
 // Wave-encoded unit: `count` consecutive blocks of at most kWaveBlock values,
@@ -427,9 +845,10 @@ __global__ __launch_bounds__(kThreads) void k_encode_waves(PlainRows rows, leco_
     encode_unit_waves<T, W>(un, u, prm, offs, out, sh);
 }
 
-// A plain column of blocks of kWaveBlock < bs <= kLargeEncodeBlock values:
-// the workgroup body alone (two cached chunks), without the wave-unit body
-// that k_encode also carries for batches of mixed block sizes.
+// A plain column of blocks of kWaveBlock < bs <= kLargeEncodeBlock values
+// (other than the stream encoder's): the workgroup body alone (two cached
+// chunks), without the wave-unit body that k_encode also carries for
+// batches of mixed block sizes.
 template <typename T, int W>
 __global__ __launch_bounds__(kThreads) void k_encode_block(PlainRows rows, leco_fix_block_param *__restrict__ prm,
                                                            uint64_t *__restrict__ offs, uint8_t *__restrict__ out) {
@@ -476,6 +895,26 @@ __global__ __launch_bounds__(kThreads) void k_encode_fixup(const leco_fix_block_
     const uint32_t w = widths ? widths[b] : width;
     const leco_fix_block_param p = prm[b];
     uint64_t end = p.payload_offset - (p.bits >= w ? 1 : 17) + block_bytes(p.bits, w, p.length);
+    if (b + 1 == nb) {
+        if (offs[0] & kTicketError) end = LECO_FIX_OFFSETS_POISON;
+        offs[0] = 0;
+    }
+    offs[b + 1] = end;
+}
+
+// The stream encoder's fix-up: the status words hold every block's
+// inclusive prefix (its end offset); payload offsets follow from them and
+// the block sizes (the ticket counters that used those fields are
+// overwritten), then offs[b+1] = end and, as k_encode_fixup, offs[0] back to
+// 0 and the poison if a look-back wait gave up.
+__global__ __launch_bounds__(kThreads) void k_encode_fixup_status(leco_fix_block_param *__restrict__ prm,
+                                                                  uint64_t *__restrict__ offs, uint64_t nb,
+                                                                  uint32_t width) {
+    const uint64_t b = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (b >= nb) return;
+    const leco_fix_block_param p = prm[b];
+    uint64_t end = offs[b + 1] & kStatusValue;
+    prm[b].payload_offset = end - block_bytes(p.bits, width, p.length) + (p.bits >= width ? 1 : 17);
     if (b + 1 == nb) {
         if (offs[0] & kTicketError) end = LECO_FIX_OFFSETS_POISON;
         offs[0] = 0;
@@ -1039,6 +1478,25 @@ __global__ __launch_bounds__(kThreads) void k_serve(const ServeColumn *__restric
 // launchers
 // ---------------------------------------------------------------------------
 
+// Workgroups of the persistent stream encoder: as many as the device holds
+// at once (per device, computed once).  More would only start after the
+// resident ones drained the tickets.
+template <typename T, int W>
+uint32_t stream_workgroups() {
+    static uint32_t cached[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+    if (!cached[dev]) {
+        int cus = 0, per = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_encode_stream<T, W>, kThreads, 0) != hipSuccess ||
+            per <= 0)
+            per = 1;
+        cached[dev] = (uint32_t)(cus * per);
+    }
+    return cached[dev];
+}
+
 template <typename T>
 hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, uint64_t *offs,
                          leco_fix_block_param *prm, hipStream_t st) {
@@ -1047,7 +1505,15 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
     PlainRows rows{(const uint8_t *)in, n, bs, W, nb};
     PlainFit fit{(const uint8_t *)in, n, bs, W, 0};
     k_fit<PlainFit><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
-    if (bs > kLargeEncodeBlock)  // one block per unit
+    if (W == 64 && bs == kStreamBlock && ((uintptr_t)in & (2 * sizeof(T) - 1)) == 0) {
+        const unsigned wgs = (unsigned)std::min<uint64_t>(nb, stream_workgroups<T, W>());
+        k_encode_stream<T, W><<<dim3(wgs), dim3(kThreads), 0, st>>>(rows, prm, offs, stream, stream_counters(nb));
+        k_encode_deferred<T, W><<<dim3((unsigned)std::min<uint64_t>((nb + kThreads - 1) / kThreads, wgs)),
+                                  dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
+        k_encode_fixup_status<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs,
+                                                                                                           nb, W);
+        return hipGetLastError();
+    } else if (bs > kLargeEncodeBlock)  // one block per unit
         k_encode_large<T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     else if (bs <= kWaveBlock)
         k_encode_waves<T, W><<<dim3((unsigned)encode_units(nb, bs)), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);

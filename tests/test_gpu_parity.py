@@ -125,6 +125,54 @@ def test_every_bit_width_small_and_large_blocks(torch_dev, oracle_mod):
             assert np.array_equal(to_host(got, vals.dtype), vals[idx]), (width, bs)
 
 
+def test_prediction_range_edges_and_alignment(torch_dev, oracle_mod):
+    # Full 4096-value blocks around the +-2^51 prediction range that the
+    # encoder's branch-free prediction covers: a fit inside the range whose
+    # recentred theta0 leaves it (one large outlier), a line that crosses
+    # 2^51 inside the block, lines just inside the range, and values above
+    # 2^63; then the same column from an address that is 8 but not 16-byte
+    # aligned.  Byte-identical to the oracle either way.
+    from leco_fix import Column
+    torch = torch_dev
+    bs = 4096
+    i = np.arange(bs, dtype=np.uint64)
+    lim = np.uint64(1 << 51)
+    blocks = []
+    out_up = np.full(bs, lim - np.uint64(1 << 30), np.uint64) + i
+    out_up[2000] += np.uint64(1 << 40)
+    blocks.append(out_up)
+    out_dn = np.full(bs, lim - np.uint64(1 << 30), np.uint64) + i
+    out_dn[100] -= np.uint64(1 << 40)
+    blocks.append(out_dn)
+    blocks.append(lim - np.uint64(2048) + i)                       # crosses 2^51
+    blocks.append(lim - np.uint64(5000) + np.uint64(3) * (i // np.uint64(3)))  # just inside
+    blocks.append(np.uint64(1 << 63) + np.uint64(12345) * i)        # above 2^63
+    blocks.append((lim - np.uint64(1)) - i)                         # decreasing from just inside
+    vals = np.concatenate(blocks + [np.arange(777, dtype=np.uint64)])
+    want_st, want_off = oracle_mod.encode_column(vals, bs, threads=8)
+    say(f"prediction range edges n={len(vals)}")
+    col = Column.encode(to_dev(torch, vals), bs)
+    assert np.array_equal(to_host(col.offsets, np.uint64), want_off)
+    assert col.stream_bytes() == want_st.tobytes()
+    assert np.array_equal(to_host(col.decode(), np.uint64), vals)
+
+    say("prediction range edges: input 8 but not 16-byte aligned")
+    padded = to_dev(torch, np.concatenate([np.zeros(1, np.uint64), vals]))
+    view = padded[1:]
+    assert view.data_ptr() % 16 == 8
+    col2 = Column.encode(view, bs)
+    assert col2.stream_bytes() == want_st.tobytes()
+    for width in (32, 64):
+        rng = np.random.default_rng(width)
+        y = np.sort(rng.integers(0, 1 << (width - 2), 5 * bs + 100, dtype=np.uint64))
+        y = y.astype(np.uint32) if width == 32 else y
+        want_st, _ = oracle_mod.encode_column(y, bs, threads=8)
+        padded = to_dev(torch, np.concatenate([np.zeros(1, y.dtype), y]))
+        say(f"unaligned sorted width={width}")
+        assert Column.encode(padded[1:], bs).stream_bytes() == want_st.tobytes()
+        assert Column.encode(padded[1:].clone(), bs).stream_bytes() == want_st.tobytes()
+
+
 def test_reference_cpu_workload_on_gpu(torch_dev, oracle_mod):
     # 10M uint32, y_i = clamp(round(1000 i + N(0, 50^2))), block size 10,000,
     # 1M uniform lookups: byte-identical to the oracle and exact round trip.
