@@ -479,7 +479,7 @@ inline uint32_t stream_counters(uint64_t nb) {
 struct StreamShared {
     EncodeShared e;                            // e.buf: image 0
     alignas(16) uint32_t img1[kLdsBytes / 4];  // image 1
-    uint64_t next_ticket;
+    uint64_t next_ticket, next_ticket2;
 };
 
 template <typename T, int W>
@@ -496,6 +496,7 @@ __global__ __launch_bounds__(kThreads, 4) void k_encode_stream(PlainRows rows, l
     auto take = [&]() -> uint64_t { return (uint64_t)atomicAdd(ctr, 1ull) * nctr + myc; };
     if (tid == 0) {
         ss.next_ticket = take();
+        ss.next_ticket2 = take();
         sh.init_reductions();
     }
     clear_image<kThreads>(sh.buf);
@@ -562,54 +563,63 @@ __global__ __launch_bounds__(kThreads, 4) void k_encode_stream(PlainRows rows, l
         pend = false;
     };
 
-    int ci = 0;       // the free image
-    uint64_t tk = 0;  // thread 0: the block after the next one
+    // Per iteration: the ticket two blocks ahead is taken first; the
+    // current block's passes; the pending block's first status window is
+    // loaded (fresh, and older than what follows: vmcnt retires in order);
+    // the current block's size is published and, once its values are folded
+    // into 32-bit fields, the next block's loads are issued; then the pack,
+    // and the pending block's look-back and store while those loads fly.
+    int ci = 0;  // the free image
+    uint64_t nxt = uniform_u64(ss.next_ticket2);
     bool have = fetch(cur);
     while (have) {
         // an opaque copy of the thread index per block: position arithmetic
         // is redone per block instead of held in registers across the loop
         int tl = tid;
         asm volatile("" : "+v"(tl));
-        if (tid == 0) tk = take();  // read back below
-        if (pend) lookback_issue<kThreads, kLookPer>(p_u, offs, stw);
+        uint64_t tk = 0;
+        if (tid == 0) tk = take();  // the block after nxt; read back at the end of the iteration
         t0 = __longlong_as_double((long long)uniform_u64((uint64_t)__double_as_longlong(t0)));
         t1 = __longlong_as_double((long long)uniform_u64((uint64_t)__double_as_longlong(t1)));
+        const uint64_t cu = cur, cb = b;
+        const uint32_t cm = m;
         double tr = t0;
+        const double t1c = t1;
         int l;
         bool packs = false;
-        uint32_t fld[VPT];
-        if (fast && near_range(t0, t1, M)) {
-            uint32_t signs;
+        uint32_t signs = 0;
+        const bool near = fast && near_range(t0, t1, M);
+        if (near) {
             l = near_passes<kThreads, VPT, W>(tl, v, tr, t1, signs, sh);
             if (l == W) tr = 0.0;
-            publish_size(cur, block_bytes((uint32_t)l, W, M), offs);
-            packs = l != W && l <= 32;
-            if (packs && l > 0) near_fields<VPT>(v, signs, l, fld);
         } else {
             const BlockWidth r = stream_block_width<T, W>(y, m, t0, t1, sh);
             l = __builtin_amdgcn_readfirstlane(r.l);
             tr = __longlong_as_double((long long)uniform_u64((uint64_t)__double_as_longlong(r.t0)));
-            publish_size(cur, block_bytes((uint32_t)l, W, m), offs);
         }
+        if (pend) lookback_issue<kThreads, kLookPer>(p_u, offs, stw);
         const bool raw = (l == W);
+        const uint64_t size = block_bytes((uint32_t)l, W, cm);
+        publish_size(cu, size, offs);
+        packs = near && !raw && l <= 32;
         if (tid == 0) {
-            prm[b].theta0 = tr;
-            if (raw) prm[b].theta1 = 0.0;
-            prm[b].bits = (uint32_t)l | (packs ? 0u : kDeferred);
+            prm[cb].theta0 = tr;
+            if (raw) prm[cb].theta1 = 0.0;
+            prm[cb].bits = (uint32_t)l | (packs ? 0u : kDeferred);
         }
-        const uint64_t size = block_bytes((uint32_t)l, W, m);
-        const uint64_t done_u = cur, done_b = b;
-        const double t1d = t1;
-        if (tid == 0) ss.next_ticket = tk;
+        uint32_t fld[VPT];
+        if (packs && l > 0) near_fields<VPT>(v, signs, l, fld);
+        cur = nxt;
+        have = fetch(cur);  // the next block's loads fly during the pack and the drain
+        if (packs) near_pack<kThreads, VPT>(tl, fld, l, tr, t1c, 0, image(ci));
+        if (pend) drain();
+        if (tid == 0) ss.next_ticket2 = tk;
         __syncthreads();
         if (tid == 0) sh.amag = 0;  // every thread has read it
-        cur = uniform_u64(ss.next_ticket);
-        have = fetch(cur);  // the next block's loads fly during the pack and the drain
-        if (packs) near_pack<kThreads, VPT>(tl, fld, l, tr, t1d, 0, image(ci));
-        if (pend) drain();
+        nxt = uniform_u64(ss.next_ticket2);
         pend = true;
-        p_u = done_u;
-        p_b = done_b;
+        p_u = cu;
+        p_b = cb;
         p_size = size;
         p_img = packs ? ci : -1;
         if (packs) ci ^= 1;
