@@ -149,6 +149,13 @@ __device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
     return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
 }
 
+// Lane L receives lane L-1's v (DPP wave_shr:1); lane 0 receives 0.
+__device__ __forceinline__ uint64_t wave_prev_u64(uint64_t v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)v, 0x138, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(v >> 32), 0x138, 0xf, 0xf, false);
+    return (uint64_t)(uint32_t)lo | ((uint64_t)(uint32_t)hi << 32);
+}
+
 // Reduction of a commutative, associative op over each row of 16 lanes;
 // every lane of the row ends with the row's result.  Lane exchanges: xor 1
 // and xor 2 (quad permutations), then the 8-lane and 16-lane mirrors, which

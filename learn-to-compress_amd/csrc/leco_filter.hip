@@ -45,8 +45,9 @@ __device__ __forceinline__ bool below(uint64_t v, uint64_t f) {
 constexpr int kWaves = kThreads / kWave;
 constexpr int kEmitPer = 8;  // rounds of kThreads candidates per compaction step when writing positions
 constexpr uint32_t kCloseRounds = 64;  // scan rounds of a tile whose ballots pass 1 parks for pass 2
-constexpr uint32_t kCloseWin = 1024;   // periods whose windows pass 1 keeps for the scan
+constexpr uint32_t kCloseWin = 2048;   // periods whose windows pass 1 keeps for the scan
 constexpr uint32_t kCloseStage = 8192; // bytes of a block's fields the scan sweep stages through LDS
+constexpr uint32_t kCloseTableLen = 32768;  // blocks shorter than this keep the window table
 
 struct FilterShared {
     uint64_t wsum[kWaves];
@@ -56,18 +57,26 @@ struct FilterShared {
     uint32_t stop_passing;  // the period the close loop stopped at still passed its test (limit cut)
     uint64_t base;
     uint64_t tile;
-    // one chunk of kThreads periods of filter_range_close
-    uint64_t pf1[kThreads], pf2[kThreads];
-    uint32_t plo[kThreads];
-    uint32_t ppre[kThreads + 1];
     // per block of the tile: the close loop's period count, how pass 2 reports it (0 walk,
     // 1 scan evaluated again, 2 scan from the ballots pass 1 parked) and its first parked round
     uint32_t bK[256];
     uint8_t bscan[256];
+    uint8_t bl[256];  // the block's header byte
     uint16_t bround[256];
     uint64_t rb[kCloseRounds * kWaves];  // parked scan ballots of (round, wave)
-    // the current block's window of each of its first kCloseWin periods (filter_range_close)
-    int32_t pw1[kCloseWin], pw2[kCloseWin];
+    // thre1, thre2 of each of the current block's first kCloseWin periods (blocks shorter than
+    // kCloseTableLen), as int16: values below -32768 are kept as -32768, which compares with
+    // every position below 2^15 as they do (as uint32 both are above every position)
+    int16_t pw1[kCloseWin], pw2[kCloseWin];
+    union {
+        struct {  // one chunk of kThreads periods of the close walk
+            uint64_t pf1[kThreads], pf2[kThreads];
+            uint32_t plo[kThreads];
+            uint32_t ppre[kThreads + 1];
+        };
+        // the scan's staged fields of one block; at the end of mark, the tile's bitmap
+        alignas(16) uint64_t stage[kCloseStage / 8];
+    };
 };
 
 // Arguments of one filter launch.
@@ -254,6 +263,11 @@ struct CloseBlock {
     double inv_base;        // 1 / base (scan mode: base >= 1)
 };
 
+// Periods whose windows close_periods keeps in the table.
+__device__ __forceinline__ uint32_t close_nwin(const CloseBlock &c, uint32_t K) {
+    return c.m < kCloseTableLen ? (K < kCloseWin ? K : kCloseWin) : 0;
+}
+
 // K: the periods the reference's loop runs (at most kFilterMaxPeriods);
 // *cut: the loop was stopped by the limit with the next period still passing.
 template <int W>
@@ -265,9 +279,9 @@ __device__ __forceinline__ uint32_t close_periods(const CloseBlock &c, FilterSha
         int32_t thre1, thre2;
         period_window<W>((c.f1 + (uint64_t)k * c.step) & wmask, (c.f2 + (uint64_t)k * c.step) & wmask, c.delta,
                          c.t0, c.t1, c.m, &thre1, &thre2);
-        if (k < kCloseWin) {
-            sh.pw1[k] = thre1;
-            sh.pw2[k] = thre2;
+        if (k < kCloseWin && c.m < kCloseTableLen) {
+            sh.pw1[k] = (int16_t)(thre1 < -32768 ? -32768 : thre1);
+            sh.pw2[k] = (int16_t)(thre2 < -32768 ? -32768 : thre2);
         }
         const bool cond = (uint64_t)(int64_t)thre1 < c.m && thre2 > 0;
         const uint64_t sb = __ballot(!cond || k >= kFilterMaxPeriods);
@@ -371,6 +385,7 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
     CloseBlock c;
     c.blk = a.stream + a.offs[b];
     c.l = c.blk[0];
+    if (!WRITE && tid == 0) sh.bl[slot] = (uint8_t)c.l;
     c.f1 = (uint64_t)((const T *)a.f1)[b];
     c.f2 = (uint64_t)((const T *)a.f2)[b];
     c.m = m;
@@ -430,7 +445,7 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
                 bool h = false;
                 if (i < mp) {
                     v = value(i);
-                    h = close_scan_hit<W>(c, K, i, v, a.base, sh, K < kCloseWin ? K : kCloseWin);
+                    h = close_scan_hit<W>(c, K, i, v, a.base, sh, close_nwin(c, K));
                 }
                 uint64_t u = (uint64_t)__shfl_down((long long)v, 1, kWave);
                 if (lane == kWave - 1 && i + 1 < mp) u = value(i + 1);
@@ -442,13 +457,87 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
                 }
             }
         };
-        if (staged)
+        // The common case, without branches: fields of at most 32 bits in LDS (one funnel shift
+        // per field), every prediction inside cvtt64's fast range, a block shorter than
+        // kCloseTableLen.  The order check takes the previous lane's value by DPP (the
+        // boundaries between waves and rounds are checked after the sweep); every lane runs
+        // the period test with a clamped quotient and compares the remainder with the
+        // period's span, (v - f1 - 1) mod base < f2 - f1 - 1, which is v < f2 + k*base.  The
+        // same hits as close_scan_hit.
+        auto fast_sweep = [&](auto value, auto quot_rem) {
+            const uint64_t base_v = (uint64_t)(uint32_t)a.base, kb = (uint64_t)K * base_v;
+            const uint32_t nwin = close_nwin(c, K);
+            const uint32_t span = (W == 64 ? c.f2 > c.f1 : (int64_t)c.f2 > (int64_t)c.f1)
+                                      ? (uint32_t)(c.f2 - c.f1 - 1) : 0u;
+            uint64_t bad = 0;
+            for (uint32_t r = 0; r < nr; ++r) {
+                const uint32_t i = r * kThreads + tid;
+                const bool on = i < mp;
+                const uint64_t v = value(on ? i : mp - 1);
+                const uint64_t pv = wave_prev_u64(v);
+                bad |= __ballot(on && (W == 64 ? pv > v : (int64_t)pv > (int64_t)v));
+                // k: the last period whose f1 lies below v; periods from K on never pass
+                const bool gt = W == 64 ? v > c.f1 : (int64_t)v > (int64_t)c.f1;
+                const uint64_t d = (W == 64 ? v - c.f1 : (uint64_t)((int64_t)v - (int64_t)c.f1)) - 1;
+                const bool in_k = on && gt && d < kb;
+                uint32_t rem;
+                const uint32_t k = quot_rem(in_k ? d : 0, &rem);
+                int32_t thre1, thre2;
+                if (k < nwin) {
+                    thre1 = sh.pw1[k];
+                    thre2 = sh.pw2[k];
+                } else {
+                    period_window<W>(c.f1 + (uint64_t)k * base_v, c.f2 + (uint64_t)k * base_v, c.delta, c.t0,
+                                     c.t1, c.m, &thre1, &thre2);
+                }
+                const bool h = in_k && rem < span && i >= (uint32_t)thre1 && i < (uint32_t)thre2;
+                cnt += h;
+                if (park) {
+                    const uint64_t bal = __ballot(h);
+                    if (lane == 0) sh.rb[(rounds_used + r) * kWaves + wid] = bal;
+                }
+            }
+            sorted = (bad & ~1ull) == 0;  // lane 0 compared nothing
+            for (uint32_t j = tid; 64 * (j + 1) < mp; j += kThreads) {
+                const uint64_t x = value(64 * j + 63), y = value(64 * j + 64);
+                sorted &= W == 64 ? x <= y : (int64_t)x <= (int64_t)y;
+            }
+        };
+        if (staged && c.l <= 32 && m < kCloseTableLen && near_range(c.t0, c.t1, mp)) {
+            const uint32_t *b32 = (const uint32_t *)stage;
+            const uint32_t lw = (uint32_t)c.l;
+            const uint32_t fmask = lw >= 32 ? ~0u : (1u << lw) - 1, mmask = (1u << (lw - 1)) - 1;
+            auto value = [&](uint32_t i) -> uint64_t {
+                const uint32_t q = fbit + i * lw;
+                const uint32_t f = __builtin_amdgcn_alignbit(b32[(q >> 5) + 1], b32[q >> 5], q & 31) & fmask;
+                const uint32_t mag = f & mmask;
+                const int32_t res = (f >> (lw - 1)) ? (int32_t)mag : -(int32_t)mag;
+                return (uint64_t)(predict_near(c.t0, c.t1, i) + (int64_t)res);
+            };
+            const uint32_t bu = (uint32_t)a.base;
+            if ((bu & (bu - 1)) == 0) {
+                const int s = __builtin_ctz(bu);
+                const uint32_t bm = bu - 1;
+                fast_sweep(value, [=](uint64_t d, uint32_t *rem) -> uint32_t {
+                    *rem = (uint32_t)d & bm;
+                    return (uint32_t)(d >> s);
+                });
+            } else {
+                const double inv = c.inv_base;
+                fast_sweep(value, [=](uint64_t d, uint32_t *rem) -> uint32_t {
+                    const uint32_t q = period_quotient(d, (uint64_t)bu, inv);
+                    *rem = (uint32_t)d - q * bu;
+                    return q;
+                });
+            }
+        } else if (staged) {
             sweep([&](uint32_t i) -> uint64_t {
                 return apply_field(predict(c.t0, c.t1, i), lds_field<W>(stage, fbit + i * (uint32_t)c.l, c.l),
                                    c.l);
             });
-        else
+        } else {
             sweep([&](uint32_t i) -> uint64_t { return close_value(c, i); });
+        }
         const uint64_t bad = __ballot(!sorted);
         if (lane == 0) sh.wfail[wid] = bad ? 1u : 0u;
         __syncthreads();
@@ -562,8 +651,7 @@ constexpr uint64_t kBitmapTile = 1ull << 63;
 template <typename T, int W, int MODE>
 __global__ __launch_bounds__(kThreads) void k_filter_close_mark(FilterArgs a) {
     __shared__ FilterShared sh;
-    // the LeCo close scan's field staging, then the tile's bitmap (a placeholder for FOR)
-    __shared__ __align__(16) uint64_t stage[MODE == 1 ? kCloseStage / 8 : 2];
+    uint64_t *const stage = sh.stage;  // the LeCo close scan's field staging, then the tile's bitmap
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     const uint64_t t = blockIdx.x;
     const uint64_t nb = (a.n + a.bs - 1) / a.bs;
@@ -591,33 +679,46 @@ __global__ __launch_bounds__(kThreads) void k_filter_close_mark(FilterArgs a) {
     // swept in scan mode with its ballots parked
     bool bm = MODE == 1 && (uint64_t)kbt * a.bs <= kFilterTile;
     for (uint32_t k = 0; k < kbt && bm; ++k) {
-        const int l = a.stream[a.offs[b0 + k]];
+        const int l = sh.bl[k];
         bm = l > 0 && l < W && (sh.bK[k] == 0 || sh.bscan[k] == 2);
     }
     const uint64_t groups = gt_groups(a.tiles);
     if constexpr (MODE == 1) {
       if (bm) {
-        uint32_t *bits = (uint32_t *)stage;  // pass 1 is done with the staged fields
-        bits[tid] = 0;
-        __syncthreads();
-        for (uint32_t k = 0; k < kbt; ++k) {
-            if (sh.bK[k] == 0) continue;
-            const uint32_t nr = (block_len(b0 + k) + kThreads - 1) / kThreads, r0 = sh.bround[k];
-            for (uint32_t r = 0; r < nr; ++r) {
-                // the wave's 64 positions start at tile bit q: lanes 0-2 OR the ballot, shifted
-                // by q % 32, into words q/32 .. q/32+2 (nonzero parts stay inside the tile)
-                const uint64_t bal = sh.rb[(r0 + r) * kWaves + wid];
-                const uint32_t q = k * a.bs + r * kThreads + (uint32_t)wid * kWave, sft = q & 31;
-                if (bal && lane < 3) {
-                    const uint32_t part = lane == 0   ? (uint32_t)(bal << sft)
-                                          : lane == 1 ? (uint32_t)((bal << sft) >> 32)
-                                                      : (sft ? (uint32_t)(bal >> (64 - sft)) : 0u);
-                    if (part) atomicOr(&bits[(q >> 5) + lane], part);
+        uint32_t *const dst = (uint32_t *)(a.work + groups + a.tiles) + t * kGtWords;
+        if (a.bs % kWave == 0) {
+            // every parked ballot covers 64 positions from a multiple of 64 of the tile, so the
+            // bitmap's 64-bit word j (positions 64j ..) is one ballot, or zero
+            if (tid < (int)(kGtWords / 2)) {
+                const uint32_t q = (uint32_t)tid * kWave, k = q / a.bs, off = q - k * a.bs;
+                uint64_t word = 0;
+                if (k < kbt && sh.bK[k] != 0 && off < block_len(b0 + k))
+                    word = sh.rb[(sh.bround[k] + off / kThreads) * kWaves + (off % kThreads) / kWave];
+                ((uint2 *)dst)[tid] = make_uint2((uint32_t)word, (uint32_t)(word >> 32));
+            }
+        } else {
+            uint32_t *bits = (uint32_t *)stage;  // pass 1 is done with the staged fields
+            bits[tid] = 0;
+            __syncthreads();
+            for (uint32_t k = 0; k < kbt; ++k) {
+                if (sh.bK[k] == 0) continue;
+                const uint32_t nr = (block_len(b0 + k) + kThreads - 1) / kThreads, r0 = sh.bround[k];
+                for (uint32_t r = 0; r < nr; ++r) {
+                    // the wave's 64 positions start at tile bit q: lanes 0-2 OR the ballot, shifted
+                    // by q % 32, into words q/32 .. q/32+2 (nonzero parts stay inside the tile)
+                    const uint64_t bal = sh.rb[(r0 + r) * kWaves + wid];
+                    const uint32_t q = k * a.bs + r * kThreads + (uint32_t)wid * kWave, sft = q & 31;
+                    if (bal && lane < 3) {
+                        const uint32_t part = lane == 0   ? (uint32_t)(bal << sft)
+                                              : lane == 1 ? (uint32_t)((bal << sft) >> 32)
+                                                          : (sft ? (uint32_t)(bal >> (64 - sft)) : 0u);
+                        if (part) atomicOr(&bits[(q >> 5) + lane], part);
+                    }
                 }
             }
+            __syncthreads();
+            dst[tid] = bits[tid];
         }
-        __syncthreads();
-        ((uint32_t *)(a.work + groups + a.tiles))[t * kGtWords + tid] = bits[tid];
       }
     }
     if (tid == 0) {
@@ -630,8 +731,7 @@ __global__ __launch_bounds__(kThreads) void k_filter_close_mark(FilterArgs a) {
 template <typename T, int W, int MODE>
 __global__ __launch_bounds__(kThreads) void k_filter_close_emit(FilterArgs a) {
     __shared__ FilterShared sh;
-    // the LeCo close scan's field staging, or a bitmap tile's words and their prefixes
-    __shared__ __align__(16) uint64_t stage[MODE == 1 ? kCloseStage / 8 : 2];
+    uint64_t *const stage = sh.stage;  // the LeCo close scan's field staging
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     const uint64_t t = blockIdx.x;
     const uint64_t nb = (a.n + a.bs - 1) / a.bs;
@@ -659,21 +759,22 @@ __global__ __launch_bounds__(kThreads) void k_filter_close_emit(FilterArgs a) {
     if (!count) return;
     if constexpr (MODE == 1) {
       if (cw & kBitmapTile) {
-        uint32_t *bits = (uint32_t *)stage, *pre = bits + kGtWords;
+        // thread tid holds bitmap word tid and the hits before it; wave w expands its own
+        // words 64w .. 64w+63 two at a time (64 positions), reading them from its lanes
         const uint32_t word = ((const uint32_t *)(a.work + groups + a.tiles))[t * kGtWords + tid];
         uint32_t all;
         const uint32_t mine = wg_exclusive_scan(sh, (uint32_t)__popc(word), &all);
-        bits[tid] = word;
-        pre[tid] = mine;
-        __syncthreads();
-        const uint32_t start = a.pos_base + (uint32_t)(b0 * a.bs);  // uint32 position arithmetic
-        const uint64_t below = (1ull << lane) - 1;
-        for (uint32_t r = 0; r < kGtWords / 2 / kWaves; ++r) {  // rounds of kThreads positions
-            const uint32_t w64 = r * kWaves + wid;               // the wave's 64 positions
-            const uint64_t bw = (uint64_t)bits[2 * w64] | (uint64_t)bits[2 * w64 + 1] << 32;
-            if ((bw >> lane) & 1) {
-                const uint64_t at = base + pre[2 * w64] + (uint64_t)__popcll(bw & below);
-                if (at < a.cap) a.out[at] = start + w64 * kWave + (uint32_t)lane;
+        const uint32_t start = a.pos_base + (uint32_t)(b0 * a.bs) + (uint32_t)wid * (kWave * 32);  // uint32 arithmetic
+#pragma unroll 4
+        for (int j = 0; j < kWave / 2; ++j) {
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)word, 2 * j);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)word, 2 * j + 1);
+            if ((lo | hi) == 0) continue;
+            const uint32_t p0 = (uint32_t)__builtin_amdgcn_readlane((int)mine, 2 * j);
+            const bool set = ((lane < 32 ? lo >> lane : hi >> (lane - 32)) & 1) != 0;
+            if (set) {
+                const uint64_t at = base + p0 + __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+                if (at < a.cap) a.out[at] = start + (uint32_t)j * kWave + (uint32_t)lane;
             }
         }
         return;
