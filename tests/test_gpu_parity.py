@@ -311,3 +311,31 @@ def test_small_block_ranges_cross_tiles(torch_dev, bs):
             for shape in (DECODE_STAGED, DECODE_VALUES):
                 got = to_host(col.decode_range(a, b, shape=shape), vals.dtype)
                 assert np.array_equal(got, vals[a:b]), (width, a, b, shape)
+
+
+@pytest.mark.parametrize("bs", [256, 1000, 2048, 4096])
+def test_decode_into_unaligned_output(torch_dev, bs):
+    # whole 64-bit chunks and small-block pairs are written with 16-byte stores
+    # when their destination is 16-byte aligned: output buffers that start 8
+    # bytes past a 16-byte boundary, ranges of even and odd starts, and both
+    # together take the per-value stores instead, with the same values
+    from leco_fix import Column, datagen
+    torch = torch_dev
+    rng = np.random.default_rng(4242 + bs)
+    n = 6 * bs + 77
+    vals = datagen.sorted_uniform(n, 64, bs)
+    say(f"unaligned output bs={bs} n={n}")
+    col = Column.encode(to_dev(torch, vals), bs)
+    backing = torch.zeros(n + 2, dtype=torch.int64, device="cuda")
+    for shift in (0, 1):
+        out = backing[shift:shift + n]
+        assert (out.data_ptr() % 16 == 0) == (shift == 0)
+        col.decode(out=out)
+        assert np.array_equal(to_host(out, np.uint64), vals), shift
+        for a, b in [(0, n), (1, n), (bs, 3 * bs), (bs + 1, 3 * bs + 1), (2 * bs - 1, n - 1)] + \
+                [tuple(sorted(rng.integers(0, n + 1, 2))) for _ in range(6)]:
+            if a == b:
+                continue
+            out = backing[shift:shift + (b - a)]
+            col.decode_range(a, b, out=out)
+            assert np.array_equal(to_host(out, np.uint64), vals[a:b]), (shift, a, b)
