@@ -630,6 +630,61 @@ static_assert(kGtWords == kThreads, "one bitmap word per thread");
 constexpr uint32_t kGtGroup = kGtGroupTiles;     // tiles per group sum
 __host__ __device__ inline uint64_t gt_groups(uint64_t tiles) { return (tiles + kGtGroup - 1) / kGtGroup; }
 
+// Positions of one tile's bitmap written to out[base ..] in position order.
+// Thread tid holds bitmap word tid (positions start + 32*tid ..) and `mine`,
+// the hits in the words before it; `total` is the tile's hit count.  Each
+// wave expands its own 64 words two at a time (64 positions per step: word
+// pair and prefix by lane reads, the slot by mbcnt) into LDS (buf: the whole
+// tile when its hits fit kEmitBuf entries, else each half of the words in
+// turn, at most kEmitBuf positions each), and the buffer is then copied out
+// with 16-byte stores.  Only out[.. cap) is written.
+constexpr uint32_t kEmitBuf = kFilterTile / 2;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void emit_bitmap(const FilterArgs &a, uint32_t word, uint32_t mine, uint32_t total,
+                                            uint64_t base, uint32_t start, uint32_t *buf, uint32_t *half) {
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    if (tid == (int)(kGtWords / 2)) *half = mine;  // hits in the first half of the words
+    __syncthreads();
+    const uint32_t split = *half;
+    const bool whole = total <= kEmitBuf;
+    for (int h = 0; h < (whole ? 1 : 2); ++h) {
+        const uint32_t h0 = whole || h == 0 ? 0u : split;     // hits before this pass
+        const uint32_t h1 = whole || h == 1 ? total : split;  // ... and up to its end
+        // waves 0-1 hold the first half of the words, waves 2-3 the second
+        if (whole || (h == 0) == (wid < kWaves / 2)) {
+            const uint32_t p0 = start + (uint32_t)wid * (kWave * 32);
+#pragma unroll 4
+            for (int j = 0; j < kWave / 2; ++j) {
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)word, 2 * j);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)word, 2 * j + 1);
+                if ((lo | hi) == 0) continue;
+                const uint32_t before = (uint32_t)__builtin_amdgcn_readlane((int)mine, 2 * j) - h0;
+                if (((lane < 32 ? lo >> lane : hi >> (lane - 32)) & 1) != 0)
+                    buf[before + __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u))] =
+                        p0 + (uint32_t)j * kWave + (uint32_t)lane;
+            }
+        }
+        __syncthreads();
+        const uint64_t dst = base + h0;
+        uint32_t n = h1 - h0;
+        if (dst >= a.cap) n = 0;
+        else if (a.cap - dst < n) n = (uint32_t)(a.cap - dst);
+        uint32_t *o = a.out + dst;
+        const uint32_t lead = (uint32_t)((4 - (((uintptr_t)o >> 2) & 3)) & 3);  // entries before a 16-byte boundary
+        const uint32_t head = lead < n ? lead : n;
+        if ((uint32_t)tid < head) o[tid] = buf[tid];
+        const uint32_t quads = (n - head) / 4;
+        u32x4 *o4 = (u32x4 *)(o + head);
+        for (uint32_t q = tid; q < quads; q += kThreads) {
+            const uint32_t i = head + 4 * q;
+            __builtin_nontemporal_store(u32x4{buf[i], buf[i + 1], buf[i + 2], buf[i + 3]}, &o4[q]);
+        }
+        const uint32_t done = head + 4 * quads;
+        if ((uint32_t)tid < n - done) o[done + tid] = buf[done + tid];
+        __syncthreads();  // buf is read before the next pass refills it
+    }
+}
+
 // The close filters, MODE 1 (LeCo filter_range_close) and 3 (FOR
 // filter_range_close), in two launches with no serial dependence between
 // tiles (a tile is filter_tile_blocks() whole blocks):
@@ -759,24 +814,12 @@ __global__ __launch_bounds__(kThreads) void k_filter_close_emit(FilterArgs a) {
     if (!count) return;
     if constexpr (MODE == 1) {
       if (cw & kBitmapTile) {
-        // thread tid holds bitmap word tid and the hits before it; wave w expands its own
-        // words 64w .. 64w+63 two at a time (64 positions), reading them from its lanes
         const uint32_t word = ((const uint32_t *)(a.work + groups + a.tiles))[t * kGtWords + tid];
         uint32_t all;
-        const uint32_t mine = wg_exclusive_scan(sh, (uint32_t)__popc(word), &all);
-        const uint32_t start = a.pos_base + (uint32_t)(b0 * a.bs) + (uint32_t)wid * (kWave * 32);  // uint32 arithmetic
-#pragma unroll 4
-        for (int j = 0; j < kWave / 2; ++j) {
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)word, 2 * j);
-            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)word, 2 * j + 1);
-            if ((lo | hi) == 0) continue;
-            const uint32_t p0 = (uint32_t)__builtin_amdgcn_readlane((int)mine, 2 * j);
-            const bool set = ((lane < 32 ? lo >> lane : hi >> (lane - 32)) & 1) != 0;
-            if (set) {
-                const uint64_t at = base + p0 + __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
-                if (at < a.cap) a.out[at] = start + (uint32_t)j * kWave + (uint32_t)lane;
-            }
-        }
+        const uint32_t mine = wg_exclusive_scan(sh, (uint32_t)__popc(word), &all);  // ends with a barrier
+        static_assert(sizeof(FilterShared) >= kEmitBuf * 4 + 4, "the shared state holds the emit buffer");
+        uint32_t *const buf = (uint32_t *)&sh;  // the shared state is free from here on
+        emit_bitmap(a, word, mine, all, base, a.pos_base + (uint32_t)(b0 * a.bs), buf, buf + kEmitBuf);
         return;
       }
     }
@@ -967,7 +1010,8 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt_mark(FilterArgs a) {
 // The positions of tile t's bitmap, at its offset (position order).
 __global__ __launch_bounds__(kThreads) void k_filter_gt_emit(FilterArgs a) {
     __shared__ GtShared sh;
-    __shared__ uint32_t pre[kGtWords];
+    __shared__ uint32_t obuf[kEmitBuf];
+    __shared__ uint32_t half;
     const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
     const uint64_t t = blockIdx.x;
     const GtTile g = gt_tile(a, t);
@@ -988,18 +1032,7 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt_emit(FilterArgs a) {
     for (int w = 0; w < kWaves; ++w) base += sh.wsum[w];
     if (t + 1 == a.tiles && tid == 0) a.result[0] = base + total;
     if (!total) return;
-    sh.bits[tid] = word;
-    pre[tid] = mine;
-    __syncthreads();
-    const uint64_t below = (1ull << lane) - 1;
-    for (uint32_t r = 0; r < kGtWords / 2 / kWaves; ++r) {  // rounds of kThreads positions
-        const uint32_t w64 = r * kWaves + wid;               // the wave's 64 positions
-        const uint64_t bits = (uint64_t)sh.bits[2 * w64] | (uint64_t)sh.bits[2 * w64 + 1] << 32;
-        if ((bits >> lane) & 1) {
-            const uint64_t at = base + pre[2 * w64] + (uint64_t)__popcll(bits & below);
-            if (at < a.cap) a.out[at] = start + w64 * kWave + (uint32_t)lane;
-        }
-    }
+    emit_bitmap(a, word, mine, total, base, start, obuf, &half);
 }
 
 __global__ __launch_bounds__(kThreads) void k_filter_clear(uint64_t *__restrict__ work, uint64_t words,
