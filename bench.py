@@ -676,7 +676,8 @@ def main() -> int:
             k = json.load(f)["kernels"].get("k_decode")
         if k:
             traffic = int(k["read_bytes"] + k["write_bytes"])
-            traffic_src = os.path.relpath(pmc_file, ROOT) + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; copy-calibrated)"
+            traffic_src = os.path.relpath(pmc_file, ROOT) + (" (rocprofv3 --pmc TCC_EA0_RDREQ_DRAM_32B_sum, "
+                                                             "TCC_EA0_WRREQ_WRITE_DRAM_32B_sum; x 32 B)")
 
     dec_gbs = raw_bytes / (ms_dec * 1e-3) / 1e9
     achieved = (comp + raw_bytes) / (ms_dec * 1e-3) / 1e9
