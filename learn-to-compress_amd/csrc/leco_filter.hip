@@ -508,7 +508,7 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
             const uint32_t lw = (uint32_t)c.l;
             const uint32_t fmask = lw >= 32 ? ~0u : (1u << lw) - 1, mmask = (1u << (lw - 1)) - 1;
             auto value = [&](uint32_t i) -> uint64_t {
-                const uint32_t q = fbit + i * lw;
+                const uint32_t q = fbit + __umul24(i, lw);  // i < 2^15: the full-rate 24-bit multiply
                 const uint32_t f = __builtin_amdgcn_alignbit(b32[(q >> 5) + 1], b32[q >> 5], q & 31) & fmask;
                 const uint32_t mag = f & mmask;
                 const int32_t res = (f >> (lw - 1)) ? (int32_t)mag : -(int32_t)mag;
@@ -520,7 +520,9 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
                 const uint32_t bm = bu - 1;
                 fast_sweep(value, [=](uint64_t d, uint32_t *rem) -> uint32_t {
                     *rem = (uint32_t)d & bm;
-                    return (uint32_t)(d >> s);
+                    // d >> s by a 32-bit funnel shift (s <= 30; the quotient is below 2^16 for d < K*base
+                    // and 0 for the d = 0 the caller passes out of range)
+                    return __builtin_amdgcn_alignbit((uint32_t)(d >> 32), (uint32_t)d, (uint32_t)s);
                 });
             } else {
                 const double inv = c.inv_base;
