@@ -613,12 +613,16 @@ def main() -> int:
         filt = filter_section(torch, vals, col, args.filter_reps)
         log(f"filter: {filt}")
 
-    copy_gbs = None
+    copy_gbs = fill_gbs = None
     if args.copy_reps > 0:
-        log("device copy reference")
+        log("device copy and fill references")
         dst = torch.empty_like(out)
         ms_cp, _ = timed(torch, lambda: dst.copy_(out), args.copy_reps, 1)
         copy_gbs = 2 * raw_bytes / (ms_cp * 1e-3) / 1e9
+        # the runtime writing the decoded column's N x 8 bytes and reading nothing: the rate the
+        # decode's output alone could be written at
+        ms_fill, _ = timed(torch, lambda: dst.fill_(7), args.copy_reps, 1)
+        fill_gbs = raw_bytes / (ms_fill * 1e-3) / 1e9
         del dst
 
     del lk, idx
@@ -709,6 +713,7 @@ def main() -> int:
             "decode_wall_ms_per_step": round(wall_dec * 1e3 / args.steps, 4),
             "decode_value_parallel_ms": round(ms_dec_values, 4),
             "device_copy_gbs": round(copy_gbs, 1) if copy_gbs else None,
+            "device_fill_gbs": round(fill_gbs, 1) if fill_gbs else None,
         },
         "roofline": {
             "bound": "hbm",

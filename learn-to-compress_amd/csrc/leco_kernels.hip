@@ -952,7 +952,7 @@ template <typename T, bool RAW>
 __device__ __forceinline__ void decode_run(const uint64_t *__restrict__ buf, uint32_t lbit, int lw, int l, double t0,
                                            double t1, uint32_t j0, uint32_t ja, uint32_t jb, T *__restrict__ o) {
     const int tid = threadIdx.x;
-    auto value = [&](uint32_t j) -> T {
+    auto one = [&](uint32_t j) {
         const uint32_t q = lbit + (j - j0) * (uint32_t)lw;
         T v;
         if constexpr (sizeof(T) == 4) {
@@ -971,29 +971,12 @@ __device__ __forceinline__ void decode_run(const uint64_t *__restrict__ buf, uin
             const uint64_t field = ((buf[w] >> sh) | ((buf[w + 1] << 1) << (63 - sh))) & low_mask(lw);
             v = RAW ? (T)field : (T)apply_field(predict(t0, t1, j), field, l);
         }
-        return v;
+        // decoded values are written once and not read back here: streaming stores
+        __builtin_nontemporal_store(v, &o[j]);
     };
-    // decoded values are written once and not read back here: streaming stores
-    auto one = [&](uint32_t j) { __builtin_nontemporal_store(value(j), &o[j]); };
     if (ja == j0 && jb == j0 + kChunk) {
-        // a whole chunk of 64-bit values: when its first value is 16-byte aligned, each
-        // thread decodes two consecutive values per step and writes them with one 16-byte
-        // store (+4% over one value per lane; four 32-bit values per lane measured no gain)
-        constexpr int kVec = 16 / (int)sizeof(T);
-        if (sizeof(T) == 8 && (((uintptr_t)(o + j0)) & 15) == 0) {
-            typedef T tvec __attribute__((ext_vector_type(kVec)));
 #pragma unroll
-            for (int kk = 0; kk < (int)(kChunk / kThreads / kVec); ++kk) {
-                const uint32_t j = j0 + (uint32_t)kVec * (tid + kk * kThreads);
-                tvec x;
-#pragma unroll
-                for (int e = 0; e < kVec; ++e) x[e] = value(j + e);
-                __builtin_nontemporal_store(x, (tvec *)&o[j]);
-            }
-        } else {
-#pragma unroll
-            for (int kk = 0; kk < (int)(kChunk / kThreads); ++kk) one(j0 + tid + kk * kThreads);
-        }
+        for (int kk = 0; kk < (int)(kChunk / kThreads); ++kk) one(j0 + tid + kk * kThreads);
     } else {
         for (uint32_t j = ja + tid; j < jb; j += kThreads) one(j);
     }
@@ -1157,15 +1140,9 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
         for (uint32_t pb = p0; pb < pend; pb += kMultiUnroll * kThreads) {
             uint32_t lb[kMultiUnroll], jj[kMultiUnroll];
             double2 th[kMultiUnroll];
-            // position of slot u: 32-bit values lane-strided; 64-bit values in pairs of
-            // consecutive positions per thread, stored together (ppos)
-            auto ppos = [&](int u) -> uint32_t {
-                return W == 64 ? pb + 2 * (uint32_t)tid + (uint32_t)(u >> 1) * 2 * kThreads + (uint32_t)(u & 1)
-                               : pb + (uint32_t)tid + (uint32_t)u * kThreads;
-            };
 #pragma unroll
             for (int u = 0; u < kMultiUnroll; ++u) {
-                uint32_t p = ppos(u);
+                uint32_t p = pb + tid + u * kThreads;
                 p = p < pend ? p : pend - 1;
                 const uint32_t k = pow2 ? p >> lsh : __umulhi(p, inv);
                 jj[u] = p - k * bs;
@@ -1216,11 +1193,9 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
                     w0[u] = sm.buf[q >> 6];
                     w1[u] = sm.buf[(q >> 6) + 1];
                 }
-                uint64_t val[kMultiUnroll];
-                bool wr[kMultiUnroll];
 #pragma unroll
                 for (int u = 0; u < kMultiUnroll; ++u) {
-                    const uint32_t p = ppos(u);
+                    const uint32_t p = pb + tid + u * kThreads;
                     const uint32_t j = jj[u];
                     const int l = (int)(lb[u] & 127);
                     const bool raw = l >= W;
@@ -1228,30 +1203,16 @@ __device__ __forceinline__ void decode_multi_body(const uint8_t *__restrict__ st
                     const int sh = (int)(((lb[u] >> 7) + j * (uint32_t)lw) & 63);
                     const uint64_t field = ((w0[u] >> sh) | ((w1[u] << 1) << (63 - sh))) & low_mask(lw);
                     if constexpr (CODEC == kCodecFor) {
-                        val[u] = raw ? field : double_to_bits(th[u].x) + field;
-                        wr[u] = p < pend;
+                        const uint64_t val = raw ? field : double_to_bits(th[u].x) + field;
+                        if (p < pend) __builtin_nontemporal_store((T)val, &o[p]);
                     } else {
                         const int64_t pr = predict(th[u].x, th[u].y, j);
                         const int lc = l > 0 ? l : 1;
                         const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
                         const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
-                        val[u] = raw ? field : (l ? v : (uint64_t)pr);
-                        wr[u] = p < pend && (l < 126 || j < (uint32_t)(128 - l));
-                    }
-                }
-                // pairs of consecutive positions: one 16-byte store when both are written and
-                // the pair is 16-byte aligned (every pair is when o + pb is: pairs start at
-                // pb + an even offset)
-                const bool al = (((uintptr_t)(o + pb)) & 15) == 0;
-#pragma unroll
-                for (int u = 0; u < kMultiUnroll; u += 2) {
-                    const uint32_t p = ppos(u);
-                    typedef uint64_t u64v2 __attribute__((ext_vector_type(2)));
-                    if (al && wr[u] && wr[u + 1]) {
-                        __builtin_nontemporal_store(u64v2{val[u], val[u + 1]}, (u64v2 *)&o[p]);
-                    } else {
-                        if (wr[u]) __builtin_nontemporal_store((T)val[u], &o[p]);
-                        if (wr[u + 1]) __builtin_nontemporal_store((T)val[u + 1], &o[p + 1]);
+                        const uint64_t val = raw ? field : (l ? v : (uint64_t)pr);
+                        const bool wr = p < pend && (l < 126 || j < (uint32_t)(128 - l));
+                        if (wr) __builtin_nontemporal_store((T)val, &o[p]);
                     }
                 }
             }

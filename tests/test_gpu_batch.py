@@ -176,3 +176,38 @@ def test_autotune_columns_picks_within_tolerance(oracle_mod):
     torch.cuda.synchronize()
     assert all(torch.equal(o, v) for o, v in zip(outs, dev))
     b.close()
+
+
+def test_batch_decode_of_long_large_block_columns():
+    # long columns of large blocks (several chunks per block, blocks both
+    # inside and over the staging buffer) through the batch decode kernels,
+    # whose output pointers come from the column table: every value, with and
+    # without graph replay, against the column itself and the plain decode
+    import torch
+    from leco_fix import workloads
+    from leco_fix.batch import Batch
+    specs = [("linear_sigma", 10_760_805, 64, 3 * 100003 + 324, 16384),
+             ("linear_sigma", 4_000_001, 64, 11, 8192),
+             ("sorted_uniform", 3_000_000, 64, 12, 2048),
+             ("linear_sigma", 5_000_000, 32, 13, 16384),
+             ("sorted_zipf", 2_000_000, 64, 14, 1024)]
+    vals = [workloads.gen_column(d, n, w, s) for d, n, w, s, _ in specs]
+    say(f"batch of {len(specs)} long columns")
+    b = Batch(vals, [bs for *_, bs in specs])
+    b.encode()
+    torch.cuda.synchronize()
+    for graph in (False, True, True):
+        for o in b.outs:
+            o.zero_()
+        outs = b.decode(graph=graph)
+        torch.cuda.synchronize()
+        for c, v in enumerate(vals):
+            bad = (outs[c] != v).nonzero()
+            assert bad.numel() == 0, (c, graph, bad[:8].flatten().tolist())
+    for o in b.outs:
+        o.zero_()
+    b.decode_per_column()
+    torch.cuda.synchronize()
+    for c, v in enumerate(vals):
+        assert torch.equal(b.outs[c], v), c
+    b.close()
