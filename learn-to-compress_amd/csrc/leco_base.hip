@@ -299,10 +299,15 @@ __global__ __launch_bounds__(kThreads) void k_base_decode(const uint8_t *__restr
 // of kb consecutive blocks, their bytes staged into LDS whole (as many blocks
 // at a time as fit kMultiLds), one wave per block.  A wave takes 16 bytes of
 // consecutive values per lane per step (4 uint32 or 2 uint64): the lane sums
-// its deltas in order, one inclusive wave scan of the lane sums plus the
-// running value places them, and each lane writes its values with one
-// 16-byte store.  Against one delta per lane and a scan per 64 values, this
-// is 1.4x faster at block sizes 256-1000.
+// its deltas in order, and one inclusive wave scan of the lane sums plus the
+// running value places them.  Against one delta per lane and a scan per 64
+// values, this is 1.4x faster at block sizes 256-1000.  The scan is DPP row
+// shifts and broadcasts (VALU), not LDS shuffles.  The step's values then
+// cross lanes (wave shuffles) so that each store instruction writes 8 bytes
+// per lane (4 for a uint32 block that starts at an odd word), contiguous
+// across the wave: a 16-byte store of the lane's own values can read its
+// upper half from the VGPRs only after the next LDS field read has refilled
+// them.
 template <typename T, int W>
 __global__ __launch_bounds__(kThreads) void k_delta_decode_multi(const uint8_t *__restrict__ st,
                                                                  const uint64_t *__restrict__ offs, uint64_t n,
@@ -359,12 +364,10 @@ __global__ __launch_bounds__(kThreads) void k_delta_decode_multi(const uint8_t *
             // Value p is y0 plus deltas 0 .. p-1 (delta d is field d), sums
             // modulo 2^W in T.  Each lane takes R consecutive values p0 ..
             // p0+R-1 of a step of R*kWave (R*sizeof(T) = 16 bytes): its deltas
-            // summed in order, then one wave scan of the lane totals; the R
-            // values go out as one 16-byte store, contiguous across the wave,
-            // when the block's output is 16-byte aligned.
+            // summed in order, then one wave scan of the lane totals.
             T carry = (T)y0;
             constexpr uint32_t R = 16 / sizeof(T);
-            const bool vec = ((uintptr_t)o & 15) == 0;
+            const bool pairs = ((uintptr_t)o & 7) == 0;  // uint32: 8-byte stores of value pairs
             for (uint32_t i0 = 0; i0 < m; i0 += R * kWave) {
                 const uint32_t p0 = i0 + R * lane;
                 T v[R];
@@ -378,31 +381,55 @@ __global__ __launch_bounds__(kThreads) void k_delta_decode_multi(const uint8_t *
                     }
                     v[k] = run;
                 }
-                T tot = run;
-                for (int sft = 1; sft < kWave; sft <<= 1) {
-                    T t;
-                    if constexpr (W == 32) t = (T)__shfl_up((int)tot, sft, kWave);
-                    else t = (T)__shfl_up((long long)tot, sft, kWave);
-                    if (lane >= sft) tot += t;
-                }
+                const T tot = wave_inclusive_sum(run);
                 const T base = carry + (tot - run);
 #pragma unroll
                 for (uint32_t k = 0; k < R; ++k) v[k] += base;
-                if (vec && p0 + R <= m) {
-                    if constexpr (W == 32) {
-                        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-                        __builtin_nontemporal_store(u32x4{v[0], v[1], v[2], v[3]}, (u32x4 *)(o + p0));
-                    } else {
-                        typedef uint64_t u64x2 __attribute__((ext_vector_type(2)));
-                        __builtin_nontemporal_store(u64x2{v[0], v[1]}, (u64x2 *)(o + p0));
+                const bool odd = lane & 1;
+                if constexpr (W == 64) {
+                    // store h writes values i0 + 64h + lane: value 2s + c of the
+                    // step sits in lane s, v[c]
+#pragma unroll
+                    for (uint32_t h = 0; h < 2; ++h) {
+                        const int src = (int)(32 * h) + (lane >> 1);
+                        const T a = (T)__shfl((long long)v[0], src, kWave);
+                        const T b = (T)__shfl((long long)v[1], src, kWave);
+                        const uint32_t p = i0 + 64 * h + lane;
+                        if (p < m) __builtin_nontemporal_store(odd ? b : a, &o[p]);
+                    }
+                } else if (pairs) {
+                    // store h writes values i0 + 128h + 2 lane and the next:
+                    // value 4s + c of the step sits in lane s, v[c]
+                    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+                    for (uint32_t h = 0; h < 2; ++h) {
+                        const int src = (int)(32 * h) + (lane >> 1);
+                        const uint32_t a0 = (uint32_t)__shfl((int)v[0], src, kWave);
+                        const uint32_t a1 = (uint32_t)__shfl((int)v[1], src, kWave);
+                        const uint32_t a2 = (uint32_t)__shfl((int)v[2], src, kWave);
+                        const uint32_t a3 = (uint32_t)__shfl((int)v[3], src, kWave);
+                        const uint32_t x = odd ? a2 : a0, y = odd ? a3 : a1;
+                        const uint32_t q = i0 + 128 * h + 2 * lane;
+                        if (q + 1 < m) __builtin_nontemporal_store(u32x2{x, y}, (u32x2 *)(o + q));
+                        else if (q < m) __builtin_nontemporal_store((T)x, &o[q]);
                     }
                 } else {
+                    // uint32 output at an odd word: store h writes values
+                    // i0 + 64h + lane, value 4s + c sitting in lane s, v[c]
 #pragma unroll
-                    for (uint32_t k = 0; k < R; ++k)
-                        if (p0 + k < m) __builtin_nontemporal_store(v[k], &o[p0 + k]);
+                    for (uint32_t h = 0; h < 4; ++h) {
+                        const int src = (int)(16 * h) + (lane >> 2);
+                        const uint32_t a0 = (uint32_t)__shfl((int)v[0], src, kWave);
+                        const uint32_t a1 = (uint32_t)__shfl((int)v[1], src, kWave);
+                        const uint32_t a2 = (uint32_t)__shfl((int)v[2], src, kWave);
+                        const uint32_t a3 = (uint32_t)__shfl((int)v[3], src, kWave);
+                        const uint32_t c = lane & 3;
+                        const uint32_t x = c == 0 ? a0 : c == 1 ? a1 : c == 2 ? a2 : a3;
+                        const uint32_t p = i0 + 64 * h + lane;
+                        if (p < m) __builtin_nontemporal_store((T)x, &o[p]);
+                    }
                 }
-                if constexpr (W == 32) carry += (T)__shfl((int)tot, kWave - 1, kWave);
-                else carry += (T)__shfl((long long)tot, kWave - 1, kWave);
+                carry += wave_last(tot);
             }
         }
         __syncthreads();

@@ -184,6 +184,51 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
     return s;
 }
 
+// Inclusive prefix sums over the wave's 64 lanes (all active), modulo 2^32 or
+// 2^64: Hillis-Steele steps inside each row of 16 lanes by DPP row shifts of
+// 1, 2, 4 and 8 (a lane whose source lies before its row adds 0), then row
+// broadcasts carry the rows' totals on: lane 15's into rows 1 and 3 (lane 47's
+// for row 3: the broadcast reads the row before), and lane 31's into rows 2
+// and 3.  The pieces of a 64-bit value move alike, and the adds carry.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_or_zero_u32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_or_zero_u64(uint64_t v) {
+    return (uint64_t)dpp_or_zero_u32<CTRL, ROWS>((uint32_t)v) |
+           ((uint64_t)dpp_or_zero_u32<CTRL, ROWS>((uint32_t)(v >> 32)) << 32);
+}
+template <typename T>
+__device__ __forceinline__ T wave_inclusive_sum(T v) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit lanes");
+    if constexpr (sizeof(T) == 4) {
+        v += (T)dpp_or_zero_u32<0x111, 0xf>((uint32_t)v);  // row_shr:1
+        v += (T)dpp_or_zero_u32<0x112, 0xf>((uint32_t)v);  // row_shr:2
+        v += (T)dpp_or_zero_u32<0x114, 0xf>((uint32_t)v);  // row_shr:4
+        v += (T)dpp_or_zero_u32<0x118, 0xf>((uint32_t)v);  // row_shr:8
+        v += (T)dpp_or_zero_u32<0x142, 0xa>((uint32_t)v);  // row_bcast:15 into rows 1, 3
+        v += (T)dpp_or_zero_u32<0x143, 0xc>((uint32_t)v);  // row_bcast:31 into rows 2, 3
+    } else {
+        v += (T)dpp_or_zero_u64<0x111, 0xf>((uint64_t)v);
+        v += (T)dpp_or_zero_u64<0x112, 0xf>((uint64_t)v);
+        v += (T)dpp_or_zero_u64<0x114, 0xf>((uint64_t)v);
+        v += (T)dpp_or_zero_u64<0x118, 0xf>((uint64_t)v);
+        v += (T)dpp_or_zero_u64<0x142, 0xa>((uint64_t)v);
+        v += (T)dpp_or_zero_u64<0x143, 0xc>((uint64_t)v);
+    }
+    return v;
+}
+
+// Lane 63's value, in every lane (scalar reads).
+template <typename T>
+__device__ __forceinline__ T wave_last(T v) {
+    if constexpr (sizeof(T) == 4) return (T)__builtin_amdgcn_readlane((int)(uint32_t)v, 63);
+    else
+        return (T)((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, 63) |
+                   ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), 63) << 32));
+}
+
 __device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
     for (int o = kWave / 2; o > 0; o >>= 1) {
         uint64_t u = (uint64_t)__shfl_xor((long long)v, o, kWave);

@@ -295,64 +295,66 @@ struct PackedBlock {
     int lbyte = -1;        // header byte 0 when it differs from the field width l
 };
 
+// Image stores are 8 bytes wide: a lane's 8 bytes are read from the VGPRs as
+// the store issues.  A 16-byte store can read its upper 8 bytes later, after
+// a load that the compiler placed behind it has already refilled the same
+// registers, and then writes the load's data.
+typedef uint32_t img_u32x2 __attribute__((ext_vector_type(2)));
+
 // Bytes [g0, g1) of the output from the LDS image `buf` of the 16-byte
-// aligned range [a0, a0 + span): 16-byte stores, byte stores at the ragged
-// ends (the neighbouring blocks' bytes are not touched).
+// aligned range [a0, a0 + span): lane-contiguous 8-byte stores, byte stores
+// at the ragged ends (the neighbouring blocks' bytes are not touched).
 template <int NT>
 __device__ __forceinline__ void store_image(const uint32_t *buf, uint64_t a0, uint64_t g0, uint64_t g1, uint32_t span,
                                             int tid, uint8_t *__restrict__ out) {
     const uint8_t *bb = (const uint8_t *)buf;
-    for (uint32_t u = tid; u < span / 16; u += NT) {
-        const uint64_t ga = a0 + 16ull * u;
-        if (ga >= g0 && ga + 16 <= g1) {
-            *(uint4 *)(out + ga) = *(const uint4 *)(bb + 16 * u);
+    for (uint32_t u = tid; u < span / 8; u += NT) {
+        const uint64_t ga = a0 + 8ull * u;
+        if (ga >= g0 && ga + 8 <= g1) {
+            *(img_u32x2 *)(out + ga) = img_u32x2{buf[2 * u], buf[2 * u + 1]};
         } else {
-            for (int k = 0; k < 16; ++k) {
+            for (int k = 0; k < 8; ++k) {
                 const uint64_t gb = ga + k;
-                if (gb >= g0 && gb < g1) out[gb] = bb[16 * u + k];
+                if (gb >= g0 && gb < g1) out[gb] = bb[8 * u + k];
             }
         }
     }
 }
 
 // Bytes [0, size) of the LDS image `img` (a block packed at lead 0) to
-// out[off, off + size): the 16-byte aligned middle with 16-byte stores, each
-// built from five image words funnel-shifted by the same byte count; the
-// ragged head and tail bytes (fewer than 16 each, shared with neighbouring
-// blocks' segments) with byte stores, one thread each.  Reads image words up
-// to (size + 19) / 4; the image has that room for the blocks stored here.
+// out[off, off + size): the 8-byte aligned middle with lane-contiguous 8-byte
+// stores, each built from three image words funnel-shifted by the same byte
+// count; the ragged head and tail bytes (fewer than 8 each, shared with
+// neighbouring blocks' segments) with byte stores, one thread each.  Reads
+// image words up to (size + 11) / 4; the image has that room for the blocks
+// stored here.
 template <int NT>
 __device__ __forceinline__ void store_image_shifted(const uint32_t *img, uint64_t off, uint64_t size, int tid,
                                                     uint8_t *__restrict__ out) {
+    static_assert(NT >= 16, "head and tail bytes take 16 threads");
     const uint64_t end = off + size;
-    const uint64_t a0 = (off + 15) & ~15ull;  // first 16-byte boundary at or after off
+    const uint64_t a0 = (off + 7) & ~7ull;  // first 8-byte boundary at or after off
     const uint64_t hend = a0 < end ? a0 : end;
-    const uint64_t a1 = a0 < end ? end & ~15ull : a0;  // the middle is [a0, a1)
+    const uint64_t a1 = a0 < end ? end & ~7ull : a0;  // the middle is [a0, a1)
     auto byte_at = [&](uint64_t g) -> uint32_t {
         const uint32_t i = (uint32_t)(g - off);
         return (img[i >> 2] >> (8 * (i & 3))) & 0xFF;
     };
-    if (tid < 16) {
+    if (tid < 8) {
         const uint64_t g = off + tid;
         if (g < hend) out[g] = (uint8_t)byte_at(g);
-    } else if (tid < 32) {
-        const uint64_t g = a1 + (tid - 16);
+    } else if (tid < 16) {
+        const uint64_t g = a1 + (tid - 8);
         if (g >= a1 && g < end && a0 < end) out[g] = (uint8_t)byte_at(g);
     }
     const uint32_t i0 = (uint32_t)(a0 - off);  // image byte of the middle's first byte
     const uint32_t r = i0 & 3;
-    const uint32_t units = (uint32_t)((a1 - a0) / 16);
+    const uint32_t units = (uint32_t)((a1 - a0) / 8);
     for (uint32_t u = tid; u < units; u += NT) {
-        const uint32_t q = (i0 + 16 * u) >> 2;
-        uint32_t w[5];
-#pragma unroll
-        for (int j = 0; j < 5; ++j) w[j] = img[q + j];
-        uint4 o;
-        o.x = __builtin_amdgcn_alignbyte(w[1], w[0], r);
-        o.y = __builtin_amdgcn_alignbyte(w[2], w[1], r);
-        o.z = __builtin_amdgcn_alignbyte(w[3], w[2], r);
-        o.w = __builtin_amdgcn_alignbyte(w[4], w[3], r);
-        *(uint4 *)(out + a0 + 16ull * u) = o;
+        const uint32_t q = (i0 + 8 * u) >> 2;
+        const uint32_t w0 = img[q], w1 = img[q + 1], w2 = img[q + 2];
+        *(img_u32x2 *)(out + a0 + 8ull * u) =
+            img_u32x2{__builtin_amdgcn_alignbyte(w1, w0, r), __builtin_amdgcn_alignbyte(w2, w1, r)};
     }
 }
 

@@ -1519,15 +1519,18 @@ hipError_t launch_encode(const T *in, uint64_t n, uint32_t bs, uint8_t *stream, 
     PlainRows rows{(const uint8_t *)in, n, bs, W, nb};
     PlainFit fit{(const uint8_t *)in, n, bs, W, 0};
     k_fit<PlainFit><<<dim3((unsigned)((nb + kFitRows - 1) / kFitRows)), dim3(kFitThreads), 0, st>>>(fit, prm, offs);
-    if (W == 64 && bs == kStreamBlock && ((uintptr_t)in & (2 * sizeof(T) - 1)) == 0) {
-        const unsigned wgs = (unsigned)std::min<uint64_t>(nb, stream_workgroups<T, W>());
-        k_encode_stream<T, W><<<dim3(wgs), dim3(kThreads), 0, st>>>(rows, prm, offs, stream, stream_counters(nb));
-        k_encode_deferred<T, W><<<dim3((unsigned)std::min<uint64_t>((nb + kThreads - 1) / kThreads, wgs)),
-                                  dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
-        k_encode_fixup_status<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(prm, offs,
-                                                                                                           nb, W);
-        return hipGetLastError();
-    } else if (bs > kLargeEncodeBlock)  // one block per unit
+    if constexpr (W == 64) {  // the stream encoder exists for uint64 only
+        if (bs == kStreamBlock && ((uintptr_t)in & (2 * sizeof(T) - 1)) == 0) {
+            const unsigned wgs = (unsigned)std::min<uint64_t>(nb, stream_workgroups<T, W>());
+            k_encode_stream<T, W><<<dim3(wgs), dim3(kThreads), 0, st>>>(rows, prm, offs, stream, stream_counters(nb));
+            k_encode_deferred<T, W><<<dim3((unsigned)std::min<uint64_t>((nb + kThreads - 1) / kThreads, wgs)),
+                                      dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
+            k_encode_fixup_status<<<dim3((unsigned)((nb + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(
+                prm, offs, nb, W);
+            return hipGetLastError();
+        }
+    }
+    if (bs > kLargeEncodeBlock)  // one block per unit
         k_encode_large<T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);
     else if (bs <= kWaveBlock)
         k_encode_waves<T, W><<<dim3((unsigned)encode_units(nb, bs)), dim3(kThreads), 0, st>>>(rows, prm, offs, stream);

@@ -184,10 +184,13 @@ __global__ __launch_bounds__(kThreads) void k_opmax_encode(const uint32_t *__res
         const bool raw = l >= 31;  // write_delta_default: the values themselves, 4 bytes each
         const int lf = raw ? 32 : l;
         if (lane == 0) {
-            prm[b].theta0 = t0;
-            prm[b].theta1 = t1;
-            prm[b].bits = (uint32_t)l;
-            prm[b].payload_offset = off + kOpmaxHdr;
+            // field by field (relaxed atomic stores are never merged): one
+            // 16-byte store of these LDS-loaded registers could read its upper
+            // half only after the next block's LDS loads refilled them
+            __hip_atomic_store((uint64_t *)&prm[b].theta0, double_to_bits(t0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store((uint64_t *)&prm[b].theta1, double_to_bits(t1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&prm[b].bits, (uint32_t)l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&prm[b].payload_offset, off + kOpmaxHdr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         PackedBlock pb{off, off + opmax_block_bytes((uint32_t)l, m), double_to_bits(t0), (uint64_t)s_t1b[k], m, lf,
                        kOpmaxHdr,
@@ -454,7 +457,9 @@ __global__ __launch_bounds__(kThreads) void k_opmax_decode(const uint8_t *__rest
 
     // The workgroup's runs cover consecutive positions [p0, p1): transpose the
     // values through LDS (free again once every thread has read its fields)
-    // and write the range with lane-contiguous 16-byte streaming stores.
+    // and write the range with lane-contiguous 8-byte streaming stores (a
+    // 16-byte store can read its upper half from the VGPRs only after the
+    // next LDS load has refilled them).
     const uint64_t p0 = b0 * bs + (uint64_t)rr0 * kOpmaxRun;
     const uint64_t gl = (g0 + kThreads < runs ? g0 + kThreads : runs) - 1;
     const uint32_t rl = rr0 + (uint32_t)(gl - g0);
@@ -470,17 +475,15 @@ __global__ __launch_bounds__(kThreads) void k_opmax_decode(const uint8_t *__rest
     for (uint32_t r = 0; r < len; ++r) lds[at((uint32_t)(i0 - p0) + r)] = o[r];
     __syncthreads();
     uint32_t *dst = out + p0;
-    uint32_t head = (uint32_t)((4 - (((uintptr_t)dst >> 2) & 3)) & 3);  // values before a 16-byte boundary
-    head = head < cnt ? head : cnt;
-    if ((uint32_t)tid < head) dst[tid] = lds[at(tid)];
-    const uint32_t nvec = (cnt - head) / 4;
-    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const uint32_t head = ((uintptr_t)dst & 4) && cnt ? 1u : 0u;  // a value before an 8-byte boundary
+    if ((uint32_t)tid < head) dst[0] = lds[at(0)];
+    const uint32_t nvec = (cnt - head) / 2;
+    typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
     for (uint32_t v = tid; v < nvec; v += kThreads) {
-        const uint32_t q = head + 4 * v;
-        __builtin_nontemporal_store(u32x4{lds[at(q)], lds[at(q + 1)], lds[at(q + 2)], lds[at(q + 3)]},
-                                    (u32x4 *)(dst + q));
+        const uint32_t q = head + 2 * v;
+        __builtin_nontemporal_store(u32x2{lds[at(q)], lds[at(q + 1)]}, (u32x2 *)(dst + q));
     }
-    for (uint32_t q = head + 4 * nvec + tid; q < cnt; q += kThreads) dst[q] = lds[at(q)];
+    for (uint32_t q = head + 2 * nvec + tid; q < cnt; q += kThreads) dst[q] = lds[at(q)];
 }
 
 // The reference's fix-up lists applied to decoded values, skipping raw blocks
