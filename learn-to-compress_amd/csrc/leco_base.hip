@@ -3,13 +3,15 @@
 // the reference), on the same column layout as the LeCo codec: the blocks'
 // byte strings back to back, block b at offs[b].
 //
-// Encode: one workgroup per block in ticket order -- block statistics (FOR:
-// min and max; Delta: the largest |y[i+1] - y[i]|), bit width and size, the
-// decoupled look-back for the block's offset, then the LDS-image packer shared
-// with the LeCo encoder.  Decode: one workgroup per block; Delta decodes by a
-// workgroup prefix sum of the signed deltas (mod 2^w, so any summation order
-// gives the reference's sequential result).  Lookups: one lane per query (FOR)
-// or one wave per query summing the deltas before it (Delta).
+// Encode: blocks in ticket order -- block statistics (FOR: min and max;
+// Delta: the largest |y[i+1] - y[i]|), bit width and size, the decoupled
+// look-back for the offset, then the LDS-image packer shared with the LeCo
+// encoder; one wave per block and a run of blocks per ticket up to 4096
+// values, one workgroup per block above.  Decode: one workgroup per block;
+// Delta decodes by a workgroup prefix sum of the signed deltas (mod 2^w, so
+// any summation order gives the reference's sequential result).  Lookups: one
+// lane per query (FOR) or one wave per query summing the deltas before it
+// (Delta).
 #include "leco_kernels.h"
 
 namespace leco {
@@ -108,13 +110,166 @@ __global__ __launch_bounds__(kThreads) void k_base_encode(const T *__restrict__ 
     }
     auto sync = [] { __syncthreads(); };
     for (uint32_t c = 0; c < pb.nchunks; ++c)
-        pack_chunk<kThreads, kChunk, false>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) -> uint64_t {
+        pack_chunk<kThreads, kChunk>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) -> uint64_t {
             const uint64_t v = (uint64_t)y[i];
             if (raw) return v;
             if (CODEC == kCodecFor) return v - lo;
             // n fields: the last one, past the reference's vector of n-1 deltas, is zero
             return i + 1 < m ? delta_field<W>(v, (uint64_t)y[i + 1], l) : 0;
         });
+}
+
+// Blocks of at most kBaseUnitBlock values are encoded one per wave, a
+// workgroup taking base_unit_blocks() consecutive blocks per ticket (about
+// 4 * kBaseUnitBlock values, at most kBaseUnitMaxBlocks blocks): one ticket
+// per block would cap the encoder at the rate of same-address atomics (about
+// one per 12 ns on MI355X, 170 GB/s of uint64 input at bs 256).
+constexpr uint32_t kBaseUnitBlock = 4096;
+constexpr uint32_t kBaseUnitMaxBlocks = 256;
+__host__ __device__ inline uint32_t base_unit_blocks(uint32_t bs) {
+    const uint32_t per = 4 * (kBaseUnitBlock / bs);
+    return per < kBaseUnitMaxBlocks ? per : kBaseUnitMaxBlocks;
+}
+
+struct BaseUnitShared {
+    alignas(16) uint32_t buf[kLdsBytes / 4];  // one LDS image of kWaveLdsBytes per wave
+    uint64_t at[kBaseUnitMaxBlocks];          // block size, then its stream offset
+    uint64_t lo[kBaseUnitMaxBlocks];          // FOR: the block's minimum; Delta: its first value
+    uint64_t hi[kBaseUnitMaxBlocks];          // FOR: the block's maximum
+    int l[kBaseUnitMaxBlocks];
+};
+
+// The same blocks as k_base_encode, `per` consecutive blocks per workgroup
+// (bs <= kBaseUnitBlock), one wave per block.  Phase A: each wave's blocks'
+// statistics, bit widths and sizes.  Then one look-back for the unit over the
+// units' status words offs[(u+1)*per] (StridedSlot); the unit's inner block
+// ends go straight to offs[b+1].  Phase B: each wave packs its blocks,
+// kWaveChunk values at a time through its LDS image.
+template <int CODEC, typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_base_encode_units(const T *__restrict__ y_all, uint64_t n, uint32_t bs,
+                                                                uint32_t per, uint64_t *__restrict__ offs,
+                                                                uint8_t *__restrict__ out) {
+    __shared__ BaseUnitShared sh;
+    __shared__ uint64_t s_u;
+    constexpr uint32_t kWaves = kThreads / kWave;
+    constexpr uint32_t tb = W / 8;
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    if (tid == 0) s_u = take_ticket(offs);  // offs[0] is the ticket
+    __syncthreads();
+    const uint64_t u = s_u;
+    const uint64_t nb = (n + bs - 1) / bs;
+    const uint64_t b0 = u * per;
+    if (b0 >= nb) return;
+    const uint32_t count = (uint32_t)(nb - b0 < per ? nb - b0 : per);
+
+    // phase A
+    for (uint32_t k = wid; k < count; k += kWaves) {
+        const uint64_t s = (b0 + k) * bs;
+        const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
+        const T *y = y_all + s;
+        uint64_t lo = ~0ull, hi = 0;
+        for (uint32_t j0 = 0; j0 < m; j0 += kWaveChunk) {
+            uint64_t v[kWaveVPL], w[kWaveVPL];
+#pragma unroll
+            for (int q = 0; q < kWaveVPL; ++q) {
+                const uint32_t j = j0 + lane + q * kWave;
+                v[q] = j < m ? (uint64_t)y[j] : 0;
+                if (CODEC == kCodecDelta) w[q] = j + 1 < m ? (uint64_t)y[j + 1] : 0;
+            }
+#pragma unroll
+            for (int q = 0; q < kWaveVPL; ++q) {
+                const uint32_t j = j0 + lane + q * kWave;
+                if (CODEC == kCodecFor) {
+                    lo = j < m && v[q] < lo ? v[q] : lo;
+                    hi = j < m && v[q] > hi ? v[q] : hi;
+                } else {
+                    const uint64_t d = (w[q] > v[q] ? w[q] - v[q] : v[q] - w[q]) & wmask<W>();
+                    hi = j + 1 < m && d > hi ? d : hi;
+                }
+            }
+        }
+        hi = wave_max_u64(hi);
+        if (CODEC == kCodecFor) lo = wave_min_u64(lo);
+        int l;
+        if (CODEC == kCodecFor) l = hi - lo ? bitlen64(hi - lo) : 0;
+        else l = hi ? bitlen64(hi) + 1 : 0;
+        if (l > W) l = W;
+        if (lane == 0) {
+            sh.at[k] = CODEC == kCodecFor ? for_block_bytes((uint32_t)l, W, m) : delta_block_bytes((uint32_t)l, W, m);
+            sh.lo[k] = CODEC == kCodecFor ? lo : (uint64_t)y[0];
+            sh.hi[k] = hi;
+            sh.l[k] = l;
+        }
+    }
+    __syncthreads();
+
+    // the unit's offset (four blocks per lane), its blocks' offsets and ends
+    if (wid == 0) {
+        uint64_t sz[4], run = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t k = 4 * lane + r;
+            sz[r] = k < count ? sh.at[k] : 0;
+            run += sz[r];
+        }
+        const uint64_t inc = wave_inclusive_sum(run);
+        const uint64_t excl = look_back(u, wave_last(inc), offs, StridedSlot{per, nb});
+        uint64_t o = excl + inc - run;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const uint32_t k = 4 * lane + r;
+            if (k < count) {
+                sh.at[k] = o;
+                o += sz[r];
+                if (k + 1 < count) offs[b0 + k + 1] = o;  // the last block's end is the unit's status word
+            }
+        }
+    }
+    __syncthreads();
+
+    // phase B
+    uint32_t *buf = sh.buf + wid * (kWaveLdsBytes / 4);
+    for (uint32_t k = wid; k < count; k += kWaves) {
+        const uint64_t s = (b0 + k) * bs;
+        const uint32_t m = (uint32_t)(n - s < bs ? n - s : bs);
+        const T *y = y_all + s;
+        const int l = __builtin_amdgcn_readfirstlane(sh.l[k]);
+        const uint64_t lo = sh.lo[k], hi = sh.hi[k], off = sh.at[k];
+        const bool raw = l == W;
+        PackedBlock pb;
+        pb.off = off;
+        pb.end = off + (CODEC == kCodecFor ? for_block_bytes((uint32_t)l, W, m) : delta_block_bytes((uint32_t)l, W, m));
+        pb.m = m;
+        pb.l = l;
+        pb.nchunks = l == 0 ? 1 : (m + kWaveChunk - 1) / kWaveChunk;
+        if (raw) {
+            pb.hdr = 1;
+            pb.t0b = pb.t1b = 0;
+        } else if (CODEC == kCodecFor) {
+            pb.hdr = 1 + 2 * tb;
+            pb.t0b = W == 64 ? lo : (lo | (hi << 32));
+            pb.t1b = W == 64 ? hi : 0;
+        } else {
+            pb.hdr = 1 + tb;
+            pb.t0b = lo;
+            pb.t1b = 0;
+        }
+        auto sync = [] { wave_lds_sync(); };
+        for (uint32_t c = 0; c < pb.nchunks; ++c) {
+            uint64_t f[kWaveVPL];
+#pragma unroll
+            for (int q = 0; q < kWaveVPL; ++q) {
+                const uint32_t i = c * kWaveChunk + lane + q * kWave;
+                const uint64_t v = i < m ? (uint64_t)y[i] : 0;
+                if (raw || l == 0) f[q] = v;
+                else if (CODEC == kCodecFor) f[q] = v - lo;
+                // n fields: the last one, past the reference's vector of n-1 deltas, is zero
+                else f[q] = i + 1 < m ? delta_field<W>(v, (uint64_t)y[i + 1], l) : 0;
+            }
+            pack_chunk<kWave, kWaveChunk>(pb, c, lane, buf, out, sync,
+                                                [&](int q, uint32_t) -> uint64_t { return f[q]; });
+        }
+    }
 }
 
 __global__ __launch_bounds__(kThreads) void k_base_clear(uint64_t *__restrict__ offs, uint64_t words) {
@@ -505,7 +660,15 @@ hipError_t launch_base_encode(int codec, const T *in, uint64_t n, uint32_t bs, u
     constexpr int W = 8 * sizeof(T);
     const uint64_t nb = (n + bs - 1) / bs;
     k_base_clear<<<dim3((unsigned)((nb + 1 + kThreads - 1) / kThreads)), dim3(kThreads), 0, st>>>(offs, nb + 1);
-    if (codec == kCodecFor)
+    if (bs <= kBaseUnitBlock) {
+        const uint32_t per = base_unit_blocks(bs);
+        const unsigned units = (unsigned)((nb + per - 1) / per);
+        if (codec == kCodecFor)
+            k_base_encode_units<kCodecFor, T, W><<<dim3(units), dim3(kThreads), 0, st>>>(in, n, bs, per, offs, stream);
+        else
+            k_base_encode_units<kCodecDelta, T, W><<<dim3(units), dim3(kThreads), 0, st>>>(in, n, bs, per, offs,
+                                                                                           stream);
+    } else if (codec == kCodecFor)
         k_base_encode<kCodecFor, T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(in, n, bs, offs, stream);
     else
         k_base_encode<kCodecDelta, T, W><<<dim3((unsigned)nb), dim3(kThreads), 0, st>>>(in, n, bs, offs, stream);

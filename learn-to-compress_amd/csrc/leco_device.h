@@ -123,14 +123,6 @@ __device__ __forceinline__ uint64_t double_to_bits(double d) { return (uint64_t)
 
 // ---- wave / workgroup reductions (wave64) ----------------------------------
 
-__device__ __forceinline__ int64_t wave_max_i64(int64_t v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) { int64_t u = __shfl_xor((long long)v, o, kWave); v = u > v ? u : v; }
-    return v;
-}
-__device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) { int64_t u = __shfl_xor((long long)v, o, kWave); v = u < v ? u : v; }
-    return v;
-}
 // A value every lane holds alike (an LDS broadcast, say), marked uniform for
 // the compiler so that branches on it stay scalar.
 __device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
@@ -229,12 +221,30 @@ __device__ __forceinline__ T wave_last(T v) {
                    ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), 63) << 32));
 }
 
-__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        uint64_t u = (uint64_t)__shfl_xor((long long)v, o, kWave);
-        v = u > v ? u : v;
+// Reduction over the wave (all 64 lanes active), returned to every lane: row
+// results by DPP (VALU, no LDS queue), then the four rows' results read from
+// lanes 0, 16, 32 and 48 (scalar).
+template <class Op>
+__device__ __forceinline__ uint64_t wave_reduce_u64(uint64_t v, Op op) {
+    v = row_reduce_u64(v, op);
+    uint64_t r = 0;
+#pragma unroll
+    for (int row = 0; row < 4; ++row) {
+        const uint32_t lo = __builtin_amdgcn_readlane((int)(uint32_t)v, 16 * row);
+        const uint32_t hi = __builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), 16 * row);
+        const uint64_t x = (uint64_t)lo | ((uint64_t)hi << 32);
+        r = row ? op(r, x) : x;
     }
-    return v;
+    return r;
+}
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t v) {
+    return wave_reduce_u64(v, [](uint64_t a, uint64_t b) { return a > b ? a : b; });
+}
+__device__ __forceinline__ int64_t wave_max_i64(int64_t v) {
+    return (int64_t)wave_reduce_u64((uint64_t)v, [](uint64_t a, uint64_t b) { return (int64_t)a > (int64_t)b ? a : b; });
+}
+__device__ __forceinline__ int64_t wave_min_i64(int64_t v) {
+    return (int64_t)wave_reduce_u64((uint64_t)v, [](uint64_t a, uint64_t b) { return (int64_t)a < (int64_t)b ? a : b; });
 }
 
 }  // namespace leco

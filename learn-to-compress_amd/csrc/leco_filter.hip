@@ -996,6 +996,9 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt_mark(FilterArgs a) {
                                                   : (sft ? (uint32_t)(bal >> (64 - sft)) : 0u);
                 if (part) atomicOr(&sh.bits[(q >> 5) + lane], part);
             }
+            // the ORs done before any later load (a queued LDS atomic can read its
+            // registers after a later load has refilled them)
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         }
     }
     __syncthreads();

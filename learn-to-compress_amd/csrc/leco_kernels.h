@@ -97,6 +97,22 @@ struct EncodeShared {
     int ul[kUnitMaxBlocks];
 };
 
+// Where unit p's look-back status word lives.  NextSlot: offs[p+1], one word
+// per unit.  StridedSlot: offs[(p+1)*stride], the last unit's at offs[last]
+// -- the units are runs of `stride` blocks, and the slots in between hold
+// the end offsets of the blocks inside a unit, written once its prefix is
+// known and never read as statuses.
+struct NextSlot {
+    __device__ uint64_t operator()(int64_t p) const { return (uint64_t)(p + 1); }
+};
+struct StridedSlot {
+    uint64_t stride, last;
+    __device__ uint64_t operator()(int64_t p) const {
+        const uint64_t s = (uint64_t)(p + 1) * stride;
+        return s < last ? s : last;
+    }
+};
+
 // One wave's part of a look-back window: PER chunks of 64 predecessors'
 // status words (lane l of chunk k holds unit wtop - 64k - l; units below 0
 // read as an inclusive prefix of 0), nearest first.  The part that counts
@@ -107,10 +123,10 @@ struct EncodeShared {
 // after each round.  Returns the sum of the counted words' values (every
 // lane); `found`: an inclusive prefix ends the part; `gave_up`: the wait
 // bound was reached with words of the part still not ready (they count 0).
-template <int PER>
+template <int PER, class Slot = NextSlot>
 __device__ __forceinline__ uint64_t resolve_window(uint64_t (&stw)[PER], int64_t wtop,
                                                    const uint64_t *__restrict__ offs, uint32_t spin_limit,
-                                                   bool &found, bool &gave_up) {
+                                                   bool &found, bool &gave_up, Slot slot = Slot()) {
     const int lane = threadIdx.x % kWave;
     int ks = PER;               // chunk of the nearest inclusive prefix
     uint64_t stop_mask = ~0ull; // lanes of chunk ks up to and including it
@@ -139,7 +155,7 @@ __device__ __forceinline__ uint64_t resolve_window(uint64_t (&stw)[PER], int64_t
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int64_t p = wtop - (k * kWave + lane);
-            if (k <= ks && (stw[k] & kStatusMask) == 0) stw[k] = load_status(&offs[p + 1]);
+            if (k <= ks && (stw[k] & kStatusMask) == 0) stw[k] = load_status(&offs[slot(p)]);
         }
     }
     found = ks < PER;
@@ -165,17 +181,18 @@ __device__ __forceinline__ uint64_t resolve_window(uint64_t (&stw)[PER], int64_t
 // keeps a broken invariant from hanging the GPU.  A wait that reaches the
 // bound counts the predecessor as empty and sets kTicketError in offs[0],
 // which poisons the launch's offsets.
-template <int PER = kLookPer>
-__device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_t *__restrict__ offs) {
+template <int PER = kLookPer, class Slot = NextSlot>
+__device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_t *__restrict__ offs,
+                                              Slot slot = Slot()) {
     constexpr int kWin = kWave * PER;
     const int lane = threadIdx.x % kWave;
     const LookbackHook hook = g_lookback_hook;
     const bool publish = b != hook.withhold;
     if (b == 0) {
-        if (lane == 0 && publish) store_status(&offs[1], kStatusInclusive | size);
+        if (lane == 0 && publish) store_status(&offs[slot(0)], kStatusInclusive | size);
         return 0;
     }
-    if (lane == 0 && publish) store_status(&offs[b + 1], kStatusAggregate | size);
+    if (lane == 0 && publish) store_status(&offs[slot((int64_t)b)], kStatusAggregate | size);
     uint64_t excl = 0;
     bool gave_up = false;
     for (int64_t top = (int64_t)b - 1;; top -= kWin) {
@@ -183,14 +200,14 @@ __device__ __forceinline__ uint64_t look_back(uint64_t b, uint64_t size, uint64_
 #pragma unroll
         for (int k = 0; k < PER; ++k) {
             const int64_t p = top - (k * kWave + lane);
-            stw[k] = p >= 0 ? load_status(&offs[p + 1]) : kStatusInclusive;
+            stw[k] = p >= 0 ? load_status(&offs[slot(p)]) : kStatusInclusive;
         }
         bool found, gu;
-        excl += resolve_window<PER>(stw, top, offs, hook.spin_limit, found, gu);
+        excl += resolve_window<PER>(stw, top, offs, hook.spin_limit, found, gu, slot);
         gave_up = gave_up || gu;
         if (found) break;
     }
-    if (lane == 0 && publish) store_status(&offs[b + 1], kStatusInclusive | (excl + size));
+    if (lane == 0 && publish) store_status(&offs[slot((int64_t)b)], kStatusInclusive | (excl + size));
     if (gave_up && lane == 0) atomicOr((unsigned long long *)offs, (unsigned long long)kTicketError);
     return excl;
 }
@@ -284,6 +301,20 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Parameter-table fields, written one by one with relaxed atomic stores: the
+// compiler never merges those into one 12- or 16-byte store, and a wide store
+// can read its upper registers after a later LDS load has refilled them.
+__device__ __forceinline__ void put_param_u64(uint64_t *p, uint64_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void put_param(leco_fix_block_param *p, double t0, bool zero_t1, uint32_t bits,
+                                          bool with_offset, uint64_t payload_offset) {
+    put_param_u64((uint64_t *)&p->theta0, double_to_bits(t0));
+    if (zero_t1) put_param_u64((uint64_t *)&p->theta1, 0);
+    __hip_atomic_store(&p->bits, bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (with_offset) put_param_u64(&p->payload_offset, payload_offset);
+}
+
 // Where a block's bytes go and what its header holds.
 struct PackedBlock {
     uint64_t off, end;     // byte range of the block in the stream
@@ -360,13 +391,19 @@ __device__ __forceinline__ void store_image_shifted(const uint32_t *img, uint64_
 
 // Chunk c (CH values) of a block, packed by NT threads (thread `tid`) through
 // the LDS image `buf`; sync() orders the group's LDS accesses.  value(kk, i)
-// is the field of block value i = chunk position tid + kk*NT.  CACHED: kk is
-// a compile-time index (fully unrolled) so that register-cached values stay
-// in registers; otherwise value() recomputes the field and the loop is kept
-// rolled, which keeps register pressure down.
-template <int NT, uint32_t CH, bool CACHED, class Sync, class Value>
+// is the field of block value i = chunk position tid + kk*NT (kk a
+// compile-time index, so register-cached values stay in registers).
+//
+// Every field is computed (any global loads in value() included) before the
+// chunk's first LDS write: an LDS instruction still queued behind others
+// (the image ORs are atomics that queue deeply under contention) can read
+// its address and data registers after a later global load has refilled
+// them, and then ORs the wrong bits at the wrong place.  No load is issued
+// between the chunk's first LDS write and the wait for all of them at the end.
+template <int NT, uint32_t CH, class Sync, class Value>
 __device__ __forceinline__ void pack_chunk(const PackedBlock &pb, uint32_t c, int tid, uint32_t *buf,
                                            uint8_t *__restrict__ out, Sync sync, Value value) {
+    constexpr int K = (int)(CH / NT);
     const int l = pb.l;
     const uint64_t cbytes = (uint64_t)(CH / 8) * l;
     const uint64_t g0 = c == 0 ? pb.off : pb.off + pb.hdr + c * cbytes;
@@ -374,6 +411,14 @@ __device__ __forceinline__ void pack_chunk(const PackedBlock &pb, uint32_t c, in
     const uint64_t a0 = g0 & ~15ull;
     const uint32_t span = (uint32_t)(((g1 + 15) & ~15ull) - a0);
     const uint32_t lead = (uint32_t)(g0 - a0);
+    const uint32_t j0 = c * CH;
+    const uint32_t jn = pb.m - j0 < CH ? pb.m - j0 : CH;
+    uint64_t field[K];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+        const uint32_t j = tid + kk * NT;  // position inside the chunk
+        field[kk] = l > 0 && j < jn ? value(kk, j0 + j) : 0;
+    }
     for (uint32_t w = tid; w < span / 4; w += NT) buf[w] = 0;
     sync();
     if (c == 0 && (uint32_t)tid < pb.hdr) {
@@ -386,34 +431,28 @@ __device__ __forceinline__ void pack_chunk(const PackedBlock &pb, uint32_t c, in
     }
     if (l > 0) {
         const uint64_t pbit = 8ull * (lead + (c == 0 ? pb.hdr : 0));
-        const uint32_t j0 = c * CH;
-        const uint32_t jn = pb.m - j0 < CH ? pb.m - j0 : CH;
-        auto put = [&](int kk) {
-            const uint32_t j = tid + kk * NT;  // position inside the chunk
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            const uint32_t j = tid + kk * NT;
             if (j < jn) {
-                const uint64_t field = value(kk, j0 + j);
                 const uint64_t q = pbit + (uint64_t)j * l;
                 const uint32_t dw = (uint32_t)(q >> 5);
                 const int shb = (int)(q & 31);
-                const uint32_t d0 = (uint32_t)(field << shb);
-                const uint32_t d1 = (uint32_t)(shb ? field >> (32 - shb) : field >> 32);
-                const uint32_t d2 = shb ? (uint32_t)(field >> (64 - shb)) : 0u;
+                const uint32_t d0 = (uint32_t)(field[kk] << shb);
+                const uint32_t d1 = (uint32_t)(shb ? field[kk] >> (32 - shb) : field[kk] >> 32);
+                const uint32_t d2 = shb ? (uint32_t)(field[kk] >> (64 - shb)) : 0u;
                 if (d0) atomicOr(&buf[dw], d0);
                 if (shb + l > 32 && d1) atomicOr(&buf[dw + 1], d1);
                 if (shb + l > 64 && d2) atomicOr(&buf[dw + 2], d2);
             }
-        };
-        if constexpr (CACHED) {
-#pragma unroll
-            for (int kk = 0; kk < (int)(CH / NT); ++kk) put(kk);
-        } else {
-#pragma unroll 2
-            for (int kk = 0; kk < (int)(CH / NT); ++kk) put(kk);
         }
     }
     sync();
     store_image<NT>(buf, a0, g0, g1, span, tid, out);
     sync();
+    // every LDS operation of the chunk done before the caller's next loads
+    // (s_waitcnt lgkmcnt(0); a wave-scope sync does not wait)
+    __builtin_amdgcn_s_waitcnt(0xC07F);
 }
 
 // Small-block decode staging: the bytes of whole blocks, their offsets and

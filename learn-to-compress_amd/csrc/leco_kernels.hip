@@ -342,10 +342,7 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
             sh.end = excl + size;
             sh.l = lb;
             sh.t0 = lb == W ? 0.0 : t0;
-            prm[b].theta0 = sh.t0;
-            if (lb == W) prm[b].theta1 = 0.0;
-            prm[b].bits = (uint32_t)lb;
-            prm[b].payload_offset = excl + (lb == W ? 1 : 17);
+            put_param(&prm[b], sh.t0, lb == W, (uint32_t)lb, true, excl + (lb == W ? 1 : 17));
         }
     }
     __syncthreads();
@@ -366,13 +363,13 @@ __device__ __forceinline__ void encode_block_body(const T *__restrict__ y, uint3
         return residual_field<W>((uint64_t)y[i], predict(t0, t1, i), l, &mg2);
     };
     auto sync = [] { __syncthreads(); };
-    pack_chunk<NT, kChunk, true>(pb, 0, tid, sh.buf, out, sync,
+    pack_chunk<NT, kChunk>(pb, 0, tid, sh.buf, out, sync,
                                        [&](int kk, uint32_t i) { return cached(kk, i); });
     if (CC > 1 && pb.nchunks > 1)
-        pack_chunk<NT, kChunk, true>(pb, 1, tid, sh.buf, out, sync,
+        pack_chunk<NT, kChunk>(pb, 1, tid, sh.buf, out, sync,
                                      [&](int kk, uint32_t i) { return cached((CC > 1 ? kChunk / NT : 0) + kk, i); });
     for (uint32_t c = CC; c < pb.nchunks; ++c)
-        pack_chunk<NT, kChunk, false>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) { return fresh(i); });
+        pack_chunk<NT, kChunk>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) { return fresh(i); });
 }
 
 // Persistent, software-pipelined encoder for a plain column of 4096-value
@@ -603,15 +600,16 @@ __global__ __launch_bounds__(kThreads, 4) void k_encode_stream(PlainRows rows, l
         publish_size(cu, size, offs);
         packs = near && !raw && l <= 32;
         if (tid == 0) {
-            prm[cb].theta0 = tr;
-            if (raw) prm[cb].theta1 = 0.0;
-            prm[cb].bits = (uint32_t)l | (packs ? 0u : kDeferred);
+            put_param(&prm[cb], tr, raw, (uint32_t)l | (packs ? 0u : kDeferred), false, 0);
         }
         uint32_t fld[VPT];
         if (packs && l > 0) near_fields<VPT>(v, signs, l, fld);
         cur = nxt;
         have = fetch(cur);  // the next block's loads fly during the pack and the drain
         if (packs) near_pack<kThreads, VPT>(tl, fld, l, tr, t1c, 0, image(ci));
+        // the image ORs done before the drain's loads: a queued LDS atomic can
+        // read its registers after a later load has refilled them
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
         if (pend) drain();
         if (tid == 0) ss.next_ticket2 = tk;
         __syncthreads();
@@ -667,7 +665,7 @@ __global__ __launch_bounds__(kThreads) void k_encode_deferred(PlainRows rows, le
                                      l == 0 ? 1 : (m + kChunk - 1) / kChunk};
                 auto sync = [] { __syncthreads(); };
                 for (uint32_t c = 0; c < pb.nchunks; ++c)
-                    pack_chunk<kThreads, kChunk, false>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) -> uint64_t {
+                    pack_chunk<kThreads, kChunk>(pb, c, tid, sh.buf, out, sync, [&](int, uint32_t i) -> uint64_t {
                         if (raw) return (uint64_t)y[i];
                         uint64_t mg2;
                         return residual_field<W>((uint64_t)y[i], predict(t0, t1, i), l, &mg2);
@@ -756,12 +754,8 @@ __device__ __forceinline__ void encode_unit_waves(const EncodeUnit &un, uint64_t
     // unit offset: exclusive scan of the sizes (one per lane) + look-back
     if (wid == 0) {
         const uint64_t mine = (uint32_t)lane < un.count ? sh.usz[lane] : 0;
-        uint64_t inc = mine;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint64_t t = (uint64_t)__shfl_up((long long)inc, o, kWave);
-            if (lane >= o) inc += t;
-        }
-        const uint64_t total = (uint64_t)__shfl((long long)inc, kWave - 1, kWave);
+        const uint64_t inc = wave_inclusive_sum(mine);
+        const uint64_t total = wave_last(inc);
         const uint64_t excl = look_back(u, total, offs);
         if ((uint32_t)lane < un.count) sh.usz[lane] = excl + inc - mine;  // now: the block's offset
     }
@@ -780,10 +774,7 @@ __device__ __forceinline__ void encode_unit_waves(const EncodeUnit &un, uint64_t
         const bool raw = (l == W);
         const uint32_t hdr = raw ? 1u : 17u;
         if (lane == 0) {
-            prm[b].theta0 = t0;
-            if (raw) prm[b].theta1 = 0.0;
-            prm[b].bits = (uint32_t)l;
-            prm[b].payload_offset = off + hdr;
+            put_param(&prm[b], t0, raw, (uint32_t)l, true, off + hdr);
         }
         const PackedBlock pb{off, off + block_bytes((uint32_t)l, W, m), double_to_bits(t0), double_to_bits(t1), m, l,
                              hdr, l == 0 ? 1 : (m + kWaveChunk - 1) / kWaveChunk};
@@ -804,7 +795,7 @@ __device__ __forceinline__ void encode_unit_waves(const EncodeUnit &un, uint64_t
                     f[q] = residual_field<W>(f[q], predict(t0, t1, i), l, &mg2);
                 }
             }
-            pack_chunk<kWave, kWaveChunk, true>(pb, c, lane, buf, out, sync,
+            pack_chunk<kWave, kWaveChunk>(pb, c, lane, buf, out, sync,
                                                 [&](int q, uint32_t) -> uint64_t { return f[q]; });
         }
     }

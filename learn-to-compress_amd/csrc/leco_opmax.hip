@@ -58,27 +58,11 @@ __device__ __forceinline__ uint64_t opmax_field(int32_t d, int l) {
     return (uint64_t)(mag & ((1u << (l - 1)) - 1)) | ((uint64_t)(d > 0) << (l - 1));
 }
 
-__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const int32_t u = __shfl_xor(v, o, kWave);
-        v = u > v ? u : v;
-    }
-    return v;
-}
-__device__ __forceinline__ int32_t wave_min_i32(int32_t v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const int32_t u = __shfl_xor(v, o, kWave);
-        v = u < v ? u : v;
-    }
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
-    for (int o = kWave / 2; o > 0; o >>= 1) {
-        const uint32_t u = (uint32_t)__shfl_xor((int)v, o, kWave);
-        v = u > v ? u : v;
-    }
-    return v;
-}
+// 32-bit wave reductions on the 64-bit DPP reduction (sign-extended for the
+// signed ones): VALU and scalar reads, no LDS queue.
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) { return (int32_t)wave_max_i64((int64_t)v); }
+__device__ __forceinline__ int32_t wave_min_i32(int32_t v) { return (int32_t)wave_min_i64((int64_t)v); }
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) { return (uint32_t)wave_max_u64((uint64_t)v); }
 
 // ---------------------------------------------------------------------------
 // Encode: encodeArray8 (:45-131) per block, one wave per block.
@@ -161,12 +145,8 @@ __global__ __launch_bounds__(kThreads) void k_opmax_encode(const uint32_t *__res
     // unit offset: exclusive scan of the sizes (one per lane) + look-back
     if (wid == 0) {
         const uint64_t mine = (uint32_t)lane < count ? sh.usz[lane] : 0;
-        uint64_t inc = mine;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint64_t t = (uint64_t)__shfl_up((long long)inc, o, kWave);
-            if (lane >= o) inc += t;
-        }
-        const uint64_t total = (uint64_t)__shfl((long long)inc, kWave - 1, kWave);
+        const uint64_t inc = wave_inclusive_sum(mine);
+        const uint64_t total = wave_last(inc);
         const uint64_t excl = look_back(u, total, offs);
         if ((uint32_t)lane < count) sh.usz[lane] = excl + inc - mine;  // now: the block's offset
     }
@@ -184,13 +164,8 @@ __global__ __launch_bounds__(kThreads) void k_opmax_encode(const uint32_t *__res
         const bool raw = l >= 31;  // write_delta_default: the values themselves, 4 bytes each
         const int lf = raw ? 32 : l;
         if (lane == 0) {
-            // field by field (relaxed atomic stores are never merged): one
-            // 16-byte store of these LDS-loaded registers could read its upper
-            // half only after the next block's LDS loads refilled them
-            __hip_atomic_store((uint64_t *)&prm[b].theta0, double_to_bits(t0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store((uint64_t *)&prm[b].theta1, double_to_bits(t1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&prm[b].bits, (uint32_t)l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&prm[b].payload_offset, off + kOpmaxHdr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            put_param(&prm[b], t0, false, (uint32_t)l, true, off + kOpmaxHdr);
+            put_param_u64((uint64_t *)&prm[b].theta1, double_to_bits(t1));
         }
         PackedBlock pb{off, off + opmax_block_bytes((uint32_t)l, m), double_to_bits(t0), (uint64_t)s_t1b[k], m, lf,
                        kOpmaxHdr,
@@ -205,7 +180,7 @@ __global__ __launch_bounds__(kThreads) void k_opmax_encode(const uint32_t *__res
                 const uint32_t v = i < m ? y[i] : 0u;
                 f[q] = (raw || lf == 0) ? (uint64_t)v : opmax_field(opmax_resid(v, opmax_pm(t0, t1, i)), lf);
             }
-            pack_chunk<kWave, kWaveChunk, true>(pb, c, lane, buf, out, sync,
+            pack_chunk<kWave, kWaveChunk>(pb, c, lane, buf, out, sync,
                                                 [&](int q, uint32_t) -> uint64_t { return f[q]; });
         }
         if (fix) {

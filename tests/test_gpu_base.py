@@ -87,6 +87,24 @@ def test_base_columns_match_oracle(oracle_mod, codec, bs):
                 assert np.array_equal(host(col.filter_range_close(f1, f2, base), np.uint32), want), (width, base)
 
 
+@pytest.mark.parametrize("codec", ["for", "delta"])
+@pytest.mark.parametrize("width,bs,n", [(64, 100, 1_000_003), (32, 256, 2_000_000), (64, 1000, 1_500_001),
+                                        (32, 4096, 3_000_017), (64, 7, 300_001), (32, 2500, 999_999)])
+def test_base_encode_many_units(oracle_mod, codec, width, bs, n):
+    # blocks of up to 4096 values are encoded in runs of blocks per workgroup, one
+    # look-back per run: many runs, a ragged last run and a ragged last block
+    import torch
+    from leco_fix.baselines import BaseColumn
+    rng = np.random.default_rng(bs * 7 + width)
+    vals = mixed(rng, n, width)
+    say(f"{codec} width={width} bs={bs} n={n} (many encode units)")
+    st, off = oracle_mod.base_encode_column(codec, vals, bs)
+    col = BaseColumn.encode(codec, to_dev(torch, vals), bs)
+    assert np.array_equal(host(col.offsets, np.uint64), off)
+    assert col.stream_bytes() == st.tobytes()
+    assert np.array_equal(host(col.decode(), vals.dtype), vals)
+
+
 def test_base_host_surface(oracle_mod):
     from leco_fix.baselines import Delta_int, FOR_int
     rng = np.random.default_rng(3)
