@@ -133,6 +133,26 @@ int leco_fix_lookup_u64(const uint8_t *d_stream, const leco_fix_block_param *d_p
                         uint64_t n, uint32_t block_size, const uint64_t *d_idx, uint64_t nq,
                         uint64_t *d_out, void *stream);
 
+/* Point lookups in large batches, the same results as leco_fix_lookup_*: the
+ * queries are first grouped by the range of 2^20 or more values they fall in
+ * (a counting sort of index/query pairs into d_work), then looked up group by
+ * group, so that each block's bytes and parameter entry are read from HBM
+ * about once per batch instead of once per query (per query, the reference's
+ * randomdecodeArray8, piecewise_fix_integer_template.h:305-337).  It moves
+ * less than half the HBM bytes of leco_fix_lookup_* on uniformly random
+ * queries, but takes longer there: each result is a scattered 8-byte write,
+ * which costs a 32-byte HBM write.  No counterpart in the reference.  d_work holds
+ * leco_fix_lookup_grouped_workspace(nq) bytes, n <= 2^33.  d_idx and d_out
+ * must not overlap d_work.  Asynchronous on `stream`; calls that share a
+ * workspace must be ordered. */
+uint64_t leco_fix_lookup_grouped_workspace(uint64_t nq);
+int leco_fix_lookup_grouped_u32(const uint8_t *d_stream, const leco_fix_block_param *d_params,
+                                uint64_t n, uint32_t block_size, const uint64_t *d_idx, uint64_t nq,
+                                uint32_t *d_out, void *d_work, uint64_t work_bytes, void *stream);
+int leco_fix_lookup_grouped_u64(const uint8_t *d_stream, const leco_fix_block_param *d_params,
+                                uint64_t n, uint32_t block_size, const uint64_t *d_idx, uint64_t nq,
+                                uint64_t *d_out, void *d_work, uint64_t work_bytes, void *stream);
+
 /* Waits for `stream` and reads the encoded size d_offsets[num_blocks] of an
  * encode enqueued on it (any codec and width): LECO_FIX_OK with
  * *stream_bytes set, or LECO_FIX_EDEVICE when the encoder poisoned the

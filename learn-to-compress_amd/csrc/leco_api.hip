@@ -55,6 +55,16 @@ int lookup(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, u
     return hip_status(leco::launch_lookup<T>(stream, prm, n, bs, idx, nq, out, (hipStream_t)st));
 }
 
+template <typename T>
+int lookup_grouped(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, uint32_t bs,
+                   const uint64_t *idx, uint64_t nq, T *out, void *work, uint64_t work_bytes, void *st) {
+    if (!valid_bs(bs) || n == 0 || n > leco::kGroupMaxValues || !stream || !prm) return LECO_FIX_EINVAL;
+    if (nq == 0) return LECO_FIX_OK;
+    if (!idx || !out || !work || work_bytes < leco::lookup_grouped_workspace(nq) || ((uintptr_t)work & 7))
+        return LECO_FIX_EINVAL;
+    return hip_status(leco::launch_lookup_grouped<T>(stream, prm, n, bs, idx, nq, out, work, (hipStream_t)st));
+}
+
 // ---- per-thread staging for the one-block host-buffer calls -------------
 
 struct Staging {
@@ -783,6 +793,19 @@ int leco_fix_lookup_u32(const uint8_t *d_stream, const leco_fix_block_param *d_p
 int leco_fix_lookup_u64(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n, uint32_t bs,
                         const uint64_t *d_idx, uint64_t nq, uint64_t *d_out, void *stream) {
     return lookup<uint64_t>(d_stream, d_params, n, bs, d_idx, nq, d_out, stream);
+}
+
+uint64_t leco_fix_lookup_grouped_workspace(uint64_t nq) { return leco::lookup_grouped_workspace(nq); }
+
+int leco_fix_lookup_grouped_u32(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n,
+                                uint32_t bs, const uint64_t *d_idx, uint64_t nq, uint32_t *d_out, void *d_work,
+                                uint64_t work_bytes, void *stream) {
+    return lookup_grouped<uint32_t>(d_stream, d_params, n, bs, d_idx, nq, d_out, d_work, work_bytes, stream);
+}
+int leco_fix_lookup_grouped_u64(const uint8_t *d_stream, const leco_fix_block_param *d_params, uint64_t n,
+                                uint32_t bs, const uint64_t *d_idx, uint64_t nq, uint64_t *d_out, void *d_work,
+                                uint64_t work_bytes, void *stream) {
+    return lookup_grouped<uint64_t>(d_stream, d_params, n, bs, d_idx, nq, d_out, d_work, work_bytes, stream);
 }
 
 int leco_fix_build_params(const uint8_t *d_stream, const uint64_t *d_offsets, uint64_t n, uint32_t bs,

@@ -213,6 +213,32 @@ hipError_t launch_decode_values(const uint8_t *stream, const leco_fix_block_para
 template <typename T>
 hipError_t launch_lookup(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, uint32_t bs,
                          const uint64_t *idx, uint64_t nq, T *out, hipStream_t st);
+
+// Grouped lookups: queries grouped by the range of 2^sh values they fall in
+// (bins groups, at most kGroupMaxBins; 2^sh >= 2^20 values), then looked up
+// group by group.  The workspace holds the group totals and cursors and one
+// 8-byte pair (index << 31 | query) per query of a pass; a pass takes at most
+// kGroupMaxQueries queries, and indices must fit 33 bits (n <= 2^33).
+constexpr uint32_t kGroupChunk = 32768;  // queries per workgroup in the counting passes
+constexpr uint32_t kXcds = 8;            // MI355X: accelerator dies; workgroup i runs on die i mod 8
+constexpr uint32_t kGroupMaxBins = 4096;
+constexpr uint64_t kGroupMaxQueries = 1ull << 31;
+constexpr uint64_t kGroupQueryMask = kGroupMaxQueries - 1;
+constexpr uint64_t kGroupMaxValues = 1ull << 33;
+struct LookupGroups {
+    uint32_t sh, bins;
+};
+__host__ __device__ inline LookupGroups lookup_groups(uint64_t n) {
+    uint32_t sh = 20;
+    while (((n + (1ull << sh) - 1) >> sh) > kGroupMaxBins) ++sh;
+    return {sh, (uint32_t)((n + (1ull << sh) - 1) >> sh)};
+}
+inline uint64_t lookup_grouped_workspace(uint64_t nq) {
+    return 2ull * kGroupMaxBins * sizeof(uint32_t) + 8ull * (nq < kGroupMaxQueries ? nq : kGroupMaxQueries);
+}
+template <typename T>
+hipError_t launch_lookup_grouped(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, uint32_t bs,
+                                 const uint64_t *idx, uint64_t nq, T *out, void *work, hipStream_t st);
 hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs,
                                uint32_t w, leco_fix_block_param *prm, hipStream_t st);
 

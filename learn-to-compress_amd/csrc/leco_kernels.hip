@@ -1326,6 +1326,150 @@ __global__ __launch_bounds__(kThreads) void k_lookup(const uint8_t *__restrict__
     out[q] = i < n ? (T)lookup_one<W>(st, prm, bs, i) : (T)0;
 }
 
+// Grouped lookups: a batch of random accesses (Leco_int<T>::randomdecodeArray8,
+// piecewise_fix_integer_template.h:305-337, once per query) with the queries
+// first grouped by the range of 2^sh values they fall in -- a counting sort of
+// (index, query) pairs -- and then looked up group by group.  A group's blocks
+// and parameter entries are then read from HBM about once for all of its
+// queries, instead of one 128-byte line (or two) per query; consecutive
+// workgroups share a group, so its bytes stay in the L2s and the Infinity
+// Cache while they run.  Queries outside [0, n) give 0, as k_lookup.
+//
+// The LDS atomics of the counting passes use registers of loaded indices:
+// each thread loads a batch of indices, issues the batch's atomics, and waits
+// for the LDS queue before the next batch's loads (a queued LDS atomic can
+// read its registers after a later load has refilled them).
+constexpr int kGroupBatch = 16;  // indices per thread loaded before their LDS atomics
+
+__global__ __launch_bounds__(kThreads) void k_group_clear(uint32_t *__restrict__ totals, uint32_t bins) {
+    for (uint32_t b = threadIdx.x; b < bins; b += kThreads) totals[b] = 0;
+}
+
+// Workgroup histogram of the groups of chunk `blockIdx.x` into hist (LDS).
+__device__ __forceinline__ void group_histogram(const uint64_t *__restrict__ idx, uint64_t q0, uint64_t q1,
+                                                uint64_t n, uint32_t sh, uint32_t *hist) {
+    for (uint64_t qb = q0 + threadIdx.x; qb < q1; qb += (uint64_t)kThreads * kGroupBatch) {
+        uint64_t v[kGroupBatch];
+#pragma unroll
+        for (int k = 0; k < kGroupBatch; ++k) {
+            const uint64_t q = qb + (uint64_t)k * kThreads;
+            v[k] = q < q1 ? idx[q] : n;
+        }
+#pragma unroll
+        for (int k = 0; k < kGroupBatch; ++k)
+            if (v[k] < n) atomicAdd(&hist[(uint32_t)(v[k] >> sh)], 1u);
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    }
+}
+
+// Pass 1: group sizes (a histogram per workgroup, added into totals).
+__global__ __launch_bounds__(kThreads) void k_group_count(const uint64_t *__restrict__ idx, uint64_t nq, uint64_t n,
+                                                          uint32_t sh, uint32_t bins, uint32_t *__restrict__ totals) {
+    __shared__ uint32_t hist[kGroupMaxBins];
+    for (uint32_t b = threadIdx.x; b < bins; b += kThreads) hist[b] = 0;
+    __syncthreads();
+    const uint64_t q0 = (uint64_t)blockIdx.x * kGroupChunk;
+    const uint64_t q1 = nq - q0 < kGroupChunk ? nq : q0 + kGroupChunk;
+    group_histogram(idx, q0, q1, n, sh, hist);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < bins; b += kThreads)
+        if (hist[b]) atomicAdd(&totals[b], hist[b]);
+}
+
+// Group starts: the exclusive scan of totals into cursor (one workgroup).
+__global__ __launch_bounds__(kThreads) void k_group_starts(const uint32_t *__restrict__ totals, uint32_t bins,
+                                                           uint32_t *__restrict__ cursor) {
+    __shared__ uint32_t wsum[kThreads / kWave];
+    constexpr uint32_t kPer = kGroupMaxBins / kThreads;
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid % kWave;
+    uint32_t s[kPer], run = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t b = tid * kPer + r;
+        s[r] = b < bins ? totals[b] : 0;
+        run += s[r];
+    }
+    const uint32_t inc = wave_inclusive_sum(run);
+    if (lane == kWave - 1) wsum[wid] = inc;
+    __syncthreads();
+    uint32_t o = inc - run;
+    for (int w = 0; w < wid; ++w) o += wsum[w];
+#pragma unroll
+    for (uint32_t r = 0; r < kPer; ++r) {
+        const uint32_t b = tid * kPer + r;
+        if (b < bins) cursor[b] = o;
+        o += s[r];
+    }
+}
+
+// Pass 2: each workgroup claims a range of every group it holds queries of
+// (one global atomic per group), then writes its pairs (index << 31 | query)
+// there, ranked by LDS atomics.  Out-of-range queries get 0 at once.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_group_scatter(const uint64_t *__restrict__ idx, uint64_t nq,
+                                                            uint64_t n, uint32_t sh, uint32_t bins,
+                                                            uint32_t *__restrict__ cursor,
+                                                            uint64_t *__restrict__ pairs, T *__restrict__ out) {
+    __shared__ uint32_t hist[kGroupMaxBins];
+    __shared__ uint32_t base[kGroupMaxBins];
+    for (uint32_t b = threadIdx.x; b < bins; b += kThreads) hist[b] = 0;
+    __syncthreads();
+    const uint64_t q0 = (uint64_t)blockIdx.x * kGroupChunk;
+    const uint64_t q1 = nq - q0 < kGroupChunk ? nq : q0 + kGroupChunk;
+    group_histogram(idx, q0, q1, n, sh, hist);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < bins; b += kThreads) {
+        const uint32_t h = hist[b];
+        base[b] = h ? atomicAdd(&cursor[b], h) : 0u;
+        hist[b] = 0;
+    }
+    __syncthreads();
+    for (uint64_t qb = q0 + threadIdx.x; qb < q1; qb += (uint64_t)kThreads * kGroupBatch) {
+        uint64_t v[kGroupBatch];
+#pragma unroll
+        for (int k = 0; k < kGroupBatch; ++k) {
+            const uint64_t q = qb + (uint64_t)k * kThreads;
+            v[k] = q < q1 ? idx[q] : n;
+        }
+#pragma unroll
+        for (int k = 0; k < kGroupBatch; ++k) {
+            const uint64_t q = qb + (uint64_t)k * kThreads;
+            if (q < q1) {
+                if (v[k] < n) {
+                    const uint32_t g = (uint32_t)(v[k] >> sh);
+                    const uint32_t r = atomicAdd(&hist[g], 1u);
+                    pairs[base[g] + r] = (v[k] << 31) | q;
+                } else {
+                    out[q] = (T)0;
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    }
+}
+
+// Pass 3: one pair per thread, in group order; the pair count is the last
+// group's end (cursor after pass 2).  Workgroups are dispatched to the 8 XCDs
+// in turn (workgroup i to XCD i mod 8), so workgroup i takes segment
+// (i mod 8) * (segments / 8) + i / 8: each XCD walks its own eighth of the
+// pairs in order, and its L2 holds only the few groups it is on (block reads
+// from HBM drop from 4.3 to 1.1 GB for 64M queries on 500M keys).  The pairs
+// stream past the caches (non-temporal); the results are plain stores, which
+// the L2 merges where it can (non-temporal 8-byte stores took 3.1 ms instead
+// of 2.1).
+template <typename T, int W>
+__global__ __launch_bounds__(kThreads) void k_group_lookup(const uint8_t *__restrict__ st,
+                                                           const leco_fix_block_param *__restrict__ prm,
+                                                           uint32_t bs, const uint64_t *__restrict__ pairs,
+                                                           const uint32_t *__restrict__ end, T *__restrict__ out) {
+    const uint32_t per_xcd = gridDim.x / kXcds;  // the grid is a multiple of kXcds
+    const uint64_t seg = (uint64_t)(blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+    const uint64_t p = seg * kThreads + threadIdx.x;
+    if (p >= *end) return;
+    const uint64_t pr = __builtin_nontemporal_load(&pairs[p]);
+    out[pr & kGroupQueryMask] = (T)lookup_one<W>(st, prm, bs, pr >> 31);
+}
+
 // Param table from block headers (for streams this library did not write).
 __global__ __launch_bounds__(kThreads) void k_build_params(const uint8_t *__restrict__ st,
                                                            const uint64_t *__restrict__ offs, uint64_t n,
@@ -1585,6 +1729,29 @@ hipError_t launch_lookup(const uint8_t *stream, const leco_fix_block_param *prm,
     return hipGetLastError();
 }
 
+template <typename T>
+hipError_t launch_lookup_grouped(const uint8_t *stream, const leco_fix_block_param *prm, uint64_t n, uint32_t bs,
+                                 const uint64_t *idx, uint64_t nq, T *out, void *work, hipStream_t st) {
+    constexpr int W = 8 * sizeof(T);
+    const LookupGroups gr = lookup_groups(n);
+    uint32_t *totals = (uint32_t *)work;
+    uint32_t *cursor = totals + kGroupMaxBins;
+    uint64_t *pairs = (uint64_t *)(cursor + kGroupMaxBins);
+    for (uint64_t c = 0; c < nq; c += kGroupMaxQueries) {
+        const uint64_t m = nq - c < kGroupMaxQueries ? nq - c : kGroupMaxQueries;
+        const unsigned chunks = (unsigned)((m + kGroupChunk - 1) / kGroupChunk);
+        k_group_clear<<<dim3(1), dim3(kThreads), 0, st>>>(totals, gr.bins);
+        k_group_count<<<dim3(chunks), dim3(kThreads), 0, st>>>(idx + c, m, n, gr.sh, gr.bins, totals);
+        k_group_starts<<<dim3(1), dim3(kThreads), 0, st>>>(totals, gr.bins, cursor);
+        k_group_scatter<T><<<dim3(chunks), dim3(kThreads), 0, st>>>(idx + c, m, n, gr.sh, gr.bins, cursor, pairs,
+                                                                    out + c);
+        const uint64_t segs = (m + kThreads - 1) / kThreads;
+        k_group_lookup<T, W><<<dim3((unsigned)((segs + kXcds - 1) / kXcds * kXcds)), dim3(kThreads), 0, st>>>(
+            stream, prm, bs, pairs, cursor + gr.bins - 1, out + c);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_build_params(const uint8_t *stream, const uint64_t *offs, uint64_t n, uint32_t bs, uint32_t w,
                                leco_fix_block_param *prm, hipStream_t st) {
     const uint64_t nb = (n + bs - 1) / bs;
@@ -1663,5 +1830,9 @@ template hipError_t launch_lookup<uint32_t>(const uint8_t *, const leco_fix_bloc
                                             const uint64_t *, uint64_t, uint32_t *, hipStream_t);
 template hipError_t launch_lookup<uint64_t>(const uint8_t *, const leco_fix_block_param *, uint64_t, uint32_t,
                                             const uint64_t *, uint64_t, uint64_t *, hipStream_t);
+template hipError_t launch_lookup_grouped<uint32_t>(const uint8_t *, const leco_fix_block_param *, uint64_t, uint32_t,
+                                                    const uint64_t *, uint64_t, uint32_t *, void *, hipStream_t);
+template hipError_t launch_lookup_grouped<uint64_t>(const uint8_t *, const leco_fix_block_param *, uint64_t, uint32_t,
+                                                    const uint64_t *, uint64_t, uint64_t *, void *, hipStream_t);
 
 }  // namespace leco

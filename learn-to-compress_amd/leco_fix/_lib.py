@@ -18,6 +18,7 @@ c_ptr = ctypes.c_void_p
 c_int = ctypes.c_int
 
 LECO_FIX_OK = 0
+LECO_FIX_EINVAL = 1
 LECO_FIX_ELIMIT = 4
 LECO_FIX_ESCRATCH = 5
 LECO_FIX_EDEVICE = 6
@@ -54,6 +55,8 @@ for _w in ("u32", "u64"):
         f"leco_fix_decode_range_ex_{_w}": (c_int, [c_ptr, c_ptr, c_ptr, c_u64, c_u32, c_u64, c_u64, c_int, c_ptr,
                                                    c_ptr]),
         f"leco_fix_lookup_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr]),
+        f"leco_fix_lookup_grouped_{_w}": (c_int, [c_ptr, c_ptr, c_u64, c_u32, c_ptr, c_u64, c_ptr, c_ptr, c_u64,
+                                                  c_ptr]),
         f"leco_fix_encode_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr, c_u64, ctypes.POINTER(c_u64)]),
         f"leco_fix_decode_block_{_w}": (c_int, [c_ptr, c_u64, c_ptr]),
         f"leco_fix_random_decode_{_w}": (c_int, [c_ptr, c_u32, c_ptr]),
@@ -69,6 +72,7 @@ for _w in ("u32", "u64"):
                                                       ctypes.POINTER(c_u64)]),
     })
 SIGNATURES["leco_fix_filter_workspace_bytes"] = (c_u64, [c_u64, c_u32])
+SIGNATURES["leco_fix_lookup_grouped_workspace"] = (c_u64, [c_u64])
 SIGNATURES["leco_fix_encode_result"] = (c_int, [c_ptr, c_u64, ctypes.POINTER(c_u64), c_ptr])
 SIGNATURES["leco_fix_test_lookback_hook"] = (c_int, [c_u32, c_u64])
 SIGNATURES["leco_fix_filter_range_noalloc"] = (c_int, [c_u32, c_ptr, c_ptr, c_u64, c_u32, c_u64, c_ptr, c_u64, c_ptr,

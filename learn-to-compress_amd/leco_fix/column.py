@@ -111,17 +111,33 @@ class Column:
                  begin, end, shape, out.data_ptr(), _stream_handle(stream)), "decode_range")
         return out
 
-    def lookup(self, idx: torch.Tensor, out: torch.Tensor | None = None, stream=None) -> torch.Tensor:
-        """Leco_int<T>::randomdecodeArray8 for every index in idx (int64 device tensor)."""
+    def lookup(self, idx: torch.Tensor, out: torch.Tensor | None = None, stream=None,
+               grouped: bool = False) -> torch.Tensor:
+        """Leco_int<T>::randomdecodeArray8 for every index in idx (int64 device tensor).
+
+        grouped: group the queries by value range first (leco_fix_lookup_grouped_*,
+        8 bytes of workspace per query), so that each block is read from HBM about
+        once per batch: 2.4x less HBM traffic, but about 30% slower on MI355X (each
+        scattered 8-byte result costs a 32-byte HBM write).  Off unless asked for."""
         idx = idx.contiguous()
         if idx.dtype not in (torch.int64, torch.uint64):
             idx = idx.to(torch.int64)
         if out is None:
             out = torch.empty(idx.numel(), dtype=_STORAGE[self.width], device=self.stream.device)
         L = lib()
+        nq = idx.numel()
+        if grouped:
+            ws = int(L.leco_fix_lookup_grouped_workspace(nq))
+            work = torch.empty(ws, dtype=torch.uint8, device=self.stream.device)
+            if stream is not None:
+                work.record_stream(stream)  # not reused before the lookups on `stream` are done
+            fn = L.leco_fix_lookup_grouped_u32 if self.width == 32 else L.leco_fix_lookup_grouped_u64
+            check(fn(self.stream.data_ptr(), self.params.data_ptr(), self.n, self.block_size, idx.data_ptr(),
+                     nq, out.data_ptr(), work.data_ptr(), ws, _stream_handle(stream)), "lookup_grouped")
+            return out
         fn = L.leco_fix_lookup_u32 if self.width == 32 else L.leco_fix_lookup_u64
         check(fn(self.stream.data_ptr(), self.params.data_ptr(), self.n, self.block_size, idx.data_ptr(),
-                 idx.numel(), out.data_ptr(), _stream_handle(stream)), "lookup")
+                 nq, out.data_ptr(), _stream_handle(stream)), "lookup")
         return out
 
     def _filter(self, close: bool, filt, f1, f2, base: int, out, capacity, stream):

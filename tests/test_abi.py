@@ -109,3 +109,21 @@ print("rc", rc)
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     assert "rc 3" in r.stdout, r.stdout
+
+
+def test_grouped_lookup_workspace_and_argument_checks():
+    # no device memory is touched: every call below is refused before any launch
+    from leco_fix._lib import LECO_FIX_EINVAL, lib
+    L = lib()
+    head = 2 * 4096 * 4
+    for nq in (0, 1, 1000, 1 << 31, (1 << 31) + 5, 1 << 40):
+        assert L.leco_fix_lookup_grouped_workspace(nq) == head + 8 * min(nq, 1 << 31)
+    fake = 1 << 20  # an address the checks reject the call before using
+    ws = L.leco_fix_lookup_grouped_workspace(100)
+    args = (fake, fake, 1000, 16, fake, 100, fake)
+    assert L.leco_fix_lookup_grouped_u64(*args, fake, ws - 1, None) == LECO_FIX_EINVAL   # short workspace
+    assert L.leco_fix_lookup_grouped_u64(*args, fake + 4, ws, None) == LECO_FIX_EINVAL   # misaligned workspace
+    assert L.leco_fix_lookup_grouped_u64(fake, fake, (1 << 33) + 1, 16, fake, 100, fake, fake, ws, None) == \
+        LECO_FIX_EINVAL                                                                   # indices past 33 bits
+    assert L.leco_fix_lookup_grouped_u32(fake, fake, 1000, 0, fake, 100, fake, fake, ws, None) == LECO_FIX_EINVAL
+    assert L.leco_fix_lookup_grouped_u32(fake, fake, 1000, 16, fake, 0, fake, fake, 0, None) == 0  # empty batch
