@@ -22,6 +22,7 @@ constexpr uint32_t kWaveChunk = kWaveVPL * kWave;   // values a wave packs throu
 constexpr int kWaveLdsBytes = kLdsBytes / (kThreads / kWave);  // one wave's share of the LDS image
 static_assert(kMultiLds + 16 <= kLdsBytes, "staged small-block bytes plus one word of field over-read fit the LDS buffer");
 constexpr int kMultiUnroll = 4;                     // positions per thread in flight in small-block decode
+constexpr uint32_t kWaveDecodeMin = 128;            // small blocks of at least this many values: one wave per block
 constexpr int kLookPer = 4;                         // predecessors read per lane per look-back step
 constexpr int kLookWin = kWave * kLookPer;          // predecessors per look-back step (wave 0)
 
