@@ -1058,6 +1058,73 @@ __device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st
     }
 }
 
+// One step of decode_multi_waves: U positions per lane, pb + lane + 64u, of
+// the block at tile position pk (header values uniform), up to b.
+template <int U, int CODEC, typename T, int W>
+__device__ __forceinline__ void decode_wave_step(const MultiShared &sm, uint32_t pb, uint32_t b, uint32_t pk,
+                                                 uint32_t fb, uint32_t lw, uint32_t l, bool raw, double t0,
+                                                 double t1, T *__restrict__ o) {
+    const int lane = threadIdx.x % kWave;
+    if constexpr (W == 32) {
+        const uint32_t *b32 = (const uint32_t *)sm.buf;
+        uint32_t x0[U], x1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            uint32_t p = pb + lane + u * kWave;
+            p = p < b ? p : b - 1;
+            const uint32_t q = fb + (p - pk) * lw;
+            x0[u] = b32[q >> 5];
+            x1[u] = b32[(q >> 5) + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t p = pb + lane + u * kWave;
+            const uint32_t j = p - pk;
+            const uint32_t q = fb + j * lw;
+            const uint32_t w = __builtin_amdgcn_alignbit(x1[u], x0[u], q & 31);
+            uint32_t val;
+            if constexpr (CODEC == kCodecFor) {
+                val = raw ? w : (uint32_t)double_to_bits(t0) + (w & ((1u << l) - 1));
+            } else {
+                const uint32_t pr = predict_lo32(t0, t1, j);
+                const uint32_t lc = l > 0 ? l - 1 : 0;
+                const uint32_t mag = w & ((1u << lc) - 1);
+                const uint32_t v = ((w >> lc) & 1) ? pr + mag : pr - mag;
+                val = raw ? w : (l ? v : pr);
+            }
+            if (p < b) __builtin_nontemporal_store((T)val, &o[p]);
+        }
+    } else {
+        uint64_t w0[U], w1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            uint32_t p = pb + lane + u * kWave;
+            p = p < b ? p : b - 1;
+            const uint32_t q = fb + (p - pk) * lw;
+            w0[u] = sm.buf[q >> 6];
+            w1[u] = sm.buf[(q >> 6) + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t p = pb + lane + u * kWave;
+            const uint32_t j = p - pk;
+            const int sh = (int)((fb + j * lw) & 63);
+            const uint64_t field = ((w0[u] >> sh) | ((w1[u] << 1) << (63 - sh))) & low_mask((int)lw);
+            uint64_t val;
+            if constexpr (CODEC == kCodecFor) {
+                val = raw ? field : double_to_bits(t0) + field;
+            } else {
+                const int64_t pr = predict(t0, t1, j);
+                const int lc = l > 0 ? (int)l : 1;
+                const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
+                const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
+                val = raw ? field : (l ? v : (uint64_t)pr);
+            }
+            if (p < b) __builtin_nontemporal_store((T)val, &o[p]);
+        }
+    }
+}
+
 // The position loop of decode_multi_body for blocks of kWaveDecodeMin or more
 // values, a multiple of 64: each wave takes whole blocks (block ks + wid, then
 // every kThreads/kWave-th), so the block's header is wave-uniform (scalar
@@ -1070,7 +1137,7 @@ __device__ __forceinline__ void decode_block_body(const uint8_t *__restrict__ st
 template <int CODEC, typename T, int W>
 __device__ __forceinline__ void decode_multi_waves(const MultiShared &sm, uint32_t ks, uint32_t ke, uint32_t bs,
                                                    uint32_t p0, uint32_t pend, T *__restrict__ o) {
-    const int wid = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int wid = threadIdx.x / kWave;
     for (uint32_t k = ks + wid; k < ke; k += kThreads / kWave) {
         const uint32_t lbk = __builtin_amdgcn_readfirstlane(sm.hlb[k - ks]);
         const double t0 = bits_to_double(uniform_u64(double_to_bits(sm.ht[k - ks].x)));
@@ -1086,66 +1153,17 @@ __device__ __forceinline__ void decode_multi_waves(const MultiShared &sm, uint32
             const uint32_t e = pk + (128 - l);
             b = b < e ? b : e;
         }
-        for (uint32_t pb = a; pb < b; pb += kMultiUnroll * kWave) {
-            if constexpr (W == 32) {
-                const uint32_t *b32 = (const uint32_t *)sm.buf;
-                uint32_t x0[kMultiUnroll], x1[kMultiUnroll];
-#pragma unroll
-                for (int u = 0; u < kMultiUnroll; ++u) {
-                    uint32_t p = pb + lane + u * kWave;
-                    p = p < b ? p : b - 1;
-                    const uint32_t q = fb + (p - pk) * lw;
-                    x0[u] = b32[q >> 5];
-                    x1[u] = b32[(q >> 5) + 1];
-                }
-#pragma unroll
-                for (int u = 0; u < kMultiUnroll; ++u) {
-                    const uint32_t p = pb + lane + u * kWave;
-                    const uint32_t j = p - pk;
-                    const uint32_t q = fb + j * lw;
-                    const uint32_t w = __builtin_amdgcn_alignbit(x1[u], x0[u], q & 31);
-                    uint32_t val;
-                    if constexpr (CODEC == kCodecFor) {
-                        val = raw ? w : (uint32_t)double_to_bits(t0) + (w & ((1u << l) - 1));
-                    } else {
-                        const uint32_t pr = predict_lo32(t0, t1, j);
-                        const uint32_t lc = l > 0 ? l - 1 : 0;
-                        const uint32_t mag = w & ((1u << lc) - 1);
-                        const uint32_t v = ((w >> lc) & 1) ? pr + mag : pr - mag;
-                        val = raw ? w : (l ? v : pr);
-                    }
-                    if (p < b) __builtin_nontemporal_store((T)val, &o[p]);
-                }
-            } else {
-                uint64_t w0[kMultiUnroll], w1[kMultiUnroll];
-#pragma unroll
-                for (int u = 0; u < kMultiUnroll; ++u) {
-                    uint32_t p = pb + lane + u * kWave;
-                    p = p < b ? p : b - 1;
-                    const uint32_t q = fb + (p - pk) * lw;
-                    w0[u] = sm.buf[q >> 6];
-                    w1[u] = sm.buf[(q >> 6) + 1];
-                }
-#pragma unroll
-                for (int u = 0; u < kMultiUnroll; ++u) {
-                    const uint32_t p = pb + lane + u * kWave;
-                    const uint32_t j = p - pk;
-                    const int sh = (int)((fb + j * lw) & 63);
-                    const uint64_t field = ((w0[u] >> sh) | ((w1[u] << 1) << (63 - sh))) & low_mask((int)lw);
-                    uint64_t val;
-                    if constexpr (CODEC == kCodecFor) {
-                        val = raw ? field : double_to_bits(t0) + field;
-                    } else {
-                        const int64_t pr = predict(t0, t1, j);
-                        const int lc = l > 0 ? (int)l : 1;
-                        const uint64_t mag = field & ((1ull << (lc - 1)) - 1);
-                        const uint64_t v = ((field >> (lc - 1)) & 1) ? (uint64_t)pr + mag : (uint64_t)pr - mag;
-                        val = raw ? field : (l ? v : (uint64_t)pr);
-                    }
-                    if (p < b) __builtin_nontemporal_store((T)val, &o[p]);
-                }
-            }
+        uint32_t pb = a;
+        for (; pb + kMultiUnroll * kWave <= b; pb += kMultiUnroll * kWave)
+            decode_wave_step<kMultiUnroll, CODEC, T, W>(sm, pb, b, pk, fb, lw, l, raw, t0, t1, o);
+        // the rest of the block (fewer than kMultiUnroll*64 positions): two or one
+        // positions per lane, so that no step runs with idle lanes when the rest is
+        // a multiple of 64
+        if (pb + 2 * kWave <= b) {
+            decode_wave_step<2, CODEC, T, W>(sm, pb, b, pk, fb, lw, l, raw, t0, t1, o);
+            pb += 2 * kWave;
         }
+        for (; pb < b; pb += kWave) decode_wave_step<1, CODEC, T, W>(sm, pb, b, pk, fb, lw, l, raw, t0, t1, o);
     }
 }
 
