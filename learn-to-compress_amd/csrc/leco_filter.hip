@@ -963,8 +963,10 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt_mark(FilterArgs a) {
         }
         // bit offset of field 0 of this block in LDS (staged) or in the block
         const int64_t fbit = words ? ((int64_t)(off + pay) - (int64_t)a0) * 8 : (int64_t)pay * 8;
-        auto field = [&](uint32_t j) -> uint64_t {
-            if (!words) return load_bits(blk, (uint64_t)fbit + (uint64_t)j * lf, (int)lf);
+        // `staged`: std::true_type (fields from the LDS copy) or std::false_type
+        // (from global memory), so that each round loop holds only its own reads
+        auto field = [&](uint32_t j, auto staged) -> uint64_t {
+            if constexpr (!decltype(staged)::value) return load_bits(blk, (uint64_t)fbit + (uint64_t)j * lf, (int)lf);
             const uint32_t q = (uint32_t)(fbit + (int64_t)j * lf), w = q >> 5, sft = q & 31;
             const uint32_t x0 = lds[w], x1 = lds[w + 1];
             uint64_t v = __builtin_amdgcn_alignbit(x1, x0, sft);
@@ -972,16 +974,16 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt_mark(FilterArgs a) {
             return v & low_mask((int)lf);
         };
         const uint32_t rel = (uint32_t)(s - tile_start);  // tile bit of block position 0
-        for (uint32_t p = lo; p < hi; p += kThreads) {
+        auto round = [&](uint32_t p, auto staged) {
             const uint32_t j = p + tid;
             bool h = false;
             if (j < hi) {
                 uint64_t v;
-                if (raw) v = field(j);
-                else if (FOR) v = l ? (base_v + field(j)) & wmask : base_v;
+                if (raw) v = field(j, staged);
+                else if (FOR) v = l ? (base_v + field(j, staged)) & wmask : base_v;
                 else {
                     const int64_t pr = predict(t0, t1, j);
-                    v = l ? apply_field(pr, field(j), l) : (uint64_t)pr;
+                    v = l ? apply_field(pr, field(j, staged), l) : (uint64_t)pr;
                 }
                 h = (raw || FOR) ? v > f : above<W>(v, f);
             }
@@ -996,9 +998,20 @@ __global__ __launch_bounds__(kThreads) void k_filter_gt_mark(FilterArgs a) {
                                                   : (sft ? (uint32_t)(bal >> (64 - sft)) : 0u);
                 if (part) atomicOr(&sh.bits[(q >> 5) + lane], part);
             }
-            // the ORs done before any later load (a queued LDS atomic can read its
-            // registers after a later load has refilled them)
+        };
+        // The ORs are done before any later load: a queued LDS atomic can read its
+        // registers after a later load has refilled them.  Fields from LDS: one wait
+        // after the block (before the next block's header loads); fields from
+        // global memory: a wait after every round (separate loops, so that no
+        // path leads from an OR to a load without one).
+        if (words) {
+            for (uint32_t p = lo; p < hi; p += kThreads) round(p, std::true_type());
             __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        } else {
+            for (uint32_t p = lo; p < hi; p += kThreads) {
+                round(p, std::false_type());
+                __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+            }
         }
     }
     __syncthreads();
