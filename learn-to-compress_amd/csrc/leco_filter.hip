@@ -469,11 +469,13 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
             const uint32_t nwin = close_nwin(c, K);
             const uint32_t span = (W == 64 ? c.f2 > c.f1 : (int64_t)c.f2 > (int64_t)c.f1)
                                       ? (uint32_t)(c.f2 - c.f1 - 1) : 0u;
-            uint64_t bad = 0;
-            for (uint32_t r = 0; r < nr; ++r) {
+            uint64_t bad = 0, hits = 0;  // hits: the wave's count, wave-uniform (from the ballots)
+            // one round of kThreads positions; FULL: every position of the round is below mp
+            auto round = [&](uint32_t r, auto full) {
+                constexpr bool FULL = decltype(full)::value;
                 const uint32_t i = r * kThreads + tid;
-                const bool on = i < mp;
-                const uint64_t v = value(on ? i : mp - 1);
+                const bool on = FULL || i < mp;
+                const uint64_t v = value(FULL || on ? i : mp - 1);
                 const uint64_t pv = wave_prev_u64(v);
                 bad |= __ballot(on && (W == 64 ? pv > v : (int64_t)pv > (int64_t)v));
                 // k: the last period whose f1 lies below v; periods from K on never pass
@@ -491,12 +493,14 @@ __device__ __forceinline__ bool close_block(const FilterArgs &a, FilterShared &s
                                      c.t1, c.m, &thre1, &thre2);
                 }
                 const bool h = in_k && rem < span && i >= (uint32_t)thre1 && i < (uint32_t)thre2;
-                cnt += h;
-                if (park) {
-                    const uint64_t bal = __ballot(h);
-                    if (lane == 0) sh.rb[(rounds_used + r) * kWaves + wid] = bal;
-                }
-            }
+                const uint64_t bal = __ballot(h);
+                hits += (uint64_t)__popcll(bal);
+                if (park && lane == 0) sh.rb[(rounds_used + r) * kWaves + wid] = bal;
+            };
+            const uint32_t nfull = mp / kThreads;
+            for (uint32_t r = 0; r < nfull; ++r) round(r, std::true_type{});
+            if (nfull < nr) round(nfull, std::false_type{});
+            if (lane == 0) cnt += hits;  // the workgroup sums every thread's count
             sorted = (bad & ~1ull) == 0;  // lane 0 compared nothing
             for (uint32_t j = tid; 64 * (j + 1) < mp; j += kThreads) {
                 const uint64_t x = value(64 * j + 63), y = value(64 * j + 64);
